@@ -1,0 +1,425 @@
+"""Precision matrices, parameter derivatives, log-likelihood and sampling.
+
+Oracle restatement (test infrastructure) of
+  * ``spdes/whittle_matern{,_anisotropic,_ha}2D.py``  makeQ  (initial-state Q0)
+  * ``spdes/advection_{i,,ha_}diffusion2D.py``       makeQ / logLike (stationary)
+  * ``spdes/var_advection_var_diffusion2D.py``         makeQ / logLike (B-spline fields)
+  * ``model.py:73-153``                                sample / update
+using scipy.sparse operations in the reference's order, so the sparsity
+pattern (scipy drops exact zeros in products and sums, keeps them in COO
+construction and ``bmat``) is the reference's.
+"""
+import numpy as np
+from scipy import sparse
+from scipy.sparse import linalg as spla
+
+from . import stencils as st
+
+
+# --------------------------------------------------------------------- H maps
+def h_map(kind, gamma, vx=0.0, vy=0.0):
+    """2x2 diffusion tensor and its derivatives w.r.t. (log gamma, vx, vy).
+
+    iso: gamma I  (``advection_idiffusion2D.py:97``)
+    ani: gamma I + v v^T  (``advection_diffusion2D.py:100``)
+    ha : gamma (cosh|v| I + sinh|v|/|v| [[vx,vy],[vy,-vx]])  (``advection_ha_diffusion2D.py:109-115``)
+    """
+    I = np.eye(2)
+    if kind == "iso":
+        return gamma * I, [gamma * I]
+    if kind == "ani":
+        v = np.array([vx, vy])
+        H = gamma * I + v[:, None] * v[None, :]
+        d = []
+        for e in (np.array([1.0, 0.0]), np.array([0.0, 1.0])):
+            d.append(e[:, None] * v[None, :] + v[:, None] * e[None, :])
+        return H, [gamma * I] + d
+    aV = np.sqrt(vx ** 2 + vy ** 2)
+    mV = np.array([[vx, vy], [vy, -vx]])
+    ch = (np.exp(aV) + np.exp(-aV)) / 2
+    sh = (np.exp(aV) - np.exp(-aV)) / 2
+    H = gamma * (ch * I + sh / aV * mV)
+    dvx = gamma / aV * (vx * sh * I + vx / aV * (ch - sh / aV) * mV + sh * np.array([[1, 0], [0, -1]]))
+    dvy = gamma / aV * (vy * sh * I + vy / aV * (ch - sh / aV) * mV + sh * np.array([[0, 1], [1, 0]]))
+    return H, [H, dvx, dvy]
+
+
+def ah_const_csc(grid2, H, bc):
+    M, N = grid2.shape[0], grid2.shape[1]
+    return st.triplets_to_csc(*st.ah_const(M, N, H, grid2.hx, grid2.hy, bc), M * N)
+
+
+def ah_var_csc(grid2, H, bc):
+    M, N = grid2.shape[0], grid2.shape[1]
+    return st.triplets_to_csc(*st.ah_var(M, N, H, grid2.hx, grid2.hy, bc), M * N)
+
+
+def aw_const_csc(grid2, G, bc, diff=3):
+    M, N = grid2.shape[0], grid2.shape[1]
+    return st.triplets_to_csc(*st.aw_const(M, N, G, grid2.hx, grid2.hy, diff, bc), M * N)
+
+
+def aw_var_csc(grid2, G, bc, diff=3, dG=None):
+    M, N = grid2.shape[0], grid2.shape[1]
+    return st.triplets_to_csc(*st.aw_var(M, N, G, grid2.hx, grid2.hy, diff, bc, dG), M * N,
+                              nan_to_zero=True)
+
+
+# ------------------------------------------------------------ Whittle-Matern
+class WhittleMatern:
+    """Spatial Matern precision Q0 = A0^T iDv A0, A0 = Dv kappa - A_H(H0)
+    (``whittle_matern2D.py:59-85``, ``whittle_matern_anisotropic2D.py:61-104``,
+    ``whittle_matern_ha2D.py:68-110``).  par = [log kappa, log gamma, (vx, vy), log tau]."""
+
+    def __init__(self, grid, kind="iso", par=None, bc=3):
+        self.grid, self.kind, self.bc = grid, kind, bc
+        self.npar = 3 if kind == "iso" else 5
+        self.par = np.asarray(par, dtype=np.float64) if par is not None else None
+        self.Q = None
+        if par is not None:
+            self.Q, _ = self.makeQ(self.par, grad=False)
+
+    def getPars(self):
+        return self.par
+
+    def A0(self, par):
+        g = self.grid
+        kappa, gamma = np.exp(par[0]), np.exp(par[1])
+        vx, vy = (par[2], par[3]) if self.kind != "iso" else (0.0, 0.0)
+        H, dH = h_map(self.kind, gamma, vx, vy)
+        Dk = kappa * sparse.eye(g.Ns)
+        return g.Dv @ Dk - ah_const_csc(g, H, self.bc), H, dH, Dk
+
+    def makeQ(self, par, grad=True):
+        g = self.grid
+        A, H, dH, Dk = self.A0(par)
+        Q = A.transpose() @ g.iDv @ A
+        if not grad:
+            return Q, None
+        dQ = []
+        dA = (g.Dv @ Dk).tocsc()
+        dQ.append(dA.T @ g.iDv @ A + A.T @ g.iDv @ dA)
+        for dh in dH:
+            dA = -ah_const_csc(g, dh, self.bc)
+            dQ.append(dA.T @ g.iDv @ A + A.T @ g.iDv @ dA)
+        return Q, dQ
+
+
+# --------------------------------------------------------- space-time blocks
+def assemble_blocks(A, Qs, Q0, iDv, T, sigma, dt, diag0=None):
+    """Block-tridiagonal space-time precision (``advection_idiffusion2D.py:110-115``):
+        Q00 = sigma dt Q0 + Qs,  Qtt = A^T iDv Qs iDv A + Qs,  Q_{T-1} = A^T iDv Qs iDv A,
+        Q_{t,t+1} = -Qs iDv A,   Q_{t+1,t} = -A^T iDv Qs,  all scaled by 1/(dt sigma).
+    One ``bmat`` call; the per-block patterns are those of the reference's
+    incremental concatenation."""
+    up = -Qs @ iDv @ A
+    lo = -A.T @ iDv @ Qs
+    mid = A.T @ iDv @ Qs @ iDv @ A + Qs
+    last = A.T @ iDv @ Qs @ iDv @ A
+    first = sigma * dt * Q0 + Qs if diag0 is None else diag0
+    blocks = [[None] * T for _ in range(T)]
+    for t in range(T):
+        if t == 0:
+            blocks[t][t] = first
+        elif t == T - 1:
+            blocks[t][t] = last
+        else:
+            blocks[t][t] = mid
+        if t + 1 < T:
+            blocks[t][t + 1] = up
+            blocks[t + 1][t] = lo
+    return sparse.bmat(blocks).tocsc()
+
+
+def d_blocks(A, Qs, iDv, T, dA, dQs=None):
+    """Derivative of the block formula for a perturbation (dA, dQs)
+    (``advection_idiffusion2D.py:120-155``), unscaled."""
+    Ns = A.shape[0]
+    if dQs is None:
+        first = sparse.csc_matrix((Ns, Ns))
+        up = -Qs @ iDv @ dA
+        lo = -dA.T @ iDv @ Qs
+        mid = dA.T @ iDv @ Qs @ iDv @ A + A.T @ iDv @ Qs @ iDv @ dA
+        last = mid
+    else:
+        first = dQs
+        up = -dQs @ iDv @ A - Qs @ iDv @ dA
+        lo = -dA.T @ iDv @ Qs - A.T @ iDv @ dQs
+        last = dA.T @ iDv @ Qs @ iDv @ A + A.T @ iDv @ dQs @ iDv @ A + A.T @ iDv @ Qs @ iDv @ dA
+        mid = last + dQs
+    blocks = [[None] * T for _ in range(T)]
+    for t in range(T):
+        blocks[t][t] = first if t == 0 else (last if t == T - 1 else mid)
+        if t + 1 < T:
+            blocks[t][t + 1] = up
+            blocks[t + 1][t] = lo
+    return sparse.bmat(blocks).tocsc()
+
+
+def pad_q0(dQ0, Ns, T):
+    """mod0 derivative placed in block (0,0) (``advection_idiffusion2D.py:157-162``)."""
+    blocks = [[None] * T for _ in range(T)]
+    blocks[0][0] = dQ0
+    for t in range(1, T):
+        blocks[t][t] = sparse.csc_matrix((Ns, Ns))
+    return sparse.bmat(blocks).tocsc()
+
+
+class _STBase:
+    def _q0(self, par_mod0, grad):
+        if par_mod0 is not None:
+            return self.mod0.makeQ(par_mod0, grad=grad)
+        return self.mod0.Q, None
+
+
+class StationaryAdvDiff(_STBase):
+    """Stationary advection-diffusion, kind in {iso, ani, ha}.
+    par (iso) = [log kappa, log gamma, wx, wy, log sigma, (mod0...), log tau]
+    par (ani/ha) = [log kappa, log gamma, vx, vy, wx, wy, log sigma, (mod0...), log tau]."""
+
+    def __init__(self, grid, mod0, kind="iso", bc=3):
+        self.grid, self.mod0, self.kind, self.bc = grid, mod0, kind, bc
+        self.nself = 6 if kind == "iso" else 8
+
+    def split(self, par):
+        par = np.asarray(par, dtype=np.float64)
+        if self.kind == "iso":
+            kap, gam, vx, vy, ws, sig = par[0], par[1], 0.0, 0.0, par[2:4], par[4]
+            p0 = par[5:] if par.size > 6 else None
+        else:
+            kap, gam, vx, vy, ws, sig = par[0], par[1], par[2], par[3], par[4:6], par[6]
+            p0 = par[7:] if par.size > 8 else None
+        return kap, gam, vx, vy, ws, sig, p0
+
+    def makeQ(self, par, grad=True):
+        g = self.grid
+        kap, gam, vx, vy, ws, sig = self.split(par)[:6]
+        p0 = self.split(par)[6]
+        dt, T, Ns, Dv, iDv = g.dt, g.T, g.Ns, g.Dv, g.iDv
+        kappa, gamma, sigma = np.exp(kap), np.exp(gam), np.exp(sig)
+        H, dH = h_map(self.kind, gamma, vx, vy)
+        Dk = kappa * sparse.eye(Ns)
+        A_H = ah_const_csc(g, H, self.bc)
+        As = Dv @ Dk
+        Qs = As.transpose() @ iDv @ As
+        A = Dv + Dv @ Dk * dt - A_H * dt + aw_const_csc(g, ws, self.bc) * dt
+        Q0, dQ0 = self._q0(p0, grad)
+        Q = 1 / (dt * sigma) * assemble_blocks(A, Qs, Q0, iDv, T, sigma, dt)
+        if not grad:
+            return Q, None
+        c = 1 / (dt * sigma)
+        dQ = []
+        dA = (Dv @ Dk * dt).tocsc()
+        dAs = Dv @ Dk
+        dQs = dAs.T @ iDv @ As + As.T @ iDv @ dAs
+        dQ.append(c * d_blocks(A, Qs, iDv, T, dA, dQs))
+        for dh in dH:
+            dQ.append(c * d_blocks(A, Qs, iDv, T, -ah_const_csc(g, dh, self.bc) * dt))
+        dQ.append(c * d_blocks(A, Qs, iDv, T, aw_const_csc(g, ws, self.bc, diff=1) * dt))
+        dQ.append(c * d_blocks(A, Qs, iDv, T, aw_const_csc(g, ws, self.bc, diff=2) * dt))
+        full = assemble_blocks(A, Qs, Q0, iDv, T, sigma, dt, diag0=Qs)
+        dQ.append(-c * full)
+        if dQ0 is not None:
+            dQ += [pad_q0(d, Ns, T) for d in dQ0]
+        return Q, dQ
+
+
+class VarAdvVarDiff(_STBase):
+    """Non-stationary model with B-spline fields (``var_advection_var_diffusion2D.py``).
+    par = [c_kappa(Np), c_gamma(Np), c_vx(Np), c_vy(Np), c_wx(Np), c_wy(Np), log sigma,
+           (mod0...), log tau]."""
+
+    def __init__(self, grid, mod0, bc=3):
+        self.grid, self.mod0, self.bc = grid, mod0, bc
+        self.Np = grid.Nbs2
+
+    def fields(self, par):
+        g, Np = self.grid, self.Np
+        kappa = np.exp(g.evalB(par[:Np]))
+        gamma = np.exp(g.evalBH(par[Np:2 * Np]))
+        vx = g.evalBH(par[2 * Np:3 * Np])
+        vy = g.evalBH(par[3 * Np:4 * Np])
+        ws = g.evalAdv(par[4 * Np:6 * Np])
+        return kappa, gamma, vx, vy, ws
+
+    @staticmethod
+    def h_field(gamma, vx, vy):
+        vv = np.stack([vx, vy], axis=2)
+        return (np.eye(2) * np.stack([gamma, gamma], axis=2)[:, :, :, None]
+                + vv[:, :, :, None] * vv[:, :, None, :])
+
+    def makeQ(self, par, grad=True):
+        g, Np = self.grid, self.Np
+        par = np.asarray(par, dtype=np.float64)
+        dt, T, Ns, Dv, iDv = g.dt, g.T, g.Ns, g.Dv, g.iDv
+        kappa, gamma, vx, vy, ws = self.fields(par)
+        sigma = np.exp(par[Np * 6])
+        Hs = self.h_field(gamma, vx, vy)
+        Dk = sparse.diags(kappa).tocsc()
+        As = Dv @ Dk
+        Qs = As.transpose() @ iDv @ As
+        A = Dv + (Dv @ Dk - ah_var_csc(g, Hs, self.bc) + aw_var_csc(g, ws, self.bc)) * dt
+        p0 = par[Np * 6 + 1:] if par.size > Np * 6 + 2 else None
+        Q0, dQ0 = self._q0(p0, grad)
+        Q = assemble_blocks(A, Qs, Q0, iDv, T, sigma, dt).tocsc() / (dt * sigma)
+        if not grad:
+            return Q, None
+        c = dt * sigma
+        dQ = []
+        for i in range(Np):
+            dk = sparse.diags(g.bs[:, i] * kappa).tocsc()
+            dA = dt * Dv @ dk
+            dAs = Dv @ dk
+            dQs = dAs.T @ iDv @ As + As.T @ iDv @ dAs
+            dQ.append(d_blocks(A, Qs, iDv, T, dA, dQs) / c)
+        for i in range(Np):
+            dHs = np.eye(2) * (np.stack([g.bsH[:, :, i] * gamma] * 2, axis=2)[:, :, :, None])
+            dQ.append(d_blocks(A, Qs, iDv, T, -ah_var_csc(g, dHs, self.bc) * dt) / c)
+        vv = np.stack([vx, vy], axis=2)
+        for comp in (0, 1):
+            for i in range(Np):
+                e = np.zeros(Np)
+                e[i] = 1.0
+                b = g.evalBH(e)
+                z = np.zeros_like(b)
+                dv = np.stack([b, z] if comp == 0 else [z, b], axis=2)
+                dHs = vv[:, :, :, None] * dv[:, :, None, :] + dv[:, :, :, None] * vv[:, :, None, :]
+                dQ.append(d_blocks(A, Qs, iDv, T, -ah_var_csc(g, dHs, self.bc) * dt) / c)
+        for i in range(2 * Np):
+            e = np.zeros(2 * Np)
+            e[i] = 1.0
+            dws = g.evalAdv(e)
+            diff = 1 if i < Np else 2
+            dQ.append(d_blocks(A, Qs, iDv, T, aw_var_csc(g, ws, self.bc, diff, dws) * dt) / c)
+        full = assemble_blocks(A, Qs, Q0, iDv, T, sigma, dt, diag0=Qs)
+        dQ.append(-full.tocsc() / c)
+        if dQ0 is not None:
+            dQ += [pad_q0(d, Ns, T) for d in dQ0]
+        return Q, dQ
+
+    def spatial_operator(self, par):
+        """A (Ns x Ns) and q = diag(Qs) for the factorised form."""
+        g, Np = self.grid, self.Np
+        kappa, gamma, vx, vy, ws = self.fields(par)
+        Dk = sparse.diags(kappa).tocsc()
+        A = g.Dv + (g.Dv @ Dk - ah_var_csc(g, self.h_field(gamma, vx, vy), self.bc)
+                    + aw_var_csc(g, ws, self.bc)) * g.dt
+        return A.tocsc(), g.V * kappa ** 2, np.exp(par[Np * 6])
+
+
+def stationary_spatial_operator(model, par):
+    g = model.grid
+    kap, gam, vx, vy, ws, sig, _ = model.split(par)
+    kappa, gamma = np.exp(kap), np.exp(gam)
+    H, _ = h_map(model.kind, gamma, vx, vy)
+    Dk = kappa * sparse.eye(g.Ns)
+    A = g.Dv + g.Dv @ Dk * g.dt - ah_const_csc(g, H, model.bc) * g.dt \
+        + aw_const_csc(g, ws, model.bc) * g.dt
+    return A.tocsc(), np.full(g.Ns, g.V * kappa ** 2), np.exp(sig)
+
+
+# ------------------------------------------------------------ factor helpers
+def logdet_spd(Q):
+    """log|Q| of a sparse SPD matrix: dense Cholesky for small n, else sparse LU
+    with symmetric pivoting (stand-in for CHOLMOD's ``Factor.logdet``)."""
+    n = Q.shape[0]
+    if n <= 4000:
+        L = np.linalg.cholesky(Q.toarray())
+        return 2.0 * np.log(np.diag(L)).sum()
+    lu = spla.splu(Q.tocsc(), permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0,
+                   options=dict(SymmetricMode=True))
+    return np.log(np.abs(lu.U.diagonal())).sum()
+
+
+def logdet_closed_form(A, q, A0, V, T, sigma, dt):
+    """log|Q| for Q = F^T D F (F block-bidiagonal with iDv^1/2 A0 and iDv A on
+    the diagonal, -I below; D = blockdiag(I, I_(T-1) x Qs/(dt sigma))):
+        2 log|det A0| - Ns log V + 2(T-1)(log|det A| - Ns log V)
+        + (T-1) sum_k log(q_k / (dt sigma))."""
+    Ns = A.shape[0]
+
+    def logabsdet(M):
+        lu = spla.splu(M.tocsc(), permc_spec="COLAMD")
+        return np.log(np.abs(lu.U.diagonal())).sum()
+    return (2 * logabsdet(A0) - Ns * np.log(V) + 2 * (T - 1) * (logabsdet(A) - Ns * np.log(V))
+            + (T - 1) * np.log(q / (dt * sigma)).sum())
+
+
+def sample_fform(A, q, A0, V, T, sigma, dt, z):
+    """Ordering-free exact sampler x = F^{-1} D^{-1/2} z:
+       x_0 = sqrt(V) A0^{-1} z_0,  x_t = V A^{-1} (x_{t-1} + sqrt(dt sigma / q) z_t)."""
+    Ns = A.shape[0]
+    z = z.reshape(T, Ns, -1)
+    luA0 = spla.splu(A0.tocsc())
+    luA = spla.splu(A.tocsc())
+    x = np.empty_like(z)
+    x[0] = np.sqrt(V) * luA0.solve(z[0])
+    s = np.sqrt(dt * sigma / q)[:, None]
+    for t in range(1, T):
+        x[t] = V * luA.solve(x[t - 1] + s * z[t])
+    return x.reshape(T * Ns, -1)
+
+
+# ----------------------------------------------------------- log-likelihood
+def loglike(Q, dQ, S, data, log_tau, trace="exact", probes=None, dense_inverse=None):
+    """Normalised log-likelihood and gradient (``advection_idiffusion2D.py:167-192``):
+        like = 1/2 r log|Q| + 1/2 m r log tau - 1/2 r log|Q_c| - 1/2 sum mu_c o (Q mu_c)
+               - tau/2 ||y - S mu_c||^2,     Q_c = Q + tau S^T S,  mu_c = Q_c^{-1} tau S^T y
+        g_i  = 1/2 r tr((Q^{-1} - Q_c^{-1}) dQ_i) - 1/2 mu_c^T dQ_i mu_c
+        g_tau = m r/2 - 1/2 r tr(Q_c^{-1} tau S^T S) - tau/2 ||y - S mu_c||^2
+    returning (-like/(m r), -g/(m r)).  trace="exact" uses dense inverses
+    (small n); trace="hutchinson" uses the probe matrix V (n x nh1) exactly as
+    the reference's estimator (``:182-189``)."""
+    tau = np.exp(log_tau)
+    y = data.reshape(data.shape[0], -1)
+    m, r = y.shape
+    StS = (S.T @ S).tocsc()
+    Qc = (Q + StS * tau).tocsc()
+    n = Q.shape[0]
+    if n <= 4000:
+        Qd, Qcd = Q.toarray(), Qc.toarray()
+        Lc = np.linalg.cholesky(Qcd)
+        mu = np.linalg.solve(Lc.T, np.linalg.solve(Lc, (S.T @ y) * tau))
+        ld, ldc = logdet_spd(Q), 2.0 * np.log(np.diag(Lc)).sum()
+        solveQ = lambda b: np.linalg.solve(Qd, b)
+        solveQc = lambda b: np.linalg.solve(Qcd, b)
+    else:
+        luQ = spla.splu(Q.tocsc(), permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0,
+                        options=dict(SymmetricMode=True))
+        luQc = spla.splu(Qc, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0,
+                         options=dict(SymmetricMode=True))
+        mu = luQc.solve((S.T @ y) * tau)
+        ld = np.log(np.abs(luQ.U.diagonal())).sum()
+        ldc = np.log(np.abs(luQc.U.diagonal())).sum()
+        solveQ, solveQc = luQ.solve, luQc.solve
+    resid = y - S @ mu
+    like = (0.5 * ld * r + m * r * np.log(tau) / 2 - 0.5 * ldc * r
+            - 0.5 * (mu * (Q @ mu)).sum() - tau / 2 * (resid ** 2).sum())
+    g = np.zeros(len(dQ) + 1)
+    if trace == "exact":
+        Sig = np.linalg.inv(Q.toarray()) if dense_inverse is None else dense_inverse[0]
+        Sigc = np.linalg.inv(Qc.toarray()) if dense_inverse is None else dense_inverse[1]
+        D = Sig - Sigc
+        for i, d in enumerate(dQ):
+            tr = d.multiply(D).sum()
+            g[i] = 0.5 * tr * r - 0.5 * (mu * (d @ mu)).sum()
+        trc = (StS * tau).multiply(Sigc).sum()
+        g[-1] = m * r / 2 - 0.5 * trc * r - tau / 2 * (resid ** 2).sum()
+    else:
+        Vp = probes
+        nh1 = Vp.shape[1]
+        TrQ = solveQ(Vp)
+        TrQc = solveQc(Vp)
+        for i, d in enumerate(dQ):
+            g[i] = 0.5 * ((TrQ - TrQc) * (d @ Vp)).sum() * r / nh1 - 0.5 * (mu * (d @ mu)).sum()
+        g[-1] = (m * r / 2 - 0.5 * (TrQc * (StS * tau @ Vp)).sum() * r / nh1
+                 - tau / 2 * (resid ** 2).sum())
+    return -like / (m * r), -g / (m * r), mu
+
+
+def reference_probes(n, nh1=100, seed=None):
+    """The reference's Rademacher probes: ``2*np.random.randint(1,3,n*nh1)-3``
+    from the global legacy RNG, reshaped (n, nh1) (``:182``)."""
+    if seed is not None:
+        np.random.seed(seed)
+    return (2 * np.random.randint(1, 3, n * nh1) - 3).reshape(n, nh1).astype(np.float64)

@@ -1,0 +1,148 @@
+"""Pins for the oracle's model layer: closed forms, brute force, finite
+differences (each chosen so a dropped term, a sign or an index error fails)."""
+import numpy as np
+import pytest
+from scipy import sparse
+
+from oracle import fixtures as F, models as Mo
+
+
+def _spatial(c):
+    m = c["model"]
+    if isinstance(m, Mo.VarAdvVarDiff):
+        A, q, s = m.spatial_operator(c["par"])
+    else:
+        A, q, s = Mo.stationary_spatial_operator(m, c["par"])
+    A0 = c["mod0"].A0(c["mod0"].par)[0]
+    return A, q, s, A0
+
+
+def _fform(c):
+    """F (block lower-bidiagonal) and diag(D) of Q = F^T D F."""
+    g = c["grid"]
+    A, q, s, A0 = _spatial(c)
+    T, Ns = g.T, g.Ns
+    blocks = [[None] * T for _ in range(T)]
+    blocks[0][0] = A0 / np.sqrt(g.V)
+    for t in range(1, T):
+        blocks[t][t] = A / g.V
+        blocks[t][t - 1] = -sparse.eye(Ns)
+    Fm = sparse.bmat(blocks).tocsr()
+    d = np.hstack([np.ones(Ns)] + [q / (g.dt * s)] * (T - 1))
+    return Fm, d
+
+
+CASES = {
+    "iso_b1": lambda: F.stationary(9, 8, 4, bc=1, m=30),
+    "ani_b3": lambda: F.stationary(8, 9, 3, bc=3, kind="ani", m=30),
+    "ha_b1": lambda: F.stationary(7, 8, 3, bc=1, kind="ha", m=30),
+    "var_b1": lambda: F.small(8, 7, 4, bc=1),
+    "var_b3": lambda: F.small(7, 8, 3, bc=3),
+    "var_b1_ext": lambda: F.small(6, 6, 3, bc=1, ne=2),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_q_equals_fform(case):
+    c = CASES[case]()
+    Q, _ = c["model"].makeQ(c["par"], grad=False)
+    Fm, d = _fform(c)
+    Qf = (Fm.T @ sparse.diags(d) @ Fm).toarray()
+    Qd = Q.toarray()
+    assert np.abs(Qd - Qd.T).max() <= 1e-12 * np.abs(Qd).max()
+    assert np.allclose(Qd, Qf, rtol=1e-12, atol=1e-12 * np.abs(Qd).max())
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_logdet_closed_form(case):
+    c = CASES[case]()
+    g = c["grid"]
+    Q, _ = c["model"].makeQ(c["par"], grad=False)
+    A, q, s, A0 = _spatial(c)
+    cf = Mo.logdet_closed_form(A, q, A0, g.V, g.T, s, g.dt)
+    dense = np.linalg.slogdet(Q.toarray())
+    assert dense[0] > 0
+    assert abs(cf - dense[1]) <= 1e-11 * abs(dense[1])
+
+
+@pytest.mark.parametrize("case", ["iso_b1", "var_b3", "ha_b1"])
+def test_loglike_is_gaussian_marginal(case):
+    """-like*m = log N(y; 0, S Q^-1 S^T + I/tau) + m/2 log 2pi  (brute force)."""
+    c = CASES[case]()
+    g = c["grid"]
+    Q, _ = c["model"].makeQ(c["par"], grad=False)
+    S = g.getS(c["idx"])
+    like, _, _ = Mo.loglike(Q, [], S, c["y"], c["par"][-1])
+    tau = np.exp(c["par"][-1])
+    m = S.shape[0]
+    C = S.toarray() @ np.linalg.solve(Q.toarray(), S.toarray().T) + np.eye(m) / tau
+    sgn, ld = np.linalg.slogdet(C)
+    y = c["y"]
+    logp = -0.5 * ld - 0.5 * y @ np.linalg.solve(C, y)
+    assert abs(-like * m - logp) <= 1e-9 * abs(logp)
+
+
+def _fd_check(c, which, h=1e-5, rtol=2e-5):
+    g = c["grid"]
+    S = g.getS(c["idx"])
+    par = c["par"]
+    Q, dQ = c["model"].makeQ(par, grad=True)
+    like, jac, _ = Mo.loglike(Q, dQ, S, c["y"], par[-1], trace="exact")
+    for i in which:
+        pp, pm = par.copy(), par.copy()
+        pp[i] += h
+        pm[i] -= h
+        lp = Mo.loglike(c["model"].makeQ(pp, grad=False)[0], [], S, c["y"], pp[-1])[0]
+        lm = Mo.loglike(c["model"].makeQ(pm, grad=False)[0], [], S, c["y"], pm[-1])[0]
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - jac[i]) <= rtol * max(abs(fd), 1e-3), (i, fd, jac[i])
+
+
+def test_gradient_fd_var_dirichlet():
+    c = CASES["var_b3"]()
+    Np = c["grid"].Nbs2
+    _fd_check(c, list(range(0, 6 * Np, 2)) + [6 * Np, 6 * Np + 1])
+
+
+def test_gradient_fd_stationary_dirichlet():
+    c = CASES["ani_b3"]()
+    _fd_check(c, range(8))
+
+
+def test_gradient_fd_stationary_neumann_non_advection():
+    """bc=1 advection derivative stencils are not the operator's derivative
+    (Acw_2D_b1.cpp:63-80), so only the other parameters are FD-checked here."""
+    c = CASES["iso_b1"]()
+    _fd_check(c, [0, 1, 4, 5])
+
+
+def test_pattern_iso_corners_vanish():
+    """Isotropic H gives exact-zero diffusion corners; scipy's sums drop them
+    so an interior row of Q has 13 + 5 + 5 entries (5-point A), not 25 + 9 + 9."""
+    c = F.stationary(9, 9, 3, bc=1, m=10)
+    Q, _ = c["model"].makeQ(c["par"], grad=False)
+    g = c["grid"]
+    k = 4 + 4 * 9 + 1 * 81
+    assert Q.tocsr()[k].nnz == 23
+    c = F.small(9, 9, 3, bc=1)
+    Q, _ = c["model"].makeQ(c["par"], grad=False)
+    assert Q.tocsr()[k].nnz == 43
+
+
+def test_fform_sampler_solves_factor():
+    c = CASES["var_b1"]()
+    g = c["grid"]
+    A, q, s, A0 = _spatial(c)
+    z = np.random.default_rng(0).normal(size=(g.n, 3))
+    x = Mo.sample_fform(A, q, A0, g.V, g.T, s, g.dt, z)
+    Fm, d = _fform(c)
+    assert np.allclose(Fm @ x, z / np.sqrt(d)[:, None], rtol=1e-10, atol=1e-10)
+
+
+def test_hutchinson_probes_stream():
+    """Legacy global RandomState stream (NEP 19): the reference's probe matrix."""
+    V1 = Mo.reference_probes(10, 4, seed=5)
+    np.random.seed(5)
+    raw = np.random.randint(1, 3, 40)
+    assert np.array_equal(V1.ravel(), 2 * raw - 3)
+    assert set(np.unique(V1)) == {-1.0, 1.0}
