@@ -1,0 +1,183 @@
+/*
+ * spdepy_amd -- MI355X-native GMRF hot path of berild/spdepy.
+ *
+ * C ABI of libspdepy_amd.so.  Plain C types only; every pointer documented as
+ * "device" is HBM memory owned by the caller (e.g. a torch tensor's
+ * data_ptr()); "host" pointers are ordinary process memory.  Every call
+ * returns SPDE_OK or an error code and sets a message readable with
+ * spde_last_error().  Calls are ordered on the HIP stream passed as `stream`
+ * (NULL = default stream).  A plan is not thread-safe: one stream per plan.
+ *
+ * Reference interfaces each entry point replaces (berild/spdepy, path:line):
+ *   stencils  -> AH_new/AH_Row/AH_Col/AH_Val/AH_delete
+ *                (src/spdepy/spdes/ccode/AcH_2D_b1.cpp:170-185, AH_2D_b1.cpp:154-169)
+ *                Aw_new/Aw_Row/Aw_Col/Aw_Val/Aw_delete
+ *                (Acw_2D_b1.cpp:117-132, Aw_2D_b1.cpp:136-151)
+ *   assembly  -> the block formula of makeQ
+ *                (spdes/advection_idiffusion2D.py:85-116,
+ *                 spdes/var_advection_var_diffusion2D.py:83-116)
+ *   factor    -> sksparse.cholmod.cholesky(Q) and Factor.logdet()
+ *                (advection_idiffusion2D.py:116,173,180)
+ *   solve     -> Factor.solve_A / solve_Lt / apply_Pt (model.py:79-80,125-126;
+ *                advection_idiffusion2D.py:176,183-184)
+ *   selinv    -> Model.qinv via R-INLA (model.py:89-118, rqinv.R:18-70) and the
+ *                trace terms of logLike (advection_idiffusion2D.py:182-189)
+ *   gradient  -> the dQ loop of logLike (advection_idiffusion2D.py:186-189,
+ *                var_advection_var_diffusion2D.py:218-221)
+ */
+#ifndef SPDEPY_AMD_H
+#define SPDEPY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPDE_OK 0
+#define SPDE_ERR_ARG 1      /* invalid argument */
+#define SPDE_ERR_HIP 2      /* HIP runtime failure */
+#define SPDE_ERR_NOT_SPD 3  /* factorisation hit a non-positive pivot */
+#define SPDE_ERR_ALLOC 4    /* device allocation failed */
+#define SPDE_ERR_STATE 5    /* call out of order (e.g. solve before factor) */
+
+const char* spde_last_error(void);
+
+/* ---------------------------------------------------------------- stencils
+ * Finite-volume diffusion stencil in the reference's slot layout: for each
+ * cell k = i + j*M nine slots [C,E,W,N,S,NE,SW,NW,SE]; row[9k+s] == M*N marks
+ * a dropped slot.  variable=0: H is the 2x2 tensor (4 doubles, row-major,
+ * AcH_2D_b{1,3}.cpp semantics); variable=1: H is (M*N,4 faces W,E,S,N,2,2)
+ * (AH_2D_b{1,2,3}.cpp semantics, including the bc=1 stale-index zeroing of
+ * AH_2D_b1.cpp:25-52).  H, row, col, val: device. */
+int spde_stencil_diffusion(int M, int N, const double* H, int variable, double hx, double hy,
+                           int bc, int32_t* row, int32_t* col, double* val, void* stream);
+
+/* Upwind advection stencil, five slots [C,E,W,N,S] per cell.  variable=0: G
+ * holds (wx, wy) (Acw_2D_b{1,2,3}.cpp); variable=1: G and dG are (M*N, 4 faces
+ * E,N,W,S) (Aw_2D_b{1,2,3}.cpp).  diff: 3 operator, 1 d/dwx, 2 d/dwy.  dG may
+ * be NULL (zeros).  All pointers device. */
+int spde_stencil_advection(int M, int N, const double* G, const double* dG, int variable,
+                           double hx, double hy, int diff, int bc, int32_t* row, int32_t* col,
+                           double* val, void* stream);
+
+/* ---------------------------------------------------------------- plan
+ * A plan holds the structural pattern of Q for an (M x N x T, bc) lattice, the
+ * nested-dissection ordering, the multifrontal schedule and the device
+ * arenas for `nslots` factorisations.  flags: SPDE_PLAN_HOST_ONLY builds the
+ * symbolic part only (no device memory; inspection and tests). */
+typedef struct spde_plan spde_plan;
+#define SPDE_PLAN_HOST_ONLY 1
+
+typedef struct spde_plan_info {
+  int64_t n, nnz_struct, nnz_L, front_doubles, z_doubles;
+  int64_t nnodes, nlevels, max_front, max_p;
+  double flops_factor, flops_selinv;
+} spde_plan_info;
+
+int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int flags,
+                     spde_plan** plan);
+void spde_plan_destroy(spde_plan* plan);
+int spde_plan_get_info(const spde_plan* plan, spde_plan_info* info);
+
+/* Host copies of the structure and the symbolic analysis (for inspection):
+ * indptr[n+1], indices[nnz], code[nnz]; perm[n]; nodes[nnodes*8] = (start,
+ * p, n, parent, child0, child1, level, nB); foff[nnodes]; bnd[sum nB];
+ * relpos[sum nB]; map[nnz] (front-arena offset of each structural entry, -1
+ * for the upper triangle).  Any pointer may be NULL. */
+int spde_plan_export(const spde_plan* plan, int64_t* indptr, int32_t* indices, uint8_t* code,
+                     int32_t* perm, int32_t* nodes, int64_t* foff, int32_t* bnd,
+                     int32_t* relpos, int64_t* map);
+
+/* ---------------------------------------------------------------- assembly
+ * Coefficients of the space-time operator.  variable=0 (stationary models):
+ * kappa_c, H_c (2x2 row-major) and G_c (wx, wy) with the constant-stencil
+ * semantics; variable=1 (B-spline fields): device arrays kappa[Ns],
+ * H[Ns*16] (faces W,E,S,N) and G[Ns*4] (faces E,N,W,S).  The initial-state
+ * block is the Whittle-Matern precision of A0 = V kappa0 I - A_H(H0)
+ * (constant stencil).  sigma = exp(log sigma). */
+typedef struct spde_coeffs {
+  int variable;
+  double hx, hy, dt, V, sigma;
+  double kappa_c, H_c[4], G_c[2];
+  const double* kappa;
+  const double* H;
+  const double* G;
+  double kappa0, H0[4];
+} spde_coeffs;
+
+/* Values of Q on the structural pattern (qval[nnz_struct], device). */
+int spde_assemble(spde_plan* plan, const spde_coeffs* c, double* qval, void* stream);
+
+/* Drop exact zeros: the reference's scipy pattern.  indptr[n+1] (int64),
+ * indices[nnz_struct], data[nnz_struct] device; *nnz (host) receives the kept
+ * count. */
+int spde_prune(spde_plan* plan, const double* qval, int64_t* indptr, int32_t* indices,
+               double* data, int64_t* nnz, void* stream);
+
+/* y = Q x for Q on the structural pattern (x, y device, n x nrhs, column-major). */
+int spde_spmv(spde_plan* plan, const double* qval, const double* x, double* y, int nrhs,
+              void* stream);
+
+/* ---------------------------------------------------------------- factor
+ * Cholesky factorisation of qval (+ diag_add on the diagonal when non-NULL,
+ * both device) into factor slot `slot`; *logdet (host) = log|Q|.  Returns
+ * SPDE_ERR_NOT_SPD (message names the failing lattice index) when a pivot is
+ * not positive. */
+int spde_factor(spde_plan* plan, int slot, const double* qval, const double* diag_add,
+                double* logdet, void* stream);
+
+/* Solves with the factor in `slot`.  mode: 0 Q^-1 B, 1 L^-1 P B,
+ * 2 P^T L^-T B (the reference's apply_Pt(solve_Lt(.))).  B device,
+ * n x nrhs column-major with leading dimension ldb, overwritten. */
+#define SPDE_SOLVE_A 0
+#define SPDE_SOLVE_L 1
+#define SPDE_SOLVE_LT 2
+int spde_solve(spde_plan* plan, int slot, int mode, double* B, int nrhs, int64_t ldb,
+               void* stream);
+
+/* Takahashi selected inverse of the factor in `slot`: Sigma = Q^-1 on the
+ * structural pattern (sigma[nnz_struct], device). */
+int spde_selinv(spde_plan* plan, int slot, double* sigma, void* stream);
+
+/* ---------------------------------------------------------------- gradient
+ * Per-cell adjoints of the log-likelihood gradient (the contraction
+ * g_i = <G, dQ_i> of the reference's dQ loop, done once through the adjoint of
+ * the assembly).  G = 1/2 r (Sigma_Q - Sigma_Qc) - 1/2 mu mu^T on the pattern
+ * (mode 0, sigma_q/sigma_qc from spde_selinv) or its Hutchinson estimate from
+ * solves trq = Q^-1 V, trqc = Q_c^-1 V with the probe matrix V (mode 1, all
+ * n x nh1 column-major, lattice order).  The last spde_assemble on this plan
+ * must have used the same coefficients.  out[Ns*40] (device) receives per
+ * cell: [0] d/dkappa, [1..16] d/dH (faces W,E,S,N, 2x2), [17..20] d/d(dG)
+ * of the x-derivative stencil, [21..24] of the y-derivative stencil, [25]
+ * d/dkappa0 / kappa0 ... see spdepy_amd/engine.py for the full layout. */
+typedef struct spde_grad_args {
+  int mode;
+  int r;
+  int nh1;
+  const double* qval;
+  const double* sigma_q;
+  const double* sigma_qc;
+  const double* trq;
+  const double* trqc;
+  const double* probes;
+  const double* mu;
+  const double* diag_add;
+  spde_coeffs coeffs;
+} spde_grad_args;
+
+int spde_grad_cells(spde_plan* plan, const spde_grad_args* args, double* out, void* stream);
+
+/* ---------------------------------------------------------------- diagnostics
+ * No counterpart in the reference: self-checks of the dense kernels.
+ * spde_debug_mfma_probe writes the 64x4 accumulator lanes of one FP64 MFMA
+ * for a fixed input pattern (out: 256 doubles, device).  spde_debug_gemm runs
+ * the library's batched GEMM kernel on one problem (column-major, device). */
+int spde_debug_mfma_probe(int which, double* out, void* stream);
+int spde_debug_gemm(int ta, int tb, int M, int N, int K, double alpha, const double* A, int lda,
+                    const double* B, int ldb, double beta, double* C, int ldc, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

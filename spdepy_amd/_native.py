@@ -1,0 +1,106 @@
+"""ctypes binding of libspdepy_amd.so (the C ABI declared in include/spdepy_amd.h).
+
+The library is built in place by ``spdepy_amd.build``; importing a compute
+entry point without it raises -- there is no CPU fallback."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libspdepy_amd.so")
+
+SPDE_OK, SPDE_ERR_ARG, SPDE_ERR_HIP, SPDE_ERR_NOT_SPD, SPDE_ERR_ALLOC, SPDE_ERR_STATE = range(6)
+SPDE_PLAN_HOST_ONLY = 1
+SOLVE_A, SOLVE_L, SOLVE_LT = 0, 1, 2
+
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+c_i32p = ctypes.POINTER(ctypes.c_int32)
+c_u8p = ctypes.POINTER(ctypes.c_uint8)
+c_dp = ctypes.POINTER(ctypes.c_double)
+vp = ctypes.c_void_p
+
+
+class SpdeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (code %d)" % (msg, code))
+        self.code = code
+
+
+class NotPositiveDefinite(SpdeError):
+    pass
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("nnz_struct", ctypes.c_int64), ("nnz_L", ctypes.c_int64),
+                ("front_doubles", ctypes.c_int64), ("z_doubles", ctypes.c_int64),
+                ("nnodes", ctypes.c_int64), ("nlevels", ctypes.c_int64), ("max_front", ctypes.c_int64),
+                ("max_p", ctypes.c_int64), ("flops_factor", ctypes.c_double),
+                ("flops_selinv", ctypes.c_double)]
+
+
+class Coeffs(ctypes.Structure):
+    _fields_ = [("variable", ctypes.c_int), ("hx", ctypes.c_double), ("hy", ctypes.c_double),
+                ("dt", ctypes.c_double), ("V", ctypes.c_double), ("sigma", ctypes.c_double),
+                ("kappa_c", ctypes.c_double), ("H_c", ctypes.c_double * 4), ("G_c", ctypes.c_double * 2),
+                ("kappa", vp), ("H", vp), ("G", vp), ("kappa0", ctypes.c_double),
+                ("H0", ctypes.c_double * 4)]
+
+
+class GradArgs(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("r", ctypes.c_int), ("nh1", ctypes.c_int), ("qval", vp),
+                ("sigma_q", vp), ("sigma_qc", vp), ("trq", vp), ("trqc", vp), ("probes", vp),
+                ("mu", vp), ("diag_add", vp), ("coeffs", Coeffs)]
+
+
+_SIGS = {
+    "spde_last_error": (ctypes.c_char_p, []),
+    "spde_stencil_diffusion": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_int, vp, vp, vp, vp]),
+    "spde_stencil_advection": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]),
+    "spde_plan_create": (ctypes.c_int, [ctypes.c_int] * 7 + [ctypes.POINTER(vp)]),
+    "spde_plan_destroy": (None, [vp]),
+    "spde_plan_get_info": (ctypes.c_int, [vp, ctypes.POINTER(PlanInfo)]),
+    "spde_plan_export": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "spde_assemble": (ctypes.c_int, [vp, ctypes.POINTER(Coeffs), vp, vp]),
+    "spde_prune": (ctypes.c_int, [vp, vp, vp, vp, vp, c_i64p, vp]),
+    "spde_spmv": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int, vp]),
+    "spde_factor": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, c_dp, vp]),
+    "spde_solve": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int64, vp]),
+    "spde_selinv": (ctypes.c_int, [vp, ctypes.c_int, vp, vp]),
+    "spde_debug_mfma_probe": (ctypes.c_int, [ctypes.c_int, vp, vp]),
+    "spde_debug_gemm": (ctypes.c_int, [ctypes.c_int] * 5 + [ctypes.c_double, vp, ctypes.c_int, vp, ctypes.c_int,
+                                                            ctypes.c_double, vp, ctypes.c_int, vp]),
+    "spde_grad_cells": (ctypes.c_int, [vp, ctypes.POINTER(GradArgs), vp, vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the native library; raises if it is not built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libspdepy_amd.so is not built; run `python -m spdepy_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    L = lib()
+    return [n for n in _SIGS if hasattr(L, n)]
+
+
+def check(rc):
+    if rc != SPDE_OK:
+        msg = lib().spde_last_error().decode()
+        if rc == SPDE_ERR_NOT_SPD:
+            raise NotPositiveDefinite(rc, msg)
+        raise SpdeError(rc, msg)

@@ -1,0 +1,274 @@
+// Gradient of the log-likelihood by the adjoint of the assembly.
+//
+// The reference forms one matrix dQ_i per parameter and contracts it with the
+// trace and data terms (advection_idiffusion2D.py:186-189,
+// var_advection_var_diffusion2D.py:218-221):
+//     g_i = <G, dQ_i>,  G = 1/2 r (Q^-1 - Q_c^-1) - 1/2 mu_c mu_c^T  (on pattern(Q)).
+// Every dQ_i is the directional derivative of the same block formula, so the
+// contraction is done once, backwards: G is reduced over time into per-cell
+// windows, differentiated through the block formula into adjoints of the
+// spatial operator A (per cell and canonical offset), of diag(Qs) and of A0,
+// and then through the per-cell stencils (unit perturbations of the very
+// functions the assembly uses, so the reference's boundary folds, the bc=1
+// stale-index zeroing and the NaN->0 rule of the advection derivative carry
+// over exactly).  Cost O(nnz(Q)) instead of O(#params * nnz(Q)).
+#pragma clang fp contract(off)
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+#include "stencil.hpp"
+
+namespace spde {
+
+struct GradKArgs {
+  int mode, r, nh1, T, variable, bc, M, N;
+  int64_t Ns, n;
+  double c, V, dt, hx, hy;
+  double H_c[4], G_c[2], kappa_c;
+  const double* kappa;
+  const double* H;
+  const double* G;
+  const int64_t* indptr;
+  const int32_t* indices;
+  const uint8_t* code;
+  const double* qval;
+  const double* sq;
+  const double* sqc;
+  const double* trq;
+  const double* trqc;
+  const double* probes;
+  const double* mu;
+  const double* diag_add;
+  const double* acan;
+  const double* a0can;
+  const double* qdiag;
+};
+
+constexpr int kWin = 60;  // Gd[25] | Gu[9] | G00[25] | Gq
+constexpr int kOut = 40;
+
+// Pass 1: per cell, reduce G over time into windows (thread per cell).
+__global__ void grad_reduce_kernel(GradKArgs a, double* __restrict__ win, double* __restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= a.Ns) return;
+  double w[kWin];
+  for (int q = 0; q < kWin; ++q) w[q] = 0.0;
+  double dotGQ = 0.0, dot00 = 0.0, tautr = 0.0;
+  const double inh = a.nh1 > 0 ? 1.0 / a.nh1 : 0.0;
+  for (int t = 0; t < a.T; ++t) {
+    const int64_t r = i + t * a.Ns;
+    for (int64_t e = a.indptr[r]; e < a.indptr[r + 1]; ++e) {
+      const int64_t col = a.indices[e];
+      const int cd = a.code[e];
+      double Ge;
+      double tr_c = 0.0;
+      if (a.mode == 0) {
+        Ge = 0.5 * a.r * (a.sq[e] - a.sqc[e]);
+        if (cd == 12) tr_c = a.sqc[e];
+      } else {
+        double s = 0.0;
+        for (int p = 0; p < a.nh1; ++p) {
+          const int64_t o = (int64_t)p * a.n;
+          s += (a.trq[r + o] - a.trqc[r + o]) * a.probes[col + o] + (a.trq[col + o] - a.trqc[col + o]) * a.probes[r + o];
+          if (cd == 12) tr_c += a.trqc[r + o] * a.probes[r + o];
+        }
+        Ge = 0.5 * a.r * inh * 0.5 * s;
+        tr_c *= inh;
+      }
+      double mm = 0.0;
+      for (int c = 0; c < a.r; ++c) mm += a.mu[r + (int64_t)c * a.n] * a.mu[col + (int64_t)c * a.n];
+      Ge -= 0.5 * mm;
+      const double qv = a.qval[e];
+      dotGQ += Ge * qv;
+      if (cd < 25) {
+        if (t >= 1) w[cd] += Ge;
+        if (t == 0) {
+          w[34 + cd] += Ge;
+          dot00 += Ge * ((cd == 12) ? qv - a.c * a.qdiag[i] : qv);
+        }
+        if (cd == 12) {
+          if (t <= a.T - 2) w[59] += Ge;
+          if (a.diag_add) tautr += a.diag_add[r] * tr_c;
+        }
+      } else if (cd >= 34 && t <= a.T - 2) {
+        w[25 + cd - 34] += Ge;
+      }
+    }
+  }
+  for (int q = 0; q < kWin; ++q) win[i * kWin + q] = w[q];
+  out[i * kOut + 36] = dotGQ;
+  out[i * kOut + 37] = dot00;
+  out[i * kOut + 38] = tautr;
+}
+
+__device__ inline int widx(int dx, int dy) { return (dy + 2) * 5 + dx + 2; }
+
+// Pass 2: per cell m, adjoints of A(m,:), q_m and A0(m,:), then through the
+// cell's own stencils.
+__global__ void grad_cell_kernel(GradKArgs a, const double* __restrict__ win, double* __restrict__ out) {
+  const int64_t mlin = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (mlin >= a.Ns) return;
+  const int m = (int)mlin;
+  const int M = a.M, N = a.N, bc = a.bc;
+  const int i = m % M, j = m / M;
+  const double* Am = a.acan + 9 * (int64_t)m;
+  const double* A0m = a.a0can + 9 * (int64_t)m;
+  const double qm = a.qdiag[m];
+  const double iV = 1.0 / a.V;
+  double abar[9], a0bar[9];
+  double qbar_quad = 0.0, qbar_u = 0.0;
+  for (int o = 0; o < 9; ++o) {
+    const int dox = o % 3 - 1, doy = o / 3 - 1;
+    double bg = 0.0, ag0 = 0.0;
+    for (int o2 = 0; o2 < 9; ++o2) {
+      const int d2x = o2 % 3 - 1, d2y = o2 / 3 - 1;
+      const int jc = nbr_cell(i, j, d2x, d2y, M, N, bc);
+      if (jc < 0) continue;
+      const int wv = widx(dox - d2x, doy - d2y);
+      bg += Am[o2] * win[(int64_t)jc * kWin + wv];
+      ag0 += A0m[o2] * win[(int64_t)jc * kWin + 34 + wv];
+    }
+    const double gu = win[mlin * kWin + 25 + o];
+    abar[o] = 2.0 * a.c * iV * qm * (iV * bg - gu);
+    a0bar[o] = 2.0 * iV * ag0;
+    qbar_quad += iV * iV * bg * Am[o];
+    qbar_u += iV * gu * Am[o];
+  }
+  const double qbar = a.c * (qbar_quad + win[mlin * kWin + 59] - 2.0 * qbar_u);
+  double* o = out + mlin * kOut;
+  for (int q = 0; q < 36; ++q) o[q] = 0.0;
+  double canv[9];
+  // initial-state operator A0 = V kappa0 I - A_H(H0)
+  o[25] = a0bar[4] * a.V;
+  for (int e = 0; e < 4; ++e) {
+    double h[4] = {0.0, 0.0, 0.0, 0.0};
+    h[e] = 1.0;
+    Cell9 c9;
+    diffusion_const(i, j, M, N, bc, a.hx, a.hy, h, c9);
+    canon9(c9, i, j, M, N, canv);
+    double s = 0.0;
+    for (int q = 0; q < 9; ++q) s += a0bar[q] * canv[q];
+    o[26 + e] = -s;
+  }
+  if (a.variable) {
+    const double kap = a.kappa[m];
+    o[0] = abar[4] * a.V * a.dt + qbar * 2.0 * a.V * kap;
+    double Hk[16];
+    for (int e = 0; e < 16; ++e) {
+      for (int q = 0; q < 16; ++q) Hk[q] = 0.0;
+      Hk[e] = 1.0;
+      Cell9 c9;
+      diffusion_var(i, j, M, N, bc, a.hx, a.hy, Hk, c9);
+      canon9(c9, i, j, M, N, canv);
+      double s = 0.0;
+      for (int q = 0; q < 9; ++q) s += abar[q] * canv[q];
+      o[1 + e] = -a.dt * s;
+    }
+    const double* Gk = a.G + 4 * (int64_t)m;
+    for (int diff = 1; diff <= 2; ++diff)
+      for (int f = 0; f < 4; ++f) {
+        double dG[4] = {0.0, 0.0, 0.0, 0.0};
+        dG[f] = 1.0;
+        Cell5 c5;
+        advection_var(i, j, M, N, bc, a.hx, a.hy, Gk, dG, diff, c5);
+        canon5(c5, i, j, M, N, canv, true);
+        double s = 0.0;
+        for (int q = 0; q < 9; ++q) s += abar[q] * canv[q];
+        o[17 + 4 * (diff - 1) + f] = a.dt * s;
+      }
+  } else {
+    o[0] = abar[4] * a.V * a.dt + qbar * 2.0 * a.V * a.kappa_c;
+    for (int e = 0; e < 4; ++e) {
+      double h[4] = {0.0, 0.0, 0.0, 0.0};
+      h[e] = 1.0;
+      Cell9 c9;
+      diffusion_const(i, j, M, N, bc, a.hx, a.hy, h, c9);
+      canon9(c9, i, j, M, N, canv);
+      double s = 0.0;
+      for (int q = 0; q < 9; ++q) s += abar[q] * canv[q];
+      o[30 + e] = -a.dt * s;
+    }
+    for (int diff = 1; diff <= 2; ++diff) {
+      Cell5 c5;
+      advection_const(i, j, M, N, bc, a.hx, a.hy, a.G_c[0], a.G_c[1], diff, c5);
+      canon5(c5, i, j, M, N, canv, false);
+      double s = 0.0;
+      for (int q = 0; q < 9; ++q) s += abar[q] * canv[q];
+      o[33 + diff] = a.dt * s;
+    }
+  }
+}
+
+}  // namespace spde
+
+using namespace spde;
+
+extern "C" int spde_grad_cells(spde_plan* pl, const spde_grad_args* g, double* out, void* stream) {
+  if (!pl || pl->host_only || !g || !out || !g->qval || !g->mu) {
+    set_error("spde_grad_cells: invalid argument");
+    return SPDE_ERR_ARG;
+  }
+  if (g->mode == 0 && (!g->sigma_q || !g->sigma_qc)) {
+    set_error("spde_grad_cells: exact mode needs both selected inverses");
+    return SPDE_ERR_ARG;
+  }
+  if (g->mode == 1 && (!g->trq || !g->trqc || !g->probes || g->nh1 < 1)) {
+    set_error("spde_grad_cells: Hutchinson mode needs trq, trqc and probes");
+    return SPDE_ERR_ARG;
+  }
+  const spde_coeffs& c = g->coeffs;
+  if (c.variable && (!c.kappa || !c.H || !c.G)) {
+    set_error("spde_grad_cells: variable coefficients need kappa, H and G");
+    return SPDE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  GradKArgs a;
+  a.mode = g->mode;
+  a.r = g->r;
+  a.nh1 = g->nh1;
+  a.T = pl->lat.T;
+  a.variable = c.variable;
+  a.bc = pl->lat.bc;
+  a.M = pl->lat.M;
+  a.N = pl->lat.N;
+  a.Ns = pl->lat.Ns();
+  a.n = pl->sym.n;
+  a.c = 1.0 / (c.dt * c.sigma);
+  a.V = c.V;
+  a.dt = c.dt;
+  a.hx = c.hx;
+  a.hy = c.hy;
+  for (int q = 0; q < 4; ++q) a.H_c[q] = c.H_c[q];
+  a.G_c[0] = c.G_c[0];
+  a.G_c[1] = c.G_c[1];
+  a.kappa_c = c.kappa_c;
+  a.kappa = c.kappa;
+  a.H = c.H;
+  a.G = c.G;
+  a.indptr = pl->d_indptr;
+  a.indices = pl->d_indices;
+  a.code = pl->d_code;
+  a.qval = g->qval;
+  a.sq = g->sigma_q;
+  a.sqc = g->sigma_qc;
+  a.trq = g->trq;
+  a.trqc = g->trqc;
+  a.probes = g->probes;
+  a.mu = g->mu;
+  a.diag_add = g->diag_add;
+  a.acan = pl->d_acan;
+  a.a0can = pl->d_a0can;
+  a.qdiag = pl->d_qdiag;
+  double* win = nullptr;
+  if (pl->work_doubles >= a.Ns * kWin) {
+    win = pl->d_work;
+  } else {
+    set_error("spde_grad_cells: workspace too small");
+    return SPDE_ERR_STATE;
+  }
+  const unsigned blocks = (unsigned)((a.Ns + 127) / 128);
+  hipLaunchKernelGGL(grad_reduce_kernel, dim3(blocks), dim3(128), 0, s, a, win, out);
+  hipLaunchKernelGGL(grad_cell_kernel, dim3(blocks), dim3(128), 0, s, a, win, out);
+  SPDE_HIP(hipGetLastError());
+  return SPDE_OK;
+}

@@ -1,0 +1,149 @@
+// Internal plan object shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/spdepy_amd.h"
+#include "plan.hpp"
+
+namespace spde {
+
+void set_error(const std::string& msg);
+
+#define SPDE_HIP(call)                                                        \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ::spde::set_error(std::string(#call) + ": " + hipGetErrorString(e_));   \
+      return SPDE_ERR_HIP;                                                    \
+    }                                                                         \
+  } while (0)
+
+constexpr int NB = 64;  // diagonal block width of the dense partial factorisation
+
+// Operand base pointers of a batched launch (selected per operand by id).
+enum Base : int8_t { kFront = 0, kZ = 1, kSigma = 2, kWork = 3, kVec = 4, kX = 5, kNumBases = 6 };
+struct Bases {
+  double* p[kNumBases];
+};
+
+// One GEMM problem: C = alpha op(A) op(B) + beta C, column-major.
+struct GemmProb {
+  int64_t a_off, b_off, c_off;
+  int32_t lda, ldb, ldc;
+  int32_t M, N, K;
+  int8_t a_base, b_base, c_base, lower;
+  double alpha, beta;
+};
+struct GemmTile {
+  int32_t prob, tm, tn, pad;
+};
+
+// A batched GEMM launch: problems + tile list (device), transposition flags.
+struct GemmLaunch {
+  bool ta = false, tb = false;
+  int ntiles = 0, nprobs = 0;
+  GemmProb* d_probs = nullptr;
+  GemmTile* d_tiles = nullptr;
+  std::vector<GemmProb> h_probs;
+  std::vector<GemmTile> h_tiles;
+  double flops = 0;
+};
+
+struct PotrfDesc {      // one diagonal block of one front
+  int64_t foff;         // front offset
+  int64_t zoff;         // Z offset of the block's node
+  int32_t ld, zld;      // front dimension, p
+  int32_t k0, kb;       // block start (within own columns) and width
+  int32_t node, colbase;
+};
+struct PanelDesc {      // a strip of rows below a diagonal block
+  int64_t foff, zoff;
+  int32_t ld, zld, k0, kb, r0, nr;
+};
+struct ExtAddDesc {     // extend-add of a child's update block into its parent
+  int64_t cfoff, pfoff, relpos_off;
+  int32_t cld, cp, pld, b;
+};
+struct ExtAddTile {
+  int32_t desc, ti, tj, pad;
+};
+
+struct Step {           // one stage of the dense partial factorisation of a level
+  int npotrf = 0, npanel = 0;
+  PotrfDesc* d_potrf = nullptr;
+  PanelDesc* d_panel = nullptr;
+  GemmLaunch update;
+};
+
+struct LevelPlan {
+  std::vector<int> nodes;
+  int next[2] = {0, 0};                 // extend-add tile counts per child slot
+  ExtAddDesc* d_ext[2] = {nullptr, nullptr};
+  ExtAddTile* d_exttile[2] = {nullptr, nullptr};
+  std::vector<ExtAddDesc> h_ext[2];
+  std::vector<ExtAddTile> h_exttile[2];
+  std::vector<Step> steps;
+  GemmLaunch syrk;
+  // solves / selinv
+  GemmLaunch fwd_y, fwd_u, bwd_t, bwd_x;   // rebuilt per nrhs
+  GemmLaunch si_x, si_s21, si_s11a, si_s11b;
+  int64_t si_work = 0;
+};
+
+struct Slot {
+  double* front = nullptr;  // factor fronts
+  double* z = nullptr;      // inverses of the diagonal blocks of L
+  bool factored = false;
+};
+
+struct NodeDev {            // per-node table on the device
+  int64_t foff, zoff, voff, relpos_off;
+  int32_t start, p, n, parent, nb, level, pad0, pad1;
+};
+
+}  // namespace spde
+
+struct spde_plan {
+  spde::Lattice lat;
+  spde::Structure st;
+  spde::Symbolic sym;
+  std::vector<int64_t> map;      // scatter map (front offsets, -1 upper)
+  std::vector<int64_t> mapsym;   // map of the lower mirror for every entry
+  bool host_only = true;
+  int nslots = 0;
+  // device structure
+  int64_t* d_indptr = nullptr;
+  int32_t* d_indices = nullptr;
+  uint8_t* d_code = nullptr;
+  int64_t* d_diag = nullptr;
+  int64_t* d_map = nullptr;
+  int64_t* d_mapsym = nullptr;
+  int32_t* d_perm = nullptr;
+  int32_t* d_relpos = nullptr;
+  int32_t* d_bnd = nullptr;       // concatenated boundary lists (elimination indices)
+  spde::NodeDev* d_nodes = nullptr;
+  double* d_tmpl = nullptr;       // per-cell value templates of the blocks of Q
+  double* d_acan = nullptr;       // canonical 9-point spatial operator A
+  double* d_a0can = nullptr;      // canonical 9-point initial-state operator A0
+  double* d_qdiag = nullptr;      // diagonal of Qs per cell
+  double* d_sigma = nullptr;      // selected-inverse fronts
+  double* d_work = nullptr;
+  int64_t work_doubles = 0;
+  double* d_vec = nullptr;        // solve front vectors
+  int64_t vec_cap = 0;            // in doubles
+  double* d_xy = nullptr;         // elimination-ordered solve buffer
+  int64_t xy_cap = 0;
+  double* d_partial = nullptr;    // per-node log-det partial sums
+  double* d_scalar = nullptr;     // small device results
+  int* d_status = nullptr;        // first non-positive pivot
+  std::vector<spde::Slot> slots;
+  std::vector<spde::LevelPlan> levels;
+  std::vector<spde::GemmLaunch> trtri;  // per recursion height: pairs (X, Z2)
+  int trtri_heights = 0;
+  int solve_nrhs = -1;            // nrhs the solve launches are built for
+  bool selinv_built = false;
+};

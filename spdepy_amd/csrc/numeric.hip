@@ -1,0 +1,782 @@
+// Plan construction (device arenas, launch schedules) and the numeric
+// multifrontal Cholesky, log-determinant, triangular solves and Takahashi
+// selected inverse.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <stdexcept>
+
+#include "internal.hpp"
+
+namespace spde {
+
+void launch_gemm(const GemmLaunch& g, const Bases& b, hipStream_t s);
+void launch_potrf(const PotrfDesc* d, int n, double* F, double* Z, int* status, hipStream_t s);
+void launch_panel(const PanelDesc* d, int n, double* F, const double* Z, hipStream_t s);
+void launch_fill(double* p, int64_t n, double v, hipStream_t s);
+void launch_scatter(const double* q, const int64_t* map, int64_t nnz, double* F, hipStream_t s);
+void launch_diag_add(const double* add, const int64_t* diag, const int64_t* map, int64_t n, double* F, hipStream_t s);
+void launch_extend_add(const ExtAddDesc* d, const ExtAddTile* t, int ntiles, const int32_t* relpos, double* F,
+                       hipStream_t s);
+void launch_logdet(const NodeDev* nodes, int nnodes, const double* F, double* partial, double* out, hipStream_t s);
+void launch_permute_in(const double* B, int64_t ldb, const int32_t* perm, int64_t n, int nrhs, double* xy,
+                       hipStream_t s);
+void launch_permute_out(const double* xy, const int32_t* perm, int64_t n, int nrhs, double* B, int64_t ldb,
+                        hipStream_t s);
+void launch_fwd_gather(const int32_t* lvl_nodes, int nn, const NodeDev* nodes, const int32_t* children,
+                       const int32_t* relpos, const double* xy, int64_t n, int nrhs, double* vec, hipStream_t s);
+void launch_bwd_gather(const int32_t* lvl_nodes, int nn, const NodeDev* nodes, const int32_t* bnd, const double* xy,
+                       int64_t n, int nrhs, double* vec, hipStream_t s);
+void launch_sigma_gather(const int32_t* lvl_nodes, const NodeDev* nodes, const int32_t* relpos, double* S,
+                         const int32_t* tile_node, const int32_t* tile_col, int ntiles, hipStream_t s);
+void launch_gather_pattern(const double* S, const int64_t* mapsym, int64_t nnz, double* out, hipStream_t s);
+
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+
+// Device-side per-level index tables that are not part of LevelPlan.
+struct LevelDev {
+  int32_t* d_nodes = nullptr;
+  int nn = 0;
+  int32_t* d_gtile_node = nullptr;
+  int32_t* d_gtile_col = nullptr;
+  int ngtiles = 0;
+};
+
+}  // namespace spde
+
+using namespace spde;
+
+namespace {
+
+struct PlanExtra {            // kept beside the plan (not in the public struct)
+  std::vector<LevelDev> lvl;
+  int32_t* d_children = nullptr;
+};
+
+std::vector<std::pair<spde_plan*, PlanExtra*>> g_extra;
+
+PlanExtra* extra(spde_plan* p) {
+  for (auto& e : g_extra)
+    if (e.first == p) return e.second;
+  return nullptr;
+}
+
+thread_local bool g_host_only_build = false;  // plan descriptors built for inspection only
+
+template <class T>
+int upload(T** dst, const std::vector<T>& src) {
+  *dst = nullptr;
+  if (src.empty() || g_host_only_build) return SPDE_OK;
+  SPDE_HIP(hipMalloc((void**)dst, src.size() * sizeof(T)));
+  SPDE_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return SPDE_OK;
+}
+
+// Arena sizes (doubles) per operand base, for the host-side bounds check.
+struct ArenaSizes {
+  int64_t sz[kNumBases];
+};
+
+bool operand_ok(int64_t off, int64_t ld, int64_t rows, int64_t cols, int64_t size) {
+  if (rows <= 0 || cols <= 0) return true;
+  if (off < 0 || ld < rows) return false;
+  return off + (rows - 1) + (cols - 1) * ld < size;
+}
+
+// Every problem of a launch must touch only its operands' arenas.
+int validate(const GemmLaunch& g, const ArenaSizes& a, const char* what) {
+  for (const GemmProb& P : g.h_probs) {
+    const int64_t ar = g.ta ? P.K : P.M, ac = g.ta ? P.M : P.K;
+    const int64_t br = g.tb ? P.N : P.K, bc = g.tb ? P.K : P.N;
+    const bool ok = operand_ok(P.a_off, P.lda, ar, ac, a.sz[P.a_base]) &&
+                    operand_ok(P.b_off, P.ldb, br, bc, a.sz[P.b_base]) &&
+                    operand_ok(P.c_off, P.ldc, P.M, P.N, a.sz[P.c_base]);
+    if (!ok) {
+      set_error(std::string("launch plan out of bounds in ") + what);
+      return SPDE_ERR_STATE;
+    }
+  }
+  for (const GemmTile& t : g.h_tiles)
+    if (t.prob < 0 || t.prob >= (int)g.h_probs.size()) {
+      set_error(std::string("tile list out of range in ") + what);
+      return SPDE_ERR_STATE;
+    }
+  return SPDE_OK;
+}
+
+ArenaSizes arena_sizes(const spde_plan* pl, int nrhs) {
+  ArenaSizes a;
+  a.sz[kFront] = pl->sym.front_doubles;
+  a.sz[kZ] = pl->sym.z_doubles;
+  a.sz[kSigma] = pl->sym.front_doubles;
+  a.sz[kWork] = pl->work_doubles;
+  a.sz[kVec] = pl->sym.vec_doubles * nrhs;
+  a.sz[kX] = pl->sym.n * nrhs;
+  return a;
+}
+
+int finalize(GemmLaunch& g) {
+  g.nprobs = (int)g.h_probs.size();
+  g.ntiles = (int)g.h_tiles.size();
+  int rc = upload(&g.d_probs, g.h_probs);
+  if (rc) return rc;
+  return upload(&g.d_tiles, g.h_tiles);
+}
+
+void add_prob(GemmLaunch& g, const GemmProb& P, bool lower) {
+  if (P.M <= 0 || P.N <= 0) return;
+  if (P.K <= 0 && P.beta == 1.0) return;
+  const int idx = (int)g.h_probs.size();
+  g.h_probs.push_back(P);
+  const int tm = (P.M + 63) / 64, tn = (P.N + 63) / 64;
+  for (int a = 0; a < tm; ++a)
+    for (int b = 0; b < tn; ++b)
+      if (!lower || a >= b) g.h_tiles.push_back({idx, a, b, 0});
+  g.flops += 2.0 * P.M * (double)P.N * P.K * (lower ? 0.5 : 1.0);
+}
+
+GemmProb prob(int8_t ab, int64_t ao, int lda, int8_t bb, int64_t bo, int ldb, int8_t cb, int64_t co, int ldc, int M,
+              int N, int K, double alpha, double beta) {
+  GemmProb P;
+  P.a_off = ao;
+  P.b_off = bo;
+  P.c_off = co;
+  P.lda = lda;
+  P.ldb = ldb;
+  P.ldc = ldc;
+  P.M = M;
+  P.N = N;
+  P.K = K;
+  P.a_base = ab;
+  P.b_base = bb;
+  P.c_base = cb;
+  P.lower = 0;
+  P.alpha = alpha;
+  P.beta = beta;
+  return P;
+}
+
+int build_factor_plan(spde_plan* pl) {
+  const Symbolic& sym = pl->sym;
+  pl->levels.assign(sym.levels.size(), LevelPlan());
+  for (size_t l = 0; l < sym.levels.size(); ++l) {
+    LevelPlan& L = pl->levels[l];
+    L.nodes = sym.levels[l];
+    for (int s = 0; s < 2; ++s) {
+      for (int id : L.nodes) {
+        const Node& nd = sym.nodes[id];
+        if (s >= nd.nchild) continue;
+        const Node& c = sym.nodes[nd.child[s]];
+        const int b = (int)c.B.size();
+        if (b == 0) continue;
+        const int di = (int)L.h_ext[s].size();
+        L.h_ext[s].push_back({c.foff, nd.foff, c.relpos_off, c.n(), c.p, nd.n(), b});
+        const int nt = (b + 63) / 64;
+        for (int a = 0; a < nt; ++a)
+          for (int bb = 0; bb <= a; ++bb) L.h_exttile[s].push_back({di, a, bb, 0});
+      }
+      L.next[s] = (int)L.h_exttile[s].size();
+      int rc = upload(&L.d_ext[s], L.h_ext[s]);
+      if (rc) return rc;
+      rc = upload(&L.d_exttile[s], L.h_exttile[s]);
+      if (rc) return rc;
+    }
+    int pmax = 0;
+    for (int id : L.nodes) pmax = std::max(pmax, sym.nodes[id].p);
+    for (int k0 = 0; k0 < pmax; k0 += NB) {
+      Step st;
+      std::vector<PotrfDesc> pd;
+      std::vector<PanelDesc> pn;
+      st.update.ta = false;
+      st.update.tb = true;
+      for (int id : L.nodes) {
+        const Node& nd = sym.nodes[id];
+        if (nd.p <= k0) continue;
+        const int n = nd.n(), kb = std::min(NB, nd.p - k0);
+        pd.push_back({nd.foff, nd.zoff, n, nd.p, k0, kb, id, nd.start + k0});
+        for (int r0 = k0 + kb; r0 < n; r0 += 64)
+          pn.push_back({nd.foff, nd.zoff, n, nd.p, k0, kb, r0, std::min(64, n - r0)});
+        const int M = n - k0 - kb, N = nd.p - k0 - kb;
+        const int64_t off = nd.foff + (k0 + kb) + (int64_t)k0 * n;
+        add_prob(st.update,
+                 prob(kFront, off, n, kFront, off, n, kFront, nd.foff + (k0 + kb) + (int64_t)(k0 + kb) * n, n, M, N, kb,
+                      -1.0, 1.0),
+                 true);
+      }
+      st.npotrf = (int)pd.size();
+      st.npanel = (int)pn.size();
+      int rc = upload(&st.d_potrf, pd);
+      if (rc) return rc;
+      rc = upload(&st.d_panel, pn);
+      if (rc) return rc;
+      rc = finalize(st.update);
+      if (rc) return rc;
+      L.steps.push_back(std::move(st));
+    }
+    L.syrk.ta = false;
+    L.syrk.tb = true;
+    for (int id : L.nodes) {
+      const Node& nd = sym.nodes[id];
+      const int n = nd.n(), b = (int)nd.B.size();
+      add_prob(L.syrk,
+               prob(kFront, nd.foff + nd.p, n, kFront, nd.foff + nd.p, n, kFront, nd.foff + nd.p + (int64_t)nd.p * n, n,
+                    b, b, nd.p, -1.0, 1.0),
+               true);
+    }
+    int rc = finalize(L.syrk);
+    if (rc) return rc;
+  }
+  // inverse of L11 for every node: recursive halving, batched by height
+  struct Op {
+    int node, a, m, b;
+  };
+  std::vector<std::vector<Op>> ops;
+  std::function<int(int, int, int)> rec = [&](int node, int a, int b) -> int {
+    if (b - a <= NB) return 0;
+    int half = (b - a + 1) / 2;
+    half = ((half + NB - 1) / NB) * NB;
+    const int m = a + half;
+    if (m >= b) return 0;
+    const int h = 1 + std::max(rec(node, a, m), rec(node, m, b));
+    if ((int)ops.size() <= h) ops.resize(h + 1);
+    ops[h].push_back({node, a, m, b});
+    return h;
+  };
+  for (size_t q = 0; q < sym.nodes.size(); ++q) rec((int)q, 0, sym.nodes[q].p);
+  pl->trtri.clear();
+  int64_t work = 0;
+  for (size_t h = 1; h < ops.size(); ++h) {
+    GemmLaunch gx, gz;
+    int64_t woff = 0;
+    for (const Op& o : ops[h]) {
+      const Node& nd = sym.nodes[o.node];
+      const int n = nd.n(), p = nd.p;
+      const int r = o.b - o.m, c = o.m - o.a;
+      add_prob(gx, prob(kFront, nd.foff + o.m + (int64_t)o.a * n, n, kZ, nd.zoff + o.a + (int64_t)o.a * p, p, kWork,
+                        woff, r, r, c, c, 1.0, 0.0),
+               false);
+      add_prob(gz, prob(kZ, nd.zoff + o.m + (int64_t)o.m * p, p, kWork, woff, r, kZ, nd.zoff + o.m + (int64_t)o.a * p,
+                        p, r, c, r, -1.0, 0.0),
+               false);
+      woff += (int64_t)r * c;
+    }
+    work = std::max(work, woff);
+    int rc = finalize(gx);
+    if (rc) return rc;
+    rc = finalize(gz);
+    if (rc) return rc;
+    pl->trtri.push_back(std::move(gx));
+    pl->trtri.push_back(std::move(gz));
+  }
+  pl->trtri_heights = (int)(pl->trtri.size() / 2);
+  // selected-inverse workspace: per level sum of b*p
+  for (auto& lv : sym.levels) {
+    int64_t w = 0;
+    for (int id : lv) w += (int64_t)sym.nodes[id].B.size() * sym.nodes[id].p;
+    work = std::max(work, w);
+  }
+  work = std::max<int64_t>(work, pl->lat.Ns() * 60);
+  pl->work_doubles = std::max<int64_t>(work, 1);
+  const ArenaSizes as = arena_sizes(pl, 1);
+  for (auto& L : pl->levels) {
+    for (auto& st : L.steps) {
+      int rc = validate(st.update, as, "factor update");
+      if (rc) return rc;
+    }
+    int rc = validate(L.syrk, as, "factor syrk");
+    if (rc) return rc;
+    for (int c = 0; c < 2; ++c)
+      for (const ExtAddDesc& d : L.h_ext[c])
+        if (d.cfoff + (int64_t)d.cld * d.cld > pl->sym.front_doubles ||
+            d.pfoff + (int64_t)d.pld * d.pld > pl->sym.front_doubles || d.cp + d.b != d.cld) {
+          set_error("extend-add descriptor out of bounds");
+          return SPDE_ERR_STATE;
+        }
+  }
+  for (auto& g : pl->trtri) {
+    int rc = validate(g, as, "inverse of L11");
+    if (rc) return rc;
+  }
+  for (const Node& nd : sym.nodes)
+    for (int rp : nd.relpos)
+      if (nd.parent < 0 || rp < 0 || rp >= sym.nodes[nd.parent].n()) {
+        set_error("relative position out of the parent front");
+        return SPDE_ERR_STATE;
+      }
+  return SPDE_OK;
+}
+
+int build_solve_plan(spde_plan* pl, int k) {
+  const Symbolic& sym = pl->sym;
+  const int n = (int)sym.n;
+  for (auto& L : pl->levels) {
+    for (GemmLaunch* g : {&L.fwd_y, &L.fwd_u, &L.bwd_t, &L.bwd_x}) {
+      if (g->d_probs) hipFree(g->d_probs);
+      if (g->d_tiles) hipFree(g->d_tiles);
+      *g = GemmLaunch();
+    }
+    L.fwd_y.ta = false, L.fwd_y.tb = false;
+    L.fwd_u.ta = false, L.fwd_u.tb = false;
+    L.bwd_t.ta = true, L.bwd_t.tb = false;
+    L.bwd_x.ta = true, L.bwd_x.tb = false;
+    for (int id : L.nodes) {
+      const Node& nd = sym.nodes[id];
+      const int nf = nd.n(), p = nd.p, b = (int)nd.B.size();
+      const int64_t v = nd.voff * k;
+      add_prob(L.fwd_y, prob(kZ, nd.zoff, p, kVec, v, nf, kX, nd.start, n, p, k, p, 1.0, 0.0), false);
+      add_prob(L.fwd_u, prob(kFront, nd.foff + p, nf, kX, nd.start, n, kVec, v + p, nf, b, k, p, -1.0, 1.0), false);
+      add_prob(L.bwd_t, prob(kFront, nd.foff + p, nf, kVec, v + p, nf, kVec, v, nf, p, k, b, -1.0, 1.0), false);
+      add_prob(L.bwd_x, prob(kZ, nd.zoff, p, kVec, v, nf, kX, nd.start, n, p, k, p, 1.0, 0.0), false);
+    }
+    const ArenaSizes as = arena_sizes(pl, k);
+    for (GemmLaunch* g : {&L.fwd_y, &L.fwd_u, &L.bwd_t, &L.bwd_x}) {
+      int rc = validate(*g, as, "solve");
+      if (rc) return rc;
+      rc = finalize(*g);
+      if (rc) return rc;
+    }
+  }
+  pl->solve_nrhs = k;
+  return SPDE_OK;
+}
+
+int build_selinv_plan(spde_plan* pl) {
+  const Symbolic& sym = pl->sym;
+  PlanExtra* ex = extra(pl);
+  for (size_t l = 0; l < pl->levels.size(); ++l) {
+    LevelPlan& L = pl->levels[l];
+    L.si_x.ta = false, L.si_x.tb = false;
+    L.si_s21.ta = false, L.si_s21.tb = false;
+    L.si_s11a.ta = true, L.si_s11a.tb = false;
+    L.si_s11b.ta = true, L.si_s11b.tb = false;
+    int64_t woff = 0;
+    std::vector<int32_t> tn, tc;
+    for (size_t q = 0; q < L.nodes.size(); ++q) {
+      const Node& nd = sym.nodes[L.nodes[q]];
+      const int nf = nd.n(), p = nd.p, b = (int)nd.B.size();
+      if (b > 0 && nd.parent >= 0)
+        for (int c = 0; c < (b + 63) / 64; ++c) {
+          tn.push_back((int32_t)q);
+          tc.push_back(c);
+        }
+      add_prob(L.si_x, prob(kFront, nd.foff + p, nf, kZ, nd.zoff, p, kWork, woff, b, b, p, p, 1.0, 0.0), false);
+      add_prob(L.si_s21, prob(kSigma, nd.foff + p + (int64_t)p * nf, nf, kWork, woff, b, kSigma, nd.foff + p, nf, b, p,
+                              b, -1.0, 0.0),
+               false);
+      add_prob(L.si_s11a, prob(kZ, nd.zoff, p, kZ, nd.zoff, p, kSigma, nd.foff, nf, p, p, p, 1.0, 0.0), true);
+      add_prob(L.si_s11b, prob(kWork, woff, b, kSigma, nd.foff + p, nf, kSigma, nd.foff, nf, p, p, b, -1.0, 1.0), true);
+      woff += (int64_t)b * p;
+    }
+    L.si_work = woff;
+    if (woff > pl->work_doubles) {
+      set_error("selected-inverse workspace too small");
+      return SPDE_ERR_STATE;
+    }
+    const ArenaSizes as = arena_sizes(pl, 1);
+    for (GemmLaunch* g : {&L.si_x, &L.si_s21, &L.si_s11a, &L.si_s11b}) {
+      int rc = validate(*g, as, "selected inverse");
+      if (rc) return rc;
+      rc = finalize(*g);
+      if (rc) return rc;
+    }
+    LevelDev& D = ex->lvl[l];
+    D.ngtiles = (int)tn.size();
+    int rc = upload(&D.d_gtile_node, tn);
+    if (rc) return rc;
+    rc = upload(&D.d_gtile_col, tc);
+    if (rc) return rc;
+  }
+  pl->selinv_built = true;
+  return SPDE_OK;
+}
+
+void free_launch(GemmLaunch& g) {
+  hipFree(g.d_probs);
+  hipFree(g.d_tiles);
+  g.d_probs = nullptr;
+  g.d_tiles = nullptr;
+}
+
+Bases make_bases(spde_plan* pl, int slot, double* x) {
+  Bases b;
+  b.p[kFront] = pl->slots[slot].front;
+  b.p[kZ] = pl->slots[slot].z;
+  b.p[kSigma] = pl->d_sigma;
+  b.p[kWork] = pl->d_work;
+  b.p[kVec] = pl->d_vec;
+  b.p[kX] = x;
+  return b;
+}
+
+int check_slot(spde_plan* pl, int slot) {
+  if (!pl || pl->host_only) {
+    set_error("plan has no device state");
+    return SPDE_ERR_STATE;
+  }
+  if (slot < 0 || slot >= pl->nslots) {
+    set_error("factor slot out of range");
+    return SPDE_ERR_ARG;
+  }
+  return SPDE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* spde_last_error(void) { return g_err.c_str(); }
+
+int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int flags, spde_plan** out) {
+  if (!out || bc < 1 || bc > 3 || M < 2 || N < 2 || T < 2 || nslots < 0) {
+    set_error("spde_plan_create: invalid argument");
+    return SPDE_ERR_ARG;
+  }
+  *out = nullptr;
+  spde_plan* pl = new spde_plan();
+  pl->lat = Lattice{M, N, T, bc};
+  try {
+    build_structure(pl->lat, pl->st);
+    build_symbolic(pl->lat, pl->st, leaf_max > 0 ? leaf_max : 128, pl->sym);
+    build_scatter_map(pl->st, pl->sym, pl->map);
+  } catch (const std::exception& e) {
+    set_error(std::string("spde_plan_create: ") + e.what());
+    delete pl;
+    return SPDE_ERR_ARG;
+  }
+  const int64_t nnz = (int64_t)pl->st.indices.size();
+  pl->mapsym.resize(nnz);
+  for (int64_t e = 0; e < nnz; ++e) pl->mapsym[e] = pl->map[e] >= 0 ? pl->map[e] : pl->map[pl->st.mirror[e]];
+  pl->host_only = (flags & SPDE_PLAN_HOST_ONLY) != 0;
+  pl->nslots = nslots;
+  PlanExtra* ex = new PlanExtra();
+  g_extra.push_back({pl, ex});
+  if (pl->host_only) {
+    // build and bounds-check every launch schedule without touching the device
+    g_host_only_build = true;
+    ex->lvl.resize(pl->sym.levels.size());
+    int rc = build_factor_plan(pl);
+    if (!rc) rc = build_solve_plan(pl, 3);
+    if (!rc) rc = build_selinv_plan(pl);
+    g_host_only_build = false;
+    if (rc) {
+      spde_plan_destroy(pl);
+      return rc;
+    }
+    *out = pl;
+    return SPDE_OK;
+  }
+  const Symbolic& sym = pl->sym;
+  int rc = SPDE_OK;
+#define TRY(x)                   \
+  do {                           \
+    rc = (x);                    \
+    if (rc) {                    \
+      spde_plan_destroy(pl);     \
+      return rc;                 \
+    }                            \
+  } while (0)
+  TRY(upload(&pl->d_indptr, pl->st.indptr));
+  TRY(upload(&pl->d_indices, pl->st.indices));
+  TRY(upload(&pl->d_code, pl->st.code));
+  TRY(upload(&pl->d_diag, pl->st.diag));
+  TRY(upload(&pl->d_map, pl->map));
+  TRY(upload(&pl->d_mapsym, pl->mapsym));
+  TRY(upload(&pl->d_perm, sym.perm));
+  std::vector<int32_t> relpos(std::max<int64_t>(sym.relpos_total, 1)), bnd(std::max<int64_t>(sym.relpos_total, 1));
+  std::vector<NodeDev> nodes(sym.nodes.size());
+  std::vector<int32_t> children(2 * sym.nodes.size(), -1);
+  for (size_t q = 0; q < sym.nodes.size(); ++q) {
+    const Node& nd = sym.nodes[q];
+    for (size_t k = 0; k < nd.B.size(); ++k) {
+      bnd[nd.relpos_off + k] = nd.B[k];
+      relpos[nd.relpos_off + k] = nd.relpos.empty() ? 0 : nd.relpos[k];
+    }
+    nodes[q] = NodeDev{nd.foff, nd.zoff, nd.voff, nd.relpos_off, nd.start, nd.p, nd.n(), nd.parent,
+                       (int32_t)nd.B.size(), nd.level, 0, 0};
+    for (int c = 0; c < nd.nchild; ++c) children[2 * q + c] = nd.child[c];
+  }
+  TRY(upload(&pl->d_relpos, relpos));
+  TRY(upload(&pl->d_bnd, bnd));
+  TRY(upload(&pl->d_nodes, nodes));
+  TRY(upload(&ex->d_children, children));
+  ex->lvl.resize(sym.levels.size());
+  for (size_t l = 0; l < sym.levels.size(); ++l) {
+    ex->lvl[l].nn = (int)sym.levels[l].size();
+    TRY(upload(&ex->lvl[l].d_nodes, sym.levels[l]));
+  }
+  TRY(build_factor_plan(pl));
+  const int64_t Ns = pl->lat.Ns();
+  SPDE_HIP(hipMalloc((void**)&pl->d_tmpl, sizeof(double) * Ns * 93));
+  SPDE_HIP(hipMalloc((void**)&pl->d_acan, sizeof(double) * Ns * 9));
+  SPDE_HIP(hipMalloc((void**)&pl->d_a0can, sizeof(double) * Ns * 9));
+  SPDE_HIP(hipMalloc((void**)&pl->d_qdiag, sizeof(double) * Ns));
+  SPDE_HIP(hipMalloc((void**)&pl->d_work, sizeof(double) * pl->work_doubles));
+  SPDE_HIP(hipMalloc((void**)&pl->d_partial, sizeof(double) * std::max<size_t>(sym.nodes.size(), 1)));
+  SPDE_HIP(hipMalloc((void**)&pl->d_scalar, sizeof(double) * 16));
+  SPDE_HIP(hipMalloc((void**)&pl->d_status, sizeof(int)));
+  SPDE_HIP(hipMemset(pl->d_status, 0, sizeof(int)));
+  pl->slots.resize(nslots);
+  for (int s = 0; s < nslots; ++s) {
+    SPDE_HIP(hipMalloc((void**)&pl->slots[s].front, sizeof(double) * std::max<int64_t>(sym.front_doubles, 1)));
+    SPDE_HIP(hipMalloc((void**)&pl->slots[s].z, sizeof(double) * std::max<int64_t>(sym.z_doubles, 1)));
+    SPDE_HIP(hipMemset(pl->slots[s].z, 0, sizeof(double) * std::max<int64_t>(sym.z_doubles, 1)));
+  }
+  *out = pl;
+  return SPDE_OK;
+#undef TRY
+}
+
+void spde_plan_destroy(spde_plan* pl) {
+  if (!pl) return;
+  if (pl->host_only) {
+    for (size_t i = 0; i < g_extra.size(); ++i)
+      if (g_extra[i].first == pl) {
+        delete g_extra[i].second;
+        g_extra.erase(g_extra.begin() + i);
+        break;
+      }
+  } else {
+    hipDeviceSynchronize();
+    for (auto* p : {(void*)pl->d_indptr, (void*)pl->d_indices, (void*)pl->d_code, (void*)pl->d_diag,
+                    (void*)pl->d_map, (void*)pl->d_mapsym, (void*)pl->d_perm, (void*)pl->d_relpos,
+                    (void*)pl->d_bnd, (void*)pl->d_nodes, (void*)pl->d_tmpl, (void*)pl->d_acan,
+                    (void*)pl->d_a0can, (void*)pl->d_qdiag, (void*)pl->d_sigma, (void*)pl->d_work, (void*)pl->d_vec,
+                    (void*)pl->d_xy, (void*)pl->d_partial, (void*)pl->d_scalar, (void*)pl->d_status})
+      hipFree(p);
+    for (auto& s : pl->slots) {
+      hipFree(s.front);
+      hipFree(s.z);
+    }
+    for (auto& L : pl->levels) {
+      for (int s = 0; s < 2; ++s) {
+        hipFree(L.d_ext[s]);
+        hipFree(L.d_exttile[s]);
+      }
+      for (auto& st : L.steps) {
+        hipFree(st.d_potrf);
+        hipFree(st.d_panel);
+        free_launch(st.update);
+      }
+      for (GemmLaunch* g : {&L.syrk, &L.fwd_y, &L.fwd_u, &L.bwd_t, &L.bwd_x, &L.si_x, &L.si_s21, &L.si_s11a,
+                            &L.si_s11b})
+        free_launch(*g);
+    }
+    for (auto& g : pl->trtri) free_launch(g);
+    for (size_t i = 0; i < g_extra.size(); ++i)
+      if (g_extra[i].first == pl) {
+        PlanExtra* ex = g_extra[i].second;
+        for (auto& d : ex->lvl) {
+          hipFree(d.d_nodes);
+          hipFree(d.d_gtile_node);
+          hipFree(d.d_gtile_col);
+        }
+        hipFree(ex->d_children);
+        delete ex;
+        g_extra.erase(g_extra.begin() + i);
+        break;
+      }
+  }
+  delete pl;
+}
+
+int spde_plan_get_info(const spde_plan* pl, spde_plan_info* info) {
+  if (!pl || !info) {
+    set_error("spde_plan_get_info: null argument");
+    return SPDE_ERR_ARG;
+  }
+  info->n = pl->sym.n;
+  info->nnz_struct = (int64_t)pl->st.indices.size();
+  info->nnz_L = pl->sym.nnz_L;
+  info->front_doubles = pl->sym.front_doubles;
+  info->z_doubles = pl->sym.z_doubles;
+  info->nnodes = (int64_t)pl->sym.nodes.size();
+  info->nlevels = (int64_t)pl->sym.levels.size();
+  info->max_front = pl->sym.max_front;
+  info->max_p = pl->sym.max_p;
+  info->flops_factor = pl->sym.flops_factor;
+  info->flops_selinv = pl->sym.flops_selinv;
+  return SPDE_OK;
+}
+
+int spde_plan_export(const spde_plan* pl, int64_t* indptr, int32_t* indices, uint8_t* code, int32_t* perm,
+                     int32_t* nodes, int64_t* foff, int32_t* bnd, int32_t* relpos, int64_t* map) {
+  if (!pl) {
+    set_error("spde_plan_export: null plan");
+    return SPDE_ERR_ARG;
+  }
+  const auto& st = pl->st;
+  const auto& sym = pl->sym;
+  if (indptr) std::memcpy(indptr, st.indptr.data(), st.indptr.size() * sizeof(int64_t));
+  if (indices) std::memcpy(indices, st.indices.data(), st.indices.size() * sizeof(int32_t));
+  if (code) std::memcpy(code, st.code.data(), st.code.size());
+  if (perm) std::memcpy(perm, sym.perm.data(), sym.perm.size() * sizeof(int32_t));
+  if (map) std::memcpy(map, pl->map.data(), pl->map.size() * sizeof(int64_t));
+  for (size_t q = 0; q < sym.nodes.size(); ++q) {
+    const Node& nd = sym.nodes[q];
+    if (nodes) {
+      int32_t* r = nodes + 8 * q;
+      r[0] = nd.start;
+      r[1] = nd.p;
+      r[2] = nd.n();
+      r[3] = nd.parent;
+      r[4] = nd.child[0];
+      r[5] = nd.child[1];
+      r[6] = nd.level;
+      r[7] = (int32_t)nd.B.size();
+    }
+    if (foff) foff[q] = nd.foff;
+    for (size_t k = 0; k < nd.B.size(); ++k) {
+      if (bnd) bnd[nd.relpos_off + k] = nd.B[k];
+      if (relpos) relpos[nd.relpos_off + k] = nd.relpos.empty() ? -1 : nd.relpos[k];
+    }
+  }
+  return SPDE_OK;
+}
+
+int spde_factor(spde_plan* pl, int slot, const double* qval, const double* diag_add, double* logdet, void* stream) {
+  int rc = check_slot(pl, slot);
+  if (rc) return rc;
+  if (!qval) {
+    set_error("spde_factor: null values");
+    return SPDE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  PlanExtra* ex = extra(pl);
+  Slot& sl = pl->slots[slot];
+  sl.factored = false;
+  const Symbolic& sym = pl->sym;
+  SPDE_HIP(hipMemsetAsync(pl->d_status, 0, sizeof(int), s));
+  launch_fill(sl.front, sym.front_doubles, 0.0, s);
+  launch_scatter(qval, pl->d_map, (int64_t)pl->st.indices.size(), sl.front, s);
+  if (diag_add) launch_diag_add(diag_add, pl->d_diag, pl->d_map, sym.n, sl.front, s);
+  Bases b = make_bases(pl, slot, nullptr);
+  for (size_t l = 0; l < pl->levels.size(); ++l) {
+    LevelPlan& L = pl->levels[l];
+    for (int c = 0; c < 2; ++c) launch_extend_add(L.d_ext[c], L.d_exttile[c], L.next[c], pl->d_relpos, sl.front, s);
+    for (Step& st : L.steps) {
+      launch_potrf(st.d_potrf, st.npotrf, sl.front, sl.z, pl->d_status, s);
+      launch_panel(st.d_panel, st.npanel, sl.front, sl.z, s);
+      launch_gemm(st.update, b, s);
+    }
+    launch_gemm(L.syrk, b, s);
+  }
+  for (auto& g : pl->trtri) launch_gemm(g, b, s);
+  launch_logdet(pl->d_nodes, (int)sym.nodes.size(), sl.front, pl->d_partial, pl->d_scalar, s);
+  int status = 0;
+  double ld = 0.0;
+  SPDE_HIP(hipMemcpyAsync(&status, pl->d_status, sizeof(int), hipMemcpyDeviceToHost, s));
+  SPDE_HIP(hipMemcpyAsync(&ld, pl->d_scalar, sizeof(double), hipMemcpyDeviceToHost, s));
+  SPDE_HIP(hipStreamSynchronize(s));
+  (void)ex;
+  if (status != 0) {
+    const int col = status - 1;
+    set_error("spde_factor: matrix is not positive definite (pivot at lattice index " +
+              std::to_string(sym.perm[col]) + ")");
+    return SPDE_ERR_NOT_SPD;
+  }
+  if (logdet) *logdet = ld;
+  sl.factored = true;
+  return SPDE_OK;
+}
+
+int spde_solve(spde_plan* pl, int slot, int mode, double* B, int nrhs, int64_t ldb, void* stream) {
+  int rc = check_slot(pl, slot);
+  if (rc) return rc;
+  if (!B || nrhs <= 0 || ldb < pl->sym.n || mode < 0 || mode > 2) {
+    set_error("spde_solve: invalid argument");
+    return SPDE_ERR_ARG;
+  }
+  if (!pl->slots[slot].factored) {
+    set_error("spde_solve: slot not factored");
+    return SPDE_ERR_STATE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  PlanExtra* ex = extra(pl);
+  const Symbolic& sym = pl->sym;
+  const int64_t n = sym.n;
+  if (pl->vec_cap < sym.vec_doubles * nrhs) {
+    SPDE_HIP(hipStreamSynchronize(s));
+    hipFree(pl->d_vec);
+    pl->vec_cap = sym.vec_doubles * nrhs;
+    SPDE_HIP(hipMalloc((void**)&pl->d_vec, sizeof(double) * pl->vec_cap));
+  }
+  if (pl->xy_cap < n * nrhs) {
+    SPDE_HIP(hipStreamSynchronize(s));
+    hipFree(pl->d_xy);
+    pl->xy_cap = n * nrhs;
+    SPDE_HIP(hipMalloc((void**)&pl->d_xy, sizeof(double) * pl->xy_cap));
+  }
+  if (pl->solve_nrhs != nrhs) {
+    SPDE_HIP(hipStreamSynchronize(s));
+    rc = build_solve_plan(pl, nrhs);
+    if (rc) return rc;
+  }
+  Bases b = make_bases(pl, slot, pl->d_xy);
+  const int nl = (int)pl->levels.size();
+  if (mode == SPDE_SOLVE_LT) {
+    SPDE_HIP(hipMemcpy2DAsync(pl->d_xy, n * sizeof(double), B, ldb * sizeof(double), n * sizeof(double), nrhs,
+                              hipMemcpyDeviceToDevice, s));
+  } else {
+    launch_permute_in(B, ldb, pl->d_perm, n, nrhs, pl->d_xy, s);
+  }
+  if (mode != SPDE_SOLVE_LT) {
+    for (int l = 0; l < nl; ++l) {
+      LevelPlan& L = pl->levels[l];
+      launch_fwd_gather(ex->lvl[l].d_nodes, ex->lvl[l].nn, pl->d_nodes, ex->d_children, pl->d_relpos, pl->d_xy, n,
+                        nrhs, pl->d_vec, s);
+      launch_gemm(L.fwd_y, b, s);
+      launch_gemm(L.fwd_u, b, s);
+    }
+  }
+  if (mode != SPDE_SOLVE_L) {
+    for (int l = nl - 1; l >= 0; --l) {
+      LevelPlan& L = pl->levels[l];
+      launch_bwd_gather(ex->lvl[l].d_nodes, ex->lvl[l].nn, pl->d_nodes, pl->d_bnd, pl->d_xy, n, nrhs, pl->d_vec, s);
+      launch_gemm(L.bwd_t, b, s);
+      launch_gemm(L.bwd_x, b, s);
+    }
+  }
+  if (mode == SPDE_SOLVE_L) {
+    SPDE_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double), pl->d_xy, n * sizeof(double), n * sizeof(double), nrhs,
+                              hipMemcpyDeviceToDevice, s));
+  } else {
+    launch_permute_out(pl->d_xy, pl->d_perm, n, nrhs, B, ldb, s);
+  }
+  return SPDE_OK;
+}
+
+int spde_selinv(spde_plan* pl, int slot, double* sigma, void* stream) {
+  int rc = check_slot(pl, slot);
+  if (rc) return rc;
+  if (!pl->slots[slot].factored) {
+    set_error("spde_selinv: slot not factored");
+    return SPDE_ERR_STATE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  PlanExtra* ex = extra(pl);
+  const Symbolic& sym = pl->sym;
+  if (!pl->d_sigma) SPDE_HIP(hipMalloc((void**)&pl->d_sigma, sizeof(double) * std::max<int64_t>(sym.front_doubles, 1)));
+  if (!pl->selinv_built) {
+    rc = build_selinv_plan(pl);
+    if (rc) return rc;
+  }
+  Bases b = make_bases(pl, slot, nullptr);
+  for (int l = (int)pl->levels.size() - 1; l >= 0; --l) {
+    LevelPlan& L = pl->levels[l];
+    LevelDev& D = ex->lvl[l];
+    launch_sigma_gather(D.d_nodes, pl->d_nodes, pl->d_relpos, pl->d_sigma, D.d_gtile_node, D.d_gtile_col, D.ngtiles,
+                        s);
+    launch_gemm(L.si_x, b, s);
+    launch_gemm(L.si_s21, b, s);
+    launch_gemm(L.si_s11a, b, s);
+    launch_gemm(L.si_s11b, b, s);
+  }
+  if (sigma) launch_gather_pattern(pl->d_sigma, pl->d_mapsym, (int64_t)pl->st.indices.size(), sigma, s);
+  return SPDE_OK;
+}
+
+}  // extern "C"

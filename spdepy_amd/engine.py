@@ -1,0 +1,138 @@
+"""GPU evaluation engine of one space-time lattice: assembly of Q, the two
+Cholesky factorisations, log-determinants, mu_c, the log-likelihood and the
+per-cell gradient adjoints (``logLike``, advection_idiffusion2D.py:167-207 and
+var_advection_var_diffusion2D.py:201-237).  Everything runs in the native
+library on the current HIP stream; torch supplies device memory only."""
+import numpy as np
+
+from . import _native as nat
+from .plan import Plan, _ptr, _stream
+
+_PLANS = {}
+
+# column layout of the per-cell adjoints written by spde_grad_cells
+OUT_KAPPA, OUT_H, OUT_DGX, OUT_DGY, OUT_KAPPA0, OUT_H0, OUT_HC, OUT_WX, OUT_WY = 0, 1, 17, 21, 25, 26, 30, 34, 35
+OUT_DOTGQ, OUT_DOT00, OUT_TAUTR, OUT_WIDTH = 36, 37, 38, 40
+
+
+def get_plan(M, N, T, bc, leaf_max=128, nslots=2):
+    key = (M, N, T, bc, leaf_max, nslots)
+    if key not in _PLANS:
+        _PLANS[key] = Plan(M, N, T, bc, leaf_max=leaf_max, nslots=nslots)
+    return _PLANS[key]
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("spdepy_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+class Coefficients:
+    """spde_coeffs plus the device tensors it points to (kept alive here)."""
+
+    def __init__(self, grid, *, variable, kappa, H, G, sigma, kappa0, H0):
+        torch = _torch()
+        c = nat.Coeffs()
+        c.variable = int(variable)
+        c.hx, c.hy, c.dt, c.V, c.sigma = grid.hx, grid.hy, grid.dt, grid.V, float(sigma)
+        self.keep = []
+        if variable:
+            self.kappa = torch.as_tensor(np.ascontiguousarray(kappa, dtype=np.float64)).cuda()
+            self.H = torch.as_tensor(np.ascontiguousarray(H, dtype=np.float64).reshape(-1)).cuda()
+            self.G = torch.as_tensor(np.ascontiguousarray(G, dtype=np.float64).reshape(-1)).cuda()
+            c.kappa, c.H, c.G = self.kappa.data_ptr(), self.H.data_ptr(), self.G.data_ptr()
+        else:
+            c.kappa_c = float(kappa)
+            Hc = np.asarray(H, dtype=np.float64).reshape(4)
+            for q in range(4):
+                c.H_c[q] = Hc[q]
+            c.G_c[0], c.G_c[1] = float(G[0]), float(G[1])
+        c.kappa0 = float(kappa0)
+        H0 = np.asarray(H0, dtype=np.float64).reshape(4)
+        for q in range(4):
+            c.H0[q] = H0[q]
+        self.c = c
+
+
+class Engine:
+    def __init__(self, grid, bc, leaf_max=128):
+        M, N, T = grid.shape
+        self.grid, self.bc = grid, bc
+        self.plan = get_plan(M, N, T, bc, leaf_max)
+        self.n, self.Ns, self.nnz = grid.n, grid.Ns, self.plan.nnz
+
+    # -------------------------------------------------------------- assembly
+    def assemble(self, coeffs):
+        torch = _torch()
+        q = torch.empty(self.nnz, dtype=torch.float64, device="cuda")
+        self.plan.assemble(coeffs.c, q)
+        return q
+
+    def to_scipy(self, qval):
+        """The reference-pattern CSC (exact zeros dropped as scipy does)."""
+        from scipy import sparse
+        indptr, indices, data = self.plan.prune(qval)
+        n = self.n
+        # Q is symmetric: its CSR arrays are the CSC arrays of Q^T = Q
+        return sparse.csc_matrix((data.cpu().numpy(), indices.cpu().numpy(), indptr.cpu().numpy()), shape=(n, n))
+
+    # ------------------------------------------------------- log-likelihood
+    def loglik(self, coeffs, obs_cols, data, log_tau, grad=True, trace="exact", probes=None):
+        """Returns (like, parts) where like is the normalised -loglik/(m r) and
+        parts holds the per-cell gradient adjoints and the scalar pieces."""
+        torch = _torch()
+        y = np.asarray(data, dtype=np.float64)
+        y2 = y.reshape(y.shape[0], -1)
+        m, r = y2.shape
+        tau = float(np.exp(log_tau))
+        n = self.n
+        qval = self.assemble(coeffs)
+        counts = np.bincount(obs_cols, minlength=n).astype(np.float64)
+        diag_add = torch.as_tensor(tau * counts).cuda()
+        b = np.zeros((n, r))
+        np.add.at(b, obs_cols, y2)
+        b *= tau
+        ld_q = self.plan.factor(0, qval)
+        ld_qc = self.plan.factor(1, qval, diag_add)
+        mu = self.plan.solve(1, torch.as_tensor(b).cuda())
+        qmu = self.plan.spmv(qval, mu)
+        cols_t = torch.as_tensor(obs_cols).cuda()
+        resid = torch.as_tensor(y2).cuda() - mu[cols_t]
+        quad = float((mu * qmu).sum())
+        rss = float((resid * resid).sum())
+        like = 0.5 * ld_q * r + m * r * np.log(tau) / 2 - 0.5 * ld_qc * r - 0.5 * quad - tau / 2 * rss
+        parts = dict(m=m, r=r, tau=tau, rss=rss, ld_q=ld_q, ld_qc=ld_qc, qval=qval, mu=mu, like_raw=like)
+        if not grad:
+            return -like / (m * r), parts
+        g = nat.GradArgs()
+        g.r = r
+        g.qval = qval.data_ptr()
+        mu_cm = mu.t().contiguous()            # n x r column-major
+        g.mu = mu_cm.data_ptr()
+        g.diag_add = diag_add.data_ptr()
+        g.coeffs = coeffs.c
+        keep = [mu_cm]
+        if trace == "exact":
+            sq = self.plan.selinv(0)
+            sqc = self.plan.selinv(1)
+            g.mode, g.nh1 = 0, 0
+            g.sigma_q, g.sigma_qc = sq.data_ptr(), sqc.data_ptr()
+            keep += [sq, sqc]
+        else:
+            V = torch.as_tensor(np.ascontiguousarray(probes, dtype=np.float64)).cuda()
+            trq = self.plan.solve(0, V).t().contiguous()
+            trqc = self.plan.solve(1, V).t().contiguous()
+            Vt = V.t().contiguous()
+            g.mode, g.nh1 = 1, V.shape[1]
+            g.trq, g.trqc, g.probes = trq.data_ptr(), trqc.data_ptr(), Vt.data_ptr()
+            keep += [trq, trqc, Vt]
+        out = torch.empty(self.Ns * OUT_WIDTH, dtype=torch.float64, device="cuda")
+        nat.check(nat.lib().spde_grad_cells(self.plan._h, g, _ptr(out), _stream(torch)))
+        cells = out.cpu().numpy().reshape(self.Ns, OUT_WIDTH)
+        parts["cells"] = cells
+        parts["g_sigma"] = -(cells[:, OUT_DOTGQ].sum() - cells[:, OUT_DOT00].sum())
+        parts["g_tau"] = m * r / 2 - 0.5 * r * cells[:, OUT_TAUTR].sum() - tau / 2 * rss
+        del keep
+        return -like / (m * r), parts

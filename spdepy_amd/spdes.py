@@ -1,0 +1,563 @@
+"""Drop-in SPDE model classes of the GMRF hot path.
+
+Same names, parameter layouts, method signatures and return conventions as
+berild/spdepy ``src/spdepy/spdes``:
+  * WhittleMatern2D / WhittleMaternAnisotropic2D / WhittleMaternHa2D
+    (``whittle_matern{,_anisotropic,_ha}2D.py``) -- the initial-state model;
+  * AdvectionIDiffusion2D / AdvectionDiffusion2D / AdvectionHaDiffusion2D
+    (``advection_{i,,ha_}diffusion2D.py``) -- stationary space-time models;
+  * VarAdvectionVarDiffusion2D (``var_advection_var_diffusion2D.py``) --
+    B-spline advection and anisotropic diffusion fields.
+``makeQ`` returns (Q, Q_fac, dQ) with Q a scipy CSC matrix carrying the
+reference's sparsity pattern; ``logLike(par, nh1=100, grad=True)`` returns
+-loglik/(m r) and its gradient.  All numerics run on the GPU through the
+native library (no CPU fallback).  ``logLike`` takes one extra keyword,
+``trace``: "exact" (default) contracts with the selected inverse;
+"hutchinson" reproduces the reference's stochastic estimator with nh1
+Rademacher probes drawn from numpy's global legacy RNG exactly as
+``advection_idiffusion2D.py:182`` does.
+"""
+import numpy as np
+from scipy import sparse
+
+from . import _native as nat
+from .engine import (Coefficients, Engine, OUT_DGX, OUT_DGY, OUT_H, OUT_H0, OUT_HC, OUT_KAPPA, OUT_KAPPA0,
+                     OUT_WX, OUT_WY)
+from .plan import _ptr, _stream
+
+
+# ------------------------------------------------------------- H tensors
+def h_tensor(kind, gamma, vx=0.0, vy=0.0):
+    """Diffusion tensor and its derivatives w.r.t. (log gamma, vx, vy):
+    iso gamma I; ani gamma I + v v^T; ha gamma(cosh|v| I + sinh|v|/|v| [[vx,vy],[vy,-vx]])
+    (advection_idiffusion2D.py:97, advection_diffusion2D.py:100,
+    advection_ha_diffusion2D.py:109-115 and their gradient blocks)."""
+    eye = np.eye(2)
+    if kind == "iso":
+        return gamma * eye, [gamma * eye]
+    if kind == "ani":
+        v = np.array([vx, vy])
+        ex, ey = np.array([1.0, 0.0]), np.array([0.0, 1.0])
+        return (gamma * eye + np.outer(v, v),
+                [gamma * eye, np.outer(ex, v) + np.outer(v, ex), np.outer(ey, v) + np.outer(v, ey)])
+    a = np.sqrt(vx * vx + vy * vy)
+    mv = np.array([[vx, vy], [vy, -vx]])
+    ch, sh = (np.exp(a) + np.exp(-a)) / 2, (np.exp(a) - np.exp(-a)) / 2
+    H = gamma * (ch * eye + sh / a * mv)
+    dvx = gamma / a * (vx * sh * eye + vx / a * (ch - sh / a) * mv + sh * np.array([[1.0, 0.0], [0.0, -1.0]]))
+    dvy = gamma / a * (vy * sh * eye + vy / a * (ch - sh / a) * mv + sh * np.array([[0.0, 1.0], [1.0, 0.0]]))
+    return H, [H, dvx, dvy]
+
+
+# ------------------------------------------------------- native stencils
+def _torch():
+    import torch
+    return torch
+
+
+def stencil_diffusion(shape2, H, hx, hy, bc, variable):
+    """Reference slot triplets of the diffusion stencil computed on the GPU
+    (replaces AH_new/AH_Row/AH_Col/AH_Val/AH_delete)."""
+    torch = _torch()
+    M, N = int(shape2[0]), int(shape2[1])
+    Ht = torch.as_tensor(np.ascontiguousarray(H, dtype=np.float64).reshape(-1)).cuda()
+    row = torch.empty(M * N * 9, dtype=torch.int32, device="cuda")
+    col = torch.empty_like(row)
+    val = torch.empty(M * N * 9, dtype=torch.float64, device="cuda")
+    nat.check(nat.lib().spde_stencil_diffusion(M, N, _ptr(Ht), int(variable), hx, hy, bc, _ptr(row), _ptr(col),
+                                               _ptr(val), _stream(torch)))
+    return row.cpu().numpy(), col.cpu().numpy(), val.cpu().numpy()
+
+
+def stencil_advection(shape2, G, hx, hy, diff, bc, variable, dG=None):
+    """Reference slot triplets of the upwind stencil computed on the GPU
+    (replaces Aw_new/Aw_Row/Aw_Col/Aw_Val/Aw_delete)."""
+    torch = _torch()
+    M, N = int(shape2[0]), int(shape2[1])
+    Gt = torch.as_tensor(np.ascontiguousarray(G, dtype=np.float64).reshape(-1)).cuda()
+    dGt = None if dG is None else torch.as_tensor(np.ascontiguousarray(dG, dtype=np.float64).reshape(-1)).cuda()
+    row = torch.empty(M * N * 5, dtype=torch.int32, device="cuda")
+    col = torch.empty_like(row)
+    val = torch.empty(M * N * 5, dtype=torch.float64, device="cuda")
+    nat.check(nat.lib().spde_stencil_advection(M, N, _ptr(Gt), _ptr(dGt), int(variable), hx, hy, diff, bc,
+                                               _ptr(row), _ptr(col), _ptr(val), _stream(torch)))
+    return row.cpu().numpy(), col.cpu().numpy(), val.cpu().numpy()
+
+
+def _triplets_csc(row, col, val, n, nan_to_zero=False):
+    keep = row != n
+    v = val[keep].copy()
+    if nan_to_zero:
+        v[np.isnan(v)] = 0.0
+    return sparse.csc_matrix((v, (row[keep], col[keep])), shape=(n, n))
+
+
+class _StencilMixin:
+    def Ah(self, Hs):
+        shp = self.grid.shape
+        n = int(shp[0] * shp[1])
+        Hs = np.asarray(Hs, dtype=np.float64)
+        variable = Hs.size != 4
+        return _triplets_csc(*stencil_diffusion(shp, Hs, self.grid.hx, self.grid.hy, self.bc, variable), n)
+
+    def Aw(self, ws, dws=None, diff=3):
+        shp = self.grid.shape
+        n = int(shp[0] * shp[1])
+        ws = np.asarray(ws, dtype=np.float64)
+        variable = ws.size != 2
+        return _triplets_csc(*stencil_advection(shp, ws, self.grid.hx, self.grid.hy, diff, self.bc, variable,
+                                                dws), n, nan_to_zero=variable)
+
+
+# -------------------------------------------------------- factor object
+class Factor:
+    """Device-resident Cholesky factor with the CHOLMOD Factor calls the
+    reference uses (``logdet``, ``solve_A``, ``solve_Lt``, ``apply_Pt``).
+    P is this library's nested-dissection permutation.  If the plan's slot
+    was reused by a later factorisation the factor is recomputed from the
+    stored values first."""
+
+    def __init__(self, engine, slot, qval, diag_add=None):
+        self.engine, self.slot, self.qval, self.diag_add = engine, slot, qval, diag_add
+        self._logdet = None
+
+    def _ensure(self):
+        plan = self.engine.plan
+        if plan.slot_owner.get(self.slot) is not self:
+            self._logdet = plan.factor(self.slot, self.qval, self.diag_add, owner=self)
+
+    def logdet(self):
+        self._ensure()
+        return self._logdet
+
+    def _solve(self, b, mode):
+        torch = _torch()
+        self._ensure()
+        arr = np.asarray(b, dtype=np.float64)
+        one = arr.ndim == 1
+        x = self.engine.plan.solve(self.slot, torch.as_tensor(arr.reshape(arr.shape[0], -1)).cuda(), mode)
+        out = x.cpu().numpy()
+        return out.reshape(-1) if one else out
+
+    def solve_A(self, b):
+        return self._solve(b, nat.SOLVE_A)
+
+    def solve_L(self, b, use_LDLt_decomposition=False):
+        """L^-1 P b (result in elimination order)."""
+        return self._solve(b, nat.SOLVE_L)
+
+    def solve_Lt(self, b, use_LDLt_decomposition=False):
+        """P^T L^-T b as one call; apply_Pt is then the identity on the result."""
+        return _PtApplied(self._solve(b, nat.SOLVE_LT))
+
+    def apply_Pt(self, b):
+        return np.asarray(b) if isinstance(b, _PtApplied) else self._permute(b, inverse=True)
+
+    def apply_P(self, b):
+        return self._permute(b, inverse=False)
+
+    def _permute(self, b, inverse):
+        perm = self.engine.plan.export()["perm"]
+        b = np.asarray(b)
+        out = np.empty_like(b)
+        if inverse:
+            out[perm] = b
+        else:
+            out[:] = b[perm]
+        return out
+
+
+class _PtApplied(np.ndarray):
+    def __new__(cls, arr):
+        return np.asarray(arr).view(cls)
+
+
+# --------------------------------------------------------- spatial model
+class _WhittleMaternBase(_StencilMixin):
+    kind = "iso"
+
+    def __init__(self, grid, par=None, bc=3):
+        self.grid, self.bc = grid, bc
+        self.Q = None
+        self.Q_fac = None
+        self.data = None
+        self.r = None
+        self.S = None
+        if par is None:
+            self.setPars(self._default())
+        else:
+            self.setQ(par=par)
+
+    def _default(self):
+        return np.array([-1, -1, np.log(100)] if self.kind == "iso" else [-1, -1, 0.1, 0.1, np.log(100)],
+                        dtype=np.float64)
+
+    def getPars(self, *args, **kwargs):
+        return np.array(self.par, dtype=np.float64)
+
+    def setPars(self, par):
+        self.par = np.array(par, dtype=np.float64)
+        self.kappa, self.gamma = self.par[0], self.par[1]
+        if self.kind != "iso":
+            self.vx, self.vy = self.par[2], self.par[3]
+        self.tau = self.par[-1]
+        self.sigma = np.log(np.sqrt(1 / np.exp(self.tau)))
+
+    def nself(self):
+        return 3 if self.kind == "iso" else 5
+
+    def h0(self, par):
+        """(kappa0, H0, [dH0/dlog gamma, dH0/dvx, dH0/dvy])."""
+        vx, vy = (par[2], par[3]) if self.kind != "iso" else (0.0, 0.0)
+        H, dH = h_tensor(self.kind, np.exp(par[1]), vx, vy)
+        return np.exp(par[0]), H, dH
+
+    def initFit(self, data, **kwargs):
+        assert data.shape[0] <= self.grid.Ns
+        self.data = data
+        self.r = data.shape[1] if data.ndim == 2 else 1
+        self.S = self.grid.getS(idxs=kwargs.get("idx"))
+        return self._default()
+
+    def setQ(self, par=None, S=None):
+        if par is None:
+            par = self.getPars()
+        else:
+            self.setPars(par)
+        self.Q, self.Q_fac, _ = self.makeQ(par=par, grad=False)
+        self.S = self.grid.getS()
+
+    def makeQ(self, par, grad=True):
+        """Q0 = A0^T iDv A0, A0 = Dv kappa - A_H(H) (whittle_matern2D.py:59-85)."""
+        g = self.grid
+        kappa, H, dH = self.h0(np.asarray(par, dtype=np.float64))
+        Dk = kappa * sparse.eye(g.Ns)
+        A = g.Dv @ Dk - self.Ah(H)
+        Q = (A.transpose() @ g.iDv @ A).tocsc()
+        if not grad:
+            return Q, None, None
+        dQ = []
+        dA = (g.Dv @ Dk).tocsc()
+        dQ.append(dA.T @ g.iDv @ A + A.T @ g.iDv @ dA)
+        for d in dH:
+            dA = -self.Ah(d)
+            dQ.append(dA.T @ g.iDv @ A + A.T @ g.iDv @ dA)
+        return Q, None, dQ
+
+    def print(self, par):
+        s = "| κ = %2.2f, γ = %2.2f" % (np.exp(par[0]), np.exp(par[1]))
+        if self.kind != "iso":
+            s += ", vx = %2.2f, vy = %2.2f" % (par[2], par[3])
+        return s + ",τ = %2.2f" % np.exp(par[-1])
+
+
+class WhittleMatern2D(_WhittleMaternBase):
+    kind = "iso"
+
+
+class WhittleMaternAnisotropic2D(_WhittleMaternBase):
+    kind = "ani"
+
+
+class WhittleMaternHa2D(_WhittleMaternBase):
+    kind = "ha"
+
+
+# ---------------------------------------------------- space-time models
+class _SpaceTimeBase(_StencilMixin):
+    """Shared logLike / makeQ plumbing of the space-time models."""
+
+    def _engine(self):
+        if getattr(self, "_eng", None) is None:
+            self._eng = Engine(self.grid, self.bc)
+        return self._eng
+
+    def _mod0_split(self, par):
+        """Initial-state parameters carried in par (fitQ0) or the model's own."""
+        p0 = self._par_mod0(par)
+        src = p0 if p0 is not None else self.mod0.getPars()
+        kappa0, H0, dH0 = self.mod0.h0(src)
+        return p0, kappa0, H0, dH0
+
+    def initFit(self, data, **kwargs):
+        assert data.shape[0] <= self.grid.n
+        if kwargs.get("fitQ0") is not None:
+            assert type(kwargs.get("fitQ0")) is bool
+            self.fitQ0 = kwargs.get("fitQ0")
+        x0 = self.mod0.getPars()[:-1] if self.fitQ0 else np.zeros(0)
+        par = np.hstack([self._init_self(), x0, [np.log(100)]]).astype(np.float64)
+        self.data = data
+        self.r = data.shape[1] if data.ndim == 2 else 1
+        self.S = self.grid.getS(idxs=kwargs.get("idx"))
+        self._obs_idx = kwargs.get("idx")
+        return par
+
+    def setQ(self, par=None, S=None):
+        if par is None:
+            par = self.getPars()
+        else:
+            self.setPars(par)
+        if S is not None:
+            self.S = S
+        self.Q, self.Q_fac, _ = self.makeQ(par=par, grad=False)
+        self.S = self.grid.getS()
+
+    def makeQ(self, par, grad=True):
+        par = np.asarray(par, dtype=np.float64)
+        eng = self._engine()
+        coeffs = self._coeffs(par)
+        qval = eng.assemble(coeffs)
+        Q = eng.to_scipy(qval)
+        fac = Factor(eng, 0, qval)
+        self._last_qval = qval
+        if not grad:
+            return Q, fac, None
+        return Q, fac, _LazyDQ(self, par)
+
+    def _obs_cols(self):
+        S = self.S.tocsr()
+        if np.any(np.diff(S.indptr) != 1):
+            raise ValueError("observation matrix must have one entry per row")
+        return S.indices.astype(np.int64)
+
+    def logLike(self, par, nh1=100, grad=True, trace="exact"):
+        par = np.asarray(par, dtype=np.float64)
+        eng = self._engine()
+        coeffs = self._coeffs(par)
+        probes = None
+        if grad and trace == "hutchinson":
+            probes = (2 * np.random.randint(1, 3, self.grid.n * nh1) - 3).reshape(self.grid.n, nh1)
+        like, parts = eng.loglik(coeffs, self._obs_cols(), self.data, par[-1], grad=grad, trace=trace,
+                                 probes=probes)
+        if not grad:
+            return like
+        g = self._contract(par, coeffs, parts)
+        m, r = parts["m"], parts["r"]
+        return like, -g / (m * r)
+
+    def _mod0_grad(self, par, parts, dH0, kappa0):
+        cells = parts["cells"]
+        if self._par_mod0(par) is None:
+            return []
+        hbar = cells[:, OUT_H0:OUT_H0 + 4].sum(0)
+        out = [kappa0 * cells[:, OUT_KAPPA0].sum()]
+        out += [float(np.dot(hbar, d.reshape(4))) for d in dH0]
+        return out
+
+
+class _StationaryAdvDiff(_SpaceTimeBase):
+    kind = "iso"
+
+    def __init__(self, grid, mod0, par=None, bc=3):
+        self.grid, self.mod0, self.bc = grid, mod0, bc
+        self.type = {"iso": "advection-idiffusion-2D-bc%d", "ani": "advection-diffusion-2D-bc%d",
+                     "ha": "advection-ha-diffusion-2D-bc%d"}[self.kind] % bc
+        self.Q = self.Q_fac = self.data = self.r = self.S = None
+        self.fitQ0 = True
+        self._eng = None
+        if par is None:
+            self.setPars(np.hstack([self._init_self(), [np.log(100)]]))
+        else:
+            self.setQ(par=par)
+
+    def _nself(self):
+        return 5 if self.kind == "iso" else 7
+
+    def _init_self(self):
+        return [-1, -1, 0.1, 0.1, 0] if self.kind == "iso" else [-1, -1, 0.1, 0.1, 0.1, 0.1, 0]
+
+    def _par_mod0(self, par):
+        k = self._nself()
+        return par[k:] if par.size > k + 1 else None
+
+    def getPars(self, onlySelf=True):
+        own = [self.kappa, self.gamma] + ([] if self.kind == "iso" else [self.vx, self.vy]) + \
+              [self.wx, self.wy, self.sigma]
+        if onlySelf:
+            return np.hstack(own + [self.tau]).astype(np.float64)
+        return np.hstack(own + [self.mod0.getPars()[:-1], self.tau]).astype(np.float64)
+
+    def setPars(self, par):
+        par = np.array(par, dtype=np.float64)
+        self.kappa, self.gamma = par[0], par[1]
+        if self.kind == "iso":
+            self.wx, self.wy, self.sigma = par[2], par[3], par[4]
+        else:
+            self.vx, self.vy, self.wx, self.wy, self.sigma = par[2], par[3], par[4], par[5], par[6]
+        if par.size > self._nself() + 1:
+            self.mod0.setPars(par[self._nself():])
+        self.tau = par[-1]
+
+    def _split(self, par):
+        if self.kind == "iso":
+            return par[0], par[1], 0.0, 0.0, par[2:4], par[4]
+        return par[0], par[1], par[2], par[3], par[4:6], par[6]
+
+    def _coeffs(self, par):
+        kap, gam, vx, vy, ws, sig = self._split(par)
+        H, _ = h_tensor(self.kind, np.exp(gam), vx, vy)
+        _, kappa0, H0, _ = self._mod0_split(par)
+        return Coefficients(self.grid, variable=False, kappa=np.exp(kap), H=H, G=ws, sigma=np.exp(sig),
+                            kappa0=kappa0, H0=H0)
+
+    def _contract(self, par, coeffs, parts):
+        kap, gam, vx, vy, ws, sig = self._split(par)
+        _, dH = h_tensor(self.kind, np.exp(gam), vx, vy)
+        _, kappa0, H0, dH0 = self._mod0_split(par)
+        cells = parts["cells"]
+        hbar = cells[:, OUT_HC:OUT_HC + 4].sum(0)
+        g = [np.exp(kap) * cells[:, OUT_KAPPA].sum()]
+        g += [float(np.dot(hbar, d.reshape(4))) for d in dH]
+        g += [cells[:, OUT_WX].sum(), cells[:, OUT_WY].sum(), parts["g_sigma"]]
+        g += self._mod0_grad(par, parts, dH0, kappa0)
+        g += [parts["g_tau"]]
+        return np.array(g, dtype=np.float64)
+
+    def print(self, par):
+        kap, gam, vx, vy, ws, sig = self._split(np.asarray(par))
+        s = "| κ = %2.2f, γ = %2.2f" % (np.exp(kap), np.exp(gam))
+        if self.kind != "iso":
+            s += ", vx = %2.2f, vy = %2.2f" % (vx, vy)
+        return s + ", wx = %2.2f, wy = %2.2f, σ = %2.2f, τ = %2.2f" % (ws[0], ws[1], np.exp(sig),
+                                                                          np.exp(par[-1]))
+
+
+class AdvectionIDiffusion2D(_StationaryAdvDiff):
+    """Params [log kappa, log gamma, wx, wy, log sigma, (mod0...), log tau]."""
+    kind = "iso"
+
+
+class AdvectionDiffusion2D(_StationaryAdvDiff):
+    """Params [log kappa, log gamma, vx, vy, wx, wy, log sigma, (mod0...), log tau]."""
+    kind = "ani"
+
+
+class AdvectionHaDiffusion2D(_StationaryAdvDiff):
+    kind = "ha"
+
+
+class VarAdvectionVarDiffusion2D(_SpaceTimeBase):
+    """B-spline fields: par = [c_kappa, c_gamma, c_vx, c_vy, c_wx, c_wy (Np each),
+    log sigma, (mod0...), log tau] (var_advection_var_diffusion2D.py:28-45)."""
+
+    def __init__(self, grid, mod0, par=None, bc=3):
+        self.grid, self.mod0, self.bc = grid, mod0, bc
+        self.type = "var-advection-var-diffusion-2D-bc%d" % bc
+        self.Np = grid.Nbs2
+        self.Q = self.Q_fac = self.data = self.r = self.S = None
+        self.fitQ0 = True
+        self._eng = None
+        if par is None:
+            self.setPars(np.hstack([self._init_self(), [np.log(100)]]))
+        else:
+            self.setQ(par=par)
+
+    def _init_self(self):
+        Np = self.Np
+        return np.hstack([[-1] * Np, [-1] * Np, [0.1] * Np * 2, [0.1] * Np * 2, [0]])
+
+    def _par_mod0(self, par):
+        k = self.Np * 6 + 1
+        return par[k:] if par.size > k + 1 else None
+
+    def getPars(self, onlySelf=True):
+        own = [self.kappa, self.gamma, self.vx, self.vy, self.wx, self.wy, self.sigma]
+        if onlySelf:
+            return np.hstack(own + [self.tau]).astype(np.float64)
+        return np.hstack(own + [self.mod0.getPars()[:-1], self.tau]).astype(np.float64)
+
+    def setPars(self, par):
+        par = np.array(par, dtype=np.float64)
+        Np = self.Np
+        self.kappa, self.gamma = par[0:Np], par[Np:2 * Np]
+        self.vx, self.vy = par[2 * Np:3 * Np], par[3 * Np:4 * Np]
+        self.wx, self.wy = par[4 * Np:5 * Np], par[5 * Np:6 * Np]
+        self.sigma = par[6 * Np]
+        if par.size > 6 * Np + 2:
+            self.mod0.setPars(par[6 * Np + 1:])
+        self.tau = par[-1]
+
+    def fields(self, par):
+        g, Np = self.grid, self.Np
+        kappa = np.exp(g.bs @ par[:Np])
+        gamma = np.exp(g.bsH @ par[Np:2 * Np])
+        vx = g.bsH @ par[2 * Np:3 * Np]
+        vy = g.bsH @ par[3 * Np:4 * Np]
+        ws = g.evalAdv(par[4 * Np:6 * Np])
+        return kappa, gamma, vx, vy, ws
+
+    def _coeffs(self, par):
+        kappa, gamma, vx, vy, ws = self.fields(par)
+        H = np.empty((self.grid.Ns, 4, 2, 2))
+        H[:, :, 0, 0] = gamma + vx * vx
+        H[:, :, 0, 1] = vx * vy
+        H[:, :, 1, 0] = vy * vx
+        H[:, :, 1, 1] = gamma + vy * vy
+        _, kappa0, H0, _ = self._mod0_split(par)
+        self._fields = (kappa, gamma, vx, vy)
+        return Coefficients(self.grid, variable=True, kappa=kappa, H=H, G=ws, sigma=np.exp(par[6 * self.Np]),
+                            kappa0=kappa0, H0=H0)
+
+    def _contract(self, par, coeffs, parts):
+        g, Np = self.grid, self.Np
+        kappa, gamma, vx, vy = self._fields
+        cells = parts["cells"]
+        Hb = cells[:, OUT_H:OUT_H + 16].reshape(-1, 4, 2, 2)
+        out = [g.bs.T @ (cells[:, OUT_KAPPA] * kappa)]
+        out.append(np.einsum("kfi,kf->i", g.bsH, (Hb[:, :, 0, 0] + Hb[:, :, 1, 1]) * gamma))
+        cross = Hb[:, :, 0, 1] + Hb[:, :, 1, 0]
+        out.append(np.einsum("kfi,kf->i", g.bsH, 2 * vx * Hb[:, :, 0, 0] + vy * cross))
+        out.append(np.einsum("kfi,kf->i", g.bsH, 2 * vy * Hb[:, :, 1, 1] + vx * cross))
+        wx_bar = cells[:, OUT_DGX:OUT_DGX + 4]
+        wy_bar = cells[:, OUT_DGY:OUT_DGY + 4]
+        if g.isExtended:
+            Ae = g.ext_map()
+            wx_bar, wy_bar = Ae.T @ wx_bar, Ae.T @ wy_bar
+        out.append(g.bsA[:, 0, :].T @ wx_bar[:, 0] + g.bsA[:, 2, :].T @ wx_bar[:, 2])
+        out.append(g.bsA[:, 1, :].T @ wy_bar[:, 1] + g.bsA[:, 3, :].T @ wy_bar[:, 3])
+        out.append([parts["g_sigma"]])
+        _, kappa0, H0, dH0 = self._mod0_split(par)
+        out.append(self._mod0_grad(par, parts, dH0, kappa0))
+        out.append([parts["g_tau"]])
+        return np.hstack(out).astype(np.float64)
+
+    def print(self, par):
+        Np = self.Np
+        return ("| κ = %2.2f, γ = %2.2f, vx = %2.2f, vy = %2.2f, wx = %2.2f, wy = %2.2f, "
+                "σ = %2.2f, τ = %2.2f" % (np.exp(par[:Np]).mean(), np.exp(par[Np:2 * Np]).mean(),
+                                                     par[2 * Np:3 * Np].mean(), par[3 * Np:4 * Np].mean(),
+                                                     par[4 * Np:5 * Np].mean(), par[5 * Np:6 * Np].mean(),
+                                                     np.exp(par[6 * Np]), np.exp(par[-1])))
+
+
+class _LazyDQ:
+    """The reference's list of dQ_i, produced on demand (one matrix per
+    parameter except tau).  Not needed by logLike, whose gradient contracts all
+    of them at once through the adjoint of the assembly."""
+
+    def __init__(self, model, par):
+        self.model, self.par = model, par
+
+    def __len__(self):
+        return self.par.size - 1
+
+    def __getitem__(self, i):
+        raise NotImplementedError("explicit dQ matrices are not materialised; logLike contracts them on the GPU")
+
+
+def spde_init(model, grid, parameters=None, ani=True, ha=True, bc=3, mod0=None):
+    """Dispatcher (spdes/__init__.py:1-103) for the classes of the hot path."""
+    if grid.sdim != 2:
+        raise NotImplementedError("only 2-D spatial and space-time grids")
+    if model in ("whittle-matern", 1):
+        cls = WhittleMaternHa2D if ha else (WhittleMaternAnisotropic2D if ani else WhittleMatern2D)
+        return cls(grid=grid, par=parameters, bc=bc)
+    if model in ("advection-diffusion", 2):
+        cls = AdvectionHaDiffusion2D if ha else (AdvectionDiffusion2D if ani else AdvectionIDiffusion2D)
+        return cls(grid=grid, mod0=mod0, par=parameters, bc=bc)
+    if model in ("var-advection-var-diffusion", 7):
+        if ha or not ani:
+            raise NotImplementedError("var-advection-var-diffusion is provided with anisotropic diffusion "
+                                      "(ha=False, anisotropic=True)")
+        return VarAdvectionVarDiffusion2D(grid=grid, mod0=mod0, par=parameters, bc=bc)
+    raise NotImplementedError("model %r is outside the GMRF hot path built here" % (model,))

@@ -1,0 +1,153 @@
+"""GPU parity of the native path against the oracle (MI355X)."""
+import os
+
+import numpy as np
+import pytest
+from scipy import sparse
+
+from oracle import fixtures as F, models as Mo, stencils as OS
+from tests.gpu_util import product_case, step
+
+pytestmark = pytest.mark.gpu
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "stencils.npz"))
+
+
+def _names():
+    return sorted({k.split("/")[0] for k in GOLD.files if "/" in k})
+
+
+def test_stencils_bitexact_vs_reference_golden():
+    from spdepy_amd import spdes
+    hx, hy = float(GOLD["hx"]), float(GOLD["hy"])
+    bad = []
+    for name in _names():
+        kind, rest = name.split("_", 1)
+        dims, b = rest.split("_b")
+        M, N = map(int, dims.split("x"))
+        bc = int(b[0])
+        step("stencil " + name)
+        if kind in ("AHv", "AHiso", "AHc", "AHciso"):
+            got = spdes.stencil_diffusion((M, N), GOLD[name + "/in"], hx, hy, bc, kind in ("AHv", "AHiso"))
+        elif kind in ("Awc", "Awc0"):
+            got = spdes.stencil_advection((M, N), GOLD[name + "/in"], hx, hy, int(rest.split("_d")[1]), bc, False)
+        else:
+            got = spdes.stencil_advection((M, N), GOLD[name + "/G"], hx, hy, int(rest.split("_d")[1]), bc, True,
+                                          GOLD[name + "/dG"])
+        row = GOLD[name + "/row"]
+        keep = row != M * N
+        ok = (np.array_equal(got[0], row) and np.array_equal(got[1][keep], GOLD[name + "/col"][keep])
+              and np.array_equal(got[2][keep], GOLD[name + "/val"][keep], equal_nan=True))
+        if not ok:
+            bad.append(name)
+    assert not bad, bad
+
+
+CASES = {
+    "var_b1": lambda: F.small(9, 8, 4, bc=1, m=40),
+    "var_b3": lambda: F.small(8, 9, 3, bc=3, m=40),
+    "var_b1_ext": lambda: F.small(7, 6, 3, bc=1, m=30, ne=2),
+    "iso_b1": lambda: F.stationary(10, 9, 4, bc=1, m=40),
+    "ani_b3": lambda: F.stationary(9, 10, 3, bc=3, kind="ani", m=40),
+    "ha_b1": lambda: F.stationary(8, 9, 4, bc=1, kind="ha", m=40),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_assembly_pattern_bitexact_values(case):
+    c = CASES[case]()
+    step("assemble " + case)
+    m = product_case(c)
+    Qg, fac, _ = m.makeQ(c["par"], grad=False)
+    Qo, _ = c["model"].makeQ(c["par"], grad=False)
+    Qo = Qo.tocsc()
+    Qo.sort_indices()
+    Qg.sort_indices()
+    assert np.array_equal(Qg.indptr, Qo.indptr)
+    assert np.array_equal(Qg.indices, Qo.indices)
+    assert np.allclose(Qg.data, Qo.data, rtol=1e-12, atol=1e-13 * np.abs(Qo.data).max())
+    step("factor " + case)
+    ld = fac.logdet()
+    ref = np.linalg.slogdet(Qo.toarray())[1]
+    assert abs(ld - ref) <= 1e-11 * abs(ref)
+
+
+@pytest.mark.parametrize("leaf", [8, 40, 1000])
+def test_factor_solve_selinv_small(leaf):
+    """Multi-level trees (small leaves) and a single dense front (large leaf)."""
+    import torch
+    from spdepy_amd.engine import Coefficients, Engine
+    c = CASES["var_b1"]()
+    m = product_case(c)
+    g = m.grid
+    eng = Engine(g, 1, leaf_max=leaf)
+    step("factor leaf=%d" % leaf)
+    qval = eng.assemble(m._coeffs(c["par"]))
+    Q = eng.to_scipy(qval).toarray()
+    ld = eng.plan.factor(0, qval)
+    assert abs(ld - np.linalg.slogdet(Q)[1]) <= 1e-11 * abs(np.linalg.slogdet(Q)[1])
+    rng = np.random.default_rng(1)
+    B = rng.normal(size=(g.n, 3))
+    step("solve leaf=%d" % leaf)
+    X = eng.plan.solve(0, torch.as_tensor(B).cuda()).cpu().numpy()
+    assert np.allclose(Q @ X, B, rtol=1e-9, atol=1e-9 * np.abs(B).max())
+    Y = eng.plan.solve(0, torch.as_tensor(B).cuda(), 1)
+    X2 = eng.plan.solve(0, Y, 2).cpu().numpy()
+    assert np.allclose(X2, X, rtol=1e-9, atol=1e-12)
+    step("selinv leaf=%d" % leaf)
+    s = eng.plan.selinv(0).cpu().numpy()
+    indptr, indices = eng.plan.structure()
+    rows = np.repeat(np.arange(g.n), np.diff(indptr))
+    inv = np.linalg.inv(Q)
+    assert np.allclose(s, inv[rows, indices], rtol=1e-9, atol=1e-12 * np.abs(inv).max())
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_loglike_exact_gradient(case):
+    c = CASES[case]()
+    g = c["grid"]
+    S = g.getS(c["idx"])
+    Qo, dQo = c["model"].makeQ(c["par"], grad=True)
+    like_o, jac_o, _ = Mo.loglike(Qo, dQo, S, c["y"], c["par"][-1], trace="exact")
+    m = product_case(c)
+    m.initFit(c["y"], idx=c["idx"], fitQ0=False)
+    step("logLike exact " + case)
+    like, jac = m.logLike(c["par"])
+    assert abs(like - like_o) <= 1e-9 * abs(like_o)
+    assert jac.shape == jac_o.shape
+    scale = np.abs(jac_o).max()
+    assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * scale), np.abs(jac - jac_o).max() / scale
+
+
+def test_loglike_hutchinson_same_probes():
+    c = CASES["var_b1"]()
+    g = c["grid"]
+    S = g.getS(c["idx"])
+    Qo, dQo = c["model"].makeQ(c["par"], grad=True)
+    V = Mo.reference_probes(g.n, 20, seed=11)
+    like_o, jac_o, _ = Mo.loglike(Qo, dQo, S, c["y"], c["par"][-1], trace="hutchinson", probes=V)
+    m = product_case(c)
+    m.initFit(c["y"], idx=c["idx"], fitQ0=False)
+    np.random.seed(11)
+    step("logLike hutchinson")
+    like, jac = m.logLike(c["par"], nh1=20, trace="hutchinson")
+    assert abs(like - like_o) <= 1e-9 * abs(like_o)
+    scale = np.abs(jac_o).max()
+    assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * scale)
+
+
+def test_loglike_with_initial_state_parameters():
+    """fitQ0: mod0 parameters inside par, their gradient from the Q0 block."""
+    c = CASES["iso_b1"]()
+    g = c["grid"]
+    S = g.getS(c["idx"])
+    par = np.hstack([c["par"][:-1], c["mod0"].par[:-1], c["par"][-1:]])
+    Qo, dQo = c["model"].makeQ(par, grad=True)
+    like_o, jac_o, _ = Mo.loglike(Qo, dQo, S, c["y"], par[-1], trace="exact")
+    m = product_case(c)
+    m.initFit(c["y"], idx=c["idx"])
+    step("logLike fitQ0")
+    like, jac = m.logLike(par)
+    assert abs(like - like_o) <= 1e-9 * abs(like_o)
+    scale = np.abs(jac_o).max()
+    assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * scale)
