@@ -168,6 +168,14 @@ typedef struct spde_grad_args {
 
 int spde_grad_cells(spde_plan* plan, const spde_grad_args* args, double* out, void* stream);
 
+/* ---------------------------------------------------------------- measurement
+ * Between spde_prof_begin and spde_prof_end every FP64 GEMM launch of the
+ * plan (factorisation, inverse of L11, solves, selected inverse) is bracketed
+ * by HIP events on its stream; spde_prof_end synchronises and returns the
+ * summed kernel time (ms), the summed algorithmic flops and the launch count. */
+int spde_prof_begin(spde_plan* plan);
+int spde_prof_end(spde_plan* plan, double* gemm_ms, double* gemm_flops, int64_t* launches);
+
 /* ---------------------------------------------------------------- diagnostics
  * No counterpart in the reference: self-checks of the dense kernels.
  * spde_debug_mfma_probe writes the 64x4 accumulator lanes of one FP64 MFMA
@@ -175,7 +183,7 @@ int spde_grad_cells(spde_plan* plan, const spde_grad_args* args, double* out, vo
  * the library's batched GEMM kernel on one problem (column-major, device). */
 int spde_debug_mfma_probe(int which, double* out, void* stream);
 int spde_debug_gemm(int ta, int tb, int M, int N, int K, double alpha, const double* A, int lda,
-                    const double* B, int ldb, double beta, double* C, int ldc, void* stream);
+                    const double* B, int ldb, double beta, double* C, int ldc, int tile, void* stream);
 
 #ifdef __cplusplus
 }

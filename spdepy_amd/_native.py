@@ -67,9 +67,11 @@ _SIGS = {
     "spde_factor": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, c_dp, vp]),
     "spde_solve": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int64, vp]),
     "spde_selinv": (ctypes.c_int, [vp, ctypes.c_int, vp, vp]),
+    "spde_prof_begin": (ctypes.c_int, [vp]),
+    "spde_prof_end": (ctypes.c_int, [vp, c_dp, c_dp, c_i64p]),
     "spde_debug_mfma_probe": (ctypes.c_int, [ctypes.c_int, vp, vp]),
     "spde_debug_gemm": (ctypes.c_int, [ctypes.c_int] * 5 + [ctypes.c_double, vp, ctypes.c_int, vp, ctypes.c_int,
-                                                            ctypes.c_double, vp, ctypes.c_int, vp]),
+                                                            ctypes.c_double, vp, ctypes.c_int, ctypes.c_int, vp]),
     "spde_grad_cells": (ctypes.c_int, [vp, ctypes.POINTER(GradArgs), vp, vp]),
 }
 

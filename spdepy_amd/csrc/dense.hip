@@ -14,18 +14,22 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 // ----------------------------------------------------------------- GEMM
 // C(M x N) = alpha op(A) op(B) + beta C, column-major, FP64 MFMA.
-// Block tile 64x64x16, four waves in a 2x2 arrangement (32x32 each, i.e. 2x2
-// MFMA 16x16 tiles).  The MFMA computes the transposed tile (B-fragment as the
-// matrix "A" input) so that the 16 lanes of a row group own 16 consecutive
-// rows of one column of C: stores are 128-byte coalesced in column-major
-// memory.  Accumulator layout of v_mfma_f64_16x16x4f64 (measured): lane l,
-// register v holds D[l/16 + 4v][l%16].
-template <bool TA, bool TB>
+// Block tile BM x BN x 16 (64 or 128), four waves in a 2x2 arrangement, each
+// wave (BM/2) x (BN/2) = (BM/32) x (BN/32) MFMA 16x16 tiles.  LDS is double
+// buffered and the next K-slice is prefetched into registers while the
+// current one feeds the MFMAs.  The MFMA computes the transposed tile
+// (B-fragment as the matrix "A" input) so that the 16 lanes of a row group own
+// 16 consecutive rows of one column of C: stores are 128-byte coalesced in
+// column-major memory.  Accumulator layout of v_mfma_f64_16x16x4f64
+// (measured): lane l, register v holds D[l/16 + 4v][l%16].
+template <bool TA, bool TB, int BM, int BN>
 __global__ __launch_bounds__(256) void gemm_kernel(const GemmProb* __restrict__ probs,
                                                    const GemmTile* __restrict__ tiles, Bases bases) {
-  constexpr int BM = 64, BN = 64, BK = 16, LDA_S = BM + 4, LDB_S = BN + 4;
-  __shared__ double As[BK][LDA_S];
-  __shared__ double Bs[BK][LDB_S];
+  constexpr int BK = 16, PAD = 4;
+  constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+  constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
+  __shared__ double As[2][BK][BM + PAD];
+  __shared__ double Bs[2][BK][BN + PAD];
   const GemmTile t = tiles[blockIdx.x];
   const GemmProb P = probs[t.prob];
   const double* __restrict__ A = bases.p[P.a_base] + P.a_off;
@@ -34,68 +38,74 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmProb* __restrict__ 
   const int M = P.M, N = P.N, K = P.K;
   const int m0 = t.tm * BM, n0 = t.tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
   const int lr = lane & 15, lk = lane >> 4;
-  d4 acc[2][2];
+  d4 acc[TN][TM];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < TN; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-
-  for (int k0 = 0; k0 < K; k0 += BK) {
+    for (int b = 0; b < TM; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  double ra[LA], rb[LB];
+  auto load = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < LA; ++q) {
       const int idx = tid + q * 256;
-      int mm, kk;
-      if (!TA) {
-        mm = idx & 63;
-        kk = idx >> 6;
-      } else {
-        kk = idx & 15;
-        mm = idx >> 4;
-      }
+      const int mm = TA ? idx / BK : idx % BM, kk = TA ? idx % BK : idx / BM;
       const int gm = m0 + mm, gk = k0 + kk;
-      double v = 0.0;
-      if (gm < M && gk < K) v = TA ? A[gk + (int64_t)gm * P.lda] : A[gm + (int64_t)gk * P.lda];
-      As[kk][mm] = v;
+      ra[q] = (gm < M && gk < K) ? (TA ? A[gk + (int64_t)gm * P.lda] : A[gm + (int64_t)gk * P.lda]) : 0.0;
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < LB; ++q) {
       const int idx = tid + q * 256;
-      int nn, kk;
-      if (TB) {
-        nn = idx & 63;
-        kk = idx >> 6;
-      } else {
-        kk = idx & 15;
-        nn = idx >> 4;
-      }
+      const int nn = TB ? idx % BN : idx / BK, kk = TB ? idx / BN : idx % BK;
       const int gn = n0 + nn, gk = k0 + kk;
-      double v = 0.0;
-      if (gn < N && gk < K) v = TB ? B[gn + (int64_t)gk * P.ldb] : B[gk + (int64_t)gn * P.ldb];
-      Bs[kk][nn] = v;
+      rb[q] = (gn < N && gk < K) ? (TB ? B[gn + (int64_t)gk * P.ldb] : B[gk + (int64_t)gn * P.ldb]) : 0.0;
     }
-    __syncthreads();
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < LA; ++q) {
+      const int idx = tid + q * 256;
+      const int mm = TA ? idx / BK : idx % BM, kk = TA ? idx % BK : idx / BM;
+      As[buf][kk][mm] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < LB; ++q) {
+      const int idx = tid + q * 256;
+      const int nn = TB ? idx % BN : idx / BK, kk = TB ? idx / BN : idx % BK;
+      Bs[buf][kk][nn] = rb[q];
+    }
+  };
+  const int nk = (K + BK - 1) / BK;
+  if (nk > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      double a[2], b[2];
+      double a[TM], b[TN];
 #pragma unroll
-      for (int mi = 0; mi < 2; ++mi) a[mi] = As[kk + lk][wm + mi * 16 + lr];
+      for (int mi = 0; mi < TM; ++mi) a[mi] = As[cur][kk + lk][wm + mi * 16 + lr];
 #pragma unroll
-      for (int ni = 0; ni < 2; ++ni) b[ni] = Bs[kk + lk][wn + ni * 16 + lr];
+      for (int ni = 0; ni < TN; ++ni) b[ni] = Bs[cur][kk + lk][wn + ni * 16 + lr];
 #pragma unroll
-      for (int ni = 0; ni < 2; ++ni)
+      for (int ni = 0; ni < TN; ++ni)
 #pragma unroll
-        for (int mi = 0; mi < 2; ++mi)
+        for (int mi = 0; mi < TM; ++mi)
           acc[ni][mi] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[ni][mi], 0, 0, 0);
     }
+    if (kt + 1 < nk) store(cur ^ 1);
     __syncthreads();
   }
   const double alpha = P.alpha, beta = P.beta;
 #pragma unroll
-  for (int ni = 0; ni < 2; ++ni)
+  for (int ni = 0; ni < TN; ++ni)
 #pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
+    for (int mi = 0; mi < TM; ++mi)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int m = m0 + wm + mi * 16 + lr;
@@ -108,73 +118,116 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmProb* __restrict__ 
       }
 }
 
-void launch_gemm(const GemmLaunch& g, const Bases& b, hipStream_t s) {
-  if (g.ntiles == 0) return;
+template <int T>
+static void launch_gemm_t(const GemmLaunch& g, const Bases& b, hipStream_t s) {
   dim3 grid(g.ntiles), block(256);
   if (!g.ta && !g.tb)
-    hipLaunchKernelGGL((gemm_kernel<false, false>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+    hipLaunchKernelGGL((gemm_kernel<false, false, T, T>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
   else if (!g.ta && g.tb)
-    hipLaunchKernelGGL((gemm_kernel<false, true>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+    hipLaunchKernelGGL((gemm_kernel<false, true, T, T>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
   else if (g.ta && !g.tb)
-    hipLaunchKernelGGL((gemm_kernel<true, false>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+    hipLaunchKernelGGL((gemm_kernel<true, false, T, T>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
   else
-    hipLaunchKernelGGL((gemm_kernel<true, true>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+    hipLaunchKernelGGL((gemm_kernel<true, true, T, T>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+}
+
+void launch_gemm(const GemmLaunch& g, const Bases& b, hipStream_t s) {
+  if (g.ntiles == 0) return;
+  if (g.tile == 128)
+    launch_gemm_t<128>(g, b, s);
+  else
+    launch_gemm_t<64>(g, b, s);
 }
 
 // ------------------------------------------------ diagonal block Cholesky
-// One wave per 64x64 diagonal block, kept in LDS.  Left-looking: lane i
-// computes L[i][j] = (A[i][j] - sum_k L[i][k] L[j][k]) / L[j][j] for column j;
-// then lane c solves L w = e_c for column c of W = L^-1, which is written to
-// the node's Z block.  Blocks narrower than 64 are padded with the identity.
-// A non-positive pivot is recorded in *status as (global elimination column
-// + 1) and replaced by 1 so the sweep completes.
-__global__ __launch_bounds__(64) void potrf_diag_kernel(const PotrfDesc* __restrict__ descs,
-                                                        double* __restrict__ F, double* __restrict__ Z,
-                                                        int* __restrict__ status) {
-  __shared__ double L[64][65];
-  __shared__ double W[64][65];
+// One 256-thread block per 64x64 diagonal block.  Thread (r, g) keeps row r,
+// columns 16g..16g+15 in registers.  Right-looking: at step j the owners of
+// column j publish it to LDS and every thread applies the rank-1 update to
+// its own entries (two barriers per column).  Then W = L^-1 by right-looking
+// forward substitution (row k published per step), written to the node's Z
+// block.  Blocks narrower than 64 are padded with the identity.  A
+// non-positive pivot is recorded in *status as (global elimination column +
+// 1) and replaced by 1 so the sweep completes.
+__global__ __launch_bounds__(256) void potrf_diag_kernel(const PotrfDesc* __restrict__ descs,
+                                                         double* __restrict__ F, double* __restrict__ Z,
+                                                         int* __restrict__ status) {
+  __shared__ double col[64];
+  __shared__ double dg[64];
+  __shared__ double Ls[64][65];
+  __shared__ double wrow[64];
   const PotrfDesc d = descs[blockIdx.x];
-  const int i = threadIdx.x;
+  const int tid = threadIdx.x, r = tid & 63, cg = tid >> 6;
   const int kb = d.kb;
   double* Fb = F + d.foff + d.k0 + (int64_t)d.k0 * d.ld;
-#pragma unroll 1
-  for (int c = 0; c < 64; ++c) {
+  double a[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int c = 16 * cg + q;
     double v;
-    if (i < kb && c < kb)
-      v = (c <= i) ? Fb[i + (int64_t)c * d.ld] : 0.0;
+    if (r < kb && c < kb)
+      v = (c <= r) ? Fb[r + (int64_t)c * d.ld] : 0.0;
     else
-      v = (i == c) ? 1.0 : 0.0;
-    L[i][c] = v;
+      v = (r == c) ? 1.0 : 0.0;
+    a[q] = v;
+  }
+#pragma unroll 1
+  for (int g = 0; g < 4; ++g) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int j = 16 * g + q;
+      if (cg == g) col[r] = a[q];
+      __syncthreads();
+      double piv = col[j];
+      if (!(piv > 0.0)) {
+        if (tid == 0 && j < kb) atomicCAS(status, 0, d.colbase + j + 1);
+        piv = 1.0;
+      }
+      const double dj = sqrt(piv), idj = 1.0 / dj;
+      const double lr = col[r] * idj;
+#pragma unroll
+      for (int q2 = 0; q2 < 16; ++q2) {
+        const int c = 16 * cg + q2;
+        if (c > j && c <= r) a[q2] -= lr * (col[c] * idj);
+      }
+      if (tid == 0) dg[j] = dj;
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int c = 16 * cg + q;
+    const double v = (r > c) ? a[q] / dg[c] : (r == c ? dg[c] : 0.0);
+    Ls[r][c] = v;
+    if (r < kb && c < kb && c <= r) Fb[r + (int64_t)c * d.ld] = v;
   }
   __syncthreads();
+  double w[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) w[q] = (r == 16 * cg + q) ? 1.0 : 0.0;
 #pragma unroll 1
-  for (int j = 0; j < 64; ++j) {
-    double s = L[i][j];
-#pragma unroll 4
-    for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
-    double piv = __shfl(s, j, 64);
-    if (!(piv > 0.0)) {
-      if (i == j && j < kb) atomicCAS(status, 0, d.colbase + j + 1);
-      piv = 1.0;
+  for (int k = 0; k < 64; ++k) {
+    if (r == k) {
+      const double inv = 1.0 / Ls[k][k];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        w[q] *= inv;
+        wrow[16 * cg + q] = w[q];
+      }
     }
-    const double dj = sqrt(piv);
-    L[i][j] = (i == j) ? dj : (i > j ? s / dj : 0.0);
+    __syncthreads();
+    if (r > k) {
+      const double l = Ls[r][k];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) w[q] -= l * wrow[16 * cg + q];
+    }
     __syncthreads();
   }
-#pragma unroll 1
-  for (int r = 0; r < 64; ++r) {
-    double s = (r == i) ? 1.0 : 0.0;
-#pragma unroll 4
-    for (int k = 0; k < r; ++k) s -= L[r][k] * W[k][i];
-    W[r][i] = s / L[r][r];
-  }
-  __syncthreads();
-  if (i < kb) {
+  if (r < kb) {
     double* Zb = Z + d.zoff + d.k0 + (int64_t)d.k0 * d.zld;
-#pragma unroll 1
-    for (int c = 0; c < kb; ++c) {
-      if (c <= i) Fb[i + (int64_t)c * d.ld] = L[i][c];
-      Zb[i + (int64_t)c * d.zld] = W[i][c];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = 16 * cg + q;
+      if (c < kb) Zb[r + (int64_t)c * d.zld] = w[q];
     }
   }
 }
@@ -232,7 +285,7 @@ __global__ __launch_bounds__(64) void mfma_probe_kernel(int which, double* __res
 }
 
 void launch_potrf(const PotrfDesc* d, int n, double* F, double* Z, int* status, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(potrf_diag_kernel, dim3(n), dim3(64), 0, s, d, F, Z, status);
+  if (n > 0) hipLaunchKernelGGL(potrf_diag_kernel, dim3(n), dim3(256), 0, s, d, F, Z, status);
 }
 void launch_panel(const PanelDesc* d, int n, double* F, const double* Z, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(panel_kernel, dim3(n), dim3(256), 0, s, d, F, Z);
@@ -259,8 +312,9 @@ extern "C" int spde_debug_mfma_probe(int which, double* out, void* stream) {
 }
 
 extern "C" int spde_debug_gemm(int ta, int tb, int M, int N, int K, double alpha, const double* A, int lda,
-                               const double* B, int ldb, double beta, double* C, int ldc, void* stream) {
-  if (M <= 0 || N <= 0 || K < 0 || !A || !B || !C || lda < (ta ? K : M) || ldb < (tb ? N : K) || ldc < M) {
+                               const double* B, int ldb, double beta, double* C, int ldc, int tile, void* stream) {
+  if (M <= 0 || N <= 0 || K < 0 || !A || !B || !C || lda < (ta ? K : M) || ldb < (tb ? N : K) || ldc < M ||
+      (tile != 64 && tile != 128)) {
     set_error("spde_debug_gemm: invalid argument");
     return SPDE_ERR_ARG;
   }
@@ -279,9 +333,10 @@ extern "C" int spde_debug_gemm(int ta, int tb, int M, int N, int K, double alpha
   P.alpha = alpha;
   P.beta = beta;
   std::vector<GemmTile> tiles;
-  for (int a = 0; a < (M + 63) / 64; ++a)
-    for (int b = 0; b < (N + 63) / 64; ++b) tiles.push_back({0, a, b, 0});
+  for (int a = 0; a < (M + tile - 1) / tile; ++a)
+    for (int b = 0; b < (N + tile - 1) / tile; ++b) tiles.push_back({0, a, b, 0});
   GemmLaunch g;
+  g.tile = tile;
   g.ta = ta != 0;
   g.tb = tb != 0;
   g.ntiles = (int)tiles.size();

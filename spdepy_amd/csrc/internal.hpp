@@ -45,6 +45,7 @@ struct GemmTile {
 // A batched GEMM launch: problems + tile list (device), transposition flags.
 struct GemmLaunch {
   bool ta = false, tb = false;
+  int tile = 64;                 // block tile edge (64 or 128), chosen when finalised
   int ntiles = 0, nprobs = 0;
   GemmProb* d_probs = nullptr;
   GemmTile* d_tiles = nullptr;
@@ -146,4 +147,10 @@ struct spde_plan {
   int trtri_heights = 0;
   int solve_nrhs = -1;            // nrhs the solve launches are built for
   bool selinv_built = false;
+  // optional event timing of the GEMM launches (measurement only)
+  bool prof_on = false;
+  std::vector<hipEvent_t> prof_ev;
+  std::vector<double> prof_flops;
+  size_t prof_n = 0;
+  hipStream_t prof_stream = nullptr;
 };

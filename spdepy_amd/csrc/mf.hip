@@ -141,28 +141,57 @@ __global__ __launch_bounds__(256) void bwd_gather_kernel(const int32_t* __restri
 
 // --------------------------------------------------------- selected inverse
 // Sigma22 of a node = symmetric gather from the lower triangle of the
-// parent's Sigma front.  One block per (node, 64-column tile).
+// parent's Sigma front.  One block per 64x64 tile (node, ti, tj) of the b x b
+// block; entries below the parent's diagonal are read with consecutive
+// threads on consecutive rows, mirrored ones with consecutive threads on
+// consecutive columns, both through an LDS tile so every global access is
+// coalesced.
 __global__ __launch_bounds__(256) void sigma_gather_kernel(const int32_t* __restrict__ lvl_nodes, const NodeDev* __restrict__ nodes,
                                                            const int32_t* __restrict__ relpos, double* __restrict__ S,
-                                                           const int32_t* __restrict__ tile_node, const int32_t* __restrict__ tile_col) {
-  const int node = lvl_nodes[tile_node[blockIdx.x]];
+                                                           const int4* __restrict__ tiles) {
+  __shared__ double tile[64][65];
+  const int4 tt = tiles[blockIdx.x];
+  const int node = lvl_nodes[tt.x];
   const NodeDev nd = nodes[node];
   const NodeDev pa = nodes[nd.parent];
   const int32_t* rp = relpos + nd.relpos_off;
   const int b = nd.nb;
-  const int j0 = tile_col[blockIdx.x] * 64;
-  double* dst = S + nd.foff + nd.p + (int64_t)nd.p * nd.n;
+  const int i0 = tt.y * 64, j0 = tt.z * 64;
   const double* src = S + pa.foff;
-  for (int jj = 0; jj < 64; ++jj) {
-    const int j = j0 + jj;
-    if (j >= b) break;
-    const int pj = rp[j];
-    for (int i = threadIdx.x; i < b; i += 256) {
-      const int pi = rp[i];
-      const int hi = pi > pj ? pi : pj, lo = pi > pj ? pj : pi;
-      dst[i + (int64_t)j * nd.n] = src[hi + (int64_t)lo * pa.n];
+  const int64_t pn = pa.n;
+  const int tid = threadIdx.x, u = tid & 63, w0 = tid >> 6;
+  // rows fastest: entries with rp[i] >= rp[j]
+  {
+    const int i = i0 + u;
+    const int pi = (i < b) ? rp[i] : -1;
+    for (int jj = w0; jj < 64; jj += 4) {
+      const int j = j0 + jj;
+      if (i < b && j < b) {
+        const int pj = rp[j];
+        if (pi >= pj) tile[u][jj] = src[pi + (int64_t)pj * pn];
+      }
     }
   }
+  // columns fastest: entries with rp[i] < rp[j]
+  {
+    const int j = j0 + u;
+    const int pj = (j < b) ? rp[j] : -1;
+    for (int ii = w0; ii < 64; ii += 4) {
+      const int i = i0 + ii;
+      if (i < b && j < b) {
+        const int pi = rp[i];
+        if (pi < pj) tile[ii][u] = src[pj + (int64_t)pi * pn];
+      }
+    }
+  }
+  __syncthreads();
+  double* dst = S + nd.foff + nd.p + (int64_t)nd.p * nd.n;
+  const int i = i0 + u;
+  if (i < b)
+    for (int jj = w0; jj < 64; jj += 4) {
+      const int j = j0 + jj;
+      if (j < b) dst[i + (int64_t)j * nd.n] = tile[u][jj];
+    }
 }
 
 // out[e] = S[mapsym[e]]
@@ -213,10 +242,9 @@ void launch_bwd_gather(const int32_t* lvl_nodes, int nn, const NodeDev* nodes, c
     hipLaunchKernelGGL(bwd_gather_kernel, dim3(nn), dim3(256), 0, s, lvl_nodes, nodes, bnd, xy, n, nrhs, vec);
 }
 void launch_sigma_gather(const int32_t* lvl_nodes, const NodeDev* nodes, const int32_t* relpos, double* S,
-                         const int32_t* tile_node, const int32_t* tile_col, int ntiles, hipStream_t s) {
+                         const int4* tiles, int ntiles, hipStream_t s) {
   if (ntiles > 0)
-    hipLaunchKernelGGL(sigma_gather_kernel, dim3(ntiles), dim3(256), 0, s, lvl_nodes, nodes, relpos, S, tile_node,
-                       tile_col);
+    hipLaunchKernelGGL(sigma_gather_kernel, dim3(ntiles), dim3(256), 0, s, lvl_nodes, nodes, relpos, S, tiles);
 }
 void launch_gather_pattern(const double* S, const int64_t* mapsym, int64_t nnz, double* out, hipStream_t s) {
   hipLaunchKernelGGL(gather_pattern_kernel, dim3(grid_for(nnz)), dim3(256), 0, s, S, mapsym, nnz, out);

@@ -31,7 +31,7 @@ void launch_fwd_gather(const int32_t* lvl_nodes, int nn, const NodeDev* nodes, c
 void launch_bwd_gather(const int32_t* lvl_nodes, int nn, const NodeDev* nodes, const int32_t* bnd, const double* xy,
                        int64_t n, int nrhs, double* vec, hipStream_t s);
 void launch_sigma_gather(const int32_t* lvl_nodes, const NodeDev* nodes, const int32_t* relpos, double* S,
-                         const int32_t* tile_node, const int32_t* tile_col, int ntiles, hipStream_t s);
+                         const int4* tiles, int ntiles, hipStream_t s);
 void launch_gather_pattern(const double* S, const int64_t* mapsym, int64_t nnz, double* out, hipStream_t s);
 
 static thread_local std::string g_err;
@@ -41,8 +41,7 @@ void set_error(const std::string& m) { g_err = m; }
 struct LevelDev {
   int32_t* d_nodes = nullptr;
   int nn = 0;
-  int32_t* d_gtile_node = nullptr;
-  int32_t* d_gtile_col = nullptr;
+  int4* d_gtiles = nullptr;
   int ngtiles = 0;
 };
 
@@ -119,7 +118,10 @@ ArenaSizes arena_sizes(const spde_plan* pl, int nrhs) {
   return a;
 }
 
+void make_tiles(GemmLaunch& g, int force_tile);
+
 int finalize(GemmLaunch& g) {
+  make_tiles(g, 0);
   g.nprobs = (int)g.h_probs.size();
   g.ntiles = (int)g.h_tiles.size();
   int rc = upload(&g.d_probs, g.h_probs);
@@ -130,13 +132,28 @@ int finalize(GemmLaunch& g) {
 void add_prob(GemmLaunch& g, const GemmProb& P, bool lower) {
   if (P.M <= 0 || P.N <= 0) return;
   if (P.K <= 0 && P.beta == 1.0) return;
-  const int idx = (int)g.h_probs.size();
-  g.h_probs.push_back(P);
-  const int tm = (P.M + 63) / 64, tn = (P.N + 63) / 64;
-  for (int a = 0; a < tm; ++a)
-    for (int b = 0; b < tn; ++b)
-      if (!lower || a >= b) g.h_tiles.push_back({idx, a, b, 0});
+  GemmProb Q = P;
+  Q.lower = lower ? 1 : 0;
+  g.h_probs.push_back(Q);
   g.flops += 2.0 * P.M * (double)P.N * P.K * (lower ? 0.5 : 1.0);
+}
+
+// Tile edge of a launch: 128 when that still gives the chip >= 2 blocks per
+// CU, else 64.  Then the tile list (tiles strictly above the diagonal of a
+// lower problem are skipped).
+void make_tiles(GemmLaunch& g, int force_tile) {
+  int64_t big = 0;
+  for (const GemmProb& P : g.h_probs) big += (int64_t)((P.M + 127) / 128) * ((P.N + 127) / 128);
+  g.tile = force_tile ? force_tile : (big >= 512 ? 128 : 64);
+  g.h_tiles.clear();
+  const int T = g.tile;
+  for (size_t i = 0; i < g.h_probs.size(); ++i) {
+    const GemmProb& P = g.h_probs[i];
+    const int tm = (P.M + T - 1) / T, tn = (P.N + T - 1) / T;
+    for (int a = 0; a < tm; ++a)
+      for (int b = 0; b < tn; ++b)
+        if (!P.lower || a >= b) g.h_tiles.push_back({(int32_t)i, a, b, 0});
+  }
 }
 
 GemmProb prob(int8_t ab, int64_t ao, int lda, int8_t bb, int64_t bo, int ldb, int8_t cb, int64_t co, int ldc, int M,
@@ -354,15 +371,13 @@ int build_selinv_plan(spde_plan* pl) {
     L.si_s11a.ta = true, L.si_s11a.tb = false;
     L.si_s11b.ta = true, L.si_s11b.tb = false;
     int64_t woff = 0;
-    std::vector<int32_t> tn, tc;
+    std::vector<int4> gt;
     for (size_t q = 0; q < L.nodes.size(); ++q) {
       const Node& nd = sym.nodes[L.nodes[q]];
       const int nf = nd.n(), p = nd.p, b = (int)nd.B.size();
       if (b > 0 && nd.parent >= 0)
-        for (int c = 0; c < (b + 63) / 64; ++c) {
-          tn.push_back((int32_t)q);
-          tc.push_back(c);
-        }
+        for (int ti = 0; ti < (b + 63) / 64; ++ti)
+          for (int tj = 0; tj < (b + 63) / 64; ++tj) gt.push_back(make_int4((int)q, ti, tj, 0));
       add_prob(L.si_x, prob(kFront, nd.foff + p, nf, kZ, nd.zoff, p, kWork, woff, b, b, p, p, 1.0, 0.0), false);
       add_prob(L.si_s21, prob(kSigma, nd.foff + p + (int64_t)p * nf, nf, kWork, woff, b, kSigma, nd.foff + p, nf, b, p,
                               b, -1.0, 0.0),
@@ -384,10 +399,8 @@ int build_selinv_plan(spde_plan* pl) {
       if (rc) return rc;
     }
     LevelDev& D = ex->lvl[l];
-    D.ngtiles = (int)tn.size();
-    int rc = upload(&D.d_gtile_node, tn);
-    if (rc) return rc;
-    rc = upload(&D.d_gtile_col, tc);
+    D.ngtiles = (int)gt.size();
+    int rc = upload(&D.d_gtiles, gt);
     if (rc) return rc;
   }
   pl->selinv_built = true;
@@ -399,6 +412,27 @@ void free_launch(GemmLaunch& g) {
   hipFree(g.d_tiles);
   g.d_probs = nullptr;
   g.d_tiles = nullptr;
+}
+
+// GEMM launch, bracketed by HIP events when the plan's timing is enabled.
+void run_gemm(spde_plan* pl, const GemmLaunch& g, const Bases& b, hipStream_t s) {
+  if (g.ntiles == 0) return;
+  if (pl->prof_on) {
+    while (pl->prof_ev.size() < 2 * (pl->prof_n + 1)) {
+      hipEvent_t e;
+      hipEventCreate(&e);
+      pl->prof_ev.push_back(e);
+    }
+    hipEventRecord(pl->prof_ev[2 * pl->prof_n], s);
+    launch_gemm(g, b, s);
+    hipEventRecord(pl->prof_ev[2 * pl->prof_n + 1], s);
+    if (pl->prof_flops.size() <= pl->prof_n) pl->prof_flops.resize(pl->prof_n + 1);
+    pl->prof_flops[pl->prof_n] = g.flops;
+    pl->prof_n++;
+    pl->prof_stream = s;
+    return;
+  }
+  launch_gemm(g, b, s);
 }
 
 Bases make_bases(spde_plan* pl, int slot, double* x) {
@@ -566,13 +600,13 @@ void spde_plan_destroy(spde_plan* pl) {
         free_launch(*g);
     }
     for (auto& g : pl->trtri) free_launch(g);
+    for (auto e : pl->prof_ev) hipEventDestroy(e);
     for (size_t i = 0; i < g_extra.size(); ++i)
       if (g_extra[i].first == pl) {
         PlanExtra* ex = g_extra[i].second;
         for (auto& d : ex->lvl) {
           hipFree(d.d_nodes);
-          hipFree(d.d_gtile_node);
-          hipFree(d.d_gtile_col);
+          hipFree(d.d_gtiles);
         }
         hipFree(ex->d_children);
         delete ex;
@@ -660,11 +694,11 @@ int spde_factor(spde_plan* pl, int slot, const double* qval, const double* diag_
     for (Step& st : L.steps) {
       launch_potrf(st.d_potrf, st.npotrf, sl.front, sl.z, pl->d_status, s);
       launch_panel(st.d_panel, st.npanel, sl.front, sl.z, s);
-      launch_gemm(st.update, b, s);
+      run_gemm(pl, st.update, b, s);
     }
-    launch_gemm(L.syrk, b, s);
+    run_gemm(pl, L.syrk, b, s);
   }
-  for (auto& g : pl->trtri) launch_gemm(g, b, s);
+  for (auto& g : pl->trtri) run_gemm(pl, g, b, s);
   launch_logdet(pl->d_nodes, (int)sym.nodes.size(), sl.front, pl->d_partial, pl->d_scalar, s);
   int status = 0;
   double ld = 0.0;
@@ -728,16 +762,16 @@ int spde_solve(spde_plan* pl, int slot, int mode, double* B, int nrhs, int64_t l
       LevelPlan& L = pl->levels[l];
       launch_fwd_gather(ex->lvl[l].d_nodes, ex->lvl[l].nn, pl->d_nodes, ex->d_children, pl->d_relpos, pl->d_xy, n,
                         nrhs, pl->d_vec, s);
-      launch_gemm(L.fwd_y, b, s);
-      launch_gemm(L.fwd_u, b, s);
+      run_gemm(pl, L.fwd_y, b, s);
+      run_gemm(pl, L.fwd_u, b, s);
     }
   }
   if (mode != SPDE_SOLVE_L) {
     for (int l = nl - 1; l >= 0; --l) {
       LevelPlan& L = pl->levels[l];
       launch_bwd_gather(ex->lvl[l].d_nodes, ex->lvl[l].nn, pl->d_nodes, pl->d_bnd, pl->d_xy, n, nrhs, pl->d_vec, s);
-      launch_gemm(L.bwd_t, b, s);
-      launch_gemm(L.bwd_x, b, s);
+      run_gemm(pl, L.bwd_t, b, s);
+      run_gemm(pl, L.bwd_x, b, s);
     }
   }
   if (mode == SPDE_SOLVE_L) {
@@ -746,6 +780,37 @@ int spde_solve(spde_plan* pl, int slot, int mode, double* B, int nrhs, int64_t l
   } else {
     launch_permute_out(pl->d_xy, pl->d_perm, n, nrhs, B, ldb, s);
   }
+  return SPDE_OK;
+}
+
+int spde_prof_begin(spde_plan* pl) {
+  if (!pl || pl->host_only) {
+    set_error("spde_prof_begin: no device plan");
+    return SPDE_ERR_ARG;
+  }
+  pl->prof_on = true;
+  pl->prof_n = 0;
+  return SPDE_OK;
+}
+
+int spde_prof_end(spde_plan* pl, double* gemm_ms, double* gemm_flops, int64_t* launches) {
+  if (!pl || pl->host_only) {
+    set_error("spde_prof_end: no device plan");
+    return SPDE_ERR_ARG;
+  }
+  pl->prof_on = false;
+  double ms = 0.0, fl = 0.0;
+  if (pl->prof_n) SPDE_HIP(hipStreamSynchronize(pl->prof_stream));
+  for (size_t i = 0; i < pl->prof_n; ++i) {
+    float t = 0.0f;
+    SPDE_HIP(hipEventElapsedTime(&t, pl->prof_ev[2 * i], pl->prof_ev[2 * i + 1]));
+    ms += t;
+    fl += pl->prof_flops[i];
+  }
+  if (gemm_ms) *gemm_ms = ms;
+  if (gemm_flops) *gemm_flops = fl;
+  if (launches) *launches = (int64_t)pl->prof_n;
+  pl->prof_n = 0;
   return SPDE_OK;
 }
 
@@ -768,12 +833,11 @@ int spde_selinv(spde_plan* pl, int slot, double* sigma, void* stream) {
   for (int l = (int)pl->levels.size() - 1; l >= 0; --l) {
     LevelPlan& L = pl->levels[l];
     LevelDev& D = ex->lvl[l];
-    launch_sigma_gather(D.d_nodes, pl->d_nodes, pl->d_relpos, pl->d_sigma, D.d_gtile_node, D.d_gtile_col, D.ngtiles,
-                        s);
-    launch_gemm(L.si_x, b, s);
-    launch_gemm(L.si_s21, b, s);
-    launch_gemm(L.si_s11a, b, s);
-    launch_gemm(L.si_s11b, b, s);
+    launch_sigma_gather(D.d_nodes, pl->d_nodes, pl->d_relpos, pl->d_sigma, D.d_gtiles, D.ngtiles, s);
+    run_gemm(pl, L.si_x, b, s);
+    run_gemm(pl, L.si_s21, b, s);
+    run_gemm(pl, L.si_s11a, b, s);
+    run_gemm(pl, L.si_s11b, b, s);
   }
   if (sigma) launch_gather_pattern(pl->d_sigma, pl->d_mapsym, (int64_t)pl->st.indices.size(), sigma, s);
   return SPDE_OK;

@@ -30,9 +30,10 @@ def test_mfma_f64_lane_layout():
     assert np.array_equal(res[1], (lane // 16 + 4 * v + 1).astype(float))
 
 
+@pytest.mark.parametrize("tile", [64, 128])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-@pytest.mark.parametrize("M,N,K", [(64, 64, 16), (100, 37, 70), (5, 130, 3), (129, 65, 200)])
-def test_gemm_matches_numpy(ta, tb, M, N, K):
+@pytest.mark.parametrize("M,N,K", [(64, 64, 16), (100, 37, 70), (5, 130, 3), (129, 65, 200), (300, 260, 33)])
+def test_gemm_matches_numpy(ta, tb, M, N, K, tile):
     import torch
     nat = _lib()
     rng = np.random.default_rng(M * 7 + N * 3 + K)
@@ -43,8 +44,8 @@ def test_gemm_matches_numpy(ta, tb, M, N, K):
     At = torch.as_tensor(np.asfortranarray(A).ravel(order="F")).cuda()
     Bt = torch.as_tensor(np.asfortranarray(B).ravel(order="F")).cuda()
     Ct = torch.as_tensor(np.asfortranarray(C).ravel(order="F")).cuda()
-    step("gemm %d%d %dx%dx%d" % (ta, tb, M, N, K))
+    step("gemm %d%d %dx%dx%d tile %d" % (ta, tb, M, N, K, tile))
     nat.check(nat.lib().spde_debug_gemm(ta, tb, M, N, K, -1.5, At.data_ptr(), A.shape[0], Bt.data_ptr(),
-                                        B.shape[0], 0.5, Ct.data_ptr(), M, None))
+                                        B.shape[0], 0.5, Ct.data_ptr(), M, tile, None))
     got = Ct.cpu().numpy().reshape(N, M).T
     assert np.allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
