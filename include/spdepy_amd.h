@@ -175,6 +175,10 @@ int spde_grad_cells(spde_plan* plan, const spde_grad_args* args, double* out, vo
  * summed kernel time (ms), the summed algorithmic flops and the launch count. */
 int spde_prof_begin(spde_plan* plan);
 int spde_prof_end(spde_plan* plan, double* gemm_ms, double* gemm_flops, int64_t* launches);
+/* Per-launch view of the timed launches so far (host arrays of length cap):
+ * kernel ms, algorithmic flops, tile count (negative for 128-wide tiles). */
+int spde_prof_dump(spde_plan* plan, double* ms, double* flops, int32_t* tiles, int64_t cap,
+                   int64_t* count);
 
 /* ---------------------------------------------------------------- diagnostics
  * No counterpart in the reference: self-checks of the dense kernels.

@@ -25,11 +25,19 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 template <bool TA, bool TB, int BM, int BN>
 __global__ __launch_bounds__(256) void gemm_kernel(const GemmProb* __restrict__ probs,
                                                    const GemmTile* __restrict__ tiles, Bases bases) {
-  constexpr int BK = 16, PAD = 4;
+  constexpr int BK = 16;
+  // LDS layout follows each operand's memory order so that both the global
+  // loads and the LDS stores are contiguous: an operand stored with m (n)
+  // contiguous goes to a k-major tile (pitch BM+16: the four lane groups of a
+  // fragment read land 32 banks apart), a k-contiguous one to an m-major
+  // (n-major) tile with pitch BK+1.
+  constexpr bool AK = !TA, BKM = TB;  // k-major tiles
+  constexpr int PA = AK ? BM + 16 : BK + 1, PB = BKM ? BN + 16 : BK + 1;
+  constexpr int SA = AK ? BK * PA : BM * PA, SB = BKM ? BK * PB : BN * PB;
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
-  __shared__ double As[2][BK][BM + PAD];
-  __shared__ double Bs[2][BK][BN + PAD];
+  __shared__ double As[2][SA];
+  __shared__ double Bs[2][SB];
   const GemmTile t = tiles[blockIdx.x];
   const GemmProb P = probs[t.prob];
   const double* __restrict__ A = bases.p[P.a_base] + P.a_off;
@@ -40,41 +48,41 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmProb* __restrict__ 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
   const int lr = lane & 15, lk = lane >> 4;
+  const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
+  const int am = TA ? tid / BK : tid % BM, ak = TA ? tid % BK : tid / BM;
+  const int bn = TB ? tid % BN : tid / BK, bk = TB ? tid / BN : tid % BK;
+  constexpr int ASTEP_M = TA ? 256 / BK : 0, ASTEP_K = TA ? 0 : 256 / BM;
+  constexpr int BSTEP_N = TB ? 0 : 256 / BK, BSTEP_K = TB ? 256 / BN : 0;
+  auto aidx = [](int k, int m) { return AK ? k * PA + m : m * PA + k; };
+  auto bidx = [](int k, int n) { return BKM ? k * PB + n : n * PB + k; };
   d4 acc[TN][TM];
 #pragma unroll
-  for (int a = 0; a < TN; ++a)
+  for (int x = 0; x < TN; ++x)
 #pragma unroll
-    for (int b = 0; b < TM; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int y = 0; y < TM; ++y) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
   double ra[LA], rb[LB];
   auto load = [&](int k0) {
+    const bool full = interior && (k0 + BK <= K);
 #pragma unroll
     for (int q = 0; q < LA; ++q) {
-      const int idx = tid + q * 256;
-      const int mm = TA ? idx / BK : idx % BM, kk = TA ? idx % BK : idx / BM;
+      const int mm = am + q * ASTEP_M, kk = ak + q * ASTEP_K;
       const int gm = m0 + mm, gk = k0 + kk;
-      ra[q] = (gm < M && gk < K) ? (TA ? A[gk + (int64_t)gm * P.lda] : A[gm + (int64_t)gk * P.lda]) : 0.0;
+      const int64_t off = TA ? gk + (int64_t)gm * P.lda : gm + (int64_t)gk * P.lda;
+      ra[q] = (full || (gm < M && gk < K)) ? A[off] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < LB; ++q) {
-      const int idx = tid + q * 256;
-      const int nn = TB ? idx % BN : idx / BK, kk = TB ? idx / BN : idx % BK;
+      const int nn = bn + q * BSTEP_N, kk = bk + q * BSTEP_K;
       const int gn = n0 + nn, gk = k0 + kk;
-      rb[q] = (gn < N && gk < K) ? (TB ? B[gn + (int64_t)gk * P.ldb] : B[gk + (int64_t)gn * P.ldb]) : 0.0;
+      const int64_t off = TB ? gn + (int64_t)gk * P.ldb : gk + (int64_t)gn * P.ldb;
+      rb[q] = (full || (gn < N && gk < K)) ? B[off] : 0.0;
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int q = 0; q < LA; ++q) {
-      const int idx = tid + q * 256;
-      const int mm = TA ? idx / BK : idx % BM, kk = TA ? idx % BK : idx / BM;
-      As[buf][kk][mm] = ra[q];
-    }
+    for (int q = 0; q < LA; ++q) As[buf][aidx(ak + q * ASTEP_K, am + q * ASTEP_M)] = ra[q];
 #pragma unroll
-    for (int q = 0; q < LB; ++q) {
-      const int idx = tid + q * 256;
-      const int nn = TB ? idx % BN : idx / BK, kk = TB ? idx / BN : idx % BK;
-      Bs[buf][kk][nn] = rb[q];
-    }
+    for (int q = 0; q < LB; ++q) Bs[buf][bidx(bk + q * BSTEP_K, bn + q * BSTEP_N)] = rb[q];
   };
   const int nk = (K + BK - 1) / BK;
   if (nk > 0) {
@@ -85,18 +93,25 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmProb* __restrict__ 
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) load((kt + 1) * BK);
+    double fa[2][TM], fb[2][TN];
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      double a[TM], b[TN];
+    for (int mi = 0; mi < TM; ++mi) fa[0][mi] = As[cur][aidx(lk, wm + mi * 16 + lr)];
 #pragma unroll
-      for (int mi = 0; mi < TM; ++mi) a[mi] = As[cur][kk + lk][wm + mi * 16 + lr];
+    for (int ni = 0; ni < TN; ++ni) fb[0][ni] = Bs[cur][bidx(lk, wn + ni * 16 + lr)];
 #pragma unroll
-      for (int ni = 0; ni < TN; ++ni) b[ni] = Bs[cur][kk + lk][wn + ni * 16 + lr];
+    for (int s = 0; s < 4; ++s) {
+      const int f = s & 1;
+      if (s < 3) {
+#pragma unroll
+        for (int mi = 0; mi < TM; ++mi) fa[f ^ 1][mi] = As[cur][aidx(4 * (s + 1) + lk, wm + mi * 16 + lr)];
+#pragma unroll
+        for (int ni = 0; ni < TN; ++ni) fb[f ^ 1][ni] = Bs[cur][bidx(4 * (s + 1) + lk, wn + ni * 16 + lr)];
+      }
 #pragma unroll
       for (int ni = 0; ni < TN; ++ni)
 #pragma unroll
         for (int mi = 0; mi < TM; ++mi)
-          acc[ni][mi] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[ni][mi], 0, 0, 0);
+          acc[ni][mi] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[f][ni], fa[f][mi], acc[ni][mi], 0, 0, 0);
     }
     if (kt + 1 < nk) store(cur ^ 1);
     __syncthreads();
@@ -110,7 +125,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmProb* __restrict__ 
       for (int v = 0; v < 4; ++v) {
         const int m = m0 + wm + mi * 16 + lr;
         const int n = n0 + wn + ni * 16 + lk + 4 * v;
-        if (m < M && n < N) {
+        if (interior || (m < M && n < N)) {
           double* c = C + m + (int64_t)n * P.ldc;
           const double r = alpha * acc[ni][mi][v];
           *c = (beta == 0.0) ? r : r + beta * (*c);
@@ -140,21 +155,61 @@ void launch_gemm(const GemmLaunch& g, const Bases& b, hipStream_t s) {
 }
 
 // ------------------------------------------------ diagonal block Cholesky
-// One 256-thread block per 64x64 diagonal block.  Thread (r, g) keeps row r,
-// columns 16g..16g+15 in registers.  Right-looking: at step j the owners of
-// column j publish it to LDS and every thread applies the rank-1 update to
-// its own entries (two barriers per column).  Then W = L^-1 by right-looking
-// forward substitution (row k published per step), written to the node's Z
-// block.  Blocks narrower than 64 are padded with the identity.  A
-// non-positive pivot is recorded in *status as (global elimination column +
-// 1) and replaced by 1 so the sweep completes.
+// One 256-thread block per 64x64 diagonal block; thread (r, g) keeps row r,
+// columns 16g..16g+15 in registers.  Right-looking in steps of four columns:
+// the owners publish the four raw columns to LDS (double buffered, one
+// barrier per step), every thread forms the 4x4 pivot block's Cholesky
+// factor itself, finishes its row's four entries and applies the rank-4
+// update to its own columns.  W = L^-1 is then built the same way four rows
+// per step and written to the node's Z block.  Blocks narrower than 64 are
+// padded with the identity.  A non-positive pivot is recorded in *status as
+// (global elimination column + 1) and replaced by 1 so the sweep completes.
+struct Chol4 {
+  double i0, i1, i2, i3, l10, l20, l30, l21, l31, l32, d0, d1, d2, d3;
+};
+
+__device__ inline double safe_sqrt(double x, bool& bad) {
+  if (!(x > 0.0)) {
+    bad = true;
+    return 1.0;
+  }
+  return sqrt(x);
+}
+
+__device__ inline void chol4(const double (*D)[5], Chol4& c, int& badcol) {
+  bool b0 = false, b1 = false, b2 = false, b3 = false;
+  c.d0 = safe_sqrt(D[0][0], b0);
+  c.i0 = 1.0 / c.d0;
+  c.l10 = D[1][0] * c.i0;
+  c.l20 = D[2][0] * c.i0;
+  c.l30 = D[3][0] * c.i0;
+  c.d1 = safe_sqrt(D[1][1] - c.l10 * c.l10, b1);
+  c.i1 = 1.0 / c.d1;
+  c.l21 = (D[2][1] - c.l20 * c.l10) * c.i1;
+  c.l31 = (D[3][1] - c.l30 * c.l10) * c.i1;
+  c.d2 = safe_sqrt(D[2][2] - c.l20 * c.l20 - c.l21 * c.l21, b2);
+  c.i2 = 1.0 / c.d2;
+  c.l32 = (D[3][2] - c.l30 * c.l20 - c.l31 * c.l21) * c.i2;
+  c.d3 = safe_sqrt(D[3][3] - c.l30 * c.l30 - c.l31 * c.l31 - c.l32 * c.l32, b3);
+  c.i3 = 1.0 / c.d3;
+  badcol = b0 ? 0 : (b1 ? 1 : (b2 ? 2 : (b3 ? 3 : -1)));
+}
+
+// y = x L_d^-T for a row x of raw values (columns j0..j0+3)
+__device__ inline void rowsolve4(const Chol4& c, double x0, double x1, double x2, double x3, double& y0, double& y1,
+                                 double& y2, double& y3) {
+  y0 = x0 * c.i0;
+  y1 = (x1 - y0 * c.l10) * c.i1;
+  y2 = (x2 - y0 * c.l20 - y1 * c.l21) * c.i2;
+  y3 = (x3 - y0 * c.l30 - y1 * c.l31 - y2 * c.l32) * c.i3;
+}
+
 __global__ __launch_bounds__(256) void potrf_diag_kernel(const PotrfDesc* __restrict__ descs,
                                                          double* __restrict__ F, double* __restrict__ Z,
                                                          int* __restrict__ status) {
-  __shared__ double col[64];
-  __shared__ double dg[64];
+  __shared__ double pub[2][64][5];
   __shared__ double Ls[64][65];
-  __shared__ double wrow[64];
+  __shared__ double wpub[2][4][64];
   const PotrfDesc d = descs[blockIdx.x];
   const int tid = threadIdx.x, r = tid & 63, cg = tid >> 6;
   const int kb = d.kb;
@@ -170,57 +225,84 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(const PotrfDesc* __rest
       v = (r == c) ? 1.0 : 0.0;
     a[q] = v;
   }
-#pragma unroll 1
-  for (int g = 0; g < 4; ++g) {
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int j0 = 4 * s, og = s >> 2, q0 = 4 * (s & 3), buf = s & 1;
+    if (cg == og) {
+      pub[buf][r][0] = a[q0];
+      pub[buf][r][1] = a[q0 + 1];
+      pub[buf][r][2] = a[q0 + 2];
+      pub[buf][r][3] = a[q0 + 3];
+    }
+    __syncthreads();
+    Chol4 c;
+    int bad;
+    chol4(&pub[buf][j0], c, bad);
+    if (bad >= 0 && tid == 0 && j0 + bad < kb) atomicCAS(status, 0, d.colbase + j0 + bad + 1);
+    double y0, y1, y2, y3;
+    rowsolve4(c, pub[buf][r][0], pub[buf][r][1], pub[buf][r][2], pub[buf][r][3], y0, y1, y2, y3);
+    if (r < j0 + 4) {  // rows of the pivot block: exact factor entries, zeros above
+      const int k = r - j0;
+      y0 = (k == 0) ? c.d0 : (k > 0 ? y0 : 0.0);
+      y1 = (k == 1) ? c.d1 : (k > 1 ? y1 : 0.0);
+      y2 = (k == 2) ? c.d2 : (k > 2 ? y2 : 0.0);
+      y3 = (k == 3) ? c.d3 : 0.0;
+    }
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      const int j = 16 * g + q;
-      if (cg == g) col[r] = a[q];
-      __syncthreads();
-      double piv = col[j];
-      if (!(piv > 0.0)) {
-        if (tid == 0 && j < kb) atomicCAS(status, 0, d.colbase + j + 1);
-        piv = 1.0;
+      const int col = 16 * cg + q;
+      if (col >= j0 + 4 && col <= r) {
+        double z0, z1, z2, z3;
+        rowsolve4(c, pub[buf][col][0], pub[buf][col][1], pub[buf][col][2], pub[buf][col][3], z0, z1, z2, z3);
+        a[q] -= y0 * z0 + y1 * z1 + y2 * z2 + y3 * z3;
       }
-      const double dj = sqrt(piv), idj = 1.0 / dj;
-      const double lr = col[r] * idj;
-#pragma unroll
-      for (int q2 = 0; q2 < 16; ++q2) {
-        const int c = 16 * cg + q2;
-        if (c > j && c <= r) a[q2] -= lr * (col[c] * idj);
-      }
-      if (tid == 0) dg[j] = dj;
-      __syncthreads();
+    }
+    if (cg == og) {
+      a[q0] = y0;
+      a[q0 + 1] = y1;
+      a[q0 + 2] = y2;
+      a[q0 + 3] = y3;
     }
   }
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int c = 16 * cg + q;
-    const double v = (r > c) ? a[q] / dg[c] : (r == c ? dg[c] : 0.0);
-    Ls[r][c] = v;
-    if (r < kb && c < kb && c <= r) Fb[r + (int64_t)c * d.ld] = v;
+    Ls[r][c] = a[q];
+    if (r < kb && c < kb && c <= r) Fb[r + (int64_t)c * d.ld] = a[q];
   }
   __syncthreads();
   double w[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) w[q] = (r == 16 * cg + q) ? 1.0 : 0.0;
-#pragma unroll 1
-  for (int k = 0; k < 64; ++k) {
-    if (r == k) {
-      const double inv = 1.0 / Ls[k][k];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        w[q] *= inv;
-        wrow[16 * cg + q] = w[q];
+  for (int s = 0; s < 16; ++s) {
+    const int k0 = 4 * s, buf = s & 1;
+    if (r >= k0 && r < k0 + 4) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) wpub[buf][r - k0][16 * cg + q] = w[q];
+    }
+    __syncthreads();
+    const double e0 = 1.0 / Ls[k0][k0], e1 = 1.0 / Ls[k0 + 1][k0 + 1];
+    const double e2 = 1.0 / Ls[k0 + 2][k0 + 2], e3 = 1.0 / Ls[k0 + 3][k0 + 3];
+    const double m10 = Ls[k0 + 1][k0], m20 = Ls[k0 + 2][k0], m21 = Ls[k0 + 2][k0 + 1];
+    const double m30 = Ls[k0 + 3][k0], m31 = Ls[k0 + 3][k0 + 1], m32 = Ls[k0 + 3][k0 + 2];
+    const bool below = r >= k0 + 4;
+    const double g0 = below ? Ls[r][k0] : 0.0, g1 = below ? Ls[r][k0 + 1] : 0.0;
+    const double g2 = below ? Ls[r][k0 + 2] : 0.0, g3 = below ? Ls[r][k0 + 3] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int col = 16 * cg + q;
+      const double z0 = wpub[buf][0][col] * e0;
+      const double z1 = (wpub[buf][1][col] - m10 * z0) * e1;
+      const double z2 = (wpub[buf][2][col] - m20 * z0 - m21 * z1) * e2;
+      const double z3 = (wpub[buf][3][col] - m30 * z0 - m31 * z1 - m32 * z2) * e3;
+      if (below) {
+        w[q] -= g0 * z0 + g1 * z1 + g2 * z2 + g3 * z3;
+      } else if (r >= k0 && r < k0 + 4) {
+        const int k = r - k0;
+        w[q] = (k == 0) ? z0 : (k == 1 ? z1 : (k == 2 ? z2 : z3));
       }
     }
-    __syncthreads();
-    if (r > k) {
-      const double l = Ls[r][k];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) w[q] -= l * wrow[16 * cg + q];
-    }
-    __syncthreads();
   }
   if (r < kb) {
     double* Zb = Z + d.zoff + d.k0 + (int64_t)d.k0 * d.zld;

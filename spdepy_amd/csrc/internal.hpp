@@ -151,6 +151,7 @@ struct spde_plan {
   bool prof_on = false;
   std::vector<hipEvent_t> prof_ev;
   std::vector<double> prof_flops;
+  std::vector<int> prof_tiles;
   size_t prof_n = 0;
   hipStream_t prof_stream = nullptr;
 };

@@ -144,7 +144,8 @@ void add_prob(GemmLaunch& g, const GemmProb& P, bool lower) {
 void make_tiles(GemmLaunch& g, int force_tile) {
   int64_t big = 0;
   for (const GemmProb& P : g.h_probs) big += (int64_t)((P.M + 127) / 128) * ((P.N + 127) / 128);
-  g.tile = force_tile ? force_tile : (big >= 512 ? 128 : 64);
+  (void)big;
+  g.tile = force_tile ? force_tile : 64;  // 64x64 tiles measured faster than 128x128 at every shape of this path
   g.h_tiles.clear();
   const int T = g.tile;
   for (size_t i = 0; i < g.h_probs.size(); ++i) {
@@ -426,8 +427,12 @@ void run_gemm(spde_plan* pl, const GemmLaunch& g, const Bases& b, hipStream_t s)
     hipEventRecord(pl->prof_ev[2 * pl->prof_n], s);
     launch_gemm(g, b, s);
     hipEventRecord(pl->prof_ev[2 * pl->prof_n + 1], s);
-    if (pl->prof_flops.size() <= pl->prof_n) pl->prof_flops.resize(pl->prof_n + 1);
+    if (pl->prof_flops.size() <= pl->prof_n) {
+      pl->prof_flops.resize(pl->prof_n + 1);
+      pl->prof_tiles.resize(pl->prof_n + 1);
+    }
     pl->prof_flops[pl->prof_n] = g.flops;
+    pl->prof_tiles[pl->prof_n] = g.ntiles * (g.tile == 128 ? -1 : 1);
     pl->prof_n++;
     pl->prof_stream = s;
     return;
@@ -790,6 +795,24 @@ int spde_prof_begin(spde_plan* pl) {
   }
   pl->prof_on = true;
   pl->prof_n = 0;
+  return SPDE_OK;
+}
+
+int spde_prof_dump(spde_plan* pl, double* ms, double* flops, int32_t* tiles, int64_t cap, int64_t* count) {
+  if (!pl || pl->host_only) {
+    set_error("spde_prof_dump: no device plan");
+    return SPDE_ERR_ARG;
+  }
+  if (pl->prof_n) SPDE_HIP(hipStreamSynchronize(pl->prof_stream));
+  const int64_t n = std::min<int64_t>(cap, (int64_t)pl->prof_n);
+  for (int64_t i = 0; i < n; ++i) {
+    float t = 0.0f;
+    SPDE_HIP(hipEventElapsedTime(&t, pl->prof_ev[2 * i], pl->prof_ev[2 * i + 1]));
+    if (ms) ms[i] = t;
+    if (flops) flops[i] = pl->prof_flops[i];
+    if (tiles) tiles[i] = pl->prof_tiles[i];
+  }
+  if (count) *count = (int64_t)pl->prof_n;
   return SPDE_OK;
 }
 
