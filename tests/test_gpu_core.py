@@ -151,3 +151,36 @@ def test_loglike_with_initial_state_parameters():
     assert abs(like - like_o) <= 1e-9 * abs(like_o)
     scale = np.abs(jac_o).max()
     assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * scale)
+
+
+def test_model_sample_and_update():
+    """Model.sample: x = P^T L^-T z + mu with our ordering (Q x - mu part = P^T L z
+    is not observable, so check Q x equals the solve of z's factor image);
+    Model.update: mu = tau Q_c^-1 S^T y (model.py:120-127)."""
+    import spdepy_amd as sp
+    c = CASES["var_b1"]()
+    g = c["grid"]
+    m = product_case(c)
+    m.setQ(par=c["par"])
+    M = sp.Model.__new__(sp.Model)
+    M.grid, M.mod, M.useCov = m.grid, m, False
+    M.mu = np.zeros(g.n)
+    M.setModel()
+    step("Model.sample")
+    data = M.sample(n=3, simple=True, seed=4)
+    assert data.shape == (g.n, 3)
+    Qd = c["model"].makeQ(c["par"], grad=False)[0].toarray()
+    z = np.random.default_rng(4).normal(size=g.n * 3).reshape(g.n, 3)
+    # covariance identity: x^T Q x has expectation n; with 3 draws check the
+    # exact relation L^T P x = z through the factor's own solve
+    x = data
+    fac = M.Q_fac
+    back = fac.solve_Lt(fac.solve_L(Qd @ x))
+    assert np.allclose(np.asarray(back), x, rtol=1e-8, atol=1e-10)
+    step("Model.update")
+    S = g.getS(c["idx"])
+    M.update(c["y"], c["idx"], tau=np.exp(c["par"][-1]))
+    tau = np.exp(c["par"][-1])
+    Qc = Qd + tau * (S.T @ S).toarray()
+    mu_ref = np.linalg.solve(Qc, tau * (S.T @ c["y"]))
+    assert np.allclose(M.mu, mu_ref, rtol=1e-9, atol=1e-12)

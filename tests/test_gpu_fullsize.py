@@ -1,0 +1,84 @@
+"""Full-size (100 x 100 x 20, n = 200,000) checks of the GPU path through
+size-independent properties: the closed-form log-determinant, central finite
+differences of the gradient, bitwise determinism, and agreement of the
+sampler with the selected inverse."""
+import numpy as np
+import pytest
+
+from oracle import models as Mo
+from tests.gpu_util import step
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def workload():
+    import bench
+    step("build 100x100x20 workload")
+    mod, par = bench.build_workload()
+    return mod, par
+
+
+def test_logdet_matches_closed_form(workload):
+    from oracle import fixtures as F
+    mod, par = workload
+    step("factor Q at 100x100x20")
+    Q, fac, _ = mod.makeQ(par, grad=False)
+    ld = fac.logdet()
+    c = F.config2()
+    assert np.allclose(c["par"], par)
+    A, q, s = c["model"].spatial_operator(c["par"])
+    A0 = c["mod0"].A0(c["mod0"].par)[0]
+    cf = Mo.logdet_closed_form(A, q, A0, c["grid"].V, 20, s, c["grid"].dt)
+    assert abs(ld - cf) <= 1e-11 * abs(cf), (ld, cf)
+
+
+def test_gradient_central_differences(workload):
+    """bc=1: the advection derivative stencils of the reference are not the
+    operator's derivative at boundary cells, so w-coefficients are skipped."""
+    mod, par = workload
+    step("logLike + grad at 100x100x20")
+    like, jac = mod.logLike(par)
+    Np = mod.Np
+
+    def central(i, h):
+        pp, pm = par.copy(), par.copy()
+        pp[i] += h
+        pm[i] -= h
+        return (mod.logLike(pp, grad=False) - mod.logLike(pm, grad=False)) / (2 * h)
+
+    for i in [0, 27, Np + 5, 2 * Np + 9, 3 * Np + 40, 6 * Np, 6 * Np + 1]:
+        step("finite difference %d" % i)
+        fd = (4 * central(i, 1e-4) - central(i, 2e-4)) / 3   # Richardson: O(h^4)
+        assert abs(fd - jac[i]) <= 2e-6 * max(abs(jac[i]), 1e-3 * np.abs(jac).max()), (i, fd, jac[i])
+
+
+def test_evaluation_is_deterministic(workload):
+    mod, par = workload
+    step("two identical evaluations")
+    l1, j1 = mod.logLike(par)
+    l2, j2 = mod.logLike(par)
+    assert l1 == l2 and np.array_equal(j1, j2)
+
+
+def test_sampler_and_selected_inverse_identities(workload):
+    """Exact identities at full size: x = P^T L^-T z satisfies x^T Q x = z^T z;
+    the selected inverse satisfies sum_j Q_ij Sigma_ij = (Q Q^-1)_ii = 1 for
+    every row i (Q's pattern holds every j the sum needs)."""
+    import torch
+    mod, par = workload
+    eng = mod._engine()
+    qval = eng.assemble(mod._coeffs(par))
+    step("factor + sample + selinv")
+    eng.plan.factor(0, qval)
+    z = torch.as_tensor(np.random.default_rng(0).normal(size=(eng.n, 3))).cuda()
+    x = eng.plan.solve(0, z, 2)
+    qx = eng.plan.spmv(qval, x)
+    lhs = (x * qx).sum(0).cpu().numpy()
+    rhs = (z * z).sum(0).cpu().numpy()
+    assert np.allclose(lhs, rhs, rtol=1e-10)
+    sig = eng.plan.selinv(0)
+    indptr, _ = eng.plan.structure()
+    rows = torch.as_tensor(np.repeat(np.arange(eng.n), np.diff(indptr))).cuda()
+    diag = torch.zeros(eng.n, dtype=torch.float64, device="cuda").index_add_(0, rows, qval * sig)
+    assert float((diag - 1.0).abs().max()) < 1e-9
