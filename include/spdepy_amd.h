@@ -127,6 +127,15 @@ int spde_spmv(spde_plan* plan, const double* qval, const double* x, double* y, i
 int spde_factor(spde_plan* plan, int slot, const double* qval, const double* diag_add,
                 double* logdet, void* stream);
 
+/* The same factorisation enqueued without waiting (factorisations of
+ * different slots may run concurrently on different streams);
+ * spde_factor_finish waits for it and returns *logdet or SPDE_ERR_NOT_SPD.
+ * Solves and selected inverses may be enqueued on the same stream before
+ * finishing. */
+int spde_factor_async(spde_plan* plan, int slot, const double* qval, const double* diag_add,
+                      void* stream);
+int spde_factor_finish(spde_plan* plan, int slot, double* logdet);
+
 /* Solves with the factor in `slot`.  mode: 0 Q^-1 B, 1 L^-1 P B,
  * 2 P^T L^-T B (the reference's apply_Pt(solve_Lt(.))).  B device,
  * n x nrhs column-major with leading dimension ldb, overwritten. */
@@ -137,7 +146,8 @@ int spde_solve(spde_plan* plan, int slot, int mode, double* B, int nrhs, int64_t
                void* stream);
 
 /* Takahashi selected inverse of the factor in `slot`: Sigma = Q^-1 on the
- * structural pattern (sigma[nnz_struct], device). */
+ * structural pattern (sigma[nnz_struct], device).  Each slot has its own
+ * arenas, so selected inverses of different slots may run concurrently. */
 int spde_selinv(spde_plan* plan, int slot, double* sigma, void* stream);
 
 /* ---------------------------------------------------------------- gradient
@@ -172,9 +182,12 @@ int spde_grad_cells(spde_plan* plan, const spde_grad_args* args, double* out, vo
  * Between spde_prof_begin and spde_prof_end every FP64 GEMM launch of the
  * plan (factorisation, inverse of L11, solves, selected inverse) is bracketed
  * by HIP events on its stream; spde_prof_end synchronises and returns the
- * summed kernel time (ms), the summed algorithmic flops and the launch count. */
+ * summed per-launch time (ms), the summed algorithmic flops, the launch count
+ * and the busy time (ms during which at least one GEMM launch was in flight;
+ * launches on different streams overlap, so busy <= summed time). */
 int spde_prof_begin(spde_plan* plan);
-int spde_prof_end(spde_plan* plan, double* gemm_ms, double* gemm_flops, int64_t* launches);
+int spde_prof_end(spde_plan* plan, double* gemm_ms, double* gemm_flops, int64_t* launches,
+                  double* busy_ms);
 /* Per-launch view of the timed launches so far (host arrays of length cap):
  * kernel ms, algorithmic flops, tile count (negative for 128-wide tiles). */
 int spde_prof_dump(spde_plan* plan, double* ms, double* flops, int32_t* tiles, int64_t cap,

@@ -259,13 +259,7 @@ extern "C" int spde_grad_cells(spde_plan* pl, const spde_grad_args* g, double* o
   a.acan = pl->d_acan;
   a.a0can = pl->d_a0can;
   a.qdiag = pl->d_qdiag;
-  double* win = nullptr;
-  if (pl->work_doubles >= a.Ns * kWin) {
-    win = pl->d_work;
-  } else {
-    set_error("spde_grad_cells: workspace too small");
-    return SPDE_ERR_STATE;
-  }
+  double* win = pl->d_win;
   const unsigned blocks = (unsigned)((a.Ns + 127) / 128);
   hipLaunchKernelGGL(grad_reduce_kernel, dim3(blocks), dim3(128), 0, s, a, win, out);
   hipLaunchKernelGGL(grad_cell_kernel, dim3(blocks), dim3(128), 0, s, a, win, out);

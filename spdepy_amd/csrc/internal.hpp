@@ -98,6 +98,16 @@ struct LevelPlan {
 struct Slot {
   double* front = nullptr;  // factor fronts
   double* z = nullptr;      // inverses of the diagonal blocks of L
+  double* sigma = nullptr;  // selected-inverse fronts (allocated on first use)
+  double* work = nullptr;   // inverse-of-L11 and selected-inverse temporaries
+  double* partial = nullptr;  // per-node log-det partial sums
+  double* ld = nullptr;     // device log-det
+  int* status = nullptr;    // first non-positive pivot (elimination column + 1)
+  double* vec = nullptr;    // solve front vectors
+  int64_t vec_cap = 0;      // in doubles
+  double* xy = nullptr;     // elimination-ordered solve buffer
+  int64_t xy_cap = 0;
+  hipStream_t stream = nullptr;  // stream of the last factorisation
   bool factored = false;
 };
 
@@ -131,16 +141,8 @@ struct spde_plan {
   double* d_acan = nullptr;       // canonical 9-point spatial operator A
   double* d_a0can = nullptr;      // canonical 9-point initial-state operator A0
   double* d_qdiag = nullptr;      // diagonal of Qs per cell
-  double* d_sigma = nullptr;      // selected-inverse fronts
-  double* d_work = nullptr;
-  int64_t work_doubles = 0;
-  double* d_vec = nullptr;        // solve front vectors
-  int64_t vec_cap = 0;            // in doubles
-  double* d_xy = nullptr;         // elimination-ordered solve buffer
-  int64_t xy_cap = 0;
-  double* d_partial = nullptr;    // per-node log-det partial sums
-  double* d_scalar = nullptr;     // small device results
-  int* d_status = nullptr;        // first non-positive pivot
+  double* d_win = nullptr;        // per-cell gradient windows
+  int64_t work_doubles = 0;       // per-slot workspace size
   std::vector<spde::Slot> slots;
   std::vector<spde::LevelPlan> levels;
   std::vector<spde::GemmLaunch> trtri;  // per recursion height: pairs (X, Z2)
@@ -150,6 +152,7 @@ struct spde_plan {
   // optional event timing of the GEMM launches (measurement only)
   bool prof_on = false;
   std::vector<hipEvent_t> prof_ev;
+  hipEvent_t prof_base = nullptr;
   std::vector<double> prof_flops;
   std::vector<int> prof_tiles;
   size_t prof_n = 0;

@@ -444,9 +444,9 @@ Bases make_bases(spde_plan* pl, int slot, double* x) {
   Bases b;
   b.p[kFront] = pl->slots[slot].front;
   b.p[kZ] = pl->slots[slot].z;
-  b.p[kSigma] = pl->d_sigma;
-  b.p[kWork] = pl->d_work;
-  b.p[kVec] = pl->d_vec;
+  b.p[kSigma] = pl->slots[slot].sigma;
+  b.p[kWork] = pl->slots[slot].work;
+  b.p[kVec] = pl->slots[slot].vec;
   b.p[kX] = x;
   return b;
 }
@@ -553,17 +553,18 @@ int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int 
   SPDE_HIP(hipMalloc((void**)&pl->d_acan, sizeof(double) * Ns * 9));
   SPDE_HIP(hipMalloc((void**)&pl->d_a0can, sizeof(double) * Ns * 9));
   SPDE_HIP(hipMalloc((void**)&pl->d_qdiag, sizeof(double) * Ns));
-  SPDE_HIP(hipMalloc((void**)&pl->d_work, sizeof(double) * pl->work_doubles));
-  SPDE_HIP(hipMalloc((void**)&pl->d_partial, sizeof(double) * std::max<size_t>(sym.nodes.size(), 1)));
-  SPDE_HIP(hipMalloc((void**)&pl->d_scalar, sizeof(double) * 16));
-  SPDE_HIP(hipMalloc((void**)&pl->d_status, sizeof(int)));
-  SPDE_HIP(hipMemset(pl->d_status, 0, sizeof(int)));
   pl->slots.resize(nslots);
   for (int s = 0; s < nslots; ++s) {
     SPDE_HIP(hipMalloc((void**)&pl->slots[s].front, sizeof(double) * std::max<int64_t>(sym.front_doubles, 1)));
     SPDE_HIP(hipMalloc((void**)&pl->slots[s].z, sizeof(double) * std::max<int64_t>(sym.z_doubles, 1)));
     SPDE_HIP(hipMemset(pl->slots[s].z, 0, sizeof(double) * std::max<int64_t>(sym.z_doubles, 1)));
+    SPDE_HIP(hipMalloc((void**)&pl->slots[s].work, sizeof(double) * pl->work_doubles));
+    SPDE_HIP(hipMalloc((void**)&pl->slots[s].partial, sizeof(double) * std::max<size_t>(sym.nodes.size(), 1)));
+    SPDE_HIP(hipMalloc((void**)&pl->slots[s].ld, sizeof(double)));
+    SPDE_HIP(hipMalloc((void**)&pl->slots[s].status, sizeof(int)));
+    SPDE_HIP(hipMemset(pl->slots[s].status, 0, sizeof(int)));
   }
+  SPDE_HIP(hipMalloc((void**)&pl->d_win, sizeof(double) * Ns * 60));
   *out = pl;
   return SPDE_OK;
 #undef TRY
@@ -583,12 +584,18 @@ void spde_plan_destroy(spde_plan* pl) {
     for (auto* p : {(void*)pl->d_indptr, (void*)pl->d_indices, (void*)pl->d_code, (void*)pl->d_diag,
                     (void*)pl->d_map, (void*)pl->d_mapsym, (void*)pl->d_perm, (void*)pl->d_relpos,
                     (void*)pl->d_bnd, (void*)pl->d_nodes, (void*)pl->d_tmpl, (void*)pl->d_acan,
-                    (void*)pl->d_a0can, (void*)pl->d_qdiag, (void*)pl->d_sigma, (void*)pl->d_work, (void*)pl->d_vec,
-                    (void*)pl->d_xy, (void*)pl->d_partial, (void*)pl->d_scalar, (void*)pl->d_status})
+                    (void*)pl->d_a0can, (void*)pl->d_qdiag, (void*)pl->d_win})
       hipFree(p);
     for (auto& s : pl->slots) {
       hipFree(s.front);
       hipFree(s.z);
+      hipFree(s.sigma);
+      hipFree(s.work);
+      hipFree(s.partial);
+      hipFree(s.ld);
+      hipFree(s.status);
+      hipFree(s.vec);
+      hipFree(s.xy);
     }
     for (auto& L : pl->levels) {
       for (int s = 0; s < 2; ++s) {
@@ -606,6 +613,7 @@ void spde_plan_destroy(spde_plan* pl) {
     }
     for (auto& g : pl->trtri) free_launch(g);
     for (auto e : pl->prof_ev) hipEventDestroy(e);
+    if (pl->prof_base) hipEventDestroy(pl->prof_base);
     for (size_t i = 0; i < g_extra.size(); ++i)
       if (g_extra[i].first == pl) {
         PlanExtra* ex = g_extra[i].second;
@@ -676,19 +684,17 @@ int spde_plan_export(const spde_plan* pl, int64_t* indptr, int32_t* indices, uin
   return SPDE_OK;
 }
 
-int spde_factor(spde_plan* pl, int slot, const double* qval, const double* diag_add, double* logdet, void* stream) {
+int spde_factor_async(spde_plan* pl, int slot, const double* qval, const double* diag_add, void* stream) {
   int rc = check_slot(pl, slot);
   if (rc) return rc;
   if (!qval) {
-    set_error("spde_factor: null values");
+    set_error("spde_factor_async: null values");
     return SPDE_ERR_ARG;
   }
   hipStream_t s = (hipStream_t)stream;
-  PlanExtra* ex = extra(pl);
   Slot& sl = pl->slots[slot];
-  sl.factored = false;
   const Symbolic& sym = pl->sym;
-  SPDE_HIP(hipMemsetAsync(pl->d_status, 0, sizeof(int), s));
+  SPDE_HIP(hipMemsetAsync(sl.status, 0, sizeof(int), s));
   launch_fill(sl.front, sym.front_doubles, 0.0, s);
   launch_scatter(qval, pl->d_map, (int64_t)pl->st.indices.size(), sl.front, s);
   if (diag_add) launch_diag_add(diag_add, pl->d_diag, pl->d_map, sym.n, sl.front, s);
@@ -697,29 +703,44 @@ int spde_factor(spde_plan* pl, int slot, const double* qval, const double* diag_
     LevelPlan& L = pl->levels[l];
     for (int c = 0; c < 2; ++c) launch_extend_add(L.d_ext[c], L.d_exttile[c], L.next[c], pl->d_relpos, sl.front, s);
     for (Step& st : L.steps) {
-      launch_potrf(st.d_potrf, st.npotrf, sl.front, sl.z, pl->d_status, s);
+      launch_potrf(st.d_potrf, st.npotrf, sl.front, sl.z, sl.status, s);
       launch_panel(st.d_panel, st.npanel, sl.front, sl.z, s);
       run_gemm(pl, st.update, b, s);
     }
     run_gemm(pl, L.syrk, b, s);
   }
   for (auto& g : pl->trtri) run_gemm(pl, g, b, s);
-  launch_logdet(pl->d_nodes, (int)sym.nodes.size(), sl.front, pl->d_partial, pl->d_scalar, s);
+  launch_logdet(pl->d_nodes, (int)sym.nodes.size(), sl.front, sl.partial, sl.ld, s);
+  SPDE_HIP(hipGetLastError());
+  sl.stream = s;
+  sl.factored = true;  // a non-positive pivot is reported by spde_factor_finish
+  return SPDE_OK;
+}
+
+int spde_factor_finish(spde_plan* pl, int slot, double* logdet) {
+  int rc = check_slot(pl, slot);
+  if (rc) return rc;
+  Slot& sl = pl->slots[slot];
   int status = 0;
   double ld = 0.0;
-  SPDE_HIP(hipMemcpyAsync(&status, pl->d_status, sizeof(int), hipMemcpyDeviceToHost, s));
-  SPDE_HIP(hipMemcpyAsync(&ld, pl->d_scalar, sizeof(double), hipMemcpyDeviceToHost, s));
-  SPDE_HIP(hipStreamSynchronize(s));
-  (void)ex;
+  SPDE_HIP(hipMemcpyAsync(&status, sl.status, sizeof(int), hipMemcpyDeviceToHost, sl.stream));
+  SPDE_HIP(hipMemcpyAsync(&ld, sl.ld, sizeof(double), hipMemcpyDeviceToHost, sl.stream));
+  SPDE_HIP(hipStreamSynchronize(sl.stream));
   if (status != 0) {
+    sl.factored = false;
     const int col = status - 1;
     set_error("spde_factor: matrix is not positive definite (pivot at lattice index " +
-              std::to_string(sym.perm[col]) + ")");
+              std::to_string(pl->sym.perm[col]) + ")");
     return SPDE_ERR_NOT_SPD;
   }
   if (logdet) *logdet = ld;
-  sl.factored = true;
   return SPDE_OK;
+}
+
+int spde_factor(spde_plan* pl, int slot, const double* qval, const double* diag_add, double* logdet, void* stream) {
+  int rc = spde_factor_async(pl, slot, qval, diag_add, stream);
+  if (rc) return rc;
+  return spde_factor_finish(pl, slot, logdet);
 }
 
 int spde_solve(spde_plan* pl, int slot, int mode, double* B, int nrhs, int64_t ldb, void* stream) {
@@ -737,36 +758,42 @@ int spde_solve(spde_plan* pl, int slot, int mode, double* B, int nrhs, int64_t l
   PlanExtra* ex = extra(pl);
   const Symbolic& sym = pl->sym;
   const int64_t n = sym.n;
-  if (pl->vec_cap < sym.vec_doubles * nrhs) {
-    SPDE_HIP(hipStreamSynchronize(s));
-    hipFree(pl->d_vec);
-    pl->vec_cap = sym.vec_doubles * nrhs;
-    SPDE_HIP(hipMalloc((void**)&pl->d_vec, sizeof(double) * pl->vec_cap));
+  Slot& sl = pl->slots[slot];
+  if (sl.vec_cap < sym.vec_doubles * nrhs || sl.xy_cap < n * nrhs || pl->solve_nrhs != nrhs) {
+    // buffers and launch lists change: let every stream drain first
+    SPDE_HIP(hipDeviceSynchronize());
+    if (sl.vec_cap < sym.vec_doubles * nrhs) {
+      hipFree(sl.vec);
+      sl.vec = nullptr;
+      sl.vec_cap = 0;
+      SPDE_HIP(hipMalloc((void**)&sl.vec, sizeof(double) * sym.vec_doubles * nrhs));
+      sl.vec_cap = sym.vec_doubles * nrhs;
+    }
+    if (sl.xy_cap < n * nrhs) {
+      hipFree(sl.xy);
+      sl.xy = nullptr;
+      sl.xy_cap = 0;
+      SPDE_HIP(hipMalloc((void**)&sl.xy, sizeof(double) * n * nrhs));
+      sl.xy_cap = n * nrhs;
+    }
+    if (pl->solve_nrhs != nrhs) {
+      rc = build_solve_plan(pl, nrhs);
+      if (rc) return rc;
+    }
   }
-  if (pl->xy_cap < n * nrhs) {
-    SPDE_HIP(hipStreamSynchronize(s));
-    hipFree(pl->d_xy);
-    pl->xy_cap = n * nrhs;
-    SPDE_HIP(hipMalloc((void**)&pl->d_xy, sizeof(double) * pl->xy_cap));
-  }
-  if (pl->solve_nrhs != nrhs) {
-    SPDE_HIP(hipStreamSynchronize(s));
-    rc = build_solve_plan(pl, nrhs);
-    if (rc) return rc;
-  }
-  Bases b = make_bases(pl, slot, pl->d_xy);
+  Bases b = make_bases(pl, slot, sl.xy);
   const int nl = (int)pl->levels.size();
   if (mode == SPDE_SOLVE_LT) {
-    SPDE_HIP(hipMemcpy2DAsync(pl->d_xy, n * sizeof(double), B, ldb * sizeof(double), n * sizeof(double), nrhs,
+    SPDE_HIP(hipMemcpy2DAsync(sl.xy, n * sizeof(double), B, ldb * sizeof(double), n * sizeof(double), nrhs,
                               hipMemcpyDeviceToDevice, s));
   } else {
-    launch_permute_in(B, ldb, pl->d_perm, n, nrhs, pl->d_xy, s);
+    launch_permute_in(B, ldb, pl->d_perm, n, nrhs, sl.xy, s);
   }
   if (mode != SPDE_SOLVE_LT) {
     for (int l = 0; l < nl; ++l) {
       LevelPlan& L = pl->levels[l];
-      launch_fwd_gather(ex->lvl[l].d_nodes, ex->lvl[l].nn, pl->d_nodes, ex->d_children, pl->d_relpos, pl->d_xy, n,
-                        nrhs, pl->d_vec, s);
+      launch_fwd_gather(ex->lvl[l].d_nodes, ex->lvl[l].nn, pl->d_nodes, ex->d_children, pl->d_relpos, sl.xy, n,
+                        nrhs, sl.vec, s);
       run_gemm(pl, L.fwd_y, b, s);
       run_gemm(pl, L.fwd_u, b, s);
     }
@@ -774,16 +801,16 @@ int spde_solve(spde_plan* pl, int slot, int mode, double* B, int nrhs, int64_t l
   if (mode != SPDE_SOLVE_L) {
     for (int l = nl - 1; l >= 0; --l) {
       LevelPlan& L = pl->levels[l];
-      launch_bwd_gather(ex->lvl[l].d_nodes, ex->lvl[l].nn, pl->d_nodes, pl->d_bnd, pl->d_xy, n, nrhs, pl->d_vec, s);
+      launch_bwd_gather(ex->lvl[l].d_nodes, ex->lvl[l].nn, pl->d_nodes, pl->d_bnd, sl.xy, n, nrhs, sl.vec, s);
       run_gemm(pl, L.bwd_t, b, s);
       run_gemm(pl, L.bwd_x, b, s);
     }
   }
   if (mode == SPDE_SOLVE_L) {
-    SPDE_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double), pl->d_xy, n * sizeof(double), n * sizeof(double), nrhs,
+    SPDE_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double), sl.xy, n * sizeof(double), n * sizeof(double), nrhs,
                               hipMemcpyDeviceToDevice, s));
   } else {
-    launch_permute_out(pl->d_xy, pl->d_perm, n, nrhs, B, ldb, s);
+    launch_permute_out(sl.xy, pl->d_perm, n, nrhs, B, ldb, s);
   }
   return SPDE_OK;
 }
@@ -793,6 +820,10 @@ int spde_prof_begin(spde_plan* pl) {
     set_error("spde_prof_begin: no device plan");
     return SPDE_ERR_ARG;
   }
+  if (!pl->prof_base) SPDE_HIP(hipEventCreate(&pl->prof_base));
+  SPDE_HIP(hipDeviceSynchronize());
+  SPDE_HIP(hipEventRecord(pl->prof_base, nullptr));
+  SPDE_HIP(hipEventSynchronize(pl->prof_base));
   pl->prof_on = true;
   pl->prof_n = 0;
   return SPDE_OK;
@@ -803,7 +834,7 @@ int spde_prof_dump(spde_plan* pl, double* ms, double* flops, int32_t* tiles, int
     set_error("spde_prof_dump: no device plan");
     return SPDE_ERR_ARG;
   }
-  if (pl->prof_n) SPDE_HIP(hipStreamSynchronize(pl->prof_stream));
+  if (pl->prof_n) SPDE_HIP(hipDeviceSynchronize());  // launches may sit on several streams
   const int64_t n = std::min<int64_t>(cap, (int64_t)pl->prof_n);
   for (int64_t i = 0; i < n; ++i) {
     float t = 0.0f;
@@ -816,23 +847,42 @@ int spde_prof_dump(spde_plan* pl, double* ms, double* flops, int32_t* tiles, int
   return SPDE_OK;
 }
 
-int spde_prof_end(spde_plan* pl, double* gemm_ms, double* gemm_flops, int64_t* launches) {
+int spde_prof_end(spde_plan* pl, double* gemm_ms, double* gemm_flops, int64_t* launches, double* busy_ms) {
   if (!pl || pl->host_only) {
     set_error("spde_prof_end: no device plan");
     return SPDE_ERR_ARG;
   }
   pl->prof_on = false;
   double ms = 0.0, fl = 0.0;
-  if (pl->prof_n) SPDE_HIP(hipStreamSynchronize(pl->prof_stream));
+  if (pl->prof_n) SPDE_HIP(hipDeviceSynchronize());  // launches may sit on several streams
+  std::vector<std::pair<float, float>> iv(pl->prof_n);
   for (size_t i = 0; i < pl->prof_n; ++i) {
-    float t = 0.0f;
+    float t = 0.0f, a = 0.0f, b = 0.0f;
     SPDE_HIP(hipEventElapsedTime(&t, pl->prof_ev[2 * i], pl->prof_ev[2 * i + 1]));
+    SPDE_HIP(hipEventElapsedTime(&a, pl->prof_base, pl->prof_ev[2 * i]));
+    SPDE_HIP(hipEventElapsedTime(&b, pl->prof_base, pl->prof_ev[2 * i + 1]));
+    iv[i] = {a, b};
     ms += t;
     fl += pl->prof_flops[i];
   }
+  // time during which at least one GEMM launch was in flight (launches on
+  // different streams overlap)
+  std::sort(iv.begin(), iv.end());
+  double busy = 0.0, cur_a = 0.0, cur_b = -1.0;
+  for (auto& p : iv) {
+    if (p.first > cur_b) {
+      if (cur_b > cur_a) busy += cur_b - cur_a;
+      cur_a = p.first;
+      cur_b = p.second;
+    } else if (p.second > cur_b) {
+      cur_b = p.second;
+    }
+  }
+  if (cur_b > cur_a) busy += cur_b - cur_a;
   if (gemm_ms) *gemm_ms = ms;
   if (gemm_flops) *gemm_flops = fl;
   if (launches) *launches = (int64_t)pl->prof_n;
+  if (busy_ms) *busy_ms = busy;
   pl->prof_n = 0;
   return SPDE_OK;
 }
@@ -847,7 +897,11 @@ int spde_selinv(spde_plan* pl, int slot, double* sigma, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   PlanExtra* ex = extra(pl);
   const Symbolic& sym = pl->sym;
-  if (!pl->d_sigma) SPDE_HIP(hipMalloc((void**)&pl->d_sigma, sizeof(double) * std::max<int64_t>(sym.front_doubles, 1)));
+  Slot& sl = pl->slots[slot];
+  if (!sl.sigma) {
+    SPDE_HIP(hipStreamSynchronize(s));
+    SPDE_HIP(hipMalloc((void**)&sl.sigma, sizeof(double) * std::max<int64_t>(sym.front_doubles, 1)));
+  }
   if (!pl->selinv_built) {
     rc = build_selinv_plan(pl);
     if (rc) return rc;
@@ -856,13 +910,13 @@ int spde_selinv(spde_plan* pl, int slot, double* sigma, void* stream) {
   for (int l = (int)pl->levels.size() - 1; l >= 0; --l) {
     LevelPlan& L = pl->levels[l];
     LevelDev& D = ex->lvl[l];
-    launch_sigma_gather(D.d_nodes, pl->d_nodes, pl->d_relpos, pl->d_sigma, D.d_gtiles, D.ngtiles, s);
+    launch_sigma_gather(D.d_nodes, pl->d_nodes, pl->d_relpos, sl.sigma, D.d_gtiles, D.ngtiles, s);
     run_gemm(pl, L.si_x, b, s);
     run_gemm(pl, L.si_s21, b, s);
     run_gemm(pl, L.si_s11a, b, s);
     run_gemm(pl, L.si_s11b, b, s);
   }
-  if (sigma) launch_gather_pattern(pl->d_sigma, pl->d_mapsym, (int64_t)pl->st.indices.size(), sigma, s);
+  if (sigma) launch_gather_pattern(sl.sigma, pl->d_mapsym, (int64_t)pl->st.indices.size(), sigma, s);
   return SPDE_OK;
 }
 

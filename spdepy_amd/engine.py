@@ -78,6 +78,13 @@ class Engine:
         # Q is symmetric: its CSR arrays are the CSC arrays of Q^T = Q
         return sparse.csc_matrix((data.cpu().numpy(), indices.cpu().numpy(), indptr.cpu().numpy()), shape=(n, n))
 
+    def _side_stream(self, torch):
+        dev = torch.cuda.current_device()
+        if getattr(self, "_side", None) is None or self._side_dev != dev:
+            self._side = torch.cuda.Stream()     # default priority
+            self._side_dev = dev
+        return self._side
+
     # ------------------------------------------------------- log-likelihood
     def loglik(self, coeffs, obs_cols, data, log_tau, grad=True, trace="exact", probes=None):
         """Returns (like, parts) where like is the normalised -loglik/(m r) and
@@ -94,12 +101,34 @@ class Engine:
         b = np.zeros((n, r))
         np.add.at(b, obs_cols, y2)
         b *= tau
-        ld_q = self.plan.factor(0, qval)
-        ld_qc = self.plan.factor(1, qval, diag_add)
+        # The factorisations (and selected inverses) of Q and Q_c are
+        # independent: Q runs on a side stream while Q_c, mu_c and the data
+        # terms run on the current one.
+        main = torch.cuda.current_stream()
+        side = self._side_stream(torch)
+        ready = torch.cuda.Event()
+        ready.record(main)
+        side.wait_event(ready)
+        exact = grad and trace == "exact"
+        if exact:
+            sq = torch.empty(self.plan.nnz, dtype=torch.float64, device="cuda")
+            sqc = torch.empty(self.plan.nnz, dtype=torch.float64, device="cuda")
+        with torch.cuda.stream(side):
+            self.plan.factor_async(0, qval)
+            if exact:
+                self.plan.selinv(0, sq)
+        self.plan.factor_async(1, qval, diag_add)
         mu = self.plan.solve(1, torch.as_tensor(b).cuda())
+        if exact:
+            self.plan.selinv(1, sqc)
         qmu = self.plan.spmv(qval, mu)
         cols_t = torch.as_tensor(obs_cols).cuda()
         resid = torch.as_tensor(y2).cuda() - mu[cols_t]
+        done = torch.cuda.Event()
+        done.record(side)
+        main.wait_event(done)
+        ld_qc = self.plan.factor_finish(1)
+        ld_q = self.plan.factor_finish(0)
         quad = float((mu * qmu).sum())
         rss = float((resid * resid).sum())
         like = 0.5 * ld_q * r + m * r * np.log(tau) / 2 - 0.5 * ld_qc * r - 0.5 * quad - tau / 2 * rss
@@ -114,9 +143,7 @@ class Engine:
         g.diag_add = diag_add.data_ptr()
         g.coeffs = coeffs.c
         keep = [mu_cm]
-        if trace == "exact":
-            sq = self.plan.selinv(0)
-            sqc = self.plan.selinv(1)
+        if exact:
             g.mode, g.nh1 = 0, 0
             g.sigma_q, g.sigma_qc = sq.data_ptr(), sqc.data_ptr()
             keep += [sq, sqc]

@@ -107,6 +107,18 @@ class Plan:
                                         _stream(torch)))
         return ld.value
 
+    def factor_async(self, slot, qval, diag_add=None, owner=None):
+        """Enqueues the factorisation on the current stream; factor_finish
+        waits for it and returns the log-determinant."""
+        import torch
+        self.slot_owner[slot] = owner
+        nat.check(nat.lib().spde_factor_async(self._h, slot, _ptr(qval), _ptr(diag_add), _stream(torch)))
+
+    def factor_finish(self, slot):
+        ld = ctypes.c_double()
+        nat.check(nat.lib().spde_factor_finish(self._h, slot, ctypes.byref(ld)))
+        return ld.value
+
     def solve(self, slot, B, mode=nat.SOLVE_A):
         """B: (n,) or (n, k) float64 cuda tensor; returns a new tensor."""
         import torch
