@@ -1,7 +1,7 @@
 // Dense FP64 kernels for the multifrontal factorisation, written for CDNA4
-// (gfx950): batched tile-list GEMM on v_mfma_f64_16x16x4f64, a single-wave
-// Cholesky + inverse of 64x64 diagonal blocks, and the row-strip triangular
-// solve of the panel below each diagonal block.
+// (gfx950): batched tile-list GEMM on v_mfma_f64_16x16x4f64 (which also does
+// the panel solve below each diagonal block, X = S W^T with W = L11^-1) and
+// the Cholesky factor + inverse of 64x64 diagonal blocks.
 #include <hip/hip_runtime.h>
 
 #include "internal.hpp"
@@ -40,9 +40,12 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmProb* __restrict__ 
   __shared__ double Bs[2][SB];
   const GemmTile t = tiles[blockIdx.x];
   const GemmProb P = probs[t.prob];
-  const double* __restrict__ A = bases.p[P.a_base] + P.a_off;
+  // A and C may be the same memory (the in-place panel solve X = S W^T,
+  // one tile column): every tile reads all of its rows of A before its
+  // epilogue writes them, and no tile reads rows another tile writes.
+  const double* A = bases.p[P.a_base] + P.a_off;
   const double* __restrict__ B = bases.p[P.b_base] + P.b_off;
-  double* __restrict__ C = bases.p[P.c_base] + P.c_off;
+  double* C = bases.p[P.c_base] + P.c_off;
   const int M = P.M, N = P.N, K = P.K;
   const int m0 = t.tm * BM, n0 = t.tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -155,194 +158,174 @@ void launch_gemm(const GemmLaunch& g, const Bases& b, hipStream_t s) {
 }
 
 // ------------------------------------------------ diagonal block Cholesky
-// One 256-thread block per 64x64 diagonal block; thread (r, g) keeps row r,
-// columns 16g..16g+15 in registers.  Right-looking in steps of four columns:
-// the owners publish the four raw columns to LDS (double buffered, one
-// barrier per step), every thread forms the 4x4 pivot block's Cholesky
-// factor itself, finishes its row's four entries and applies the rank-4
-// update to its own columns.  W = L^-1 is then built the same way four rows
-// per step and written to the node's Z block.  Blocks narrower than 64 are
-// padded with the identity.  A non-positive pivot is recorded in *status as
-// (global elimination column + 1) and replaced by 1 so the sweep completes.
-struct Chol4 {
+// One 256-thread block per 64x64 diagonal block.  Thread (r, g) (lane r of
+// wave g) keeps row r, columns c = 4q + g (q = 0..15), in registers, so each
+// step of four pivot columns j0..j0+3 has exactly one of them in every
+// thread.  Step: the threads publish their pivot-column entry (one LDS write
+// each, double buffered), every thread forms the 4x4 pivot block's factor
+// itself, finishes its row's four entries, writes its own one to the L copy
+// Y in LDS, and after one more barrier applies the rank-4 update to its
+// columns right of the pivot block with the finished rows read back from Y
+// (broadcast reads, no recomputation).  Entries right of the diagonal are
+// never read, so the update runs unconditionally over them.  W = L^-1 is
+// then built per wave without barriers (below).  Pivots use v_rsq_f64 with
+// two Newton steps (relative error below 2^-52) instead of a square root and
+// a division.  Blocks narrower than 64 are padded with the identity.  A
+// non-positive pivot is recorded in *status as (global elimination column
+// + 1) and replaced by 1 so the sweep completes.
+struct Piv4 {
   double i0, i1, i2, i3, l10, l20, l30, l21, l31, l32, d0, d1, d2, d3;
 };
 
-__device__ inline double safe_sqrt(double x, bool& bad) {
+// Keeps an updated register value where it is computed: without it the
+// compiler defers the rank-4 updates of later columns to their first use and
+// keeps every step's operands live (hundreds of registers, spilled).
+__device__ inline void pin(double& v) { asm volatile("" : "+v"(v)); }
+
+// Value of v in lane `lane` of the wave (lane is a compile-time constant).
+__device__ inline double lane_value(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// 1/sqrt(x) for x > 0 (hardware estimate refined twice), x for the factor.
+__device__ inline double pivot_rsq(double x, bool& bad) {
   if (!(x > 0.0)) {
     bad = true;
-    return 1.0;
+    x = 1.0;
   }
-  return sqrt(x);
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = y * (1.5 - hx * y * y);
+  y = y * (1.5 - hx * y * y);
+  return y;
 }
 
-__device__ inline void chol4(const double (*D)[5], Chol4& c, int& badcol) {
+// 4x4 Cholesky factor of D(i, j) = D[i * P + j] (lower part read).
+template <int P>
+__device__ inline void chol4(const double* D, Piv4& c, int& badcol) {
   bool b0 = false, b1 = false, b2 = false, b3 = false;
-  c.d0 = safe_sqrt(D[0][0], b0);
-  c.i0 = 1.0 / c.d0;
-  c.l10 = D[1][0] * c.i0;
-  c.l20 = D[2][0] * c.i0;
-  c.l30 = D[3][0] * c.i0;
-  c.d1 = safe_sqrt(D[1][1] - c.l10 * c.l10, b1);
-  c.i1 = 1.0 / c.d1;
-  c.l21 = (D[2][1] - c.l20 * c.l10) * c.i1;
-  c.l31 = (D[3][1] - c.l30 * c.l10) * c.i1;
-  c.d2 = safe_sqrt(D[2][2] - c.l20 * c.l20 - c.l21 * c.l21, b2);
-  c.i2 = 1.0 / c.d2;
-  c.l32 = (D[3][2] - c.l30 * c.l20 - c.l31 * c.l21) * c.i2;
-  c.d3 = safe_sqrt(D[3][3] - c.l30 * c.l30 - c.l31 * c.l31 - c.l32 * c.l32, b3);
-  c.i3 = 1.0 / c.d3;
+  double x = D[0];
+  c.i0 = pivot_rsq(x, b0);
+  c.d0 = b0 ? 1.0 : x * c.i0;
+  c.l10 = D[P] * c.i0;
+  c.l20 = D[2 * P] * c.i0;
+  c.l30 = D[3 * P] * c.i0;
+  x = D[P + 1] - c.l10 * c.l10;
+  c.i1 = pivot_rsq(x, b1);
+  c.d1 = b1 ? 1.0 : x * c.i1;
+  c.l21 = (D[2 * P + 1] - c.l20 * c.l10) * c.i1;
+  c.l31 = (D[3 * P + 1] - c.l30 * c.l10) * c.i1;
+  x = D[2 * P + 2] - c.l20 * c.l20 - c.l21 * c.l21;
+  c.i2 = pivot_rsq(x, b2);
+  c.d2 = b2 ? 1.0 : x * c.i2;
+  c.l32 = (D[3 * P + 2] - c.l30 * c.l20 - c.l31 * c.l21) * c.i2;
+  x = D[3 * P + 3] - c.l30 * c.l30 - c.l31 * c.l31 - c.l32 * c.l32;
+  c.i3 = pivot_rsq(x, b3);
+  c.d3 = b3 ? 1.0 : x * c.i3;
   badcol = b0 ? 0 : (b1 ? 1 : (b2 ? 2 : (b3 ? 3 : -1)));
-}
-
-// y = x L_d^-T for a row x of raw values (columns j0..j0+3)
-__device__ inline void rowsolve4(const Chol4& c, double x0, double x1, double x2, double x3, double& y0, double& y1,
-                                 double& y2, double& y3) {
-  y0 = x0 * c.i0;
-  y1 = (x1 - y0 * c.l10) * c.i1;
-  y2 = (x2 - y0 * c.l20 - y1 * c.l21) * c.i2;
-  y3 = (x3 - y0 * c.l30 - y1 * c.l31 - y2 * c.l32) * c.i3;
 }
 
 __global__ __launch_bounds__(256) void potrf_diag_kernel(const PotrfDesc* __restrict__ descs,
                                                          double* __restrict__ F, double* __restrict__ Z,
                                                          int* __restrict__ status) {
-  __shared__ double pub[2][64][5];
-  __shared__ double Ls[64][65];
-  __shared__ double wpub[2][4][64];
+  constexpr int YP = 66;                // pitch of Y (16-byte aligned rows)
+  __shared__ double pub[2][64 * 4];     // pivot columns, row-major [r][0..3]
+  __shared__ double Y[64 * YP];         // finished L (zeros right of the diagonal)
+  __shared__ double idg[64];            // 1 / L(k, k)
   const PotrfDesc d = descs[blockIdx.x];
-  const int tid = threadIdx.x, r = tid & 63, cg = tid >> 6;
+  const int tid = threadIdx.x, r = tid & 63, g = tid >> 6;
   const int kb = d.kb;
   double* Fb = F + d.foff + d.k0 + (int64_t)d.k0 * d.ld;
   double a[16];
+  const int rr = r < kb ? r : kb - 1;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
-    const int c = 16 * cg + q;
-    double v;
-    if (r < kb && c < kb)
-      v = (c <= r) ? Fb[r + (int64_t)c * d.ld] : 0.0;
-    else
-      v = (r == c) ? 1.0 : 0.0;
-    a[q] = v;
+    const int c = 4 * q + g;
+    const int cc = c < kb ? c : kb - 1;
+    const double v = Fb[rr + (int64_t)cc * d.ld];  // clamped: always inside the block
+    a[q] = (r < kb && c < kb) ? (c <= r ? v : 0.0) : (r == c ? 1.0 : 0.0);
   }
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
-    const int j0 = 4 * s, og = s >> 2, q0 = 4 * (s & 3), buf = s & 1;
-    if (cg == og) {
-      pub[buf][r][0] = a[q0];
-      pub[buf][r][1] = a[q0 + 1];
-      pub[buf][r][2] = a[q0 + 2];
-      pub[buf][r][3] = a[q0 + 3];
-    }
+    const int j0 = 4 * s, buf = s & 1;
+    pub[buf][4 * r + g] = a[s];
     __syncthreads();
-    Chol4 c;
+    Piv4 c;
     int bad;
-    chol4(&pub[buf][j0], c, bad);
+    chol4<4>(&pub[buf][4 * j0], c, bad);
     if (bad >= 0 && tid == 0 && j0 + bad < kb) atomicCAS(status, 0, d.colbase + j0 + bad + 1);
-    double y0, y1, y2, y3;
-    rowsolve4(c, pub[buf][r][0], pub[buf][r][1], pub[buf][r][2], pub[buf][r][3], y0, y1, y2, y3);
-    if (r < j0 + 4) {  // rows of the pivot block: exact factor entries, zeros above
-      const int k = r - j0;
-      y0 = (k == 0) ? c.d0 : (k > 0 ? y0 : 0.0);
-      y1 = (k == 1) ? c.d1 : (k > 1 ? y1 : 0.0);
-      y2 = (k == 2) ? c.d2 : (k > 2 ? y2 : 0.0);
-      y3 = (k == 3) ? c.d3 : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int col = 16 * cg + q;
-      if (col >= j0 + 4 && col <= r) {
-        double z0, z1, z2, z3;
-        rowsolve4(c, pub[buf][col][0], pub[buf][col][1], pub[buf][col][2], pub[buf][col][3], z0, z1, z2, z3);
-        a[q] -= y0 * z0 + y1 * z1 + y2 * z2 + y3 * z3;
-      }
-    }
-    if (cg == og) {
-      a[q0] = y0;
-      a[q0 + 1] = y1;
-      a[q0 + 2] = y2;
-      a[q0 + 3] = y3;
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int c = 16 * cg + q;
-    Ls[r][c] = a[q];
-    if (r < kb && c < kb && c <= r) Fb[r + (int64_t)c * d.ld] = a[q];
-  }
-  __syncthreads();
-  double w[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) w[q] = (r == 16 * cg + q) ? 1.0 : 0.0;
-#pragma unroll
-  for (int s = 0; s < 16; ++s) {
-    const int k0 = 4 * s, buf = s & 1;
-    if (r >= k0 && r < k0 + 4) {
-#pragma unroll
-      for (int q = 0; q < 16; ++q) wpub[buf][r - k0][16 * cg + q] = w[q];
-    }
+    // row r's four entries; on the pivot rows this yields the factor's
+    // entries left of and on the diagonal, the rest (and all of the rows
+    // above) are cleared
+    const double* pr = &pub[buf][4 * r];
+    double y0 = pr[0] * c.i0;
+    double y1 = (pr[1] - y0 * c.l10) * c.i1;
+    double y2 = (pr[2] - y0 * c.l20 - y1 * c.l21) * c.i2;
+    double y3 = (pr[3] - y0 * c.l30 - y1 * c.l31 - y2 * c.l32) * c.i3;
+    y0 = (r >= j0) ? y0 : 0.0;
+    y1 = (r >= j0 + 1) ? y1 : 0.0;
+    y2 = (r >= j0 + 2) ? y2 : 0.0;
+    y3 = (r >= j0 + 3) ? y3 : 0.0;
+    const double mine = (g == 0) ? y0 : (g == 1 ? y1 : (g == 2 ? y2 : y3));
+    a[s] = mine;
+    Y[r * YP + j0 + g] = mine;
+    if (tid < 4) idg[j0 + tid] = (tid == 0) ? c.i0 : (tid == 1 ? c.i1 : (tid == 2 ? c.i2 : c.i3));
     __syncthreads();
-    const double e0 = 1.0 / Ls[k0][k0], e1 = 1.0 / Ls[k0 + 1][k0 + 1];
-    const double e2 = 1.0 / Ls[k0 + 2][k0 + 2], e3 = 1.0 / Ls[k0 + 3][k0 + 3];
-    const double m10 = Ls[k0 + 1][k0], m20 = Ls[k0 + 2][k0], m21 = Ls[k0 + 2][k0 + 1];
-    const double m30 = Ls[k0 + 3][k0], m31 = Ls[k0 + 3][k0 + 1], m32 = Ls[k0 + 3][k0 + 2];
-    const bool below = r >= k0 + 4;
-    const double g0 = below ? Ls[r][k0] : 0.0, g1 = below ? Ls[r][k0 + 1] : 0.0;
-    const double g2 = below ? Ls[r][k0 + 2] : 0.0, g3 = below ? Ls[r][k0 + 3] : 0.0;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int col = 16 * cg + q;
-      const double z0 = wpub[buf][0][col] * e0;
-      const double z1 = (wpub[buf][1][col] - m10 * z0) * e1;
-      const double z2 = (wpub[buf][2][col] - m20 * z0 - m21 * z1) * e2;
-      const double z3 = (wpub[buf][3][col] - m30 * z0 - m31 * z1 - m32 * z2) * e3;
-      if (below) {
-        w[q] -= g0 * z0 + g1 * z1 + g2 * z2 + g3 * z3;
-      } else if (r >= k0 && r < k0 + 4) {
-        const int k = r - k0;
-        w[q] = (k == 0) ? z0 : (k == 1 ? z1 : (k == 2 ? z2 : z3));
-      }
+    for (int q = s + 1; q < 16; ++q) {
+      const double* yc = &Y[(4 * q + g) * YP + j0];
+      a[q] -= y0 * yc[0] + y1 * yc[1] + y2 * yc[2] + y3 * yc[3];
+      pin(a[q]);
     }
   }
   if (r < kb) {
-    double* Zb = Z + d.zoff + d.k0 + (int64_t)d.k0 * d.zld;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      const int c = 16 * cg + q;
-      if (c < kb) Zb[r + (int64_t)c * d.zld] = w[q];
+      const int c = 4 * q + g;
+      if (c < kb && c <= r) Fb[r + (int64_t)c * d.ld] = a[q];
     }
   }
-}
-
-// ------------------------------------------------------------ panel solve
-// Rows r0..r0+nr of the panel below a diagonal block: X = S W^T, in place.
-__global__ __launch_bounds__(256) void panel_kernel(const PanelDesc* __restrict__ descs, double* __restrict__ F,
-                                                    const double* __restrict__ Z) {
-  __shared__ double S[64][65];
-  __shared__ double W[64][65];
-  const PanelDesc d = descs[blockIdx.x];
-  const int tid = threadIdx.x;
-  double* Fp = F + d.foff + d.r0 + (int64_t)d.k0 * d.ld;
-  const double* Zb = Z + d.zoff + d.k0 + (int64_t)d.k0 * d.zld;
-  for (int idx = tid; idx < 64 * 64; idx += 256) {
-    const int rr = idx & 63, k = idx >> 6;
-    S[rr][k] = (rr < d.nr && k < d.kb) ? Fp[rr + (int64_t)k * d.ld] : 0.0;
-    W[rr][k] = (rr < d.kb && k < d.kb && k <= rr) ? Zb[rr + (int64_t)k * d.zld] : 0.0;
+  // W = L^-1, four rows per step; w[q] = W(r, 4q + g).  Column 4q + g of W
+  // lives in wave g alone (lane = row), so the sweep needs no barrier: the
+  // four residual entries of the pivot rows are read across lanes, the 4x4
+  // solve is done on them (uniform values) and every row is updated; the
+  // pivot rows solve for their own entry in the same expression (their
+  // diagonal term left out, scaled by 1/L(r, r)), rows above read zeros
+  // from Y and keep their values.
+  double w[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) w[q] = (r == 4 * q + g) ? 1.0 : 0.0;
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int k0 = 4 * s;
+    const double* L0 = &Y[k0 * YP + k0];
+    const double e0 = idg[k0], e1 = idg[k0 + 1], e2 = idg[k0 + 2], e3 = idg[k0 + 3];
+    const double m10 = L0[YP], m20 = L0[2 * YP], m21 = L0[2 * YP + 1];
+    const double m30 = L0[3 * YP], m31 = L0[3 * YP + 1], m32 = L0[3 * YP + 2];
+    const double* Lr = &Y[r * YP + k0];
+    const int k = r - k0;
+    const double g0 = (k == 0) ? 0.0 : Lr[0], g1 = (k == 1) ? 0.0 : Lr[1];
+    const double g2 = (k == 2) ? 0.0 : Lr[2], g3 = (k == 3) ? 0.0 : Lr[3];
+    const double h = (k >= 0 && k < 4) ? idg[r] : 1.0;
+#pragma unroll
+    for (int q = 0; q <= s; ++q) {
+      const double z0 = lane_value(w[q], k0) * e0;
+      const double z1 = (lane_value(w[q], k0 + 1) - m10 * z0) * e1;
+      const double z2 = (lane_value(w[q], k0 + 2) - m20 * z0 - m21 * z1) * e2;
+      const double z3 = (lane_value(w[q], k0 + 3) - m30 * z0 - m31 * z1 - m32 * z2) * e3;
+      w[q] = (w[q] - (g0 * z0 + g1 * z1 + g2 * z2 + g3 * z3)) * h;
+      pin(w[q]);
+    }
   }
-  __syncthreads();
-  const int rr = tid & 63, cb = (tid >> 6) * 16;
-  double out[16];
+  double* Zb = Z + d.zoff + d.k0 + (int64_t)d.k0 * d.zld;
+  if (r < kb) {
 #pragma unroll
-  for (int q = 0; q < 16; ++q) out[q] = 0.0;
-  for (int k = 0; k < 64; ++k) {
-    const double s = S[rr][k];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) out[q] += s * W[cb + q][k];
-  }
-  if (rr < d.nr) {
-#pragma unroll
-    for (int q = 0; q < 16; ++q)
-      if (cb + q < d.kb) Fp[rr + (int64_t)(cb + q) * d.ld] = out[q];
+    for (int q = 0; q < 16; ++q) {
+      const int c = 4 * q + g;
+      if (c < kb) Zb[r + (int64_t)c * d.zld] = (c <= r) ? w[q] : 0.0;
+    }
   }
 }
 
@@ -368,9 +351,6 @@ __global__ __launch_bounds__(64) void mfma_probe_kernel(int which, double* __res
 
 void launch_potrf(const PotrfDesc* d, int n, double* F, double* Z, int* status, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(potrf_diag_kernel, dim3(n), dim3(256), 0, s, d, F, Z, status);
-}
-void launch_panel(const PanelDesc* d, int n, double* F, const double* Z, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(panel_kernel, dim3(n), dim3(256), 0, s, d, F, Z);
 }
 void launch_fill(double* p, int64_t n, double v, hipStream_t s) {
   if (n <= 0) return;

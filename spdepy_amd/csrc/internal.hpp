@@ -61,10 +61,6 @@ struct PotrfDesc {      // one diagonal block of one front
   int32_t k0, kb;       // block start (within own columns) and width
   int32_t node, colbase;
 };
-struct PanelDesc {      // a strip of rows below a diagonal block
-  int64_t foff, zoff;
-  int32_t ld, zld, k0, kb, r0, nr;
-};
 struct ExtAddDesc {     // extend-add of a child's update block into its parent
   int64_t cfoff, pfoff, relpos_off;
   int32_t cld, cp, pld, b;
@@ -74,10 +70,10 @@ struct ExtAddTile {
 };
 
 struct Step {           // one stage of the dense partial factorisation of a level
-  int npotrf = 0, npanel = 0;
+  int npotrf = 0;
   PotrfDesc* d_potrf = nullptr;
-  PanelDesc* d_panel = nullptr;
-  GemmLaunch update;
+  GemmLaunch panel;     // rows below the diagonal blocks: X = S W^T, in place
+  GemmLaunch update;    // trailing update of the nodes' own columns
 };
 
 struct LevelPlan {

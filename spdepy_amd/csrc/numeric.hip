@@ -15,7 +15,6 @@ namespace spde {
 
 void launch_gemm(const GemmLaunch& g, const Bases& b, hipStream_t s);
 void launch_potrf(const PotrfDesc* d, int n, double* F, double* Z, int* status, hipStream_t s);
-void launch_panel(const PanelDesc* d, int n, double* F, const double* Z, hipStream_t s);
 void launch_fill(double* p, int64_t n, double v, hipStream_t s);
 void launch_scatter(const double* q, const int64_t* map, int64_t nnz, double* F, hipStream_t s);
 void launch_diag_add(const double* add, const int64_t* diag, const int64_t* map, int64_t n, double* F, hipStream_t s);
@@ -129,13 +128,14 @@ int finalize(GemmLaunch& g) {
   return upload(&g.d_tiles, g.h_tiles);
 }
 
-void add_prob(GemmLaunch& g, const GemmProb& P, bool lower) {
+// flop_scale: algorithmic share of 2MNK (0.5 for a triangular operand)
+void add_prob(GemmLaunch& g, const GemmProb& P, bool lower, double flop_scale = 1.0) {
   if (P.M <= 0 || P.N <= 0) return;
   if (P.K <= 0 && P.beta == 1.0) return;
   GemmProb Q = P;
   Q.lower = lower ? 1 : 0;
   g.h_probs.push_back(Q);
-  g.flops += 2.0 * P.M * (double)P.N * P.K * (lower ? 0.5 : 1.0);
+  g.flops += 2.0 * P.M * (double)P.N * P.K * (lower ? 0.5 : 1.0) * flop_scale;
 }
 
 // Tile edge of a launch: 128 when that still gives the chip >= 2 blocks per
@@ -208,7 +208,8 @@ int build_factor_plan(spde_plan* pl) {
     for (int k0 = 0; k0 < pmax; k0 += NB) {
       Step st;
       std::vector<PotrfDesc> pd;
-      std::vector<PanelDesc> pn;
+      st.panel.ta = false;
+      st.panel.tb = true;
       st.update.ta = false;
       st.update.tb = true;
       for (int id : L.nodes) {
@@ -216,21 +217,27 @@ int build_factor_plan(spde_plan* pl) {
         if (nd.p <= k0) continue;
         const int n = nd.n(), kb = std::min(NB, nd.p - k0);
         pd.push_back({nd.foff, nd.zoff, n, nd.p, k0, kb, id, nd.start + k0});
-        for (int r0 = k0 + kb; r0 < n; r0 += 64)
-          pn.push_back({nd.foff, nd.zoff, n, nd.p, k0, kb, r0, std::min(64, n - r0)});
         const int M = n - k0 - kb, N = nd.p - k0 - kb;
         const int64_t off = nd.foff + (k0 + kb) + (int64_t)k0 * n;
+        // in place (kb <= 64: one tile column); W is lower triangular
+        add_prob(st.panel, prob(kFront, off, n, kZ, nd.zoff + k0 + (int64_t)k0 * nd.p, nd.p, kFront, off, n, M, kb, kb,
+                                1.0, 0.0),
+                 false, 0.5);
         add_prob(st.update,
                  prob(kFront, off, n, kFront, off, n, kFront, nd.foff + (k0 + kb) + (int64_t)(k0 + kb) * n, n, M, N, kb,
                       -1.0, 1.0),
                  true);
       }
       st.npotrf = (int)pd.size();
-      st.npanel = (int)pn.size();
       int rc = upload(&st.d_potrf, pd);
       if (rc) return rc;
-      rc = upload(&st.d_panel, pn);
+      rc = finalize(st.panel);
       if (rc) return rc;
+      for (const GemmProb& P : st.panel.h_probs)
+        if (P.N > st.panel.tile) {  // the in-place solve needs one tile column
+          set_error("panel solve wider than a tile");
+          return SPDE_ERR_STATE;
+        }
       rc = finalize(st.update);
       if (rc) return rc;
       L.steps.push_back(std::move(st));
@@ -302,7 +309,9 @@ int build_factor_plan(spde_plan* pl) {
   const ArenaSizes as = arena_sizes(pl, 1);
   for (auto& L : pl->levels) {
     for (auto& st : L.steps) {
-      int rc = validate(st.update, as, "factor update");
+      int rc = validate(st.panel, as, "factor panel");
+      if (rc) return rc;
+      rc = validate(st.update, as, "factor update");
       if (rc) return rc;
     }
     int rc = validate(L.syrk, as, "factor syrk");
@@ -604,7 +613,7 @@ void spde_plan_destroy(spde_plan* pl) {
       }
       for (auto& st : L.steps) {
         hipFree(st.d_potrf);
-        hipFree(st.d_panel);
+        free_launch(st.panel);
         free_launch(st.update);
       }
       for (GemmLaunch* g : {&L.syrk, &L.fwd_y, &L.fwd_u, &L.bwd_t, &L.bwd_x, &L.si_x, &L.si_s21, &L.si_s11a,
@@ -704,7 +713,7 @@ int spde_factor_async(spde_plan* pl, int slot, const double* qval, const double*
     for (int c = 0; c < 2; ++c) launch_extend_add(L.d_ext[c], L.d_exttile[c], L.next[c], pl->d_relpos, sl.front, s);
     for (Step& st : L.steps) {
       launch_potrf(st.d_potrf, st.npotrf, sl.front, sl.z, sl.status, s);
-      launch_panel(st.d_panel, st.npanel, sl.front, sl.z, s);
+      run_gemm(pl, st.panel, b, s);
       run_gemm(pl, st.update, b, s);
     }
     run_gemm(pl, L.syrk, b, s);
