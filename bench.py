@@ -187,7 +187,7 @@ def main():
                    "path": "exact"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(),
-                     "kernel": "gemm_kernel (batched FP64 MFMA supernode updates)",
+                     "kernel": "gemm64_kernel + gemm128_kernel (batched FP64 MFMA tile-list GEMM)",
                      "launches_per_step": launches,
                      "achieved_while_busy": achieved_busy,
                      "gemm_avg_launch_ms": gms.value / max(nl.value, 1)},

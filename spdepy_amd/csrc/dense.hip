@@ -23,8 +23,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // column-major memory.  Accumulator layout of v_mfma_f64_16x16x4f64
 // (measured): lane l, register v holds D[l/16 + 4v][l%16].
 template <bool TA, bool TB, int BM, int BN>
-__global__ __launch_bounds__(256) void gemm_kernel(const GemmProb* __restrict__ probs,
-                                                   const GemmTile* __restrict__ tiles, Bases bases) {
+__device__ __forceinline__ void gemm_body(const GemmProb* __restrict__ probs, const GemmTile* __restrict__ tiles,
+                                          const Bases& bases) {
   constexpr int BK = 16;
   // LDS layout follows each operand's memory order so that both the global
   // loads and the LDS stores are contiguous: an operand stored with m (n)
@@ -136,25 +136,44 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmProb* __restrict__ 
       }
 }
 
-template <int T>
-static void launch_gemm_t(const GemmLaunch& g, const Bases& b, hipStream_t s) {
-  dim3 grid(g.ntiles), block(256);
-  if (!g.ta && !g.tb)
-    hipLaunchKernelGGL((gemm_kernel<false, false, T, T>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
-  else if (!g.ta && g.tb)
-    hipLaunchKernelGGL((gemm_kernel<false, true, T, T>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
-  else if (g.ta && !g.tb)
-    hipLaunchKernelGGL((gemm_kernel<true, false, T, T>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
-  else
-    hipLaunchKernelGGL((gemm_kernel<true, true, T, T>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+// 64x64 tiles: four blocks per CU (LDS 38 KB, ~120 registers).
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm64_kernel(const GemmProb* __restrict__ probs,
+                                                     const GemmTile* __restrict__ tiles, Bases bases) {
+  gemm_body<TA, TB, 64, 64>(probs, tiles, bases);
+}
+
+// 128x128 tiles (each wave 64x64: half the LDS reads per MFMA): two blocks
+// per CU (LDS 74 KB); the register budget is pinned to two waves per SIMD,
+// without it the compiler takes ~360 registers and one wave.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm128_kernel(
+    const GemmProb* __restrict__ probs, const GemmTile* __restrict__ tiles, Bases bases) {
+  gemm_body<TA, TB, 128, 128>(probs, tiles, bases);
 }
 
 void launch_gemm(const GemmLaunch& g, const Bases& b, hipStream_t s) {
   if (g.ntiles == 0) return;
-  if (g.tile == 128)
-    launch_gemm_t<128>(g, b, s);
-  else
-    launch_gemm_t<64>(g, b, s);
+  dim3 grid(g.ntiles), block(256);
+  if (g.tile == 128) {
+    if (!g.ta && !g.tb)
+      hipLaunchKernelGGL((gemm128_kernel<false, false>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+    else if (!g.ta && g.tb)
+      hipLaunchKernelGGL((gemm128_kernel<false, true>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+    else if (g.ta && !g.tb)
+      hipLaunchKernelGGL((gemm128_kernel<true, false>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+    else
+      hipLaunchKernelGGL((gemm128_kernel<true, true>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+  } else {
+    if (!g.ta && !g.tb)
+      hipLaunchKernelGGL((gemm64_kernel<false, false>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+    else if (!g.ta && g.tb)
+      hipLaunchKernelGGL((gemm64_kernel<false, true>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+    else if (g.ta && !g.tb)
+      hipLaunchKernelGGL((gemm64_kernel<true, false>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+    else
+      hipLaunchKernelGGL((gemm64_kernel<true, true>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+  }
 }
 
 // ------------------------------------------------ diagonal block Cholesky

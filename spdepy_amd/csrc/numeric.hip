@@ -138,14 +138,23 @@ void add_prob(GemmLaunch& g, const GemmProb& P, bool lower, double flop_scale = 
   g.flops += 2.0 * P.M * (double)P.N * P.K * (lower ? 0.5 : 1.0) * flop_scale;
 }
 
-// Tile edge of a launch: 128 when that still gives the chip >= 2 blocks per
-// CU, else 64.  Then the tile list (tiles strictly above the diagonal of a
-// lower problem are skipped).
+// Tile edge of a launch: 128 when the launch has no lower-triangular
+// problem, a mean depth K of at least 256 and >= 512 tiles of 128 (two
+// rounds of the chip at two blocks per CU); else 64 (short or triangular
+// problems keep more blocks in flight and waste less on the diagonal).
+// Then the tile list (tiles strictly above the diagonal of a lower problem
+// are skipped).
 void make_tiles(GemmLaunch& g, int force_tile) {
-  int64_t big = 0;
-  for (const GemmProb& P : g.h_probs) big += (int64_t)((P.M + 127) / 128) * ((P.N + 127) / 128);
-  (void)big;
-  g.tile = force_tile ? force_tile : 64;  // 64x64 tiles measured faster than 128x128 at every shape of this path
+  int64_t t128 = 0;
+  double ksum = 0.0;
+  bool any_lower = false;
+  for (const GemmProb& P : g.h_probs) {
+    t128 += (int64_t)((P.M + 127) / 128) * ((P.N + 127) / 128);
+    ksum += P.K;
+    any_lower = any_lower || P.lower;
+  }
+  const bool big = !any_lower && !g.h_probs.empty() && ksum / g.h_probs.size() >= 256.0 && t128 >= 512;
+  g.tile = force_tile ? force_tile : (big ? 128 : 64);
   g.h_tiles.clear();
   const int T = g.tile;
   for (size_t i = 0; i < g.h_probs.size(); ++i) {
