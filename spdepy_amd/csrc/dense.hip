@@ -120,6 +120,9 @@ __device__ __forceinline__ void gemm_body(const GemmProb* __restrict__ probs, co
     __syncthreads();
   }
   const double alpha = P.alpha, beta = P.beta;
+  // lower problems: the part of a diagonal tile above the diagonal is left
+  // untouched (it is never read, and stays as initialised)
+  const bool diag_tile = P.lower && t.tm == t.tn;
 #pragma unroll
   for (int ni = 0; ni < TN; ++ni)
 #pragma unroll
@@ -128,7 +131,7 @@ __device__ __forceinline__ void gemm_body(const GemmProb* __restrict__ probs, co
       for (int v = 0; v < 4; ++v) {
         const int m = m0 + wm + mi * 16 + lr;
         const int n = n0 + wn + ni * 16 + lk + 4 * v;
-        if (interior || (m < M && n < N)) {
+        if ((interior || (m < M && n < N)) && !(diag_tile && m < n)) {
           double* c = C + m + (int64_t)n * P.ldc;
           const double r = alpha * acc[ni][mi][v];
           *c = (beta == 0.0) ? r : r + beta * (*c);

@@ -46,6 +46,22 @@ __global__ __launch_bounds__(256) void extend_add_kernel(const ExtAddDesc* __res
   }
 }
 
+// Zero the lower triangle of a front, one 64x64 tile (node, ti, tj) per
+// block (tiles above the diagonal are not listed; the parts of the fronts
+// above the diagonal are never written after the arena's first clearing).
+__global__ __launch_bounds__(256) void zero_lower_kernel(const NodeDev* __restrict__ nodes, const int4* __restrict__ tiles,
+                                                         double* __restrict__ F) {
+  const int4 t = tiles[blockIdx.x];
+  const NodeDev nd = nodes[t.x];
+  double* f = F + nd.foff;
+  const int i = t.y * 64 + (threadIdx.x & 63);
+  if (i >= nd.n) return;
+  for (int jj = threadIdx.x >> 6; jj < 64; jj += 4) {
+    const int j = t.z * 64 + jj;
+    if (j <= i) f[i + (int64_t)j * nd.n] = 0.0;
+  }
+}
+
 // partial[k] = sum_j log L_jj over the own columns of node k (one block per node).
 __global__ __launch_bounds__(256) void logdet_partial_kernel(const NodeDev* __restrict__ nodes, const double* __restrict__ F,
                                                              double* __restrict__ partial) {
@@ -160,8 +176,10 @@ __global__ __launch_bounds__(256) void sigma_gather_kernel(const int32_t* __rest
   const double* src = S + pa.foff;
   const int64_t pn = pa.n;
   const int tid = threadIdx.x, u = tid & 63, w0 = tid >> 6;
-  // rows fastest: entries with rp[i] >= rp[j]
-  {
+  // relpos increases, so a tile below the diagonal (ti > tj) lies wholly in
+  // the parent's lower triangle, one above it wholly in the upper (mirrored)
+  // part; only diagonal tiles need both passes
+  if (tt.y >= tt.z) {  // rows fastest: entries with rp[i] >= rp[j]
     const int i = i0 + u;
     const int pi = (i < b) ? rp[i] : -1;
     for (int jj = w0; jj < 64; jj += 4) {
@@ -172,8 +190,7 @@ __global__ __launch_bounds__(256) void sigma_gather_kernel(const int32_t* __rest
       }
     }
   }
-  // columns fastest: entries with rp[i] < rp[j]
-  {
+  if (tt.y <= tt.z) {  // columns fastest: entries with rp[i] < rp[j]
     const int j = j0 + u;
     const int pj = (j < b) ? rp[j] : -1;
     for (int ii = w0; ii < 64; ii += 4) {
@@ -245,6 +262,9 @@ void launch_sigma_gather(const int32_t* lvl_nodes, const NodeDev* nodes, const i
                          const int4* tiles, int ntiles, hipStream_t s) {
   if (ntiles > 0)
     hipLaunchKernelGGL(sigma_gather_kernel, dim3(ntiles), dim3(256), 0, s, lvl_nodes, nodes, relpos, S, tiles);
+}
+void launch_zero_lower(const NodeDev* nodes, const int4* tiles, int ntiles, double* F, hipStream_t s) {
+  if (ntiles > 0) hipLaunchKernelGGL(zero_lower_kernel, dim3(ntiles), dim3(256), 0, s, nodes, tiles, F);
 }
 void launch_gather_pattern(const double* S, const int64_t* mapsym, int64_t nnz, double* out, hipStream_t s) {
   hipLaunchKernelGGL(gather_pattern_kernel, dim3(grid_for(nnz)), dim3(256), 0, s, S, mapsym, nnz, out);

@@ -32,6 +32,7 @@ void launch_bwd_gather(const int32_t* lvl_nodes, int nn, const NodeDev* nodes, c
 void launch_sigma_gather(const int32_t* lvl_nodes, const NodeDev* nodes, const int32_t* relpos, double* S,
                          const int4* tiles, int ntiles, hipStream_t s);
 void launch_gather_pattern(const double* S, const int64_t* mapsym, int64_t nnz, double* out, hipStream_t s);
+void launch_zero_lower(const NodeDev* nodes, const int4* tiles, int ntiles, double* F, hipStream_t s);
 
 static thread_local std::string g_err;
 void set_error(const std::string& m) { g_err = m; }
@@ -53,6 +54,8 @@ namespace {
 struct PlanExtra {            // kept beside the plan (not in the public struct)
   std::vector<LevelDev> lvl;
   int32_t* d_children = nullptr;
+  int4* d_ztiles = nullptr;    // lower 64x64 tiles of every front (node, ti, tj)
+  int nztiles = 0;
 };
 
 std::vector<std::pair<spde_plan*, PlanExtra*>> g_extra;
@@ -214,42 +217,67 @@ int build_factor_plan(spde_plan* pl) {
     }
     int pmax = 0;
     for (int id : L.nodes) pmax = std::max(pmax, sym.nodes[id].p);
-    for (int k0 = 0; k0 < pmax; k0 += NB) {
+    // Recursive column blocking of the nodes' own columns: rec(a, b) factors
+    // columns [a, b) as rec(a, m), one update of columns [m, b) (all rows
+    // below m) by columns [a, m) with depth m - a, rec(m, b); leaves are
+    // 64-column steps (diagonal-block Cholesky + panel solve).  Same flops
+    // as the right-looking sweep, but the trailing updates are few and deep
+    // instead of one depth-64 pass over the trailing matrix per step.
+    std::function<int(int, int)> rec = [&](int a, int b) -> int {
       Step st;
-      std::vector<PotrfDesc> pd;
       st.panel.ta = false;
       st.panel.tb = true;
       st.update.ta = false;
       st.update.tb = true;
+      if (b - a <= NB) {
+        const int k0 = a;
+        std::vector<PotrfDesc> pd;
+        for (int id : L.nodes) {
+          const Node& nd = sym.nodes[id];
+          if (nd.p <= k0) continue;
+          const int n = nd.n(), kb = std::min(NB, nd.p - k0);
+          pd.push_back({nd.foff, nd.zoff, n, nd.p, k0, kb, id, nd.start + k0});
+          const int M = n - k0 - kb;
+          const int64_t off = nd.foff + (k0 + kb) + (int64_t)k0 * n;
+          // in place (kb <= 64: one tile column); W is lower triangular
+          add_prob(st.panel,
+                   prob(kFront, off, n, kZ, nd.zoff + k0 + (int64_t)k0 * nd.p, nd.p, kFront, off, n, M, kb, kb, 1.0, 0.0),
+                   false, 0.5);
+        }
+        st.npotrf = (int)pd.size();
+        int rc = upload(&st.d_potrf, pd);
+        if (rc) return rc;
+        rc = finalize(st.panel);
+        if (rc) return rc;
+        for (const GemmProb& P : st.panel.h_probs)
+          if (P.N > st.panel.tile) {  // the in-place solve needs one tile column
+            set_error("panel solve wider than a tile");
+            return SPDE_ERR_STATE;
+          }
+        L.steps.push_back(std::move(st));
+        return SPDE_OK;
+      }
+      int half = (b - a + 1) / 2;
+      half = ((half + NB - 1) / NB) * NB;
+      const int m = a + half;
+      int rc = rec(a, m);
+      if (rc) return rc;
       for (int id : L.nodes) {
         const Node& nd = sym.nodes[id];
-        if (nd.p <= k0) continue;
-        const int n = nd.n(), kb = std::min(NB, nd.p - k0);
-        pd.push_back({nd.foff, nd.zoff, n, nd.p, k0, kb, id, nd.start + k0});
-        const int M = n - k0 - kb, N = nd.p - k0 - kb;
-        const int64_t off = nd.foff + (k0 + kb) + (int64_t)k0 * n;
-        // in place (kb <= 64: one tile column); W is lower triangular
-        add_prob(st.panel, prob(kFront, off, n, kZ, nd.zoff + k0 + (int64_t)k0 * nd.p, nd.p, kFront, off, n, M, kb, kb,
-                                1.0, 0.0),
-                 false, 0.5);
-        add_prob(st.update,
-                 prob(kFront, off, n, kFront, off, n, kFront, nd.foff + (k0 + kb) + (int64_t)(k0 + kb) * n, n, M, N, kb,
-                      -1.0, 1.0),
+        if (nd.p <= m) continue;
+        const int n = nd.n(), M = n - m, N = std::min(b, nd.p) - m, K = m - a;
+        const int64_t ab = nd.foff + m + (int64_t)a * n;
+        add_prob(st.update, prob(kFront, ab, n, kFront, ab, n, kFront, nd.foff + m + (int64_t)m * n, n, M, N, K, -1.0, 1.0),
                  true);
       }
-      st.npotrf = (int)pd.size();
-      int rc = upload(&st.d_potrf, pd);
-      if (rc) return rc;
-      rc = finalize(st.panel);
-      if (rc) return rc;
-      for (const GemmProb& P : st.panel.h_probs)
-        if (P.N > st.panel.tile) {  // the in-place solve needs one tile column
-          set_error("panel solve wider than a tile");
-          return SPDE_ERR_STATE;
-        }
       rc = finalize(st.update);
       if (rc) return rc;
       L.steps.push_back(std::move(st));
+      return rec(m, b);
+    };
+    if (pmax > 0) {
+      int rc = rec(0, pmax);
+      if (rc) return rc;
     }
     L.syrk.ta = false;
     L.syrk.tb = true;
@@ -560,6 +588,20 @@ int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int 
   TRY(upload(&pl->d_bnd, bnd));
   TRY(upload(&pl->d_nodes, nodes));
   TRY(upload(&ex->d_children, children));
+  {
+    std::vector<int4> zt;
+    for (size_t q = 0; q < sym.nodes.size(); ++q) {
+      const int nt = (sym.nodes[q].n() + 63) / 64;
+      for (int a = 0; a < nt; ++a)
+        for (int b = 0; b <= a; ++b) zt.push_back(int4{(int)q, a, b, 0});
+    }
+    if (zt.size() > (size_t)INT32_MAX) {
+      set_error("front tile list too long");
+      return SPDE_ERR_STATE;
+    }
+    ex->nztiles = (int)zt.size();
+    TRY(upload(&ex->d_ztiles, zt));
+  }
   ex->lvl.resize(sym.levels.size());
   for (size_t l = 0; l < sym.levels.size(); ++l) {
     ex->lvl[l].nn = (int)sym.levels[l].size();
@@ -574,6 +616,9 @@ int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int 
   pl->slots.resize(nslots);
   for (int s = 0; s < nslots; ++s) {
     SPDE_HIP(hipMalloc((void**)&pl->slots[s].front, sizeof(double) * std::max<int64_t>(sym.front_doubles, 1)));
+    // cleared once: later factorisations clear only the lower triangles,
+    // and nothing writes above a front's diagonal
+    SPDE_HIP(hipMemset(pl->slots[s].front, 0, sizeof(double) * std::max<int64_t>(sym.front_doubles, 1)));
     SPDE_HIP(hipMalloc((void**)&pl->slots[s].z, sizeof(double) * std::max<int64_t>(sym.z_doubles, 1)));
     SPDE_HIP(hipMemset(pl->slots[s].z, 0, sizeof(double) * std::max<int64_t>(sym.z_doubles, 1)));
     SPDE_HIP(hipMalloc((void**)&pl->slots[s].work, sizeof(double) * pl->work_doubles));
@@ -640,6 +685,7 @@ void spde_plan_destroy(spde_plan* pl) {
           hipFree(d.d_gtiles);
         }
         hipFree(ex->d_children);
+        hipFree(ex->d_ztiles);
         delete ex;
         g_extra.erase(g_extra.begin() + i);
         break;
@@ -712,15 +758,16 @@ int spde_factor_async(spde_plan* pl, int slot, const double* qval, const double*
   hipStream_t s = (hipStream_t)stream;
   Slot& sl = pl->slots[slot];
   const Symbolic& sym = pl->sym;
+  PlanExtra* ex = extra(pl);
   SPDE_HIP(hipMemsetAsync(sl.status, 0, sizeof(int), s));
-  launch_fill(sl.front, sym.front_doubles, 0.0, s);
+  launch_zero_lower(pl->d_nodes, ex->d_ztiles, ex->nztiles, sl.front, s);
   launch_scatter(qval, pl->d_map, (int64_t)pl->st.indices.size(), sl.front, s);
   if (diag_add) launch_diag_add(diag_add, pl->d_diag, pl->d_map, sym.n, sl.front, s);
   Bases b = make_bases(pl, slot, nullptr);
   for (size_t l = 0; l < pl->levels.size(); ++l) {
     LevelPlan& L = pl->levels[l];
     for (int c = 0; c < 2; ++c) launch_extend_add(L.d_ext[c], L.d_exttile[c], L.next[c], pl->d_relpos, sl.front, s);
-    for (Step& st : L.steps) {
+    for (Step& st : L.steps) {  // a leaf (potrf + panel) or an update
       launch_potrf(st.d_potrf, st.npotrf, sl.front, sl.z, sl.status, s);
       run_gemm(pl, st.panel, b, s);
       run_gemm(pl, st.update, b, s);

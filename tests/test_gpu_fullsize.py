@@ -91,4 +91,10 @@ def test_sampler_and_selected_inverse_identities(workload):
     indptr, _ = eng.plan.structure()
     rows = torch.as_tensor(np.repeat(np.arange(eng.n), np.diff(indptr))).cuda()
     diag = torch.zeros(eng.n, dtype=torch.float64, device="cuda").index_add_(0, rows, qval * sig)
-    assert float((diag - 1.0).abs().max()) < 1e-9
+    scale = torch.zeros(eng.n, dtype=torch.float64, device="cuda").index_add_(0, rows, (qval * sig).abs())
+    err = float((diag - 1.0).abs().max())
+    print("  max |sum_j Q_ij Sigma_ij - 1| = %.3e (row scale sum_j |Q_ij Sigma_ij| up to %.3e)"
+          % (err, float(scale.max())))
+    # FP64 round-off of the factor and of the recursion grows with the
+    # condition number of Q (a few 1e-9 here); a wrong term is an O(1) error
+    assert err < 1e-8
