@@ -109,6 +109,18 @@ typedef struct spde_coeffs {
 /* Values of Q on the structural pattern (qval[nnz_struct], device). */
 int spde_assemble(spde_plan* plan, const spde_coeffs* c, double* qval, void* stream);
 
+/* Spatial matrices of the closed-form log-determinant, on a one-slice plan
+ * (spde_plan_create with T = 1, same M, N, bc): which = 0 writes
+ * Q0 = A0^T A0 / V, which = 1 writes B^T B with B = A / V, both from the last
+ * spde_assemble on the space-time plan `st` (out[nnz_struct of sp], device).
+ * log|Q| = log|Q0| + (T-1) log|B^T B| + (T-1) sum_k log(q_k c), c = 1/(dt
+ * sigma), q the diagonal of Qs: the reference factorises Q itself
+ * (advection_idiffusion2D.py:173, var_advection_var_diffusion2D.py:205); the
+ * two give the same value (Q = F^T D F, F block bidiagonal). */
+int spde_assemble_spatial(spde_plan* st, spde_plan* sp, int which, double* out, void* stream);
+/* *out (host) = sum_k log(q_k c) for the last spde_assemble on the plan. */
+int spde_logq_sum(spde_plan* plan, double c, double* out, void* stream);
+
 /* Drop exact zeros: the reference's scipy pattern.  indptr[n+1] (int64),
  * indices[nnz_struct], data[nnz_struct] device; *nnz (host) receives the kept
  * count. */
@@ -174,6 +186,13 @@ typedef struct spde_grad_args {
   const double* mu;
   const double* diag_add;
   spde_coeffs coeffs;
+  /* Closed-form log|Q| (mode 0 with sigma_q == NULL): the Sigma_Q term of G
+   * is replaced by the derivative of log|Q| = log|Q0| + (T-1) log|B^T B| +
+   * (T-1) sum_k log(q_k c), from the selected inverses of the spatial
+   * matrices (spde_assemble_spatial on the one-slice plan `spatial`). */
+  const double* sigma_gram;
+  const double* sigma_q0;
+  spde_plan* spatial;
 } spde_grad_args;
 
 int spde_grad_cells(spde_plan* plan, const spde_grad_args* args, double* out, void* stream);

@@ -48,7 +48,8 @@ class Coeffs(ctypes.Structure):
 class GradArgs(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("r", ctypes.c_int), ("nh1", ctypes.c_int), ("qval", vp),
                 ("sigma_q", vp), ("sigma_qc", vp), ("trq", vp), ("trqc", vp), ("probes", vp),
-                ("mu", vp), ("diag_add", vp), ("coeffs", Coeffs)]
+                ("mu", vp), ("diag_add", vp), ("coeffs", Coeffs),
+                ("sigma_gram", vp), ("sigma_q0", vp), ("spatial", vp)]
 
 
 _SIGS = {
@@ -62,6 +63,8 @@ _SIGS = {
     "spde_plan_get_info": (ctypes.c_int, [vp, ctypes.POINTER(PlanInfo)]),
     "spde_plan_export": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "spde_assemble": (ctypes.c_int, [vp, ctypes.POINTER(Coeffs), vp, vp]),
+    "spde_assemble_spatial": (ctypes.c_int, [vp, vp, ctypes.c_int, vp, vp]),
+    "spde_logq_sum": (ctypes.c_int, [vp, ctypes.c_double, c_dp, vp]),
     "spde_prune": (ctypes.c_int, [vp, vp, vp, vp, vp, c_i64p, vp]),
     "spde_spmv": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int, vp]),
     "spde_factor": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, c_dp, vp]),

@@ -123,18 +123,22 @@ __global__ void canonical_kernel(int M, int N, int bc, CoeffArgs a, double* __re
   for (int o = 0; o < 9; ++o) a0can[9 * (int64_t)k + o] = (o == 4) ? V * a.kappa0 - a0[o] : -a0[o];
 }
 
-// Template layout per cell (93 doubles): T0[25] | Tmid[25] | Tlast[25] | Tup[9] | Tlo[9]
+// Template layout per cell (kTmpl = 143 doubles): T0[25] | Tmid[25] | Tlast[25]
+// | Tup[9] | Tlo[9] | Gram[25] | Q0[25].  Gram = B^T B with B = A / V and
+// Q0 = A0^T A0 / V are the spatial matrices of the closed-form log|Q|
+// (Q = F^T D F with F block bidiagonal: log|Q| = log|Q0| + (T-1) log|B^T B|
+// + (T-1) sum_k log(q_k c)).
 __global__ void template_kernel(int M, int N, int bc, double V, double c, const double* __restrict__ acan,
                                 const double* __restrict__ a0can, const double* __restrict__ q,
                                 double* __restrict__ tmpl) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= M * N) return;
   const int i = k % M, j = k / M;
-  double* T = tmpl + 93 * (int64_t)k;
+  double* T = tmpl + kTmpl * (int64_t)k;
   const double iV = 1.0 / V;
   for (int wy = -2; wy <= 2; ++wy)
     for (int wx = -2; wx <= 2; ++wx) {
-      double x = 0.0, x0 = 0.0;
+      double x = 0.0, x0 = 0.0, xg = 0.0;
       for (int d1y = -1; d1y <= 1; ++d1y)
         for (int d1x = -1; d1x <= 1; ++d1x) {
           const int d2x = wx + d1x, d2y = wy + d1y;
@@ -146,12 +150,15 @@ __global__ void template_kernel(int M, int N, int bc, double V, double c, const 
           const double* bm = a0can + 9 * (int64_t)m;
           x += ((am[o1] * iV) * q[m]) * iV * am[o2];
           x0 += (bm[o1] * iV) * bm[o2];
+          xg += (am[o1] * iV) * (am[o2] * iV);
         }
       const int w = (wy + 2) * 5 + wx + 2;
       const double diag = (wx == 0 && wy == 0) ? q[k] : 0.0;
       T[w] = x0 + c * diag;
       T[25 + w] = c * (x + diag);
       T[50 + w] = c * x;
+      T[93 + w] = xg;
+      T[118 + w] = x0;
     }
   for (int dy = -1; dy <= 1; ++dy)
     for (int dx = -1; dx <= 1; ++dx) {
@@ -185,7 +192,7 @@ __global__ __launch_bounds__(256) void expand_kernel(int64_t n, int64_t Ns, int 
     const int64_t cell = r % Ns;
     const int t = (int)(r / Ns);
     const int cd = code[e];
-    const double* Tc = tmpl + 93 * cell;
+    const double* Tc = tmpl + kTmpl * cell;
     double v;
     if (cd < 25)
       v = (t == 0) ? Tc[cd] : ((t == T - 1) ? Tc[50 + cd] : Tc[25 + cd]);
@@ -195,6 +202,31 @@ __global__ __launch_bounds__(256) void expand_kernel(int64_t n, int64_t Ns, int 
       v = Tc[75 + (cd - 34)];
     qval[e] = v;
   }
+}
+
+// Spatial matrix (Gram or Q0 block of the templates) onto the structural
+// CSR of a one-slice plan (row = cell, code = 5x5 offset).
+__global__ void spatial_expand_kernel(int64_t Ns, const int64_t* __restrict__ indptr, const uint8_t* __restrict__ code,
+                                      const double* __restrict__ tmpl, int off, double* __restrict__ out) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= Ns) return;
+  const double* Tc = tmpl + kTmpl * r + off;
+  for (int64_t e = indptr[r]; e < indptr[r + 1]; ++e) out[e] = Tc[code[e]];
+}
+
+// out[0] = sum_k log(q_k c), one block, fixed order (deterministic).
+__global__ __launch_bounds__(1024) void logq_sum_kernel(int64_t Ns, const double* __restrict__ q, double c,
+                                                        double* __restrict__ out) {
+  __shared__ double red[1024];
+  double acc = 0.0;
+  for (int64_t k = threadIdx.x; k < Ns; k += 1024) acc += log(q[k] * c);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
 }
 
 // --------------------------------------------------------- prune and SpMV
@@ -326,6 +358,36 @@ int spde_assemble(spde_plan* pl, const spde_coeffs* c, double* qval, void* strea
   hipLaunchKernelGGL(expand_kernel, dim3(blocks_for(n, 128)), dim3(256), 0, s, n, Ns, pl->lat.T, pl->d_indptr,
                      pl->d_code, pl->d_tmpl, qval);
   SPDE_HIP(hipGetLastError());
+  return SPDE_OK;
+}
+
+int spde_assemble_spatial(spde_plan* st, spde_plan* sp, int which, double* out, void* stream) {
+  if (!st || st->host_only || !sp || sp->host_only || !out || which < 0 || which > 1) {
+    set_error("spde_assemble_spatial: invalid argument");
+    return SPDE_ERR_ARG;
+  }
+  if (sp->lat.T != 1 || sp->lat.M != st->lat.M || sp->lat.N != st->lat.N || sp->lat.bc != st->lat.bc) {
+    set_error("spde_assemble_spatial: the spatial plan must be the one-slice lattice of the space-time plan");
+    return SPDE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t Ns = sp->lat.Ns();
+  hipLaunchKernelGGL(spatial_expand_kernel, dim3(blocks_for(Ns, 128)), dim3(128), 0, s, Ns, sp->d_indptr, sp->d_code,
+                     st->d_tmpl, which == 0 ? 118 : 93, out);
+  SPDE_HIP(hipGetLastError());
+  return SPDE_OK;
+}
+
+int spde_logq_sum(spde_plan* pl, double c, double* out, void* stream) {
+  if (!pl || pl->host_only || !out || !(c > 0.0)) {
+    set_error("spde_logq_sum: invalid argument");
+    return SPDE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(logq_sum_kernel, dim3(1), dim3(1024), 0, s, pl->lat.Ns(), pl->d_qdiag, c, pl->d_red);
+  SPDE_HIP(hipGetLastError());
+  SPDE_HIP(hipMemcpyAsync(out, pl->d_red, sizeof(double), hipMemcpyDeviceToHost, s));
+  SPDE_HIP(hipStreamSynchronize(s));
   return SPDE_OK;
 }
 

@@ -42,6 +42,12 @@ struct GradKArgs {
   const double* acan;
   const double* a0can;
   const double* qdiag;
+  // closed-form log|Q| (mode 0 without Sigma_Q): selected inverses of the
+  // spatial matrices B^T B and Q0 on the one-slice pattern
+  int closed;
+  const double* sgram;
+  const double* sq0;
+  const int32_t* pos25;
 };
 
 constexpr int kWin = 60;  // Gd[25] | Gu[9] | G00[25] | Gq
@@ -63,7 +69,7 @@ __global__ void grad_reduce_kernel(GradKArgs a, double* __restrict__ win, double
       double Ge;
       double tr_c = 0.0;
       if (a.mode == 0) {
-        Ge = 0.5 * a.r * (a.sq[e] - a.sqc[e]);
+        Ge = 0.5 * a.r * ((a.closed ? 0.0 : a.sq[e]) - a.sqc[e]);
         if (cd == 12) tr_c = a.sqc[e];
       } else {
         double s = 0.0;
@@ -134,7 +140,30 @@ __global__ void grad_cell_kernel(GradKArgs a, const double* __restrict__ win, do
     qbar_quad += iV * iV * bg * Am[o];
     qbar_u += iV * gu * Am[o];
   }
-  const double qbar = a.c * (qbar_quad + win[mlin * kWin + 59] - 2.0 * qbar_u);
+  double qbar = a.c * (qbar_quad + win[mlin * kWin + 59] - 2.0 * qbar_u);
+  if (a.closed) {
+    // 1/2 r d log|Q| with log|Q| = log|Q0| + (T-1) log|B^T B| + (T-1) sum log(q c):
+    // d log|B^T B| / dA(m,k) = 2 (A^-T)(m,k) = 2 (A Sigma_B)(m,k) / V^2,
+    // d log|Q0| / dA0(m,k) = 2 (A0 Sigma_0)(m,k) / V, d/dq_m = (T-1) / q_m.
+    const double st = (double)a.r * (a.T - 1);
+    for (int o = 0; o < 9; ++o) {
+      const int dox = o % 3 - 1, doy = o / 3 - 1;
+      if (nbr_cell(i, j, dox, doy, M, N, bc) < 0) continue;
+      double rt = 0.0, r0 = 0.0;
+      for (int o1 = 0; o1 < 9; ++o1) {
+        const int d1x = o1 % 3 - 1, d1y = o1 / 3 - 1;
+        const int j1 = nbr_cell(i, j, d1x, d1y, M, N, bc);
+        if (j1 < 0) continue;
+        const int e = a.pos25[(int64_t)j1 * 25 + (doy - d1y + 2) * 5 + (dox - d1x + 2)];
+        if (e < 0) continue;
+        rt += Am[o1] * a.sgram[e];
+        r0 += A0m[o1] * a.sq0[e];
+      }
+      abar[o] += st * rt * iV * iV;
+      a0bar[o] += a.r * r0 * iV;
+    }
+    qbar += 0.5 * st / qm;
+  }
   double* o = out + mlin * kOut;
   for (int q = 0; q < 36; ++q) o[q] = 0.0;
   double canv[9];
@@ -208,9 +237,19 @@ extern "C" int spde_grad_cells(spde_plan* pl, const spde_grad_args* g, double* o
     set_error("spde_grad_cells: invalid argument");
     return SPDE_ERR_ARG;
   }
-  if (g->mode == 0 && (!g->sigma_q || !g->sigma_qc)) {
-    set_error("spde_grad_cells: exact mode needs both selected inverses");
+  const bool closed = g->mode == 0 && !g->sigma_q;
+  if (g->mode == 0 && !g->sigma_qc) {
+    set_error("spde_grad_cells: exact mode needs the selected inverse of Q_c");
     return SPDE_ERR_ARG;
+  }
+  if (closed) {
+    const spde_plan* sp = g->spatial;
+    if (!g->sigma_gram || !g->sigma_q0 || !sp || sp->host_only || sp->lat.T != 1 || sp->lat.M != pl->lat.M ||
+        sp->lat.N != pl->lat.N || sp->lat.bc != pl->lat.bc || !sp->d_pos25) {
+      set_error("spde_grad_cells: without sigma_q, the spatial plan and the selected inverses of B^T B and Q0 "
+                "are needed");
+      return SPDE_ERR_ARG;
+    }
   }
   if (g->mode == 1 && (!g->trq || !g->trqc || !g->probes || g->nh1 < 1)) {
     set_error("spde_grad_cells: Hutchinson mode needs trq, trqc and probes");
@@ -259,6 +298,10 @@ extern "C" int spde_grad_cells(spde_plan* pl, const spde_grad_args* g, double* o
   a.acan = pl->d_acan;
   a.a0can = pl->d_a0can;
   a.qdiag = pl->d_qdiag;
+  a.closed = closed ? 1 : 0;
+  a.sgram = closed ? g->sigma_gram : nullptr;
+  a.sq0 = closed ? g->sigma_q0 : nullptr;
+  a.pos25 = closed ? g->spatial->d_pos25 : nullptr;
   double* win = pl->d_win;
   const unsigned blocks = (unsigned)((a.Ns + 127) / 128);
   hipLaunchKernelGGL(grad_reduce_kernel, dim3(blocks), dim3(128), 0, s, a, win, out);

@@ -23,6 +23,7 @@ void set_error(const std::string& msg);
   } while (0)
 
 constexpr int NB = 64;  // diagonal block width of the dense partial factorisation
+constexpr int kTmpl = 143;  // per-cell assembly template (doubles)
 
 // Operand base pointers of a batched launch (selected per operand by id).
 enum Base : int8_t { kFront = 0, kZ = 1, kSigma = 2, kWork = 3, kVec = 4, kX = 5, kNumBases = 6 };
@@ -138,6 +139,8 @@ struct spde_plan {
   double* d_a0can = nullptr;      // canonical 9-point initial-state operator A0
   double* d_qdiag = nullptr;      // diagonal of Qs per cell
   double* d_win = nullptr;        // per-cell gradient windows
+  double* d_red = nullptr;        // one device scalar (reductions)
+  int32_t* d_pos25 = nullptr;     // one-slice plans: entry index of (cell, 5x5 code), -1 if absent
   int64_t work_doubles = 0;       // per-slot workspace size
   std::vector<spde::Slot> slots;
   std::vector<spde::LevelPlan> levels;

@@ -516,7 +516,7 @@ extern "C" {
 const char* spde_last_error(void) { return g_err.c_str(); }
 
 int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int flags, spde_plan** out) {
-  if (!out || bc < 1 || bc > 3 || M < 2 || N < 2 || T < 2 || nslots < 0) {
+  if (!out || bc < 1 || bc > 3 || M < 2 || N < 2 || T < 1 || nslots < 0) {
     set_error("spde_plan_create: invalid argument");
     return SPDE_ERR_ARG;
   }
@@ -609,7 +609,15 @@ int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int 
   }
   TRY(build_factor_plan(pl));
   const int64_t Ns = pl->lat.Ns();
-  SPDE_HIP(hipMalloc((void**)&pl->d_tmpl, sizeof(double) * Ns * 93));
+  SPDE_HIP(hipMalloc((void**)&pl->d_tmpl, sizeof(double) * Ns * kTmpl));
+  SPDE_HIP(hipMalloc((void**)&pl->d_red, sizeof(double)));
+  if (pl->lat.T == 1) {
+    std::vector<int32_t> pos((size_t)Ns * 25, -1);
+    for (int64_t r = 0; r < Ns; ++r)
+      for (int64_t e = pl->st.indptr[r]; e < pl->st.indptr[r + 1]; ++e)
+        if (pl->st.code[e] < 25) pos[(size_t)r * 25 + pl->st.code[e]] = (int32_t)e;
+    TRY(upload(&pl->d_pos25, pos));
+  }
   SPDE_HIP(hipMalloc((void**)&pl->d_acan, sizeof(double) * Ns * 9));
   SPDE_HIP(hipMalloc((void**)&pl->d_a0can, sizeof(double) * Ns * 9));
   SPDE_HIP(hipMalloc((void**)&pl->d_qdiag, sizeof(double) * Ns));
@@ -647,7 +655,7 @@ void spde_plan_destroy(spde_plan* pl) {
     for (auto* p : {(void*)pl->d_indptr, (void*)pl->d_indices, (void*)pl->d_code, (void*)pl->d_diag,
                     (void*)pl->d_map, (void*)pl->d_mapsym, (void*)pl->d_perm, (void*)pl->d_relpos,
                     (void*)pl->d_bnd, (void*)pl->d_nodes, (void*)pl->d_tmpl, (void*)pl->d_acan,
-                    (void*)pl->d_a0can, (void*)pl->d_qdiag, (void*)pl->d_win})
+                    (void*)pl->d_a0can, (void*)pl->d_qdiag, (void*)pl->d_win, (void*)pl->d_red, (void*)pl->d_pos25})
       hipFree(p);
     for (auto& s : pl->slots) {
       hipFree(s.front);

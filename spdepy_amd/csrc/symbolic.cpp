@@ -15,7 +15,7 @@
 namespace spde {
 
 void build_structure(const Lattice& lat, Structure& s) {
-  if (lat.M < 2 || lat.N < 2 || lat.T < 2) throw std::runtime_error("lattice must be at least 2x2x2");
+  if (lat.M < 2 || lat.N < 2 || lat.T < 1) throw std::runtime_error("lattice must be at least 2x2x1");
   if (lat.bc == 2 && (lat.M < 5 || lat.N < 5))
     throw std::runtime_error("periodic lattice needs M, N >= 5");
   const int64_t Ns = lat.Ns(), n = lat.n();

@@ -62,6 +62,16 @@ class Engine:
         self.grid, self.bc = grid, bc
         self.plan = get_plan(M, N, T, bc, leaf_max)
         self.n, self.Ns, self.nnz = grid.n, grid.Ns, self.plan.nnz
+        self._spatial = None
+
+    @property
+    def spatial(self):
+        """One-slice plan of the lattice (slots: 0 Q0, 1 B^T B) for the
+        closed-form log|Q|."""
+        if self._spatial is None:
+            M, N, _ = self.plan.shape
+            self._spatial = get_plan(M, N, 1, self.bc, nslots=2)
+        return self._spatial
 
     # -------------------------------------------------------------- assembly
     def assemble(self, coeffs):
@@ -86,40 +96,63 @@ class Engine:
         return self._side
 
     # ------------------------------------------------------- log-likelihood
-    def loglik(self, coeffs, obs_cols, data, log_tau, grad=True, trace="exact", probes=None):
+    def loglik(self, coeffs, obs_cols, data, log_tau, grad=True, trace="exact", probes=None, logdet="closed"):
         """Returns (like, parts) where like is the normalised -loglik/(m r) and
-        parts holds the per-cell gradient adjoints and the scalar pieces."""
+        parts holds the per-cell gradient adjoints and the scalar pieces.
+
+        logdet="closed" (exact trace only): log|Q| and its gradient come from
+        Q = F^T D F (F block bidiagonal in time): log|Q| = log|Q0| + (T-1)
+        log|B^T B| + (T-1) sum_k log(q_k c), B = A / V, with the spatial
+        matrices factorised and selectively inverted on the one-slice plan;
+        only Q_c is factorised at full size.  logdet="factor" factorises Q
+        as the reference does (advection_idiffusion2D.py:173)."""
         torch = _torch()
         y = np.asarray(data, dtype=np.float64)
         y2 = y.reshape(y.shape[0], -1)
         m, r = y2.shape
         tau = float(np.exp(log_tau))
         n = self.n
+        exact = trace == "exact"
+        closed = logdet == "closed" and (exact or not grad)
         qval = self.assemble(coeffs)
         counts = np.bincount(obs_cols, minlength=n).astype(np.float64)
         diag_add = torch.as_tensor(tau * counts).cuda()
         b = np.zeros((n, r))
         np.add.at(b, obs_cols, y2)
         b *= tau
-        # The factorisations (and selected inverses) of Q and Q_c are
-        # independent: Q runs on a side stream while Q_c, mu_c and the data
-        # terms run on the current one.
+        # The log|Q| side (full factorisation of Q, or the spatial matrices)
+        # is independent of Q_c: it runs on a side stream while Q_c, mu_c and
+        # the data terms run on the current one.
         main = torch.cuda.current_stream()
         side = self._side_stream(torch)
         ready = torch.cuda.Event()
         ready.record(main)
         side.wait_event(ready)
-        exact = grad and trace == "exact"
-        if exact:
-            sq = torch.empty(self.plan.nnz, dtype=torch.float64, device="cuda")
+        sel = grad and exact
+        sp = self.spatial if closed else None
+        if sel:
             sqc = torch.empty(self.plan.nnz, dtype=torch.float64, device="cuda")
+            if closed:
+                s_q0 = torch.empty(sp.nnz, dtype=torch.float64, device="cuda")
+                s_gram = torch.empty(sp.nnz, dtype=torch.float64, device="cuda")
+            else:
+                sq = torch.empty(self.plan.nnz, dtype=torch.float64, device="cuda")
         with torch.cuda.stream(side):
-            self.plan.factor_async(0, qval)
-            if exact:
-                self.plan.selinv(0, sq)
+            if closed:
+                q0 = sp.assemble_spatial(self.plan, 0)
+                gram = sp.assemble_spatial(self.plan, 1)
+                sp.factor_async(0, q0, owner=self)
+                sp.factor_async(1, gram, owner=self)
+                if sel:
+                    sp.selinv(0, s_q0)
+                    sp.selinv(1, s_gram)
+            else:
+                self.plan.factor_async(0, qval)
+                if sel:
+                    self.plan.selinv(0, sq)
         self.plan.factor_async(1, qval, diag_add)
         mu = self.plan.solve(1, torch.as_tensor(b).cuda())
-        if exact:
+        if sel:
             self.plan.selinv(1, sqc)
         qmu = self.plan.spmv(qval, mu)
         cols_t = torch.as_tensor(obs_cols).cuda()
@@ -128,7 +161,14 @@ class Engine:
         done.record(side)
         main.wait_event(done)
         ld_qc = self.plan.factor_finish(1)
-        ld_q = self.plan.factor_finish(0)
+        T = self.plan.shape[2]
+        if closed:
+            ld_q0 = sp.factor_finish(0)
+            ld_gram = sp.factor_finish(1)
+            logq = self.plan.logq_sum(1.0 / (coeffs.c.dt * coeffs.c.sigma))
+            ld_q = ld_q0 + (T - 1) * (ld_gram + logq)
+        else:
+            ld_q = self.plan.factor_finish(0)
         quad = float((mu * qmu).sum())
         rss = float((resid * resid).sum())
         like = 0.5 * ld_q * r + m * r * np.log(tau) / 2 - 0.5 * ld_qc * r - 0.5 * quad - tau / 2 * rss
@@ -145,8 +185,15 @@ class Engine:
         keep = [mu_cm]
         if exact:
             g.mode, g.nh1 = 0, 0
-            g.sigma_q, g.sigma_qc = sq.data_ptr(), sqc.data_ptr()
-            keep += [sq, sqc]
+            g.sigma_qc = sqc.data_ptr()
+            keep.append(sqc)
+            if closed:
+                g.sigma_q = None
+                g.sigma_gram, g.sigma_q0, g.spatial = s_gram.data_ptr(), s_q0.data_ptr(), sp._h
+                keep += [s_gram, s_q0]
+            else:
+                g.sigma_q = sq.data_ptr()
+                keep.append(sq)
         else:
             V = torch.as_tensor(np.ascontiguousarray(probes, dtype=np.float64)).cuda()
             trq = self.plan.solve(0, V).t().contiguous()
@@ -160,6 +207,9 @@ class Engine:
         cells = out.cpu().numpy().reshape(self.Ns, OUT_WIDTH)
         parts["cells"] = cells
         parts["g_sigma"] = -(cells[:, OUT_DOTGQ].sum() - cells[:, OUT_DOT00].sum())
+        if exact and closed:
+            # 1/2 r d log|Q| / d log sigma = -1/2 r (T-1) Ns
+            parts["g_sigma"] -= 0.5 * r * (T - 1) * self.plan.shape[0] * self.plan.shape[1]
         parts["g_tau"] = m * r / 2 - 0.5 * r * cells[:, OUT_TAUTR].sum() - tau / 2 * rss
         del keep
         return -like / (m * r), parts

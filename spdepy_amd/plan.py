@@ -107,6 +107,22 @@ class Plan:
                                         _stream(torch)))
         return ld.value
 
+    def assemble_spatial(self, st_plan, which, out=None):
+        """One-slice plan: Q0 (which=0) or B^T B (which=1) of the last
+        assembly on the space-time plan `st_plan`."""
+        import torch
+        if out is None:
+            out = torch.empty(self.nnz, dtype=torch.float64, device="cuda")
+        nat.check(nat.lib().spde_assemble_spatial(st_plan._h, self._h, which, _ptr(out), _stream(torch)))
+        return out
+
+    def logq_sum(self, c):
+        """sum_k log(q_k c) for the last assembly (synchronises the stream)."""
+        import torch
+        v = ctypes.c_double()
+        nat.check(nat.lib().spde_logq_sum(self._h, float(c), ctypes.byref(v), _stream(torch)))
+        return v.value
+
     def factor_async(self, slot, qval, diag_add=None, owner=None):
         """Enqueues the factorisation on the current stream; factor_finish
         waits for it and returns the log-determinant."""

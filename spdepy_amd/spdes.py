@@ -320,7 +320,12 @@ class _SpaceTimeBase(_StencilMixin):
             raise ValueError("observation matrix must have one entry per row")
         return S.indices.astype(np.int64)
 
-    def logLike(self, par, nh1=100, grad=True, trace="exact"):
+    def logLike(self, par, nh1=100, grad=True, trace="exact", logdet="closed"):
+        """The reference's logLike (advection_idiffusion2D.py:167-207,
+        var_advection_var_diffusion2D.py:201-237).  trace="exact" uses the
+        selected inverse of Q_c; logdet="closed" takes log|Q| and its
+        gradient from the spatial matrices of Q = F^T D F, logdet="factor"
+        factorises Q as the reference does (same values to rounding)."""
         par = np.asarray(par, dtype=np.float64)
         eng = self._engine()
         coeffs = self._coeffs(par)
@@ -328,7 +333,7 @@ class _SpaceTimeBase(_StencilMixin):
         if grad and trace == "hutchinson":
             probes = (2 * np.random.randint(1, 3, self.grid.n * nh1) - 3).reshape(self.grid.n, nh1)
         like, parts = eng.loglik(coeffs, self._obs_cols(), self.data, par[-1], grad=grad, trace=trace,
-                                 probes=probes)
+                                 probes=probes, logdet=logdet)
         if not grad:
             return like
         g = self._contract(par, coeffs, parts)

@@ -119,6 +119,20 @@ def test_loglike_exact_gradient(case):
     assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * scale), np.abs(jac - jac_o).max() / scale
 
 
+@pytest.mark.parametrize("case", ["var_b1", "ani_b3", "iso_b1"])
+def test_closed_form_and_factorised_logdet_agree(case):
+    """The two routes to log|Q| and its gradient (spatial matrices of
+    Q = F^T D F, or the full factorisation of Q) give the same result."""
+    c = CASES[case]()
+    m = product_case(c)
+    m.initFit(c["y"], idx=c["idx"], fitQ0=False)
+    step("logLike closed vs factor " + case)
+    l1, j1 = m.logLike(c["par"], logdet="closed")
+    l2, j2 = m.logLike(c["par"], logdet="factor")
+    assert abs(l1 - l2) <= 1e-11 * abs(l2)
+    assert np.allclose(j1, j2, rtol=1e-10, atol=1e-10 * np.abs(j2).max())
+
+
 def test_loglike_hutchinson_same_probes():
     c = CASES["var_b1"]()
     g = c["grid"]

@@ -63,6 +63,21 @@ def test_gradient_central_differences(workload):
     assert not bad, bad
 
 
+def test_closed_form_route_matches_full_factorisation(workload):
+    """log|Q| and its gradient from the spatial matrices of Q = F^T D F agree
+    with the full factorisation and selected inverse of Q at 100x100x20."""
+    mod, par = workload
+    step("logLike closed vs factor at 100x100x20")
+    l1, j1 = mod.logLike(par, logdet="closed")
+    l2, j2 = mod.logLike(par, logdet="factor")
+    print("  like closed %.15e factor %.15e; max grad diff / scale %.2e"
+          % (l1, l2, np.abs(j1 - j2).max() / np.abs(j2).max()))
+    assert abs(l1 - l2) <= 1e-11 * abs(l2)
+    # the factorised route's gradient subtracts two full-size traces whose
+    # selected inverses carry ~1e-9 round-off (see the identity test below)
+    assert np.allclose(j1, j2, rtol=1e-8, atol=1e-8 * np.abs(j2).max())
+
+
 def test_evaluation_is_deterministic(workload):
     mod, par = workload
     step("two identical evaluations")

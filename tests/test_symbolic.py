@@ -123,6 +123,29 @@ def test_plan_reproduces_dense_factor_and_inverse(M, N, T, bc, leaf):
     assert np.allclose(got, want, rtol=1e-9, atol=1e-12 * np.abs(inv).max())
 
 
+@pytest.mark.parametrize("M,N,bc,leaf", [(7, 6, 1, 16), (9, 8, 3, 20), (8, 8, 2, 12), (12, 10, 1, 40)])
+def test_one_slice_plan_reproduces_dense_factor_and_inverse(M, N, bc, leaf):
+    """One-slice lattices (the spatial matrices of the closed-form
+    log-determinant): 25-point pattern, random SPD values on it."""
+    pl = Plan(M, N, 1, bc, leaf_max=leaf, nslots=0, host_only=True)
+    e = pl.export()
+    n = M * N
+    assert np.all(e["code"] < 25)
+    S = sparse.csr_matrix((np.ones(len(e["indices"])), e["indices"], e["indptr"]), shape=(n, n))
+    R = sparse.random(n, n, density=1.0, random_state=3).multiply(S)
+    A = sparse.csr_matrix((R @ R.T).multiply(S) + sparse.eye(n) * 50.0)
+    logdet, Sig = _mf_factor(e, A)
+    dense = A.toarray()
+    sign, ref = np.linalg.slogdet(dense)
+    assert sign > 0 and abs(logdet - ref) <= 1e-10 * abs(ref)
+    inv = np.linalg.inv(dense)
+    nodes = e["nodes"]
+    flat = np.concatenate([Sig[k].ravel(order="F") for k in range(nodes.shape[0])])
+    rows = np.repeat(np.arange(n), np.diff(e["indptr"]))
+    m = e["map"] >= 0
+    assert np.allclose(flat[e["map"][m]], inv[rows[m], e["indices"][m]], rtol=1e-9, atol=1e-12 * np.abs(inv).max())
+
+
 def test_config2_plan_statistics():
     """100x100x20 lattice: the analysis the GPU factorisation is sized from."""
     pl = Plan(100, 100, 20, 1, leaf_max=128, nslots=0, host_only=True)
