@@ -100,13 +100,14 @@ def cpu_baseline():
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gemm_fp64_100x100x20_hbm_traffic.txt")
 
 
-def measured_traffic():
-    """HBM bytes per GEMM launch from the committed counter summary
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
     (FETCH_SIZE and WRITE_SIZE runs, scaled by a plain copy's readings)."""
+    key = "traffic_bytes_per_launch " + kernel.replace(" ", "") + " ="
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch"):
+                if line.replace(" ", "").startswith(key.replace(" ", "")):
                     return float(line.split("=")[1])
     except OSError:
         pass
@@ -144,7 +145,9 @@ def main():
     for _ in range(args.warmup):
         mod.logLike(par)
     torch.cuda.synchronize()
-    nat.check(nat.lib().spde_prof_begin(plan._h))
+    plans = [plan, eng.spatial]   # space-time plan (Q_c) and one-slice plan (B^T B, Q0)
+    for p in plans:
+        nat.check(nat.lib().spde_prof_begin(p._h))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -152,24 +155,44 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     import ctypes
-    gms, gfl, nl, busy = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
-    nat.check(nat.lib().spde_prof_end(plan._h, ctypes.byref(gms), ctypes.byref(gfl), ctypes.byref(nl),
-                                      ctypes.byref(busy)))
+    # per-launch records of both plans: (kind, ms, flops, bytes)
+    recs = []
+    for p in plans:
+        cap = 1 << 20
+        ms = np.zeros(cap)
+        fl = np.zeros(cap)
+        by = np.zeros(cap)
+        kd = np.zeros(cap, np.int32)
+        cnt = ctypes.c_int64()
+        nat.check(nat.lib().spde_prof_dump(p._h, ms.ctypes.data, fl.ctypes.data, by.ctypes.data, kd.ctypes.data,
+                                           None, cap, ctypes.byref(cnt)))
+        k = min(cnt.value, cap)
+        recs.append((kd[:k], ms[:k], fl[:k], by[:k]))
+        nat.check(nat.lib().spde_prof_end(p._h, None, None, None, None, None))
+    kind = np.concatenate([r[0] for r in recs])
+    lms = np.concatenate([r[1] for r in recs])
+    lfl = np.concatenate([r[2] for r in recs])
+    lby = np.concatenate([r[3] for r in recs])
     ms_per_step = 1e3 * elapsed / args.steps
     value = args.steps / elapsed
-    launches = nl.value / args.steps
-    # per-launch rate: algorithmic flops / summed launch durations (the
-    # launches of the two concurrent factorisations share the chip, so this
-    # is what one launch achieves while the other stream also runs); the busy
-    # rate divides by the time any GEMM was in flight instead
-    achieved = gfl.value / (gms.value * 1e-3) / 1e12 if gms.value > 0 else 0.0
-    achieved_busy = gfl.value / (busy.value * 1e-3) / 1e12 if busy.value > 0 else 0.0
+    # dominant kernel: the GEMM instantiation with the largest summed time;
+    # achieved = its algorithmic flops per launch / its mean launch duration
+    # (HIP events on each launch's own stream)
+    kinds = sorted(set(kind.tolist()), key=lambda q: -lms[kind == q].sum())
+    dom = kinds[0]
+    sel = kind == dom
+    dom_name = "gemm%d_kernel<%s, %s>" % (128 if dom & 4 else 64, "true" if dom & 2 else "false",
+                                        "true" if dom & 1 else "false")
+    dom_avg_ms = lms[sel].mean()
+    achieved = lfl[sel].mean() / (dom_avg_ms * 1e-3) / 1e12
+    family = lfl.sum() / (lms.sum() * 1e-3) / 1e12
     info = plan.info
     if args.stages:
         print(json.dumps({"like": like, "jac_norm": float(np.linalg.norm(jac)), "plan": info,
-                          "gemm_ms_per_eval": gms.value / args.steps, "gemm_launches_per_eval": launches,
-                          "gemm_busy_ms_per_eval": busy.value / args.steps,
-                          "gemm_flops_per_eval": gfl.value / args.steps}), file=sys.stderr)
+                          "gemm_ms_per_eval": lms.sum() / args.steps, "gemm_launches_per_eval": len(lms) / args.steps,
+                          "gemm_flops_per_eval": lfl.sum() / args.steps,
+                          "per_kind_ms_per_eval": {int(q): lms[kind == q].sum() / args.steps for q in kinds}}),
+              file=sys.stderr)
     base = None if args.no_cpu_baseline else cpu_baseline()
     out = {
         "metric": METRIC,
@@ -189,11 +212,15 @@ def main():
                                "Cholesky + selected inverse of Q_c for log|Q_c|, mu_c and the gradient",
                    "path": "exact"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(),
-                     "kernel": "gemm64_kernel + gemm128_kernel (batched FP64 MFMA tile-list GEMM)",
-                     "launches_per_step": launches,
-                     "achieved_while_busy": achieved_busy,
-                     "gemm_avg_launch_ms": gms.value / max(nl.value, 1)},
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(dom_name),
+                     "kernel": dom_name + " (batched FP64 MFMA tile-list GEMM)",
+                     "launches_per_step": int(sel.sum()) // args.steps,
+                     "flop_per_launch": lfl[sel].mean(),
+                     "algorithmic_bytes_per_launch": lby[sel].mean(),
+                     "avg_launch_ms": dom_avg_ms,
+                     "gemm_family": {"achieved": family, "launches_per_step": len(lms) / args.steps,
+                                     "ms_per_step": lms.sum() / args.steps,
+                                     "flop_per_step": lfl.sum() / args.steps}},
         "cpu_baseline": base,
     }
     print(json.dumps(out))

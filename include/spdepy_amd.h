@@ -202,15 +202,18 @@ int spde_grad_cells(spde_plan* plan, const spde_grad_args* args, double* out, vo
  * plan (factorisation, inverse of L11, solves, selected inverse) is bracketed
  * by HIP events on its stream; spde_prof_end synchronises and returns the
  * summed per-launch time (ms), the summed algorithmic flops, the launch count
- * and the busy time (ms during which at least one GEMM launch was in flight;
- * launches on different streams overlap, so busy <= summed time). */
+ * the busy time (ms during which at least one GEMM launch was in flight;
+ * launches on different streams overlap, so busy <= summed time) and the
+ * algorithmic bytes (A, B and C read once, C written once). */
 int spde_prof_begin(spde_plan* plan);
 int spde_prof_end(spde_plan* plan, double* gemm_ms, double* gemm_flops, int64_t* launches,
-                  double* busy_ms);
-/* Per-launch view of the timed launches so far (host arrays of length cap):
- * kernel ms, algorithmic flops, tile count (negative for 128-wide tiles). */
-int spde_prof_dump(spde_plan* plan, double* ms, double* flops, int32_t* tiles, int64_t cap,
-                   int64_t* count);
+                  double* busy_ms, double* gemm_bytes);
+/* Per-launch view of the timed launches so far (host arrays of length cap,
+ * each may be NULL): kernel ms, algorithmic flops, algorithmic bytes, kernel
+ * kind (4 for 128-wide tiles, + 2 when A is transposed, + 1 when B is:
+ * gemm128_kernel<false, false> is kind 4) and tile count. */
+int spde_prof_dump(spde_plan* plan, double* ms, double* flops, double* bytes, int32_t* kind,
+                   int32_t* tiles, int64_t cap, int64_t* count);
 
 /* ---------------------------------------------------------------- diagnostics
  * No counterpart in the reference: self-checks of the dense kernels.

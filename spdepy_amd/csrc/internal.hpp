@@ -53,6 +53,7 @@ struct GemmLaunch {
   std::vector<GemmProb> h_probs;
   std::vector<GemmTile> h_tiles;
   double flops = 0;
+  double bytes = 0;   // algorithmic HBM bytes: A, B, C read once (C only if beta != 0), C written once
 };
 
 struct PotrfDesc {      // one diagonal block of one front
@@ -153,7 +154,9 @@ struct spde_plan {
   std::vector<hipEvent_t> prof_ev;
   hipEvent_t prof_base = nullptr;
   std::vector<double> prof_flops;
+  std::vector<double> prof_bytes;
   std::vector<int> prof_tiles;
+  std::vector<int> prof_kind;
   size_t prof_n = 0;
   hipStream_t prof_stream = nullptr;
 };

@@ -139,6 +139,8 @@ void add_prob(GemmLaunch& g, const GemmProb& P, bool lower, double flop_scale = 
   Q.lower = lower ? 1 : 0;
   g.h_probs.push_back(Q);
   g.flops += 2.0 * P.M * (double)P.N * P.K * (lower ? 0.5 : 1.0) * flop_scale;
+  const double cmn = P.M * (double)P.N * (lower ? 0.5 : 1.0);
+  g.bytes += 8.0 * (P.M * (double)P.K + P.N * (double)P.K + cmn * (P.beta != 0.0 ? 2.0 : 1.0));
 }
 
 // Tile edge of a launch: 128 when the launch has no lower-triangular
@@ -475,10 +477,14 @@ void run_gemm(spde_plan* pl, const GemmLaunch& g, const Bases& b, hipStream_t s)
     hipEventRecord(pl->prof_ev[2 * pl->prof_n + 1], s);
     if (pl->prof_flops.size() <= pl->prof_n) {
       pl->prof_flops.resize(pl->prof_n + 1);
+      pl->prof_bytes.resize(pl->prof_n + 1);
       pl->prof_tiles.resize(pl->prof_n + 1);
+      pl->prof_kind.resize(pl->prof_n + 1);
     }
     pl->prof_flops[pl->prof_n] = g.flops;
-    pl->prof_tiles[pl->prof_n] = g.ntiles * (g.tile == 128 ? -1 : 1);
+    pl->prof_bytes[pl->prof_n] = g.bytes;
+    pl->prof_tiles[pl->prof_n] = g.ntiles;
+    pl->prof_kind[pl->prof_n] = (g.tile == 128 ? 4 : 0) + (g.ta ? 2 : 0) + (g.tb ? 1 : 0);
     pl->prof_n++;
     pl->prof_stream = s;
     return;
@@ -902,7 +908,8 @@ int spde_prof_begin(spde_plan* pl) {
   return SPDE_OK;
 }
 
-int spde_prof_dump(spde_plan* pl, double* ms, double* flops, int32_t* tiles, int64_t cap, int64_t* count) {
+int spde_prof_dump(spde_plan* pl, double* ms, double* flops, double* bytes, int32_t* kind, int32_t* tiles, int64_t cap,
+                   int64_t* count) {
   if (!pl || pl->host_only) {
     set_error("spde_prof_dump: no device plan");
     return SPDE_ERR_ARG;
@@ -914,19 +921,22 @@ int spde_prof_dump(spde_plan* pl, double* ms, double* flops, int32_t* tiles, int
     SPDE_HIP(hipEventElapsedTime(&t, pl->prof_ev[2 * i], pl->prof_ev[2 * i + 1]));
     if (ms) ms[i] = t;
     if (flops) flops[i] = pl->prof_flops[i];
+    if (bytes) bytes[i] = pl->prof_bytes[i];
+    if (kind) kind[i] = pl->prof_kind[i];
     if (tiles) tiles[i] = pl->prof_tiles[i];
   }
   if (count) *count = (int64_t)pl->prof_n;
   return SPDE_OK;
 }
 
-int spde_prof_end(spde_plan* pl, double* gemm_ms, double* gemm_flops, int64_t* launches, double* busy_ms) {
+int spde_prof_end(spde_plan* pl, double* gemm_ms, double* gemm_flops, int64_t* launches, double* busy_ms,
+                  double* gemm_bytes) {
   if (!pl || pl->host_only) {
     set_error("spde_prof_end: no device plan");
     return SPDE_ERR_ARG;
   }
   pl->prof_on = false;
-  double ms = 0.0, fl = 0.0;
+  double ms = 0.0, fl = 0.0, by = 0.0;
   if (pl->prof_n) SPDE_HIP(hipDeviceSynchronize());  // launches may sit on several streams
   std::vector<std::pair<float, float>> iv(pl->prof_n);
   for (size_t i = 0; i < pl->prof_n; ++i) {
@@ -937,6 +947,7 @@ int spde_prof_end(spde_plan* pl, double* gemm_ms, double* gemm_flops, int64_t* l
     iv[i] = {a, b};
     ms += t;
     fl += pl->prof_flops[i];
+    by += pl->prof_bytes[i];
   }
   // time during which at least one GEMM launch was in flight (launches on
   // different streams overlap)
@@ -956,6 +967,7 @@ int spde_prof_end(spde_plan* pl, double* gemm_ms, double* gemm_flops, int64_t* l
   if (gemm_flops) *gemm_flops = fl;
   if (launches) *launches = (int64_t)pl->prof_n;
   if (busy_ms) *busy_ms = busy;
+  if (gemm_bytes) *gemm_bytes = by;
   pl->prof_n = 0;
   return SPDE_OK;
 }

@@ -151,9 +151,15 @@ class Engine:
                 if sel:
                     self.plan.selinv(0, sq)
         self.plan.factor_async(1, qval, diag_add)
-        mu = self.plan.solve(1, torch.as_tensor(b).cuda())
         if sel:
-            self.plan.selinv(1, sqc)
+            # the selected inverse of Q_c (large GEMMs) runs on the side stream
+            # while the solve for mu_c (latency-bound) runs here
+            factored = torch.cuda.Event()
+            factored.record(main)
+            side.wait_event(factored)
+            with torch.cuda.stream(side):
+                self.plan.selinv(1, sqc)
+        mu = self.plan.solve(1, torch.as_tensor(b).cuda())
         qmu = self.plan.spmv(qval, mu)
         cols_t = torch.as_tensor(obs_cols).cuda()
         resid = torch.as_tensor(y2).cuda() - mu[cols_t]
