@@ -197,6 +197,26 @@ typedef struct spde_grad_args {
 
 int spde_grad_cells(spde_plan* plan, const spde_grad_args* args, double* out, void* stream);
 
+/* ---------------------------------------------------------------- B-spline fields
+ * Coefficient fields of the variable model from its basis coefficients
+ * (var_advection_var_diffusion2D.py:91-99, spat2Dtemp_regular_mesh.py:251-275),
+ * and the chain rule back onto the coefficients (the contraction the
+ * reference's per-parameter dQ loop performs, :131-183).  Bases dense
+ * row-major on the device: bs[Ns][nb], bsH[Ns][4][nb] (faces W,E,S,N),
+ * bsA[Ns][4][nb] (faces E,N,W,S).  coef[6 nb] = kappa | gamma | vx | vy | wx |
+ * wy.  Outputs (device): kappa[Ns], H[Ns][4][2][2], G[Ns][4] (the layouts of
+ * spde_coeffs), fgv[Ns][4][3] = (gamma, vx, vy) per face for the contraction. */
+int spde_fields_eval(int64_t Ns, int nb, const double* bs, const double* bsH, const double* bsA,
+                     const double* coef, double* kappa, double* H, double* G, double* fgv,
+                     void* stream);
+/* out[6 nb + 40] (device) = d/dcoef of <cells, d(kappa, H, G)/dcoef> in the
+ * coef layout, then the column sums of cells[Ns][40] (spde_grad_cells).
+ * work: device doubles, count from spde_fields_contract_workspace. */
+int spde_fields_contract_workspace(int64_t Ns, int nb, int64_t* doubles);
+int spde_fields_contract(int64_t Ns, int nb, const double* bs, const double* bsH, const double* bsA,
+                         const double* cells, const double* kappa, const double* fgv, double* work,
+                         double* out, void* stream);
+
 /* ---------------------------------------------------------------- measurement
  * Between spde_prof_begin and spde_prof_end every FP64 GEMM launch of the
  * plan (factorisation, inverse of L11, solves, selected inverse) is bracketed

@@ -65,6 +65,9 @@ _SIGS = {
     "spde_assemble": (ctypes.c_int, [vp, ctypes.POINTER(Coeffs), vp, vp]),
     "spde_assemble_spatial": (ctypes.c_int, [vp, vp, ctypes.c_int, vp, vp]),
     "spde_logq_sum": (ctypes.c_int, [vp, ctypes.c_double, c_dp, vp]),
+    "spde_fields_eval": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int] + [vp] * 8 + [vp]),
+    "spde_fields_contract_workspace": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, c_i64p]),
+    "spde_fields_contract": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int] + [vp] * 8 + [vp]),
     "spde_prune": (ctypes.c_int, [vp, vp, vp, vp, vp, c_i64p, vp]),
     "spde_spmv": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int, vp]),
     "spde_factor": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, c_dp, vp]),

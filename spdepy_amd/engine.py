@@ -39,9 +39,11 @@ class Coefficients:
         c.hx, c.hy, c.dt, c.V, c.sigma = grid.hx, grid.hy, grid.dt, grid.V, float(sigma)
         self.keep = []
         if variable:
-            self.kappa = torch.as_tensor(np.ascontiguousarray(kappa, dtype=np.float64)).cuda()
-            self.H = torch.as_tensor(np.ascontiguousarray(H, dtype=np.float64).reshape(-1)).cuda()
-            self.G = torch.as_tensor(np.ascontiguousarray(G, dtype=np.float64).reshape(-1)).cuda()
+            def dev(a):   # device tensors (fields evaluated on the GPU) are used as they are
+                if isinstance(a, torch.Tensor):
+                    return a.contiguous()
+                return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64).reshape(-1)).cuda()
+            self.kappa, self.H, self.G = dev(kappa), dev(H), dev(G)
             c.kappa, c.H, c.G = self.kappa.data_ptr(), self.H.data_ptr(), self.G.data_ptr()
         else:
             c.kappa_c = float(kappa)
@@ -96,7 +98,8 @@ class Engine:
         return self._side
 
     # ------------------------------------------------------- log-likelihood
-    def loglik(self, coeffs, obs_cols, data, log_tau, grad=True, trace="exact", probes=None, logdet="closed"):
+    def loglik(self, coeffs, obs_cols, data, log_tau, grad=True, trace="exact", probes=None, logdet="closed",
+               contract=None):
         """Returns (like, parts) where like is the normalised -loglik/(m r) and
         parts holds the per-cell gradient adjoints and the scalar pieces.
 
@@ -105,7 +108,11 @@ class Engine:
         log|B^T B| + (T-1) sum_k log(q_k c), B = A / V, with the spatial
         matrices factorised and selectively inverted on the one-slice plan;
         only Q_c is factorised at full size.  logdet="factor" factorises Q
-        as the reference does (advection_idiffusion2D.py:173)."""
+        as the reference does (advection_idiffusion2D.py:173).
+
+        contract(cells_device) -> (coef_grad, column_sums): when given, the
+        per-cell adjoints stay on the device and only the contraction onto
+        the model's coefficients and the 40 column sums come back."""
         torch = _torch()
         y = np.asarray(data, dtype=np.float64)
         y2 = y.reshape(y.shape[0], -1)
@@ -210,12 +217,17 @@ class Engine:
             keep += [trq, trqc, Vt]
         out = torch.empty(self.Ns * OUT_WIDTH, dtype=torch.float64, device="cuda")
         nat.check(nat.lib().spde_grad_cells(self.plan._h, g, _ptr(out), _stream(torch)))
-        cells = out.cpu().numpy().reshape(self.Ns, OUT_WIDTH)
-        parts["cells"] = cells
-        parts["g_sigma"] = -(cells[:, OUT_DOTGQ].sum() - cells[:, OUT_DOT00].sum())
+        if contract is not None:
+            parts["coef_grad"], colsum = contract(out)
+        else:
+            cells = out.cpu().numpy().reshape(self.Ns, OUT_WIDTH)
+            parts["cells"] = cells
+            colsum = cells.sum(0)
+        parts["colsum"] = colsum
+        parts["g_sigma"] = -(colsum[OUT_DOTGQ] - colsum[OUT_DOT00])
         if exact and closed:
             # 1/2 r d log|Q| / d log sigma = -1/2 r (T-1) Ns
             parts["g_sigma"] -= 0.5 * r * (T - 1) * self.plan.shape[0] * self.plan.shape[1]
-        parts["g_tau"] = m * r / 2 - 0.5 * r * cells[:, OUT_TAUTR].sum() - tau / 2 * rss
+        parts["g_tau"] = m * r / 2 - 0.5 * r * colsum[OUT_TAUTR] - tau / 2 * rss
         del keep
         return -like / (m * r), parts

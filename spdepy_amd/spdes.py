@@ -17,12 +17,14 @@ native library (no CPU fallback).  ``logLike`` takes one extra keyword,
 Rademacher probes drawn from numpy's global legacy RNG exactly as
 ``advection_idiffusion2D.py:182`` does.
 """
+import ctypes
+
 import numpy as np
 from scipy import sparse
 
 from . import _native as nat
 from .engine import (Coefficients, Engine, OUT_DGX, OUT_DGY, OUT_H, OUT_H0, OUT_HC, OUT_KAPPA, OUT_KAPPA0,
-                     OUT_WX, OUT_WY)
+                     OUT_WIDTH, OUT_WX, OUT_WY)
 from .plan import _ptr, _stream
 
 
@@ -90,6 +92,10 @@ def _triplets_csc(row, col, val, n, nan_to_zero=False):
     if nan_to_zero:
         v[np.isnan(v)] = 0.0
     return sparse.csc_matrix((v, (row[keep], col[keep])), shape=(n, n))
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
 
 
 class _StencilMixin:
@@ -333,19 +339,24 @@ class _SpaceTimeBase(_StencilMixin):
         if grad and trace == "hutchinson":
             probes = (2 * np.random.randint(1, 3, self.grid.n * nh1) - 3).reshape(self.grid.n, nh1)
         like, parts = eng.loglik(coeffs, self._obs_cols(), self.data, par[-1], grad=grad, trace=trace,
-                                 probes=probes, logdet=logdet)
+                                 probes=probes, logdet=logdet, contract=self._device_contract())
         if not grad:
             return like
         g = self._contract(par, coeffs, parts)
         m, r = parts["m"], parts["r"]
         return like, -g / (m * r)
 
+    def _device_contract(self):
+        """Contraction of the per-cell adjoints on the device, or None (the
+        host contracts them)."""
+        return None
+
     def _mod0_grad(self, par, parts, dH0, kappa0):
-        cells = parts["cells"]
+        colsum = parts["colsum"]
         if self._par_mod0(par) is None:
             return []
-        hbar = cells[:, OUT_H0:OUT_H0 + 4].sum(0)
-        out = [kappa0 * cells[:, OUT_KAPPA0].sum()]
+        hbar = colsum[OUT_H0:OUT_H0 + 4]
+        out = [kappa0 * colsum[OUT_KAPPA0]]
         out += [float(np.dot(hbar, d.reshape(4))) for d in dH0]
         return out
 
@@ -409,11 +420,11 @@ class _StationaryAdvDiff(_SpaceTimeBase):
         kap, gam, vx, vy, ws, sig = self._split(par)
         _, dH = h_tensor(self.kind, np.exp(gam), vx, vy)
         _, kappa0, H0, dH0 = self._mod0_split(par)
-        cells = parts["cells"]
-        hbar = cells[:, OUT_HC:OUT_HC + 4].sum(0)
-        g = [np.exp(kap) * cells[:, OUT_KAPPA].sum()]
+        colsum = parts["colsum"]
+        hbar = colsum[OUT_HC:OUT_HC + 4]
+        g = [np.exp(kap) * colsum[OUT_KAPPA]]
         g += [float(np.dot(hbar, d.reshape(4))) for d in dH]
-        g += [cells[:, OUT_WX].sum(), cells[:, OUT_WY].sum(), parts["g_sigma"]]
+        g += [colsum[OUT_WX], colsum[OUT_WY], parts["g_sigma"]]
         g += self._mod0_grad(par, parts, dH0, kappa0)
         g += [parts["g_tau"]]
         return np.array(g, dtype=np.float64)
@@ -491,20 +502,76 @@ class VarAdvectionVarDiffusion2D(_SpaceTimeBase):
         ws = g.evalAdv(par[4 * Np:6 * Np])
         return kappa, gamma, vx, vy, ws
 
+    def _dev_bases(self):
+        """Bases on the device (None on extended grids, whose advection
+        faces pass through the host border extrapolation)."""
+        g = self.grid
+        if g.isExtended:
+            return None
+        if getattr(self, "_bases", None) is None:
+            import torch
+            from . import _native as nat
+            dev = [torch.as_tensor(np.ascontiguousarray(b, dtype=np.float64)).cuda() for b in (g.bs, g.bsH, g.bsA)]
+            wd = ctypes.c_int64()
+            nat.check(nat.lib().spde_fields_contract_workspace(g.Ns, self.Np, ctypes.byref(wd)))
+            work = torch.empty(int(wd.value), dtype=torch.float64, device="cuda")
+            self._bases = (dev, work)
+        return self._bases
+
     def _coeffs(self, par):
+        _, kappa0, H0, _ = self._mod0_split(par)
+        bases = self._dev_bases()
+        if bases is not None:
+            import torch
+            from . import _native as nat
+            (bs, bsH, bsA), _ = bases
+            Ns, nb = self.grid.Ns, self.Np
+            coef = torch.as_tensor(np.ascontiguousarray(par[:6 * nb], dtype=np.float64)).cuda()
+            kappa = torch.empty(Ns, dtype=torch.float64, device="cuda")
+            H = torch.empty(Ns * 16, dtype=torch.float64, device="cuda")
+            G = torch.empty(Ns * 4, dtype=torch.float64, device="cuda")
+            fgv = torch.empty(Ns * 12, dtype=torch.float64, device="cuda")
+            st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+            nat.check(nat.lib().spde_fields_eval(Ns, nb, _p(bs), _p(bsH), _p(bsA), _p(coef), _p(kappa), _p(H),
+                                                 _p(G), _p(fgv), st))
+            self._fields_dev = (kappa, fgv, coef)
+            return Coefficients(self.grid, variable=True, kappa=kappa, H=H, G=G, sigma=np.exp(par[6 * self.Np]),
+                                kappa0=kappa0, H0=H0)
         kappa, gamma, vx, vy, ws = self.fields(par)
         H = np.empty((self.grid.Ns, 4, 2, 2))
         H[:, :, 0, 0] = gamma + vx * vx
         H[:, :, 0, 1] = vx * vy
         H[:, :, 1, 0] = vy * vx
         H[:, :, 1, 1] = gamma + vy * vy
-        _, kappa0, H0, _ = self._mod0_split(par)
         self._fields = (kappa, gamma, vx, vy)
         return Coefficients(self.grid, variable=True, kappa=kappa, H=H, G=ws, sigma=np.exp(par[6 * self.Np]),
                             kappa0=kappa0, H0=H0)
 
+    def _device_contract(self):
+        bases = self._dev_bases()
+        if bases is None:
+            return None
+
+        def contract(cells):
+            import torch
+            from . import _native as nat
+            (bs, bsH, bsA), work = bases
+            kappa, fgv, _ = self._fields_dev
+            Ns, nb = self.grid.Ns, self.Np
+            res = torch.empty(6 * nb + OUT_WIDTH, dtype=torch.float64, device="cuda")
+            st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+            nat.check(nat.lib().spde_fields_contract(Ns, nb, _p(bs), _p(bsH), _p(bsA), _p(cells), _p(kappa), _p(fgv),
+                                                     _p(work), _p(res), st))
+            h = res.cpu().numpy()
+            return h[:6 * nb], h[6 * nb:]
+        return contract
+
     def _contract(self, par, coeffs, parts):
         g, Np = self.grid, self.Np
+        _, kappa0, H0, dH0 = self._mod0_split(par)
+        if "coef_grad" in parts:
+            out = [parts["coef_grad"], [parts["g_sigma"]], self._mod0_grad(par, parts, dH0, kappa0), [parts["g_tau"]]]
+            return np.hstack(out).astype(np.float64)
         kappa, gamma, vx, vy = self._fields
         cells = parts["cells"]
         Hb = cells[:, OUT_H:OUT_H + 16].reshape(-1, 4, 2, 2)
@@ -521,7 +588,6 @@ class VarAdvectionVarDiffusion2D(_SpaceTimeBase):
         out.append(g.bsA[:, 0, :].T @ wx_bar[:, 0] + g.bsA[:, 2, :].T @ wx_bar[:, 2])
         out.append(g.bsA[:, 1, :].T @ wy_bar[:, 1] + g.bsA[:, 3, :].T @ wy_bar[:, 3])
         out.append([parts["g_sigma"]])
-        _, kappa0, H0, dH0 = self._mod0_split(par)
         out.append(self._mod0_grad(par, parts, dH0, kappa0))
         out.append([parts["g_tau"]])
         return np.hstack(out).astype(np.float64)
