@@ -30,19 +30,23 @@ __global__ __launch_bounds__(256) void extend_add_kernel(const ExtAddDesc* __res
                                                          const ExtAddTile* __restrict__ tiles,
                                                          const int32_t* __restrict__ relpos,
                                                          double* __restrict__ F) {
+  __shared__ int rpj[64];
   const ExtAddTile t = tiles[blockIdx.x];
   const ExtAddDesc d = descs[t.desc];
   const int32_t* rp = relpos + d.relpos_off;
   const double* U = F + d.cfoff + d.cp + (int64_t)d.cp * d.cld;
   double* P = F + d.pfoff;
   const int tid = threadIdx.x;
+  if (tid < 64) rpj[tid] = (t.tj * 64 + tid < d.b) ? rp[t.tj * 64 + tid] : -1;
+  __syncthreads();
   const int i = t.ti * 64 + (tid & 63);
   if (i >= d.b) return;
   const int pi = rp[i];
+#pragma unroll 4
   for (int jj = (tid >> 6); jj < 64; jj += 4) {
     const int j = t.tj * 64 + jj;
     if (j > i || j >= d.b) continue;
-    P[pi + (int64_t)rp[j] * d.pld] += U[i + (int64_t)j * d.cld];
+    P[pi + (int64_t)rpj[jj] * d.pld] += U[i + (int64_t)j * d.cld];
   }
 }
 
@@ -166,6 +170,7 @@ __global__ __launch_bounds__(256) void sigma_gather_kernel(const int32_t* __rest
                                                            const int32_t* __restrict__ relpos, double* __restrict__ S,
                                                            const int4* __restrict__ tiles) {
   __shared__ double tile[64][65];
+  __shared__ int rpi[64], rpj[64];
   const int4 tt = tiles[blockIdx.x];
   const int node = lvl_nodes[tt.x];
   const NodeDev nd = nodes[node];
@@ -176,35 +181,34 @@ __global__ __launch_bounds__(256) void sigma_gather_kernel(const int32_t* __rest
   const double* src = S + pa.foff;
   const int64_t pn = pa.n;
   const int tid = threadIdx.x, u = tid & 63, w0 = tid >> 6;
+  // parent positions of the tile's rows and columns (-1 outside the block)
+  if (tid < 64) rpi[tid] = (i0 + tid < b) ? rp[i0 + tid] : -1;
+  else if (tid < 128) rpj[tid - 64] = (j0 + tid - 64 < b) ? rp[j0 + tid - 64] : -1;
+  __syncthreads();
   // relpos increases, so a tile below the diagonal (ti > tj) lies wholly in
   // the parent's lower triangle, one above it wholly in the upper (mirrored)
   // part; only diagonal tiles need both passes
   if (tt.y >= tt.z) {  // rows fastest: entries with rp[i] >= rp[j]
-    const int i = i0 + u;
-    const int pi = (i < b) ? rp[i] : -1;
+    const int pi = rpi[u];
+#pragma unroll 4
     for (int jj = w0; jj < 64; jj += 4) {
-      const int j = j0 + jj;
-      if (i < b && j < b) {
-        const int pj = rp[j];
-        if (pi >= pj) tile[u][jj] = src[pi + (int64_t)pj * pn];
-      }
+      const int pj = rpj[jj];
+      if (pi >= 0 && pj >= 0 && pi >= pj) tile[u][jj] = src[pi + (int64_t)pj * pn];
     }
   }
   if (tt.y <= tt.z) {  // columns fastest: entries with rp[i] < rp[j]
-    const int j = j0 + u;
-    const int pj = (j < b) ? rp[j] : -1;
+    const int pj = rpj[u];
+#pragma unroll 4
     for (int ii = w0; ii < 64; ii += 4) {
-      const int i = i0 + ii;
-      if (i < b && j < b) {
-        const int pi = rp[i];
-        if (pi < pj) tile[ii][u] = src[pj + (int64_t)pi * pn];
-      }
+      const int pi = rpi[ii];
+      if (pi >= 0 && pj >= 0 && pi < pj) tile[ii][u] = src[pj + (int64_t)pi * pn];
     }
   }
   __syncthreads();
   double* dst = S + nd.foff + nd.p + (int64_t)nd.p * nd.n;
   const int i = i0 + u;
   if (i < b)
+#pragma unroll 4
     for (int jj = w0; jj < 64; jj += 4) {
       const int j = j0 + jj;
       if (j < b) dst[i + (int64_t)j * nd.n] = tile[u][jj];

@@ -69,6 +69,32 @@ class Model:
             data += self.grid.getS()[:, :np.prod(self.grid.shape)] @ z[:np.prod(self.grid.shape)] / np.sqrt(self.tau)
         return data
 
+    def qinv(self, simple=False):
+        """Marginal variances (model.py:89-118).  simple=True: variance of
+        1000 unconditional samples, as the reference.  Otherwise Q^-1 on the
+        pattern of Q from the Takahashi selected inverse of the GPU factor
+        (in place of R-INLA's qinv, kept in FP64 instead of the reference's
+        float32 cast) in ``self.pQinv`` (CSC); returns its diagonal."""
+        if simple:
+            z = self.sample(n=1000, simple=True)
+            return z.var(axis=1)
+        from scipy import sparse
+        if self.Q is None:
+            self.setModel()
+        if self.useCov:
+            raise NotImplementedError("covariate-augmented models are outside the hot path")
+        eng = self._engine()
+        fac = self._factor()
+        sig = fac.selinv().cpu().numpy()
+        indptr, indices = eng.plan.structure()
+        n = eng.n
+        full = sparse.csr_matrix((sig, indices, indptr), shape=(n, n))
+        pattern = self.Q.tocsc()
+        mask = sparse.csc_matrix((np.ones(pattern.nnz), pattern.indices, pattern.indptr), shape=pattern.shape)
+        self.pQinv = full.tocsc().multiply(mask).tocsc()
+        self.mvar = self.pQinv.diagonal()
+        return self.mvar
+
     def update(self, y, idx, tau=None):
         """Q <- Q + tau S^T S, mu <- mu + tau Q^-1 S^T (y - S mu) (model.py:120-127)."""
         if tau is None:

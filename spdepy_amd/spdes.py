@@ -136,6 +136,11 @@ class Factor:
         self._ensure()
         return self._logdet
 
+    def selinv(self):
+        """Q^-1 on the structural pattern (device tensor, structural order)."""
+        self._ensure()
+        return self.engine.plan.selinv(self.slot)
+
     def _solve(self, b, mode):
         torch = _torch()
         self._ensure()

@@ -198,3 +198,30 @@ def test_model_sample_and_update():
     Qc = Qd + tau * (S.T @ S).toarray()
     mu_ref = np.linalg.solve(Qc, tau * (S.T @ c["y"]))
     assert np.allclose(M.mu, mu_ref, rtol=1e-9, atol=1e-12)
+
+
+def test_model_qinv_selected_inverse():
+    """Model.qinv (model.py:89-118): Q^-1 on the pattern of Q and its diagonal
+    against the dense inverse; after an update the conditional precision's."""
+    import spdepy_amd as sp
+    c = CASES["var_b1"]()
+    g = c["grid"]
+    m = product_case(c)
+    m.initFit(c["y"], idx=c["idx"], fitQ0=False)
+    m.setQ(par=c["par"])
+    mod = sp.Model.__new__(sp.Model)
+    mod.grid, mod.mod, mod.Q, mod.mvar, mod.useCov = m.grid, m, None, None, False
+    mod.setModel()
+    step("qinv")
+    mvar = mod.qinv()
+    Qd = mod.Q.toarray()
+    inv = np.linalg.inv(Qd)
+    assert np.allclose(mvar, np.diag(inv), rtol=1e-10)
+    P = mod.pQinv.tocoo()
+    assert np.allclose(P.data, inv[P.row, P.col], rtol=1e-9, atol=1e-12 * np.abs(inv).max())
+    assert set(zip(*mod.Q.nonzero())) == set(zip(P.row, P.col))
+    step("qinv after update")
+    mod.update(c["y"][:5], c["idx"][:5])
+    mvar2 = mod.qinv()
+    inv2 = np.linalg.inv(mod.Q.toarray())
+    assert np.allclose(mvar2, np.diag(inv2), rtol=1e-10)
