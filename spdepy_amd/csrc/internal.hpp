@@ -66,9 +66,10 @@ struct PotrfDesc {      // one diagonal block of one front
 struct ExtAddDesc {     // extend-add of a child's update block into its parent
   int64_t cfoff, pfoff, relpos_off;
   int32_t cld, cp, pld, b;
+  int32_t split, pad;   // child columns [0, split) land in the parent's own columns
 };
 struct ExtAddTile {
-  int32_t desc, ti, tj, pad;
+  int32_t desc, ti, tj, phase;  // phase 0: columns < split, 1: columns >= split
 };
 
 struct Step {           // one stage of the dense partial factorisation of a level
@@ -80,11 +81,14 @@ struct Step {           // one stage of the dense partial factorisation of a lev
 
 struct LevelPlan {
   std::vector<int> nodes;
-  int next[2] = {0, 0};                 // extend-add tile counts per child slot
+  // extend-add per child slot and phase: phase 0 (into the own columns) before
+  // the level's factorisation, phase 1 (into the update blocks) after its
+  // SYRK, which writes the update blocks without reading them
+  int next[2][2] = {{0, 0}, {0, 0}};    // [phase][slot] tile counts
   ExtAddDesc* d_ext[2] = {nullptr, nullptr};
-  ExtAddTile* d_exttile[2] = {nullptr, nullptr};
+  ExtAddTile* d_exttile[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   std::vector<ExtAddDesc> h_ext[2];
-  std::vector<ExtAddTile> h_exttile[2];
+  std::vector<ExtAddTile> h_exttile[2][2];
   std::vector<Step> steps;
   GemmLaunch syrk;
   // solves / selinv

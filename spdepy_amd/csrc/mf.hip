@@ -25,7 +25,9 @@ __global__ void diag_add_kernel(const double* __restrict__ add, const int64_t* _
 
 // Extend-add of the lower triangle of a child's update block (rows/cols p..n
 // of its front) into the parent front at relpos.  Tile 64x64 of the child's
-// b x b block; tiles above the diagonal are not listed.
+// b x b block; tiles above the diagonal are not listed.  Phase 0 adds the
+// columns that land in the parent's own columns, phase 1 the rest (into the
+// parent's update block, after the parent's SYRK has written it).
 __global__ __launch_bounds__(256) void extend_add_kernel(const ExtAddDesc* __restrict__ descs,
                                                          const ExtAddTile* __restrict__ tiles,
                                                          const int32_t* __restrict__ relpos,
@@ -42,10 +44,11 @@ __global__ __launch_bounds__(256) void extend_add_kernel(const ExtAddDesc* __res
   const int i = t.ti * 64 + (tid & 63);
   if (i >= d.b) return;
   const int pi = rp[i];
+  const int jlo = t.phase ? d.split : 0, jhi = t.phase ? d.b : d.split;
 #pragma unroll 4
   for (int jj = (tid >> 6); jj < 64; jj += 4) {
     const int j = t.tj * 64 + jj;
-    if (j > i || j >= d.b) continue;
+    if (j > i || j < jlo || j >= jhi) continue;
     P[pi + (int64_t)rpj[jj] * d.pld] += U[i + (int64_t)j * d.cld];
   }
 }

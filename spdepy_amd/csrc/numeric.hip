@@ -206,16 +206,23 @@ int build_factor_plan(spde_plan* pl) {
         const int b = (int)c.B.size();
         if (b == 0) continue;
         const int di = (int)L.h_ext[s].size();
-        L.h_ext[s].push_back({c.foff, nd.foff, c.relpos_off, c.n(), c.p, nd.n(), b});
+        int split = 0;  // relpos increases: the first `split` columns map into the parent's own columns
+        while (split < b && c.relpos[split] < nd.p) ++split;
+        L.h_ext[s].push_back({c.foff, nd.foff, c.relpos_off, c.n(), c.p, nd.n(), b, split, 0});
         const int nt = (b + 63) / 64;
         for (int a = 0; a < nt; ++a)
-          for (int bb = 0; bb <= a; ++bb) L.h_exttile[s].push_back({di, a, bb, 0});
+          for (int bb = 0; bb <= a; ++bb) {
+            if (bb * 64 < split) L.h_exttile[0][s].push_back({di, a, bb, 0});
+            if ((bb + 1) * 64 > split) L.h_exttile[1][s].push_back({di, a, bb, 1});
+          }
       }
-      L.next[s] = (int)L.h_exttile[s].size();
       int rc = upload(&L.d_ext[s], L.h_ext[s]);
       if (rc) return rc;
-      rc = upload(&L.d_exttile[s], L.h_exttile[s]);
-      if (rc) return rc;
+      for (int ph = 0; ph < 2; ++ph) {
+        L.next[ph][s] = (int)L.h_exttile[ph][s].size();
+        rc = upload(&L.d_exttile[ph][s], L.h_exttile[ph][s]);
+        if (rc) return rc;
+      }
     }
     int pmax = 0;
     for (int id : L.nodes) pmax = std::max(pmax, sym.nodes[id].p);
@@ -286,9 +293,11 @@ int build_factor_plan(spde_plan* pl) {
     for (int id : L.nodes) {
       const Node& nd = sym.nodes[id];
       const int n = nd.n(), b = (int)nd.B.size();
+      // beta = 0: the update block is written here, the children's
+      // contributions to it are added afterwards (extend-add phase 1)
       add_prob(L.syrk,
                prob(kFront, nd.foff + nd.p, n, kFront, nd.foff + nd.p, n, kFront, nd.foff + nd.p + (int64_t)nd.p * n, n,
-                    b, b, nd.p, -1.0, 1.0),
+                    b, b, nd.p, -1.0, 0.0),
                true);
     }
     int rc = finalize(L.syrk);
@@ -597,9 +606,11 @@ int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int 
   {
     std::vector<int4> zt;
     for (size_t q = 0; q < sym.nodes.size(); ++q) {
-      const int nt = (sym.nodes[q].n() + 63) / 64;
+      // lower triangle of the own columns only (the update block is
+      // written by the SYRK before anything adds to it)
+      const int nt = (sym.nodes[q].n() + 63) / 64, pt = (sym.nodes[q].p + 63) / 64;
       for (int a = 0; a < nt; ++a)
-        for (int b = 0; b <= a; ++b) zt.push_back(int4{(int)q, a, b, 0});
+        for (int b = 0; b <= a && b < pt; ++b) zt.push_back(int4{(int)q, a, b, 0});
     }
     if (zt.size() > (size_t)INT32_MAX) {
       set_error("front tile list too long");
@@ -677,7 +688,8 @@ void spde_plan_destroy(spde_plan* pl) {
     for (auto& L : pl->levels) {
       for (int s = 0; s < 2; ++s) {
         hipFree(L.d_ext[s]);
-        hipFree(L.d_exttile[s]);
+        hipFree(L.d_exttile[0][s]);
+        hipFree(L.d_exttile[1][s]);
       }
       for (auto& st : L.steps) {
         hipFree(st.d_potrf);
@@ -780,13 +792,14 @@ int spde_factor_async(spde_plan* pl, int slot, const double* qval, const double*
   Bases b = make_bases(pl, slot, nullptr);
   for (size_t l = 0; l < pl->levels.size(); ++l) {
     LevelPlan& L = pl->levels[l];
-    for (int c = 0; c < 2; ++c) launch_extend_add(L.d_ext[c], L.d_exttile[c], L.next[c], pl->d_relpos, sl.front, s);
+    for (int c = 0; c < 2; ++c) launch_extend_add(L.d_ext[c], L.d_exttile[0][c], L.next[0][c], pl->d_relpos, sl.front, s);
     for (Step& st : L.steps) {  // a leaf (potrf + panel) or an update
       launch_potrf(st.d_potrf, st.npotrf, sl.front, sl.z, sl.status, s);
       run_gemm(pl, st.panel, b, s);
       run_gemm(pl, st.update, b, s);
     }
     run_gemm(pl, L.syrk, b, s);
+    for (int c = 0; c < 2; ++c) launch_extend_add(L.d_ext[c], L.d_exttile[1][c], L.next[1][c], pl->d_relpos, sl.front, s);
   }
   for (auto& g : pl->trtri) run_gemm(pl, g, b, s);
   launch_logdet(pl->d_nodes, (int)sym.nodes.size(), sl.front, sl.partial, sl.ld, s);
