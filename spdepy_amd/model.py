@@ -4,6 +4,7 @@ update with new observations.  Factorisations and solves run on the GPU."""
 import numpy as np
 
 from .engine import Engine
+from .optim import Optimize
 from .spdes import Factor, spde_init
 
 
@@ -31,10 +32,22 @@ class Model:
             mod0 = spde_init(model="whittle-matern", grid=grid, parameters=parameters, ani=ani, ha=ha, bc=bc)
         self.mod = spde_init(model=spde, grid=grid, parameters=parameters, ani=ani, ha=ha, bc=bc, mod0=mod0)
         self.spde_type = self.mod.type
+        self.optim = Optimize(self.mod.logLike)
 
     def fit(self, data, **kwargs):
-        raise NotImplementedError("the optimisation loop (Optimize.fit) calls logLike; it is not part of this "
-                                  "GMRF hot path build")
+        """model.py:49-71: initFit, then the optimiser over logLike (one GPU
+        log-likelihood + gradient evaluation per step), then setQ at the
+        Polyak-averaged parameters.  kwargs as Optimize.fit (x0, stepType,
+        lr, max_steps, pol, fix, verbose, end) plus initFit's (idx, fitQ0)."""
+        assert self.mod is not None
+        x0 = self.mod.initFit(data, **kwargs)
+        if kwargs.get("x0") is None:
+            kwargs["x0"] = x0
+        if kwargs.get("verbose") is not None:
+            kwargs["print"] = self.mod.print
+        res = self.optim.fit(**kwargs)
+        self.setQ(par=res["x"])
+        return res
 
     def _engine(self):
         return self.mod._engine()

@@ -225,3 +225,22 @@ def test_model_qinv_selected_inverse():
     mvar2 = mod.qinv()
     inv2 = np.linalg.inv(mod.Q.toarray())
     assert np.allclose(mvar2, np.diag(inv2), rtol=1e-10)
+
+
+def test_model_fit_lowers_the_objective():
+    """Model.fit (model.py:49-71): the optimiser over the GPU logLike lowers
+    the normalised negative log-likelihood from a perturbed start."""
+    import spdepy_amd as sp
+    from spdepy_amd.optim import Optimize
+    c = CASES["var_b1"]()
+    m = product_case(c)
+    mod = sp.Model.__new__(sp.Model)
+    mod.grid, mod.mod, mod.useCov = m.grid, m, False
+    mod.optim = Optimize(m.logLike)
+    x0 = c["par"] + 0.2
+    step("Model.fit")
+    res = mod.fit(c["y"], idx=c["idx"], fitQ0=False, x0=x0.copy(), stepType="adam", lr=0.02, max_steps=12, pol=1)
+    f0 = m.logLike(x0, grad=False)
+    f1 = m.logLike(res["x"], grad=False)
+    assert np.isfinite(f1) and f1 < f0, (f0, f1)
+    assert len(mod.optim.histF) == 12
