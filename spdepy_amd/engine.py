@@ -59,8 +59,14 @@ class Coefficients:
 
 
 class Engine:
+    """Space-time grids: Q on the M x N x T lattice.  Spatial grids
+    (Whittle-Matern models, whittle_matern2D.py:59-126): Q = Q0 = A0^T A0 / V
+    on the one-slice lattice, from the A0 coefficients (kappa0, H0)."""
+
     def __init__(self, grid, bc, leaf_max=128):
-        M, N, T = grid.shape
+        shape = list(grid.shape)
+        self.is_spatial = len(shape) == 2
+        M, N, T = (shape + [1]) if self.is_spatial else shape
         self.grid, self.bc = grid, bc
         self.plan = get_plan(M, N, T, bc, leaf_max)
         self.n, self.Ns, self.nnz = grid.n, grid.Ns, self.plan.nnz
@@ -80,6 +86,8 @@ class Engine:
         torch = _torch()
         q = torch.empty(self.nnz, dtype=torch.float64, device="cuda")
         self.plan.assemble(coeffs.c, q)
+        if self.is_spatial:  # the Q0 block of the templates is the model's Q
+            self.plan.assemble_spatial(self.plan, 0, q)
         return q
 
     def to_scipy(self, qval):
@@ -120,7 +128,7 @@ class Engine:
         tau = float(np.exp(log_tau))
         n = self.n
         exact = trace == "exact"
-        closed = logdet == "closed" and (exact or not grad)
+        closed = logdet == "closed" and (exact or not grad) and not self.is_spatial
         qval = self.assemble(coeffs)
         counts = np.bincount(obs_cols, minlength=n).astype(np.float64)
         diag_add = torch.as_tensor(tau * counts).cuda()

@@ -178,6 +178,14 @@ class Factor:
         return out
 
 
+class _SpatialGeom:
+    """Geometry of a spatial grid as the coefficient block expects it (a
+    unit time step; only the initial-state block A0 is used)."""
+
+    def __init__(self, g):
+        self.hx, self.hy, self.V, self.dt = g.hx, g.hy, g.V, 1.0
+
+
 class _PtApplied(np.ndarray):
     def __new__(cls, arr):
         return np.asarray(arr).view(cls)
@@ -254,6 +262,48 @@ class _WhittleMaternBase(_StencilMixin):
             dA = -self.Ah(d)
             dQ.append(dA.T @ g.iDv @ A + A.T @ g.iDv @ dA)
         return Q, None, dQ
+
+    # ---------------------------------------------------- log-likelihood
+    def _engine(self):
+        if getattr(self, "_eng", None) is None:
+            self._eng = Engine(self.grid, self.bc)
+        return self._eng
+
+    def _coeffs(self, par):
+        kappa, H, _ = self.h0(par)
+        g = self.grid
+        c = Coefficients(_SpatialGeom(g), variable=False, kappa=1.0, H=np.eye(2), G=(0.0, 0.0), sigma=1.0,
+                         kappa0=kappa, H0=H)
+        return c
+
+    def _obs_cols(self):
+        S = self.S.tocsr()
+        if np.any(np.diff(S.indptr) != 1):
+            raise ValueError("observation matrix must have one entry per row")
+        return S.indices.astype(np.int64)
+
+    def logLike(self, par, nh1=100, grad=True, trace="exact"):
+        """whittle_matern2D.py:90-126: -loglik/(m r) and its gradient for
+        par = [log kappa, log gamma, (vx, vy,) log tau].  trace="exact" uses
+        the selected inverses of Q and Q_c; "hutchinson" the reference's
+        estimator with the same numpy probe stream (:103)."""
+        par = np.asarray(par, dtype=np.float64)
+        eng = self._engine()
+        coeffs = self._coeffs(par)
+        probes = None
+        if grad and trace == "hutchinson":
+            probes = (2 * np.random.randint(1, 3, self.grid.n * nh1) - 3).reshape(self.grid.n, nh1)
+        like, parts = eng.loglik(coeffs, self._obs_cols(), self.data, par[-1], grad=grad, trace=trace,
+                                 probes=probes, logdet="factor")
+        if not grad:
+            return like
+        colsum = parts["colsum"]
+        kappa, _, dH = self.h0(par)
+        g = [kappa * colsum[OUT_KAPPA0]]
+        g += [float(np.dot(colsum[OUT_H0:OUT_H0 + 4], d.reshape(4))) for d in dH]
+        g += [parts["g_tau"]]
+        m, r = parts["m"], parts["r"]
+        return like, -np.array(g) / (m * r)
 
     def print(self, par):
         s = "| κ = %2.2f, γ = %2.2f" % (np.exp(par[0]), np.exp(par[1]))

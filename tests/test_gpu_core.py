@@ -240,7 +240,36 @@ def test_model_fit_lowers_the_objective():
     x0 = c["par"] + 0.2
     step("Model.fit")
     res = mod.fit(c["y"], idx=c["idx"], fitQ0=False, x0=x0.copy(), stepType="adam", lr=0.02, max_steps=12, pol=1)
+    m.initFit(c["y"], idx=c["idx"], fitQ0=False)   # fit ends with setQ, which resets S as the reference does
     f0 = m.logLike(x0, grad=False)
     f1 = m.logLike(res["x"], grad=False)
     assert np.isfinite(f1) and f1 < f0, (f0, f1)
     assert len(mod.optim.histF) == 12
+
+
+@pytest.mark.parametrize("kind,bc", [("iso", 1), ("ani", 3), ("ha", 1)])
+def test_spatial_whittle_matern_loglike(kind, bc):
+    """Spatial Whittle-Matern logLike (whittle_matern2D.py:90-126) on the
+    one-slice plan: value and gradient against the oracle (exact trace)."""
+    from oracle import grid as OG
+    import spdepy_amd as sp
+    from spdepy_amd import spdes
+    M, N = 11, 9
+    g0 = OG.SpatialGrid(F.axis(M), F.axis(N), Nbs=3)
+    par = np.array([0.2, np.log(0.7), -0.1]) if kind == "iso" else np.array([0.2, np.log(0.7), 0.3, -0.4, -0.1])
+    par[-1] = np.log(50.0)
+    wm = Mo.WhittleMatern(g0, kind, par=par.copy(), bc=bc)
+    Q, dQ = wm.makeQ(par, grad=True)
+    rng = np.random.default_rng(9)
+    idx = np.sort(rng.choice(M * N, 40, replace=False))
+    x = np.linalg.solve(np.linalg.cholesky(Q.toarray()).T, rng.normal(size=M * N))
+    y = x[g0.n2e(idx)] + 0.1 * rng.normal(size=idx.size)
+    like_o, jac_o, _ = Mo.loglike(Q, dQ, g0.getS(idx), y, par[-1], trace="exact")
+    pg = sp.grid(x=F.axis(M), y=F.axis(N), Nbs=3)
+    cls = {"iso": spdes.WhittleMatern2D, "ani": spdes.WhittleMaternAnisotropic2D, "ha": spdes.WhittleMaternHa2D}[kind]
+    m = cls(grid=pg, par=par.copy(), bc=bc)
+    m.initFit(y, idx=idx)
+    step("spatial logLike " + kind)
+    like, jac = m.logLike(par)
+    assert abs(like - like_o) <= 1e-9 * abs(like_o), (like, like_o)
+    assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * np.abs(jac_o).max()), (jac, jac_o)
