@@ -104,6 +104,10 @@ typedef struct spde_coeffs {
   const double* H;
   const double* G;
   double kappa0, H0[4];
+  /* 1: A0 = Dv kappa - A_H(H) from the variable fields kappa[Ns], H[Ns*16]
+   * (variable Whittle-Matern models, var_whittle_matern2D.py:61-65) instead
+   * of the constant kappa0, H0. */
+  int a0_variable;
 } spde_coeffs;
 
 /* Values of Q on the structural pattern (qval[nnz_struct], device). */

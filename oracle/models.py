@@ -105,6 +105,82 @@ class WhittleMatern:
         return Q, dQ
 
 
+class VarWhittleMatern:
+    """Variable-coefficient spatial Matern precision (``var_whittle_matern2D.py:
+    61-66,131-152``, ``var_whittle_matern_anisotropic2D.py``,
+    ``var_whittle_matern_ha2D.py:83-123``): Q = A^T iDv A, A = Dv Dk - A_H(H)
+    with the variable diffusion stencil; kappa = exp(bs c_k) at the centres,
+    gamma = exp(bsH c_g) on the faces and per face H = gamma I (iso),
+    gamma I + v v^T (ani), gamma expm([[vx, vy], [vy, -vx]]) (ha, written as
+    cosh/sinh).  par = [c_k, c_g, (c_vx, c_vy,) log tau] (Np each)."""
+
+    def __init__(self, grid, kind="iso", bc=3):
+        self.grid, self.kind, self.bc = grid, kind, bc
+        self.Np = grid.Nbs2
+
+    def split(self, par):
+        Np = self.Np
+        ck, cg = par[:Np], par[Np:2 * Np]
+        cvx, cvy = (par[2 * Np:3 * Np], par[3 * Np:4 * Np]) if self.kind != "iso" else (None, None)
+        return ck, cg, cvx, cvy
+
+    def fields(self, par):
+        g = self.grid
+        ck, cg, cvx, cvy = self.split(par)
+        kappa = np.exp(g.evalB(ck))
+        gam = np.exp(g.evalBH(cg))
+        Ns = g.Ns
+        I = np.eye(2)
+        H = gam[:, :, None, None] * I
+        dgam = H.copy()                       # d H / d log gamma at each face
+        dvx = dvy = None
+        if self.kind == "ani":
+            vx, vy = g.evalBH(cvx), g.evalBH(cvy)
+            vv = np.stack([vx, vy], axis=2)
+            H = H + vv[:, :, :, None] * vv[:, :, None, :]
+            ex = np.zeros_like(vv); ex[:, :, 0] = 1.0
+            ey = np.zeros_like(vv); ey[:, :, 1] = 1.0
+            dvx = vv[:, :, :, None] * ex[:, :, None, :] + ex[:, :, :, None] * vv[:, :, None, :]
+            dvy = vv[:, :, :, None] * ey[:, :, None, :] + ey[:, :, :, None] * vv[:, :, None, :]
+        elif self.kind == "ha":
+            vx, vy = g.evalBH(cvx), g.evalBH(cvy)
+            a = np.sqrt(vx ** 2 + vy ** 2)
+            mV = np.zeros((Ns, 4, 2, 2))
+            mV[:, :, 0, 0], mV[:, :, 0, 1], mV[:, :, 1, 0], mV[:, :, 1, 1] = vx, vy, vy, -vx
+            ch, sh = (np.exp(a) + np.exp(-a)) / 2, (np.exp(a) - np.exp(-a)) / 2
+            H = (gam * ch)[:, :, None, None] * I + (gam * sh / a)[:, :, None, None] * mV
+            dgam = H.copy()
+            Dx = np.zeros_like(mV); Dx[:, :, 0, 0], Dx[:, :, 1, 1] = 1.0, -1.0
+            Dy = np.zeros_like(mV); Dy[:, :, 0, 1], Dy[:, :, 1, 0] = 1.0, 1.0
+            dvx = ((gam * sh * vx / a)[:, :, None, None] * I + (gam * (ch * vx - sh * vx / a) / a ** 2)[:, :, None, None] * mV
+                   + (gam * sh / a)[:, :, None, None] * Dx)
+            dvy = ((gam * sh * vy / a)[:, :, None, None] * I + (gam * (ch * vy - sh * vy / a) / a ** 2)[:, :, None, None] * mV
+                   + (gam * sh / a)[:, :, None, None] * Dy)
+        return kappa, H, dgam, dvx, dvy
+
+    def A(self, par):
+        g = self.grid
+        kappa, H, dgam, dvx, dvy = self.fields(par)
+        return g.Dv @ sparse.diags(kappa) - ah_var_csc(g, H, self.bc), kappa, H, dgam, dvx, dvy
+
+    def makeQ(self, par, grad=True):
+        g = self.grid
+        A, kappa, H, dgam, dvx, dvy = self.A(par)
+        Q = A.transpose() @ g.iDv @ A
+        if not grad:
+            return Q, None
+        dQ = []
+
+        def add(dA):
+            dQ.append((dA.transpose() @ g.iDv @ A + A.transpose() @ g.iDv @ dA).tocsc())
+        for i in range(self.Np):
+            add(g.Dv @ sparse.diags(g.bs[:, i] * kappa))
+        for d in [dgam] + ([dvx, dvy] if self.kind != "iso" else []):
+            for i in range(self.Np):
+                add(-ah_var_csc(g, g.bsH[:, :, i][:, :, None, None] * d, self.bc))
+        return Q, dQ
+
+
 # --------------------------------------------------------- space-time blocks
 def assemble_blocks(A, Qs, Q0, iDv, T, sigma, dt, diag0=None):
     """Block-tridiagonal space-time precision (``advection_idiffusion2D.py:110-115``):

@@ -69,6 +69,7 @@ struct CoeffArgs {
   const double* H;
   const double* G;
   double kappa0, H0[4];
+  int a0_variable;
 };
 
 // A = Dv + (Dv Dk - A_H + A_w) dt (B-spline models) or
@@ -112,6 +113,10 @@ __global__ void canonical_kernel(int M, int N, int bc, CoeffArgs a, double* __re
     acan[9 * (int64_t)k + o] = v;
   }
   q[k] = ((V * kap) * (1.0 / V)) * (V * kap);
+  if (a.a0_variable) {  // A0 = Dv Dk - A_H(H) with the variable fields
+    for (int o = 0; o < 9; ++o) a0can[9 * (int64_t)k + o] = (o == 4) ? V * kap - ah[o] : -ah[o];
+    return;
+  }
   diffusion_const(i, j, M, N, bc, a.hx, a.hy, a.H0, d);
   double a0[9];
   for (int o = 0; o < 9; ++o) a0[o] = 0.0;
@@ -348,6 +353,11 @@ int spde_assemble(spde_plan* pl, const spde_coeffs* c, double* qval, void* strea
   a.H = c->H;
   a.G = c->G;
   a.kappa0 = c->kappa0;
+  a.a0_variable = c->a0_variable;
+  if (a.a0_variable && !c->variable) {
+    set_error("spde_assemble: a0_variable needs the variable fields");
+    return SPDE_ERR_ARG;
+  }
   double* qdiag = pl->d_qdiag;
   hipLaunchKernelGGL(canonical_kernel, dim3(blocks_for(Ns, 128)), dim3(128), 0, s, M, N, bc, a, pl->d_acan,
                      pl->d_a0can, qdiag);

@@ -21,7 +21,7 @@
 namespace spde {
 
 struct GradKArgs {
-  int mode, r, nh1, T, variable, bc, M, N;
+  int mode, r, nh1, T, variable, bc, M, N, a0_variable;
   int64_t Ns, n;
   double c, V, dt, hx, hy;
   double H_c[4], G_c[2], kappa_c;
@@ -205,6 +205,19 @@ __global__ void grad_cell_kernel(GradKArgs a, const double* __restrict__ win, do
         for (int q = 0; q < 9; ++q) s += abar[q] * canv[q];
         o[17 + 4 * (diff - 1) + f] = a.dt * s;
       }
+    if (a.a0_variable) {  // A0 = Dv Dk - A_H(H): kappa and H adjoints through A0
+      o[0] += a0bar[4] * a.V;
+      for (int e = 0; e < 16; ++e) {
+        for (int q = 0; q < 16; ++q) Hk[q] = 0.0;
+        Hk[e] = 1.0;
+        Cell9 c9;
+        diffusion_var(i, j, M, N, bc, a.hx, a.hy, Hk, c9);
+        canon9(c9, i, j, M, N, canv);
+        double s = 0.0;
+        for (int q = 0; q < 9; ++q) s += a0bar[q] * canv[q];
+        o[1 + e] -= s;
+      }
+    }
   } else {
     o[0] = abar[4] * a.V * a.dt + qbar * 2.0 * a.V * a.kappa_c;
     for (int e = 0; e < 4; ++e) {
@@ -267,6 +280,7 @@ extern "C" int spde_grad_cells(spde_plan* pl, const spde_grad_args* g, double* o
   a.nh1 = g->nh1;
   a.T = pl->lat.T;
   a.variable = c.variable;
+  a.a0_variable = c.a0_variable;
   a.bc = pl->lat.bc;
   a.M = pl->lat.M;
   a.N = pl->lat.N;

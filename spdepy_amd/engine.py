@@ -32,7 +32,7 @@ def _torch():
 class Coefficients:
     """spde_coeffs plus the device tensors it points to (kept alive here)."""
 
-    def __init__(self, grid, *, variable, kappa, H, G, sigma, kappa0, H0):
+    def __init__(self, grid, *, variable, kappa, H, G, sigma, kappa0, H0, a0_variable=False):
         torch = _torch()
         c = nat.Coeffs()
         c.variable = int(variable)
@@ -52,6 +52,7 @@ class Coefficients:
                 c.H_c[q] = Hc[q]
             c.G_c[0], c.G_c[1] = float(G[0]), float(G[1])
         c.kappa0 = float(kappa0)
+        c.a0_variable = int(a0_variable)
         H0 = np.asarray(H0, dtype=np.float64).reshape(4)
         for q in range(4):
             c.H0[q] = H0[q]

@@ -324,6 +324,227 @@ class WhittleMaternHa2D(_WhittleMaternBase):
     kind = "ha"
 
 
+# ------------------------------------------- variable Whittle-Matern models
+class _VarWhittleMaternBase(_StencilMixin):
+    """Spatial Matern models with B-spline fields (var_whittle_matern2D.py,
+    var_whittle_matern_anisotropic2D.py, var_whittle_matern_ha2D.py):
+    Q = A^T iDv A, A = Dv Dk - A_H(H) with the variable diffusion stencil,
+    kappa = exp(bs c_kappa), gamma = exp(bsH c_gamma) on the faces and per face
+    H = gamma I (iso), gamma I + v v^T (ani), gamma expm([[vx, vy], [vy, -vx]])
+    (ha).  par = [c_kappa, c_gamma, (c_vx, c_vy,) log tau] (Np = Nbs^2 each;
+    the reference's ha class fixes Nbs = 3).  On the GPU, A0 of the one-slice
+    plan is assembled from the fields (spde_coeffs.a0_variable)."""
+    kind = "iso"
+    label = "isotropic"
+
+    def __init__(self, grid, par=None, bc=3):
+        self.grid, self.bc = grid, bc
+        self.type = "var-whittle-matern-%s-2D-bc%d" % (self.label, bc)
+        self.Np = grid.Nbs2
+        self.Q = self.Q_fac = self.data = self.r = self.S = None
+        self._eng = None
+        if par is None:
+            self.setPars(self._default())
+        else:
+            self.setQ(par=par)
+
+    def _nfield(self):
+        return 2 if self.kind == "iso" else 4
+
+    def _default(self):
+        Np = self.Np
+        v = [[-1.0] * Np, [-1.0] * Np] + ([[0.1] * Np, [0.1] * Np] if self.kind != "iso" else [])
+        return np.hstack(v + [[np.log(100)]]).astype(np.float64)
+
+    def getPars(self, *args, **kwargs):
+        return np.hstack([self.coef, [self.tau]]).astype(np.float64)
+
+    def setPars(self, par):
+        par = np.asarray(par, dtype=np.float64)
+        self.coef = par[:-1].copy()
+        self.kappa, self.gamma = par[:self.Np], par[self.Np:2 * self.Np]
+        self.tau = par[-1]
+        self.sigma = np.log(np.sqrt(1 / np.exp(self.tau)))
+
+    def initFit(self, data, **kwargs):
+        assert data.shape[0] <= self.grid.Ns
+        self.data = data
+        self.r = data.shape[1] if data.ndim == 2 else 1
+        self.S = self.grid.getS(idxs=kwargs.get("idx"))
+        return self._default()
+
+    def _obs_cols(self):
+        S = self.S.tocsr()
+        if np.any(np.diff(S.indptr) != 1):
+            raise ValueError("observation matrix must have one entry per row")
+        return S.indices.astype(np.int64)
+
+    def _engine(self):
+        if self._eng is None:
+            self._eng = Engine(self.grid, self.bc)
+        return self._eng
+
+    # fields --------------------------------------------------------------
+    def _host_fields(self, par):
+        """kappa[Ns], H[Ns,4,2,2] and dH/d(log gamma, vx, vy) per face."""
+        g, Np = self.grid, self.Np
+        kappa = np.exp(g.bs @ par[:Np])
+        gam = np.exp(g.bsH @ par[Np:2 * Np])
+        I = np.eye(2)
+        if self.kind != "ha":
+            H = gam[:, :, None, None] * I
+            d = [H.copy()]
+            if self.kind == "ani":
+                v = np.stack([g.bsH @ par[2 * Np:3 * Np], g.bsH @ par[3 * Np:4 * Np]], axis=2)
+                H = H + v[:, :, :, None] * v[:, :, None, :]
+                for a in range(2):
+                    e = np.zeros_like(v)
+                    e[:, :, a] = 1.0
+                    d.append(v[:, :, :, None] * e[:, :, None, :] + e[:, :, :, None] * v[:, :, None, :])
+            return kappa, H, d
+        vx, vy = g.bsH @ par[2 * Np:3 * Np], g.bsH @ par[3 * Np:4 * Np]
+        a = np.sqrt(vx ** 2 + vy ** 2)
+        mV = np.zeros(vx.shape + (2, 2))
+        mV[..., 0, 0], mV[..., 0, 1], mV[..., 1, 0], mV[..., 1, 1] = vx, vy, vy, -vx
+        ch, sh = np.cosh(a), np.sinh(a)
+        H = (gam * ch)[..., None, None] * I + (gam * sh / a)[..., None, None] * mV
+        Dx = np.zeros_like(mV)
+        Dx[..., 0, 0], Dx[..., 1, 1] = 1.0, -1.0
+        Dy = np.zeros_like(mV)
+        Dy[..., 0, 1], Dy[..., 1, 0] = 1.0, 1.0
+        d = [H.copy()]
+        for u, D in ((vx, Dx), (vy, Dy)):
+            d.append((gam * sh * u / a)[..., None, None] * I + (gam * (ch * u - sh * u / a) / a ** 2)[..., None, None] * mV
+                     + (gam * sh / a)[..., None, None] * D)
+        return kappa, H, d
+
+    def _dev_bases(self):
+        if self.kind == "ha" or self.grid.Ne:
+            return None
+        if getattr(self, "_bases", None) is None:
+            import torch
+            g = self.grid
+            bs = torch.as_tensor(np.ascontiguousarray(g.bs, dtype=np.float64)).cuda()
+            bsH = torch.as_tensor(np.ascontiguousarray(g.bsH, dtype=np.float64)).cuda()
+            wd = ctypes.c_int64()
+            nat.check(nat.lib().spde_fields_contract_workspace(g.Ns, self.Np, ctypes.byref(wd)))
+            work = torch.empty(int(wd.value), dtype=torch.float64, device="cuda")
+            self._bases = ((bs, bsH, bsH), work)   # no advection faces: zero coefficients
+        return self._bases
+
+    def _coeffs(self, par):
+        geom = _SpatialGeom(self.grid)
+        bases = self._dev_bases()
+        if bases is None:
+            kappa, H, d = self._host_fields(par)
+            self._hfields = (kappa, H, d)
+            return Coefficients(geom, variable=True, kappa=kappa, H=H, G=np.zeros((self.grid.Ns, 4)), sigma=1.0,
+                                kappa0=1.0, H0=np.eye(2), a0_variable=True)
+        import torch
+        (bs, bsH, bsA), _ = bases
+        Ns, nb = self.grid.Ns, self.Np
+        c6 = np.zeros(6 * nb)
+        c6[:self._nfield() * nb] = par[:self._nfield() * nb]
+        coef = torch.as_tensor(c6).cuda()
+        kappa = torch.empty(Ns, dtype=torch.float64, device="cuda")
+        H = torch.empty(Ns * 16, dtype=torch.float64, device="cuda")
+        G = torch.empty(Ns * 4, dtype=torch.float64, device="cuda")
+        fgv = torch.empty(Ns * 12, dtype=torch.float64, device="cuda")
+        st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        nat.check(nat.lib().spde_fields_eval(Ns, nb, _p(bs), _p(bsH), _p(bsA), _p(coef), _p(kappa), _p(H), _p(G),
+                                             _p(fgv), st))
+        self._fields_dev = (kappa, fgv)
+        return Coefficients(geom, variable=True, kappa=kappa, H=H, G=G, sigma=1.0, kappa0=1.0, H0=np.eye(2),
+                            a0_variable=True)
+
+    def _device_contract(self):
+        bases = self._dev_bases()
+        if bases is None:
+            return None
+
+        def contract(cells):
+            import torch
+            (bs, bsH, bsA), work = bases
+            kappa, fgv = self._fields_dev
+            Ns, nb = self.grid.Ns, self.Np
+            res = torch.empty(6 * nb + OUT_WIDTH, dtype=torch.float64, device="cuda")
+            st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+            nat.check(nat.lib().spde_fields_contract(Ns, nb, _p(bs), _p(bsH), _p(bsA), _p(cells), _p(kappa), _p(fgv),
+                                                     _p(work), _p(res), st))
+            h = res.cpu().numpy()
+            return h[:6 * nb], h[6 * nb:]
+        return contract
+
+    # precision and likelihood ------------------------------------------
+    def makeQ(self, par, grad=True):
+        """Q on the GPU with the reference's scipy pattern; dQ is not
+        materialised (logLike contracts it)."""
+        par = np.asarray(par, dtype=np.float64)
+        eng = self._engine()
+        qval = eng.assemble(self._coeffs(par))
+        Q = eng.to_scipy(qval)
+        self._last_qval = qval
+        return Q, Factor(eng, 0, qval), (_LazyDQ(self, par) if grad else None)
+
+    def setQ(self, par=None, S=None, simple=False):
+        if par is None:
+            par = self.getPars()
+        else:
+            self.setPars(par)
+        if S is not None:
+            self.S = S
+        self.Q, self.Q_fac, _ = self.makeQ(par=par, grad=False)
+        self.S = self.grid.getS()
+
+    def logLike(self, par, nh1=100, grad=True, trace="exact"):
+        """var_whittle_matern2D.py:83-130 (and the anisotropic / ha files):
+        -loglik/(m r) and its gradient; trace="exact" uses the selected
+        inverses of Q and Q_c, "hutchinson" the reference's estimator."""
+        par = np.asarray(par, dtype=np.float64)
+        eng = self._engine()
+        coeffs = self._coeffs(par)
+        probes = None
+        if grad and trace == "hutchinson":
+            probes = (2 * np.random.randint(1, 3, self.grid.n * nh1) - 3).reshape(self.grid.n, nh1)
+        like, parts = eng.loglik(coeffs, self._obs_cols(), self.data, par[-1], grad=grad, trace=trace,
+                                 probes=probes, logdet="factor", contract=self._device_contract())
+        if not grad:
+            return like
+        nf, Np = self._nfield(), self.Np
+        if "coef_grad" in parts:
+            g = list(parts["coef_grad"][:nf * Np])
+        else:
+            gr = self.grid
+            cells = parts["cells"]
+            kappa, H, d = self._hfields
+            Hb = cells[:, OUT_H:OUT_H + 16].reshape(-1, 4, 2, 2)
+            g = list(gr.bs.T @ (cells[:, OUT_KAPPA] * kappa))
+            for dH in d:
+                g += list(np.einsum("kfi,kf->i", gr.bsH, (Hb * dH).sum(axis=(2, 3))))
+        g.append(parts["g_tau"])
+        m, r = parts["m"], parts["r"]
+        return like, -np.array(g) / (m * r)
+
+    def print(self, par):
+        Np = self.Np
+        s = "| κ = %2.2f, γ = %2.2f" % (np.exp(par[:Np]).mean(), np.exp(par[Np:2 * Np]).mean())
+        if self.kind != "iso":
+            s += ", vx = %2.2f, vy = %2.2f" % (par[2 * Np:3 * Np].mean(), par[3 * Np:4 * Np].mean())
+        return s + ",τ = %2.2f" % np.exp(par[-1])
+
+
+class VarWhittleMatern2D(_VarWhittleMaternBase):
+    kind, label = "iso", "isotropic"
+
+
+class VarWhittleMaternAnisotropic2D(_VarWhittleMaternBase):
+    kind, label = "ani", "anisotropic"
+
+
+class VarWhittleMaternHa2D(_VarWhittleMaternBase):
+    kind, label = "ha", "ha"
+
+
 # ---------------------------------------------------- space-time models
 class _SpaceTimeBase(_StencilMixin):
     """Shared logLike / makeQ plumbing of the space-time models."""
@@ -677,6 +898,9 @@ def spde_init(model, grid, parameters=None, ani=True, ha=True, bc=3, mod0=None):
         raise NotImplementedError("only 2-D spatial and space-time grids")
     if model in ("whittle-matern", 1):
         cls = WhittleMaternHa2D if ha else (WhittleMaternAnisotropic2D if ani else WhittleMatern2D)
+        return cls(grid=grid, par=parameters, bc=bc)
+    if model in ("var-whittle-matern", -1):
+        cls = VarWhittleMaternHa2D if ha else (VarWhittleMaternAnisotropic2D if ani else VarWhittleMatern2D)
         return cls(grid=grid, par=parameters, bc=bc)
     if model in ("advection-diffusion", 2):
         cls = AdvectionHaDiffusion2D if ha else (AdvectionDiffusion2D if ani else AdvectionIDiffusion2D)

@@ -273,3 +273,39 @@ def test_spatial_whittle_matern_loglike(kind, bc):
     like, jac = m.logLike(par)
     assert abs(like - like_o) <= 1e-9 * abs(like_o), (like, like_o)
     assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * np.abs(jac_o).max()), (jac, jac_o)
+
+
+@pytest.mark.parametrize("kind,bc", [("iso", 1), ("ani", 3), ("ha", 1)])
+def test_var_whittle_matern_q_and_loglike(kind, bc):
+    """Variable Whittle-Matern models (var_whittle_matern*2D.py): Q (pattern
+    and values) and the exact-trace logLike value and gradient against the
+    oracle, with A0 assembled from the fields on the one-slice plan."""
+    from oracle import grid as OG
+    import spdepy_amd as sp
+    from spdepy_amd import spdes
+    M, N = 11, 9
+    g0 = OG.SpatialGrid(F.axis(M), F.axis(N), Nbs=3)
+    wm = Mo.VarWhittleMatern(g0, kind, bc=bc)
+    rng = np.random.default_rng(4)
+    par = 0.3 * rng.normal(size=wm.Np * (2 if kind == "iso" else 4) + 1)
+    par[-1] = np.log(50.0)
+    Qo, dQo = wm.makeQ(par)
+    idx = np.sort(rng.choice(M * N, 45, replace=False))
+    x = np.linalg.solve(np.linalg.cholesky(Qo.toarray()).T, rng.normal(size=M * N))
+    y = x[g0.n2e(idx)] + 0.1 * rng.normal(size=idx.size)
+    like_o, jac_o, _ = Mo.loglike(Qo, dQo, g0.getS(idx), y, par[-1], trace="exact")
+    pg = sp.grid(x=F.axis(M), y=F.axis(N), Nbs=3)
+    cls = {"iso": spdes.VarWhittleMatern2D, "ani": spdes.VarWhittleMaternAnisotropic2D,
+           "ha": spdes.VarWhittleMaternHa2D}[kind]
+    m = cls(grid=pg, par=par.copy(), bc=bc)
+    step("var Whittle-Matern Q " + kind)
+    Q = m.Q.tocsc()
+    Qo = Qo.tocsc()
+    Qo.sort_indices(); Q.sort_indices()
+    assert np.array_equal(Q.indptr, Qo.indptr) and np.array_equal(Q.indices, Qo.indices)
+    assert np.allclose(Q.data, Qo.data, rtol=1e-12, atol=1e-12 * abs(Qo.data).max())
+    m.initFit(y, idx=idx)
+    step("var Whittle-Matern logLike " + kind)
+    like, jac = m.logLike(par)
+    assert abs(like - like_o) <= 1e-9 * abs(like_o), (like, like_o)
+    assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * np.abs(jac_o).max())

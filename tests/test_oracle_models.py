@@ -146,3 +146,30 @@ def test_hutchinson_probes_stream():
     raw = np.random.randint(1, 3, 40)
     assert np.array_equal(V1.ravel(), 2 * raw - 3)
     assert set(np.unique(V1)) == {-1.0, 1.0}
+
+
+@pytest.mark.parametrize("kind", ["iso", "ani", "ha"])
+def test_var_whittle_matern_derivatives_and_h_map(kind):
+    """Variable Whittle-Matern oracle: every dQ_i against central differences
+    of Q, and the ha face tensor against gamma * expm([[vx, vy], [vy, -vx]])."""
+    from scipy.linalg import expm
+    from oracle import grid as OG
+    g = OG.SpatialGrid(F.axis(9), F.axis(8), Nbs=3)
+    m = Mo.VarWhittleMatern(g, kind, bc=1)
+    rng = np.random.default_rng(1)
+    par = 0.3 * rng.normal(size=m.Np * (2 if kind == "iso" else 4) + 1)
+    Q, dQ = m.makeQ(par)
+    for i in range(0, par.size - 1, 4):
+        h = 1e-6
+        pp, pm = par.copy(), par.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (m.makeQ(pp, False)[0] - m.makeQ(pm, False)[0]) / (2 * h)
+        assert abs(fd - dQ[i]).max() <= 1e-6 * abs(dQ[i]).max(), i
+    if kind == "ha":
+        _, H, *_ = m.fields(par)
+        _, cg, cvx, cvy = m.split(par)
+        vx, vy, gam = g.evalBH(cvx), g.evalBH(cvy), np.exp(g.evalBH(cg))
+        for k, f in [(0, 0), (17, 2), (40, 3)]:
+            ref = gam[k, f] * expm(np.array([[vx[k, f], vy[k, f]], [vy[k, f], -vx[k, f]]]))
+            assert np.allclose(H[k, f], ref, rtol=1e-13, atol=1e-15)
