@@ -108,6 +108,10 @@ typedef struct spde_coeffs {
    * (variable Whittle-Matern models, var_whittle_matern2D.py:61-65) instead
    * of the constant kappa0, H0. */
   int a0_variable;
+  /* With variable = 1: 0 every field per cell; 1 constant advection G_c with
+   * the constant stencil (advection_var_*diffusion2D.py); 2 constant kappa_c
+   * and H_c with the constant stencil (var_advection_*diffusion2D.py). */
+  int mix;
 } spde_coeffs;
 
 /* Values of Q on the structural pattern (qval[nnz_struct], device). */

@@ -7,6 +7,8 @@ config1: stationary isotropic advection-diffusion, 30x30 cells x 10 steps
 config2: B-spline (Nbs=8) advection and anisotropic diffusion on 100x100 x 20
          steps (n = 200,000), 386 parameters, 5,000 observations.
 small(): the config-2 model class on reduced grids, for dense checks.
+field_model(): the families with some coefficients as B-spline fields and
+         the rest constant, on small grids (dense sampling).
 """
 import numpy as np
 from . import grid as G
@@ -76,3 +78,27 @@ def config2():
 
 def small(M=8, N=7, T=4, Nbs=3, bc=1, m=40, ne=None):
     return var_model(M, N, T, Nbs=Nbs, bc=bc, m=m, ne=ne)
+
+
+def field_par(model, rng, sd=0.3):
+    """kappa = 1, gamma = 0.5 plus field noise; random v and w."""
+    nd, nh, nw = model.sizes()
+    blocks = [np.zeros(nd) + (sd * rng.normal(size=nd) if nd > 1 else 0.0),
+              np.full(nd, np.log(0.5)) + (sd * rng.normal(size=nd) if nd > 1 else 0.0)]
+    blocks += [sd * rng.normal(size=nd) + 0.2 for _ in range(nh - 1)]
+    blocks.append(sd * rng.normal(size=nw))
+    return np.hstack(blocks + [[0.0, np.log(100.0)]])
+
+
+def field_model(M=8, N=7, T=4, kind="iso", dvar=True, gvar=False, bc=3, m=40, seeds=(8, 9, 10, 11)):
+    g = G.STGrid(axis(M), axis(N), np.arange(float(T)), Nbs=3)
+    mod0 = mod0_iso(M, N, bc=bc)
+    model = Mo.FieldAdvDiff(g, mod0, kind=kind, dvar=dvar, gvar=gvar, bc=bc)
+    par = field_par(model, np.random.default_rng(seeds[0]))
+    Q, _ = model.makeQ(par, grad=False)
+    L = np.linalg.cholesky(Q.toarray())
+    z = np.random.default_rng(seeds[2]).normal(size=(g.n, 1))
+    x = np.linalg.solve(L.T, z)
+    idx, y = observations(M * N * T, m, x, g, np.random.default_rng(seeds[1]),
+                          np.random.default_rng(seeds[3]))
+    return dict(grid=g, mod0=mod0, model=model, par=par, idx=idx, y=y, x=x)

@@ -383,6 +383,136 @@ class VarAdvVarDiff(_STBase):
         return A.tocsc(), g.V * kappa ** 2, np.exp(par[Np * 6])
 
 
+
+def _face_tensors(kind, gamma, vx, vy):
+    """Per-face H and its derivatives in (log gamma, vx, vy) for B-spline
+    diffusion fields (the H maps of the advection-var-* and var-advection-var-*
+    files): gamma I, gamma I + v v^T, gamma (cosh a I + sinh a / a mV)."""
+    I = np.eye(2)
+    H = gamma[:, :, None, None] * I
+    if kind == "iso":
+        return H, [H.copy()]
+    if kind == "ani":
+        vv = np.stack([vx, vy], axis=2)
+        d = [H.copy()]
+        for a in range(2):
+            e = np.zeros_like(vv)
+            e[:, :, a] = 1.0
+            d.append(vv[:, :, :, None] * e[:, :, None, :] + e[:, :, :, None] * vv[:, :, None, :])
+        return H + vv[:, :, :, None] * vv[:, :, None, :], d
+    a = np.sqrt(vx ** 2 + vy ** 2)
+    mV = np.zeros(vx.shape + (2, 2))
+    mV[..., 0, 0], mV[..., 0, 1], mV[..., 1, 0], mV[..., 1, 1] = vx, vy, vy, -vx
+    ch, sh = (np.exp(a) + np.exp(-a)) / 2, (np.exp(a) - np.exp(-a)) / 2
+    H = (gamma * ch)[..., None, None] * I + (gamma * sh / a)[..., None, None] * mV
+    Dx = np.zeros_like(mV)
+    Dx[..., 0, 0], Dx[..., 1, 1] = 1.0, -1.0
+    Dy = np.zeros_like(mV)
+    Dy[..., 0, 1], Dy[..., 1, 0] = 1.0, 1.0
+    d = [H.copy()]
+    for u, D in ((vx, Dx), (vy, Dy)):
+        d.append((gamma * sh * u / a)[..., None, None] * I + (gamma * (ch * u - sh * u / a) / a ** 2)[..., None, None] * mV
+                 + (gamma * sh / a)[..., None, None] * D)
+    return H, d
+
+
+class FieldAdvDiff(_STBase):
+    """The families between the stationary and the fully variable model:
+    advection_var_{i,,ha_}diffusion2D.py (B-spline kappa and diffusion,
+    constant advection w) and var_advection_{i,,ha_}diffusion2D.py,
+    var_advection_var_{i,ha_}diffusion2D.py (B-spline advection; diffusion
+    constant or B-spline).  A = Dv + (Dv Dk - A_H + A_w) dt.  par = [kappa
+    block, gamma block, (vx, vy blocks,) w block, log sigma, (mod0...),
+    log tau]; a block is Np coefficients when its field is a B-spline field
+    (2 Np for w), else one log-scalar (two scalars for w)."""
+
+    def __init__(self, grid, mod0, kind="iso", dvar=True, gvar=True, bc=3):
+        self.grid, self.mod0, self.kind, self.bc = grid, mod0, kind, bc
+        self.dvar, self.gvar = dvar, gvar
+        self.Np = grid.Nbs2
+
+    def sizes(self):
+        nd = self.Np if self.dvar else 1
+        nh = 1 if self.kind == "iso" else 3
+        return nd, nh, (2 * self.Np if self.gvar else 2)
+
+    def split(self, par):
+        par = np.asarray(par, dtype=np.float64)
+        nd, nh, nw = self.sizes()
+        k = 0
+        blocks = []
+        for size in [nd] * (1 + nh) + [nw]:
+            blocks.append(par[k:k + size])
+            k += size
+        sig = par[k]
+        p0 = par[k + 1:] if par.size > k + 2 else None
+        return blocks, sig, p0
+
+    def makeQ(self, par, grad=True):
+        g = self.grid
+        dt, T, Ns, Dv, iDv = g.dt, g.T, g.Ns, g.Dv, g.iDv
+        blocks, sig, p0 = self.split(par)
+        sigma = np.exp(sig)
+        ck, cw = blocks[0], blocks[-1]
+        if self.dvar:
+            kappa = np.exp(g.evalB(ck))
+            gam = np.exp(g.evalBH(blocks[1]))
+            vx = g.evalBH(blocks[2]) if self.kind != "iso" else None
+            vy = g.evalBH(blocks[3]) if self.kind != "iso" else None
+            Hs, dH = _face_tensors(self.kind, gam, vx, vy)
+            A_H = ah_var_csc(g, Hs, self.bc)
+        else:
+            kappa = np.exp(ck[0]) * np.ones(Ns)
+            vx, vy = (blocks[2][0], blocks[3][0]) if self.kind != "iso" else (0.0, 0.0)
+            Hc, dHc = h_map(self.kind, np.exp(blocks[1][0]), vx, vy)
+            A_H = ah_const_csc(g, Hc, self.bc)
+        if self.gvar:
+            ws = g.evalAdv(cw)
+            A_w = aw_var_csc(g, ws, self.bc)
+        else:
+            A_w = aw_const_csc(g, cw, self.bc)
+        Dk = sparse.diags(kappa).tocsc()
+        As = Dv @ Dk
+        Qs = As.transpose() @ iDv @ As
+        A = Dv + (Dv @ Dk - A_H + A_w) * dt
+        Q0, dQ0 = self._q0(p0, grad)
+        Q = assemble_blocks(A, Qs, Q0, iDv, T, sigma, dt).tocsc() / (dt * sigma)
+        if not grad:
+            return Q, None
+        c = dt * sigma
+        dQ = []
+
+        def dkappa(dk):
+            dk = sparse.diags(dk).tocsc()
+            dAs = Dv @ dk
+            dQs = dAs.T @ iDv @ As + As.T @ iDv @ dAs
+            dQ.append(d_blocks(A, Qs, iDv, T, dt * Dv @ dk, dQs) / c)
+        if self.dvar:
+            for i in range(self.Np):
+                dkappa(g.bs[:, i] * kappa)
+            for d in dH:
+                for i in range(self.Np):
+                    dQ.append(d_blocks(A, Qs, iDv, T, -ah_var_csc(g, g.bsH[:, :, i][:, :, None, None] * d, self.bc)
+                                       * dt) / c)
+        else:
+            dkappa(kappa)
+            for d in dHc:
+                dQ.append(d_blocks(A, Qs, iDv, T, -ah_const_csc(g, d, self.bc) * dt) / c)
+        if self.gvar:
+            for i in range(2 * self.Np):
+                e = np.zeros(2 * self.Np)
+                e[i] = 1.0
+                dQ.append(d_blocks(A, Qs, iDv, T, aw_var_csc(g, ws, self.bc, 1 if i < self.Np else 2, g.evalAdv(e))
+                                   * dt) / c)
+        else:
+            for diff in (1, 2):
+                dQ.append(d_blocks(A, Qs, iDv, T, aw_const_csc(g, cw, self.bc, diff=diff) * dt) / c)
+        full = assemble_blocks(A, Qs, Q0, iDv, T, sigma, dt, diag0=Qs)
+        dQ.append(-full.tocsc() / c)
+        if dQ0 is not None:
+            dQ += [pad_q0(d, Ns, T) for d in dQ0]
+        return Q, dQ
+
 def stationary_spatial_operator(model, par):
     g = model.grid
     kap, gam, vx, vy, ws, sig, _ = model.split(par)

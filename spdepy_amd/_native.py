@@ -42,7 +42,8 @@ class Coeffs(ctypes.Structure):
                 ("dt", ctypes.c_double), ("V", ctypes.c_double), ("sigma", ctypes.c_double),
                 ("kappa_c", ctypes.c_double), ("H_c", ctypes.c_double * 4), ("G_c", ctypes.c_double * 2),
                 ("kappa", vp), ("H", vp), ("G", vp), ("kappa0", ctypes.c_double),
-                ("H0", ctypes.c_double * 4), ("a0_variable", ctypes.c_int)]
+                ("H0", ctypes.c_double * 4), ("a0_variable", ctypes.c_int),
+                ("mix", ctypes.c_int)]
 
 
 class GradArgs(ctypes.Structure):

@@ -70,6 +70,7 @@ struct CoeffArgs {
   const double* G;
   double kappa0, H0[4];
   int a0_variable;
+  int mix;  // with variable: 0 all fields, 1 constant advection G_c, 2 constant kappa_c and H_c
 };
 
 // A = Dv + (Dv Dk - A_H + A_w) dt (B-spline models) or
@@ -82,15 +83,16 @@ __global__ void canonical_kernel(int M, int N, int bc, CoeffArgs a, double* __re
   Cell9 d;
   Cell5 w;
   double kap;
-  if (a.variable) {
+  const bool kvar = a.variable && a.mix != 2, dvar = a.variable && a.mix != 2, gvar = a.variable && a.mix != 1;
+  if (dvar)
     diffusion_var(i, j, M, N, bc, a.hx, a.hy, a.H + 16 * (int64_t)k, d);
-    advection_var(i, j, M, N, bc, a.hx, a.hy, a.G + 4 * (int64_t)k, nullptr, 3, w);
-    kap = a.kappa[k];
-  } else {
+  else
     diffusion_const(i, j, M, N, bc, a.hx, a.hy, a.H_c, d);
+  if (gvar)
+    advection_var(i, j, M, N, bc, a.hx, a.hy, a.G + 4 * (int64_t)k, nullptr, 3, w);
+  else
     advection_const(i, j, M, N, bc, a.hx, a.hy, a.G_c[0], a.G_c[1], 3, w);
-    kap = a.kappa_c;
-  }
+  kap = kvar ? a.kappa[k] : a.kappa_c;
   double ah[9], aw[9];
   for (int o = 0; o < 9; ++o) ah[o] = aw[o] = 0.0;
   for (int s = 0; s < 9; ++s)
@@ -331,8 +333,8 @@ int spde_assemble(spde_plan* pl, const spde_coeffs* c, double* qval, void* strea
     set_error("spde_assemble: invalid argument");
     return SPDE_ERR_ARG;
   }
-  if (c->variable && (!c->kappa || !c->H || !c->G)) {
-    set_error("spde_assemble: variable coefficients need kappa, H and G");
+  if (c->variable && (c->mix < 0 || c->mix > 2 || (c->mix != 2 && (!c->kappa || !c->H)) || (c->mix != 1 && !c->G))) {
+    set_error("spde_assemble: variable coefficients need their fields (kappa, H unless mix = 2; G unless mix = 1)");
     return SPDE_ERR_ARG;
   }
   hipStream_t s = (hipStream_t)stream;
@@ -354,6 +356,7 @@ int spde_assemble(spde_plan* pl, const spde_coeffs* c, double* qval, void* strea
   a.G = c->G;
   a.kappa0 = c->kappa0;
   a.a0_variable = c->a0_variable;
+  a.mix = c->variable ? c->mix : 0;
   if (a.a0_variable && !c->variable) {
     set_error("spde_assemble: a0_variable needs the variable fields");
     return SPDE_ERR_ARG;

@@ -21,7 +21,7 @@
 namespace spde {
 
 struct GradKArgs {
-  int mode, r, nh1, T, variable, bc, M, N, a0_variable;
+  int mode, r, nh1, T, variable, bc, M, N, a0_variable, mix;
   int64_t Ns, n;
   double c, V, dt, hx, hy;
   double H_c[4], G_c[2], kappa_c;
@@ -179,9 +179,11 @@ __global__ void grad_cell_kernel(GradKArgs a, const double* __restrict__ win, do
     for (int q = 0; q < 9; ++q) s += a0bar[q] * canv[q];
     o[26 + e] = -s;
   }
-  if (a.variable) {
-    const double kap = a.kappa[m];
-    o[0] = abar[4] * a.V * a.dt + qbar * 2.0 * a.V * kap;
+  // kappa, diffusion and advection each constant or a per-cell field
+  const bool kvar = a.variable && a.mix != 2, dvar = a.variable && a.mix != 2, gvar = a.variable && a.mix != 1;
+  const double kap = kvar ? a.kappa[m] : a.kappa_c;
+  o[0] = abar[4] * a.V * a.dt + qbar * 2.0 * a.V * kap;
+  if (dvar) {
     double Hk[16];
     for (int e = 0; e < 16; ++e) {
       for (int q = 0; q < 16; ++q) Hk[q] = 0.0;
@@ -189,10 +191,28 @@ __global__ void grad_cell_kernel(GradKArgs a, const double* __restrict__ win, do
       Cell9 c9;
       diffusion_var(i, j, M, N, bc, a.hx, a.hy, Hk, c9);
       canon9(c9, i, j, M, N, canv);
+      double s = 0.0, s0 = 0.0;
+      for (int q = 0; q < 9; ++q) {
+        s += abar[q] * canv[q];
+        s0 += a0bar[q] * canv[q];
+      }
+      o[1 + e] = -a.dt * s;
+      if (a.a0_variable) o[1 + e] -= s0;  // A0 = Dv Dk - A_H(H) from the same fields
+    }
+    if (a.a0_variable) o[0] += a0bar[4] * a.V;
+  } else {
+    for (int e = 0; e < 4; ++e) {
+      double h[4] = {0.0, 0.0, 0.0, 0.0};
+      h[e] = 1.0;
+      Cell9 c9;
+      diffusion_const(i, j, M, N, bc, a.hx, a.hy, h, c9);
+      canon9(c9, i, j, M, N, canv);
       double s = 0.0;
       for (int q = 0; q < 9; ++q) s += abar[q] * canv[q];
-      o[1 + e] = -a.dt * s;
+      o[30 + e] = -a.dt * s;
     }
+  }
+  if (gvar) {
     const double* Gk = a.G + 4 * (int64_t)m;
     for (int diff = 1; diff <= 2; ++diff)
       for (int f = 0; f < 4; ++f) {
@@ -205,31 +225,7 @@ __global__ void grad_cell_kernel(GradKArgs a, const double* __restrict__ win, do
         for (int q = 0; q < 9; ++q) s += abar[q] * canv[q];
         o[17 + 4 * (diff - 1) + f] = a.dt * s;
       }
-    if (a.a0_variable) {  // A0 = Dv Dk - A_H(H): kappa and H adjoints through A0
-      o[0] += a0bar[4] * a.V;
-      for (int e = 0; e < 16; ++e) {
-        for (int q = 0; q < 16; ++q) Hk[q] = 0.0;
-        Hk[e] = 1.0;
-        Cell9 c9;
-        diffusion_var(i, j, M, N, bc, a.hx, a.hy, Hk, c9);
-        canon9(c9, i, j, M, N, canv);
-        double s = 0.0;
-        for (int q = 0; q < 9; ++q) s += a0bar[q] * canv[q];
-        o[1 + e] -= s;
-      }
-    }
   } else {
-    o[0] = abar[4] * a.V * a.dt + qbar * 2.0 * a.V * a.kappa_c;
-    for (int e = 0; e < 4; ++e) {
-      double h[4] = {0.0, 0.0, 0.0, 0.0};
-      h[e] = 1.0;
-      Cell9 c9;
-      diffusion_const(i, j, M, N, bc, a.hx, a.hy, h, c9);
-      canon9(c9, i, j, M, N, canv);
-      double s = 0.0;
-      for (int q = 0; q < 9; ++q) s += abar[q] * canv[q];
-      o[30 + e] = -a.dt * s;
-    }
     for (int diff = 1; diff <= 2; ++diff) {
       Cell5 c5;
       advection_const(i, j, M, N, bc, a.hx, a.hy, a.G_c[0], a.G_c[1], diff, c5);
@@ -269,8 +265,8 @@ extern "C" int spde_grad_cells(spde_plan* pl, const spde_grad_args* g, double* o
     return SPDE_ERR_ARG;
   }
   const spde_coeffs& c = g->coeffs;
-  if (c.variable && (!c.kappa || !c.H || !c.G)) {
-    set_error("spde_grad_cells: variable coefficients need kappa, H and G");
+  if (c.variable && (c.mix < 0 || c.mix > 2 || (c.mix != 2 && (!c.kappa || !c.H)) || (c.mix != 1 && !c.G))) {
+    set_error("spde_grad_cells: variable coefficients need their fields (kappa, H unless mix = 2; G unless mix = 1)");
     return SPDE_ERR_ARG;
   }
   hipStream_t s = (hipStream_t)stream;
@@ -281,6 +277,7 @@ extern "C" int spde_grad_cells(spde_plan* pl, const spde_grad_args* g, double* o
   a.T = pl->lat.T;
   a.variable = c.variable;
   a.a0_variable = c.a0_variable;
+  a.mix = c.variable ? c.mix : 0;
   a.bc = pl->lat.bc;
   a.M = pl->lat.M;
   a.N = pl->lat.N;

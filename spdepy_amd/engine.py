@@ -32,7 +32,11 @@ def _torch():
 class Coefficients:
     """spde_coeffs plus the device tensors it points to (kept alive here)."""
 
-    def __init__(self, grid, *, variable, kappa, H, G, sigma, kappa0, H0, a0_variable=False):
+    def __init__(self, grid, *, variable, kappa, H, G, sigma, kappa0, H0, a0_variable=False, mix=0,
+                 kappa_c=None, H_c=None, G_c=None):
+        """variable=True: per-cell kappa[Ns], H[Ns,4,2,2], G[Ns,4] (or device
+        tensors); mix=1 takes the constant advection G_c instead of G, mix=2
+        the constant kappa_c and H_c instead of kappa and H."""
         torch = _torch()
         c = nat.Coeffs()
         c.variable = int(variable)
@@ -40,11 +44,23 @@ class Coefficients:
         self.keep = []
         if variable:
             def dev(a):   # device tensors (fields evaluated on the GPU) are used as they are
+                if a is None:
+                    return None
                 if isinstance(a, torch.Tensor):
                     return a.contiguous()
                 return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64).reshape(-1)).cuda()
             self.kappa, self.H, self.G = dev(kappa), dev(H), dev(G)
-            c.kappa, c.H, c.G = self.kappa.data_ptr(), self.H.data_ptr(), self.G.data_ptr()
+            c.kappa = self.kappa.data_ptr() if self.kappa is not None else None
+            c.H = self.H.data_ptr() if self.H is not None else None
+            c.G = self.G.data_ptr() if self.G is not None else None
+            c.mix = int(mix)
+            if mix == 1:
+                c.G_c[0], c.G_c[1] = float(G_c[0]), float(G_c[1])
+            elif mix == 2:
+                c.kappa_c = float(kappa_c)
+                Hc = np.asarray(H_c, dtype=np.float64).reshape(4)
+                for q in range(4):
+                    c.H_c[q] = Hc[q]
         else:
             c.kappa_c = float(kappa)
             Hc = np.asarray(H, dtype=np.float64).reshape(4)

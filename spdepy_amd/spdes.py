@@ -877,6 +877,195 @@ class VarAdvectionVarDiffusion2D(_SpaceTimeBase):
                                                      np.exp(par[6 * Np]), np.exp(par[-1])))
 
 
+class _FieldAdvDiff(_SpaceTimeBase):
+    """Space-time families with some coefficients as B-spline fields and the
+    rest constant (advection_var_{i,,ha_}diffusion2D.py: B-spline kappa and
+    diffusion, constant advection; var_advection_{i,,ha_}diffusion2D.py:
+    constant kappa and diffusion, B-spline advection;
+    var_advection_var_{i,ha_}diffusion2D.py: all B-spline).  par = [kappa
+    block, gamma block, (vx, vy blocks,) w block, log sigma, (mod0...),
+    log tau], a block being Np = Nbs^2 coefficients for a field (2 Np for w)
+    or one log-scalar (two scalars for w) for a constant; the reference's
+    files fix Nbs = 3.  The per-cell adjoints are contracted on the host."""
+    kind, dvar, gvar, name = "iso", True, False, ""
+    w0 = 0.1    # initial advection coefficients
+
+    def __init__(self, grid, mod0, par=None, bc=3):
+        self.grid, self.mod0, self.bc = grid, mod0, bc
+        self.type = "%s-bc%d" % (self.name, bc)
+        self.Np = grid.Nbs2
+        self.Q = self.Q_fac = self.data = self.r = self.S = None
+        self.fitQ0 = True
+        self._eng = None
+        if par is None:
+            self.setPars(np.hstack([self._init_self(), [np.log(100)]]))
+        else:
+            self.setQ(par=par)
+
+    def _sizes(self):
+        nd = self.Np if self.dvar else 1
+        return nd, (1 if self.kind == "iso" else 3), (2 * self.Np if self.gvar else 2)
+
+    def _nself(self):
+        nd, nh, nw = self._sizes()
+        return nd * (1 + nh) + nw + 1
+
+    def _init_self(self):
+        nd, nh, nw = self._sizes()
+        v = [[-1.0] * nd, [-1.0] * nd] + [[0.1] * nd] * (nh - 1) + [[self.w0] * nw, [0.0]]
+        return np.hstack(v)
+
+    def _split(self, par):
+        nd, nh, nw = self._sizes()
+        blocks, k = [], 0
+        for size in [nd] * (1 + nh) + [nw]:
+            blocks.append(np.asarray(par[k:k + size], dtype=np.float64))
+            k += size
+        return blocks, par[k]
+
+    def _par_mod0(self, par):
+        k = self._nself()
+        return par[k:] if par.size > k + 1 else None
+
+    def getPars(self, onlySelf=True):
+        own = np.asarray(self.own, dtype=np.float64)
+        if onlySelf:
+            return np.hstack([own, [self.tau]])
+        return np.hstack([own, self.mod0.getPars()[:-1], [self.tau]])
+
+    def setPars(self, par):
+        par = np.asarray(par, dtype=np.float64)
+        k = self._nself()
+        self.own = par[:k].copy()
+        self.sigma = par[k - 1]
+        if par.size > k + 1:
+            self.mod0.setPars(par[k:])
+        self.tau = par[-1]
+
+    def _face_fields(self, blocks):
+        """Per-face H[Ns,4,2,2] and its derivatives in (log gamma, vx, vy)."""
+        g = self.grid
+        gam = np.exp(g.bsH @ blocks[1])
+        I = np.eye(2)
+        H = gam[:, :, None, None] * I
+        if self.kind == "iso":
+            return H, [H.copy()]
+        vx, vy = g.bsH @ blocks[2], g.bsH @ blocks[3]
+        if self.kind == "ani":
+            v = np.stack([vx, vy], axis=2)
+            d = [H.copy()]
+            for a in range(2):
+                e = np.zeros_like(v)
+                e[:, :, a] = 1.0
+                d.append(v[:, :, :, None] * e[:, :, None, :] + e[:, :, :, None] * v[:, :, None, :])
+            return H + v[:, :, :, None] * v[:, :, None, :], d
+        a = np.sqrt(vx ** 2 + vy ** 2)
+        mV = np.zeros(vx.shape + (2, 2))
+        mV[..., 0, 0], mV[..., 0, 1], mV[..., 1, 0], mV[..., 1, 1] = vx, vy, vy, -vx
+        ch, sh = np.cosh(a), np.sinh(a)
+        H = (gam * ch)[..., None, None] * I + (gam * sh / a)[..., None, None] * mV
+        Dx = np.zeros_like(mV)
+        Dx[..., 0, 0], Dx[..., 1, 1] = 1.0, -1.0
+        Dy = np.zeros_like(mV)
+        Dy[..., 0, 1], Dy[..., 1, 0] = 1.0, 1.0
+        d = [H.copy()]
+        for u, D in ((vx, Dx), (vy, Dy)):
+            d.append((gam * sh * u / a)[..., None, None] * I + (gam * (ch * u - sh * u / a) / a ** 2)[..., None, None] * mV
+                     + (gam * sh / a)[..., None, None] * D)
+        return H, d
+
+    def _coeffs(self, par):
+        g = self.grid
+        blocks, lsig = self._split(par)
+        _, kappa0, H0, _ = self._mod0_split(par)
+        kw = dict(variable=True, sigma=np.exp(lsig), kappa0=kappa0, H0=H0, kappa=None, H=None, G=None)
+        if self.dvar:
+            kw["kappa"] = np.exp(g.bs @ blocks[0])
+            H, dH = self._face_fields(blocks)
+            kw["H"] = H
+        else:
+            vx, vy = (blocks[2][0], blocks[3][0]) if self.kind != "iso" else (0.0, 0.0)
+            Hc, dH = h_tensor(self.kind, np.exp(blocks[1][0]), vx, vy)
+            kw["kappa_c"], kw["H_c"] = np.exp(blocks[0][0]), Hc
+        if self.gvar:
+            kw["G"] = g.evalAdv(blocks[-1])
+        else:
+            kw["G_c"] = blocks[-1]
+        kw["mix"] = 0 if (self.dvar and self.gvar) else (1 if self.dvar else 2)
+        self._fields = (kw.get("kappa"), kw.get("kappa_c"), dH)
+        return Coefficients(g, **kw)
+
+    def _contract(self, par, coeffs, parts):
+        g = self.grid
+        _, kappa0, H0, dH0 = self._mod0_split(par)
+        kappa, kappa_c, dH = self._fields
+        colsum = parts["colsum"]
+        out = []
+        if self.dvar:
+            cells = parts["cells"]
+            out.append(g.bs.T @ (cells[:, OUT_KAPPA] * kappa))
+            Hb = cells[:, OUT_H:OUT_H + 16].reshape(-1, 4, 2, 2)
+            for d in dH:
+                out.append(np.einsum("kfi,kf->i", g.bsH, (Hb * d).sum(axis=(2, 3))))
+        else:
+            out.append([kappa_c * colsum[OUT_KAPPA]])
+            hbar = colsum[OUT_HC:OUT_HC + 4]
+            out.append([float(np.dot(hbar, d.reshape(4))) for d in dH])
+        if self.gvar:
+            cells = parts["cells"]
+            wx_bar = cells[:, OUT_DGX:OUT_DGX + 4]
+            wy_bar = cells[:, OUT_DGY:OUT_DGY + 4]
+            if g.isExtended:
+                Ae = g.ext_map()
+                wx_bar, wy_bar = Ae.T @ wx_bar, Ae.T @ wy_bar
+            out.append(g.bsA[:, 0, :].T @ wx_bar[:, 0] + g.bsA[:, 2, :].T @ wx_bar[:, 2])
+            out.append(g.bsA[:, 1, :].T @ wy_bar[:, 1] + g.bsA[:, 3, :].T @ wy_bar[:, 3])
+        else:
+            out.append([colsum[OUT_WX], colsum[OUT_WY]])
+        out.append([parts["g_sigma"]])
+        out.append(self._mod0_grad(par, parts, dH0, kappa0))
+        out.append([parts["g_tau"]])
+        return np.hstack(out).astype(np.float64)
+
+    def print(self, par):
+        blocks, lsig = self._split(np.asarray(par))
+        return "| κ = %2.2f, γ = %2.2f, σ = %2.2f, τ = %2.2f" % (np.exp(blocks[0]).mean(), np.exp(blocks[1]).mean(),
+                                                             np.exp(lsig), np.exp(par[-1]))
+
+
+class AdvectionVarIDiffusion2D(_FieldAdvDiff):
+    kind, dvar, gvar, name = "iso", True, False, "advection-var-diffusion-isotropic-2D"
+
+
+class AdvectionVarDiffusion2D(_FieldAdvDiff):
+    kind, dvar, gvar, name = "ani", True, False, "advection-var-diffusion-anisotropic-2D"
+
+
+class AdvectionVarHaDiffusion2D(_FieldAdvDiff):
+    kind, dvar, gvar, name = "ha", True, False, "advection-var-diffusion-ha-2D"
+
+
+class VarAdvectionIDiffusion2D(_FieldAdvDiff):
+    kind, dvar, gvar, name = "iso", False, True, "var-advection-diffusion-isotropic-2D"
+    w0 = 0.01   # var_advection_idiffusion2D.py:22
+
+
+class VarAdvectionDiffusion2D(_FieldAdvDiff):
+    kind, dvar, gvar, name = "ani", False, True, "var-advection-diffusion-anisotropic-2D"
+
+
+class VarAdvectionHaDiffusion2D(_FieldAdvDiff):
+    kind, dvar, gvar, name = "ha", False, True, "var-advection-diffusion-ha-2D"
+
+
+class VarAdvectionVarIDiffusion2D(_FieldAdvDiff):
+    kind, dvar, gvar, name = "iso", True, True, "var-advection-var-diffusion-isotropic-2D"
+
+
+class VarAdvectionVarHaDiffusion2D(_FieldAdvDiff):
+    kind, dvar, gvar, name = "ha", True, True, "var-advection-var-diffusion-ha-2D"
+
+
 class _LazyDQ:
     """The reference's list of dQ_i, produced on demand (one matrix per
     parameter except tau).  Not needed by logLike, whose gradient contracts all
@@ -905,9 +1094,14 @@ def spde_init(model, grid, parameters=None, ani=True, ha=True, bc=3, mod0=None):
     if model in ("advection-diffusion", 2):
         cls = AdvectionHaDiffusion2D if ha else (AdvectionDiffusion2D if ani else AdvectionIDiffusion2D)
         return cls(grid=grid, mod0=mod0, par=parameters, bc=bc)
+    if model in ("advection-var-diffusion", 3):
+        cls = AdvectionVarHaDiffusion2D if ha else (AdvectionVarDiffusion2D if ani else AdvectionVarIDiffusion2D)
+        return cls(grid=grid, mod0=mod0, par=parameters, bc=bc)
+    if model in ("var-advection-diffusion", 6):
+        cls = VarAdvectionHaDiffusion2D if ha else (VarAdvectionDiffusion2D if ani else VarAdvectionIDiffusion2D)
+        return cls(grid=grid, mod0=mod0, par=parameters, bc=bc)
     if model in ("var-advection-var-diffusion", 7):
-        if ha or not ani:
-            raise NotImplementedError("var-advection-var-diffusion is provided with anisotropic diffusion "
-                                      "(ha=False, anisotropic=True)")
-        return VarAdvectionVarDiffusion2D(grid=grid, mod0=mod0, par=parameters, bc=bc)
+        cls = (VarAdvectionVarHaDiffusion2D if ha else
+               (VarAdvectionVarDiffusion2D if ani else VarAdvectionVarIDiffusion2D))
+        return cls(grid=grid, mod0=mod0, par=parameters, bc=bc)
     raise NotImplementedError("model %r is outside the GMRF hot path built here" % (model,))

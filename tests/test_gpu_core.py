@@ -309,3 +309,40 @@ def test_var_whittle_matern_q_and_loglike(kind, bc):
     like, jac = m.logLike(par)
     assert abs(like - like_o) <= 1e-9 * abs(like_o), (like, like_o)
     assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * np.abs(jac_o).max())
+
+
+FIELD_CASES = [(k, dv, gv, 3) for k in ("iso", "ani", "ha") for dv, gv in ((True, False), (False, True), (True, True))]
+FIELD_CASES += [("iso", True, False, 1), ("ha", False, True, 1), ("ani", True, True, 1)]
+
+
+@pytest.mark.parametrize("kind,dvar,gvar,bc", FIELD_CASES)
+def test_field_family_q_and_loglike(kind, dvar, gvar, bc):
+    """Families with some coefficients as B-spline fields and the rest
+    constant (advection_var_*, var_advection_*, var_advection_var_*): Q
+    (pattern and values) and the exact-trace logLike value and gradient
+    against the oracle."""
+    import spdepy_amd as sp
+    from spdepy_amd import spdes
+    c = F.field_model(9, 8, 4, kind=kind, dvar=dvar, gvar=gvar, bc=bc, m=45)
+    g, par = c["grid"], c["par"]
+    Qo, dQo = c["model"].makeQ(par, grad=True)
+    like_o, jac_o, _ = Mo.loglike(Qo, dQo, g.getS(c["idx"]), c["y"], par[-1], trace="exact")
+    pg = sp.grid(x=g.x, y=g.y, t=g.t, Nbs=3)
+    mod0 = spdes.WhittleMatern2D(grid=sp.grid(x=g.x, y=g.y, Nbs=3), par=np.array(c["mod0"].par), bc=bc)
+    spde = {(True, False): "advection-var-diffusion", (False, True): "var-advection-diffusion",
+            (True, True): "var-advection-var-diffusion"}[(dvar, gvar)]
+    m = spdes.spde_init(spde, pg, ani=kind != "iso", ha=kind == "ha", bc=bc, mod0=mod0)
+    step("field family Q %s dvar=%d gvar=%d bc=%d" % (kind, dvar, gvar, bc))
+    Qg, _, _ = m.makeQ(par, grad=False)
+    Qo = Qo.tocsc()
+    Qo.sort_indices()
+    Qg.sort_indices()
+    assert np.array_equal(Qg.indptr, Qo.indptr) and np.array_equal(Qg.indices, Qo.indices)
+    assert np.allclose(Qg.data, Qo.data, rtol=1e-12, atol=1e-13 * np.abs(Qo.data).max())
+    m.initFit(c["y"], idx=c["idx"], fitQ0=False)
+    step("field family logLike %s dvar=%d gvar=%d bc=%d" % (kind, dvar, gvar, bc))
+    like, jac = m.logLike(par)
+    assert abs(like - like_o) <= 1e-9 * abs(like_o), (like, like_o)
+    scale = np.abs(jac_o).max()
+    assert jac.shape == jac_o.shape
+    assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * scale), np.abs(jac - jac_o).max() / scale

@@ -173,3 +173,55 @@ def test_var_whittle_matern_derivatives_and_h_map(kind):
         for k, f in [(0, 0), (17, 2), (40, 3)]:
             ref = gam[k, f] * expm(np.array([[vx[k, f], vy[k, f]], [vy[k, f], -vx[k, f]]]))
             assert np.allclose(H[k, f], ref, rtol=1e-13, atol=1e-15)
+
+
+FIELD_COMBOS = [(k, dv, gv) for k in ("iso", "ani", "ha") for dv, gv in ((True, False), (False, True), (True, True))]
+
+
+@pytest.mark.parametrize("kind,dvar,gvar", FIELD_COMBOS)
+def test_field_families_reduce_to_stationary(kind, dvar, gvar):
+    """With every B-spline field constant (the bases sum to one) and bc=3 the
+    field families are the stationary model of the same kind: Q agrees to
+    rounding.  (At bc=1 the variable stencils' border rows differ from the
+    constant ones in the reference as well, so this pin is a bc=3 one.)"""
+    from oracle import grid as OG
+    g = OG.STGrid(F.axis(8), F.axis(7), np.arange(4.0), Nbs=3)
+    mod0 = F.mod0_iso(8, 7, bc=3)
+    Np = g.Nbs2
+    nh = 1 if kind == "iso" else 3
+    ps = (np.array([0.2, np.log(0.5), 0.3, -0.2, 0.1, np.log(100.0)]) if kind == "iso" else
+          np.array([0.2, np.log(0.5), 0.4, -0.3, 0.3, -0.2, 0.1, np.log(100.0)]))
+    Qs, _ = Mo.StationaryAdvDiff(g, mod0, kind=kind, bc=3).makeQ(ps, grad=False)
+    fm = Mo.FieldAdvDiff(g, mod0, kind, dvar, gvar, bc=3)
+    nd = Np if dvar else 1
+    blocks = [np.full(nd, ps[i]) for i in range(1 + nh)]
+    w = ps[1 + nh:3 + nh]
+    blocks.append(np.repeat(w, Np) if gvar else w)
+    Qf, _ = fm.makeQ(np.hstack(blocks + [ps[-2:]]), grad=False)
+    assert abs(Qf - Qs).max() <= 1e-13 * abs(Qs).max()
+
+
+def test_field_family_ani_all_fields_is_var_model():
+    """All-field anisotropic case = the config-2 model (pinned separately)."""
+    c = F.small(8, 7, 4, bc=1)
+    fm = Mo.FieldAdvDiff(c["grid"], c["mod0"], "ani", True, True, bc=1)
+    Qv, _ = c["model"].makeQ(c["par"], grad=False)
+    Qf, _ = fm.makeQ(c["par"], grad=False)
+    assert abs(Qf - Qv).max() == 0.0
+
+
+@pytest.mark.parametrize("kind,dvar,gvar", FIELD_COMBOS)
+def test_field_family_derivatives(kind, dvar, gvar):
+    """Every 3rd dQ_i against central differences of Q (bc=3, where the
+    advection stencils' derivatives are the operator's)."""
+    c = F.field_model(kind=kind, dvar=dvar, gvar=gvar, bc=3)
+    m, par = c["model"], c["par"]
+    Q, dQ = m.makeQ(par, grad=True)
+    assert len(dQ) == par.size - 1
+    for i in list(range(0, par.size - 1, 3)) + [par.size - 2]:
+        h = 1e-6
+        pp, pm = par.copy(), par.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (m.makeQ(pp, False)[0] - m.makeQ(pm, False)[0]) / (2 * h)
+        assert abs(fd - dQ[i]).max() <= 1e-6 * max(abs(dQ[i]).max(), 1e-3 * abs(Q).max()), i
