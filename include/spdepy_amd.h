@@ -68,6 +68,11 @@ int spde_stencil_advection(int M, int N, const double* G, const double* dG, int 
  * symbolic part only (no device memory; inspection and tests). */
 typedef struct spde_plan spde_plan;
 #define SPDE_PLAN_HOST_ONLY 1
+/* Couple neighbouring time slices through the 5x5 spatial window (the pattern
+ * of Qt (x) Qs for the separable models) instead of the 3x3 window of the
+ * advection-diffusion models.  Such a plan takes its values from
+ * spde_assemble_kron only. */
+#define SPDE_PLAN_WIDE_TIME 2
 
 typedef struct spde_plan_info {
   int64_t n, nnz_struct, nnz_L, front_doubles, z_doubles;
@@ -201,9 +206,39 @@ typedef struct spde_grad_args {
   const double* sigma_gram;
   const double* sigma_q0;
   spde_plan* spatial;
+  /* One-slice plans only: when set, G is these per-entry weights (on the
+   * plan's structural pattern, both triangles) and sigma_q, sigma_qc, mu,
+   * trq, trqc, probes are not read; the outputs are the adjoints of
+   * <weights, dQ> (the separable models' spatial gradient, spde_kron_reduce). */
+  const double* weights;
 } spde_grad_args;
 
 int spde_grad_cells(spde_plan* plan, const spde_grad_args* args, double* out, void* stream);
+
+/* ---------------------------------------------------------------- separable models
+ * Q = Qt (x) Qs (seperable_spatial_temporal2D.py:80-81, _ha2D.py, _idiffusion2D.py:
+ * sparse.kron(Qt, Qs)) on a plan created with SPDE_PLAN_WIDE_TIME; `spatial` is
+ * the one-slice plan of the same M, N, bc holding Qs (its structural values,
+ * qs[nnz(spatial)], e.g. from spde_assemble + spde_assemble_spatial of the
+ * variable Whittle-Matern coefficients).  qt[2T-1] (device) = the diagonal of
+ * the tridiagonal Qt, then its off-diagonal. */
+int spde_assemble_kron(spde_plan* plan, spde_plan* spatial, const double* qs, const double* qt,
+                       double* out, void* stream);
+/* The contractions that replace the reference's dQ loop for dQ = Qt (x) dQs and
+ * dQ = dQt (x) Qs (seperable_spatial_temporal2D.py:84-118,143-146), with
+ * G = 1/2 r (Q^-1 - Q_c^-1) - 1/2 mu_c mu_c^T and Q^-1 = Qt^-1 (x) Qs^-1:
+ *   weights[s] = ws sigma_s[s] + sum_{t,t'} Qt[t,t'] (-1/2 r sigma_c - 1/2 sum_c mu mu^T)[(t,i),(t',j)]
+ * for every structural entry s = (i,j) of the one-slice plan (ws = r T / 2 and
+ * sigma_s the selected inverse of Qs give the log|Q| term; pass the weights to
+ * spde_grad_cells on the one-slice plan), and per time pair (t, t+d), d = -1, 0, 1:
+ *   blocks[(3t + d + 1) * 3 + 0] = sum_{i,j} sigma_c[(t,i),(t+d,j)] Qs[i,j]
+ *   blocks[(3t + d + 1) * 3 + 1] = sum_{c,i,j} mu[(t,i),c] mu[(t+d,j),c] Qs[i,j]
+ *   blocks[(3t + 1) * 3 + 2]     = sum_i diag_add[(t,i)] sigma_c[(t,i),(t,i)]
+ * (zeros where t+d is outside [0,T)).  sigma_c is the selected inverse of Q_c
+ * on this plan, mu[n x r] column-major, diag_add[n]; blocks[9T] (device). */
+int spde_kron_reduce(spde_plan* plan, spde_plan* spatial, const double* qs, const double* qt,
+                     const double* sigma_s, double ws, const double* sigma_c, const double* mu, int r,
+                     const double* diag_add, double* weights, double* blocks, void* stream);
 
 /* ---------------------------------------------------------------- B-spline fields
  * Coefficient fields of the variable model from its basis coefficients

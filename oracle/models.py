@@ -182,6 +182,67 @@ class VarWhittleMatern:
 
 
 # --------------------------------------------------------- space-time blocks
+class Separable:
+    """Separable space-time precision Q = Qt (x) Qs
+    (``seperable_spatial_temporal2D.py:64-124``, ``_ha2D.py``,
+    ``_idiffusion2D.py:62-98``).  Qs is the variable Whittle-Matern precision
+    of the same kind (A = Dv Dk - A_H(H), Qs = A^T iDv A); Qt is tridiagonal:
+      ani:      par[.] = log rho,  Qt = tridiag(-rho; 1, 1+rho^2, .., 1+rho^2, 1) / (1 - rho^2)
+      iso, ha:  par[.] = a, log sigma,  Qt = sigma tridiag(-a; 1, 1+a^2, .., 1+a^2, 1)
+    (``makeQt``, ``:186-211`` and ``_ha2D.py:202-225``; the latter's loop is
+    written for T = 10, the form here holds for every T).  par = [spatial
+    fields (2 or 4 Np), temporal (1 or 2), log tau]; dQ in the same order."""
+
+    def __init__(self, grid, kind="ani", bc=3):
+        from .grid import SpatialGrid
+        self.grid, self.kind, self.bc = grid, kind, bc
+        self.sgrid = SpatialGrid(grid.x, grid.y, extend=grid.Ne if grid.Ne else None, Nbs=grid.Nbs)
+        self.wm = VarWhittleMatern(self.sgrid, kind, bc)
+        self.Np = grid.Nbs2
+        self.ns = self.Np * (2 if kind == "iso" else 4)
+
+    def qt(self, tp, T, diff=False):
+        """Qt, or its derivative in log rho (ani) / in a (iso, ha)."""
+        d, o = np.zeros(T), np.zeros(T - 1)
+        if self.kind == "ani":
+            rho = np.exp(tp[0])
+            den = 1.0 - rho ** 2
+            if diff:
+                d[:] = 4 * rho ** 2 / den ** 2
+                d[0] = d[-1] = 2 * rho ** 2 / den ** 2
+                o[:] = -rho * (1 + rho ** 2) / den ** 2
+            else:
+                d[:] = (1 + rho ** 2) / den
+                d[0] = d[-1] = 1.0 / den
+                o[:] = -rho / den
+        else:
+            a, sig = tp[0], np.exp(tp[1])
+            if diff:
+                d[:] = 2 * a * sig
+                d[0] = d[-1] = 0.0
+                o[:] = -sig
+            else:
+                d[:] = (1 + a ** 2) * sig
+                d[0] = d[-1] = sig
+                o[:] = -a * sig
+        return sparse.diags([o, d, o], [-1, 0, 1]).tocsc()
+
+    def makeQ(self, par, grad=True):
+        par = np.asarray(par, dtype=np.float64)
+        T = self.grid.T
+        Qs, dQs = self.wm.makeQ(np.hstack([par[:self.ns], [0.0]]), grad)
+        tp = par[self.ns:-1]
+        Qt = self.qt(tp, T)
+        Q = sparse.kron(Qt, Qs).tocsc()
+        if not grad:
+            return Q, None
+        dQ = [sparse.kron(Qt, d).tocsc() for d in dQs]
+        dQ.append(sparse.kron(self.qt(tp, T, diff=True), Qs).tocsc())
+        if self.kind != "ani":
+            dQ.append(Q.copy())
+        return Q, dQ
+
+
 def assemble_blocks(A, Qs, Q0, iDv, T, sigma, dt, diag0=None):
     """Block-tridiagonal space-time precision (``advection_idiffusion2D.py:110-115``):
         Q00 = sigma dt Q0 + Qs,  Qtt = A^T iDv Qs iDv A + Qs,  Q_{T-1} = A^T iDv Qs iDv A,

@@ -10,6 +10,7 @@ LIB_PATH = os.path.join(_HERE, "libspdepy_amd.so")
 
 SPDE_OK, SPDE_ERR_ARG, SPDE_ERR_HIP, SPDE_ERR_NOT_SPD, SPDE_ERR_ALLOC, SPDE_ERR_STATE = range(6)
 SPDE_PLAN_HOST_ONLY = 1
+SPDE_PLAN_WIDE_TIME = 2
 SOLVE_A, SOLVE_L, SOLVE_LT = 0, 1, 2
 
 c_i64p = ctypes.POINTER(ctypes.c_int64)
@@ -50,7 +51,7 @@ class GradArgs(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("r", ctypes.c_int), ("nh1", ctypes.c_int), ("qval", vp),
                 ("sigma_q", vp), ("sigma_qc", vp), ("trq", vp), ("trqc", vp), ("probes", vp),
                 ("mu", vp), ("diag_add", vp), ("coeffs", Coeffs),
-                ("sigma_gram", vp), ("sigma_q0", vp), ("spatial", vp)]
+                ("sigma_gram", vp), ("sigma_q0", vp), ("spatial", vp), ("weights", vp)]
 
 
 _SIGS = {
@@ -66,6 +67,9 @@ _SIGS = {
     "spde_assemble": (ctypes.c_int, [vp, ctypes.POINTER(Coeffs), vp, vp]),
     "spde_assemble_spatial": (ctypes.c_int, [vp, vp, ctypes.c_int, vp, vp]),
     "spde_logq_sum": (ctypes.c_int, [vp, ctypes.c_double, c_dp, vp]),
+    "spde_assemble_kron": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+    "spde_kron_reduce": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_double, vp, vp, ctypes.c_int, vp, vp, vp,
+                                        vp]),
     "spde_fields_eval": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int] + [vp] * 8 + [vp]),
     "spde_fields_contract_workspace": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, c_i64p]),
     "spde_fields_contract": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int] + [vp] * 8 + [vp]),
@@ -94,6 +98,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("libspdepy_amd.so is not built; run `python -m spdepy_amd.build`")
+        # torch brings its own HIP runtime: it is loaded and initialised before
+        # this library makes its first HIP call, so that torch's device
+        # initialisation (which then shares the device with the library's
+        # allocations and streams) always comes first in the process.
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name, None)

@@ -337,6 +337,10 @@ int spde_assemble(spde_plan* pl, const spde_coeffs* c, double* qval, void* strea
     set_error("spde_assemble: variable coefficients need their fields (kappa, H unless mix = 2; G unless mix = 1)");
     return SPDE_ERR_ARG;
   }
+  if (pl->lat.wide) {
+    set_error("spde_assemble: a wide-time plan takes its values from spde_assemble_kron");
+    return SPDE_ERR_ARG;
+  }
   hipStream_t s = (hipStream_t)stream;
   const int M = pl->lat.M, N = pl->lat.N, bc = pl->lat.bc;
   const int64_t Ns = pl->lat.Ns();
@@ -379,7 +383,8 @@ int spde_assemble_spatial(spde_plan* st, spde_plan* sp, int which, double* out, 
     set_error("spde_assemble_spatial: invalid argument");
     return SPDE_ERR_ARG;
   }
-  if (sp->lat.T != 1 || sp->lat.M != st->lat.M || sp->lat.N != st->lat.N || sp->lat.bc != st->lat.bc) {
+  if (sp->lat.T != 1 || sp->lat.M != st->lat.M || sp->lat.N != st->lat.N || sp->lat.bc != st->lat.bc ||
+      st->lat.wide) {
     set_error("spde_assemble_spatial: the spatial plan must be the one-slice lattice of the space-time plan");
     return SPDE_ERR_ARG;
   }
@@ -392,7 +397,7 @@ int spde_assemble_spatial(spde_plan* st, spde_plan* sp, int which, double* out, 
 }
 
 int spde_logq_sum(spde_plan* pl, double c, double* out, void* stream) {
-  if (!pl || pl->host_only || !out || !(c > 0.0)) {
+  if (!pl || pl->host_only || !out || !(c > 0.0) || pl->lat.wide) {
     set_error("spde_logq_sum: invalid argument");
     return SPDE_ERR_ARG;
   }

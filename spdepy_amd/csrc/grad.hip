@@ -48,6 +48,7 @@ struct GradKArgs {
   const double* sgram;
   const double* sq0;
   const int32_t* pos25;
+  const double* wts;  // one-slice weights in place of G (separable models)
 };
 
 constexpr int kWin = 60;  // Gd[25] | Gu[9] | G00[25] | Gq
@@ -68,7 +69,9 @@ __global__ void grad_reduce_kernel(GradKArgs a, double* __restrict__ win, double
       const int cd = a.code[e];
       double Ge;
       double tr_c = 0.0;
-      if (a.mode == 0) {
+      if (a.wts) {
+        Ge = a.wts[e];
+      } else if (a.mode == 0) {
         Ge = 0.5 * a.r * ((a.closed ? 0.0 : a.sq[e]) - a.sqc[e]);
         if (cd == 12) tr_c = a.sqc[e];
       } else {
@@ -81,9 +84,11 @@ __global__ void grad_reduce_kernel(GradKArgs a, double* __restrict__ win, double
         Ge = 0.5 * a.r * inh * 0.5 * s;
         tr_c *= inh;
       }
-      double mm = 0.0;
-      for (int c = 0; c < a.r; ++c) mm += a.mu[r + (int64_t)c * a.n] * a.mu[col + (int64_t)c * a.n];
-      Ge -= 0.5 * mm;
+      if (!a.wts) {
+        double mm = 0.0;
+        for (int c = 0; c < a.r; ++c) mm += a.mu[r + (int64_t)c * a.n] * a.mu[col + (int64_t)c * a.n];
+        Ge -= 0.5 * mm;
+      }
       const double qv = a.qval[e];
       dotGQ += Ge * qv;
       if (cd < 25) {
@@ -242,12 +247,16 @@ __global__ void grad_cell_kernel(GradKArgs a, const double* __restrict__ win, do
 using namespace spde;
 
 extern "C" int spde_grad_cells(spde_plan* pl, const spde_grad_args* g, double* out, void* stream) {
-  if (!pl || pl->host_only || !g || !out || !g->qval || !g->mu) {
+  if (!pl || pl->host_only || !g || !out || !g->qval || (!g->mu && !g->weights) || pl->lat.wide) {
     set_error("spde_grad_cells: invalid argument");
     return SPDE_ERR_ARG;
   }
-  const bool closed = g->mode == 0 && !g->sigma_q;
-  if (g->mode == 0 && !g->sigma_qc) {
+  if (g->weights && (pl->lat.T != 1 || g->mode != 0)) {
+    set_error("spde_grad_cells: weights are taken on one-slice plans in mode 0");
+    return SPDE_ERR_ARG;
+  }
+  const bool closed = g->mode == 0 && !g->sigma_q && !g->weights;
+  if (g->mode == 0 && !g->sigma_qc && !g->weights) {
     set_error("spde_grad_cells: exact mode needs the selected inverse of Q_c");
     return SPDE_ERR_ARG;
   }
@@ -313,6 +322,7 @@ extern "C" int spde_grad_cells(spde_plan* pl, const spde_grad_args* g, double* o
   a.sgram = closed ? g->sigma_gram : nullptr;
   a.sq0 = closed ? g->sigma_q0 : nullptr;
   a.pos25 = closed ? g->spatial->d_pos25 : nullptr;
+  a.wts = g->weights;
   double* win = pl->d_win;
   const unsigned blocks = (unsigned)((a.Ns + 127) / 128);
   hipLaunchKernelGGL(grad_reduce_kernel, dim3(blocks), dim3(128), 0, s, a, win, out);

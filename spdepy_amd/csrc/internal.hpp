@@ -146,6 +146,8 @@ struct spde_plan {
   double* d_win = nullptr;        // per-cell gradient windows
   double* d_red = nullptr;        // one device scalar (reductions)
   int32_t* d_pos25 = nullptr;     // one-slice plans: entry index of (cell, 5x5 code), -1 if absent
+  int32_t* d_sprow = nullptr;     // one-slice plans: row and column of each entry (Kronecker kernels)
+  int32_t* d_spcol = nullptr;
   int64_t work_doubles = 0;       // per-slot workspace size
   std::vector<spde::Slot> slots;
   std::vector<spde::LevelPlan> levels;

@@ -538,8 +538,10 @@ int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int 
   *out = nullptr;
   spde_plan* pl = new spde_plan();
   pl->lat = Lattice{M, N, T, bc};
+  pl->lat.wide = (flags & SPDE_PLAN_WIDE_TIME) ? 1 : 0;
   try {
     build_structure(pl->lat, pl->st);
+    if (pl->lat.wide) check_wide_layout(pl->lat, pl->st);
     build_symbolic(pl->lat, pl->st, leaf_max > 0 ? leaf_max : 128, pl->sym);
     build_scatter_map(pl->st, pl->sym, pl->map);
   } catch (const std::exception& e) {
@@ -672,7 +674,8 @@ void spde_plan_destroy(spde_plan* pl) {
     for (auto* p : {(void*)pl->d_indptr, (void*)pl->d_indices, (void*)pl->d_code, (void*)pl->d_diag,
                     (void*)pl->d_map, (void*)pl->d_mapsym, (void*)pl->d_perm, (void*)pl->d_relpos,
                     (void*)pl->d_bnd, (void*)pl->d_nodes, (void*)pl->d_tmpl, (void*)pl->d_acan,
-                    (void*)pl->d_a0can, (void*)pl->d_qdiag, (void*)pl->d_win, (void*)pl->d_red, (void*)pl->d_pos25})
+                    (void*)pl->d_a0can, (void*)pl->d_qdiag, (void*)pl->d_win, (void*)pl->d_red, (void*)pl->d_pos25,
+                    (void*)pl->d_sprow, (void*)pl->d_spcol})
       hipFree(p);
     for (auto& s : pl->slots) {
       hipFree(s.front);

@@ -12,10 +12,14 @@ namespace spde {
 
 // Code of a structural entry of Q: 0..24 same time slice (5x5 window,
 // (dy,dx) lexicographic), 25..33 previous slice (3x3), 34..42 next slice.
-enum : int { kDiagSlots = 25, kPrevBase = 25, kNextBase = 34, kSlots = 43 };
+// Wide-time lattices (separable models, Qt (x) Qs) couple the neighbouring
+// slices through the 5x5 window too: 43..67 previous, 68..92 next slice.
+enum : int { kDiagSlots = 25, kPrevBase = 25, kNextBase = 34, kSlots = 43, kPrevWide = 43, kNextWide = 68,
+             kSlotsWide = 93 };
 
 struct Lattice {
   int M, N, T, bc;            // bc: 1 zero-flux, 2 periodic, 3 Dirichlet
+  int wide = 0;               // 1: neighbouring slices coupled through the 5x5 window
   int64_t Ns() const { return (int64_t)M * N; }
   int64_t n() const { return Ns() * T; }
   // neighbour of cell (i,j) at offset (dx,dy) under the boundary rule, -1 if none
@@ -68,6 +72,10 @@ struct Symbolic {
 };
 
 void build_structure(const Lattice& lat, Structure& s);
+// Wide-time lattices: every row (t,i) holds the same spatial neighbours in
+// each of its slices t-1, t, t+1, in the same order (throws otherwise).  The
+// Kronecker kernels index Q's entries by that layout.
+void check_wide_layout(const Lattice& lat, const Structure& s);
 // Geometric nested dissection + multifrontal symbolic analysis.
 // Throws std::runtime_error if the separator property is violated.
 void build_symbolic(const Lattice& lat, const Structure& s, int leaf_max, Symbolic& sym);

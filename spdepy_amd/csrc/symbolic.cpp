@@ -22,19 +22,23 @@ void build_structure(const Lattice& lat, Structure& s) {
   s.indptr.assign(n + 1, 0);
   s.indices.clear();
   s.code.clear();
-  s.indices.reserve(n * 43);
-  s.code.reserve(n * 43);
+  s.indices.reserve(n * (lat.wide ? 75 : 43));
+  s.code.reserve(n * (lat.wide ? 75 : 43));
   std::vector<std::pair<int32_t, uint8_t>> row;
   for (int t = 0; t < lat.T; ++t)
     for (int j = 0; j < lat.N; ++j)
       for (int i = 0; i < lat.M; ++i) {
         row.clear();
         const int64_t r = i + (int64_t)j * lat.M + (int64_t)t * Ns;
+        const int rt = lat.wide ? 2 : 1;  // reach of the coupling to the neighbouring slices
+        auto tcode = [&](int base_narrow, int base_wide, int dx, int dy) {
+          return (uint8_t)(lat.wide ? base_wide + (dy + 2) * 5 + dx + 2 : base_narrow + (dy + 1) * 3 + dx + 1);
+        };
         if (t > 0)
-          for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
+          for (int dy = -rt; dy <= rt; ++dy)
+            for (int dx = -rt; dx <= rt; ++dx) {
               int64_t nb = lat.nbr(i, j, dx, dy);
-              if (nb >= 0) row.push_back({(int32_t)(nb + (t - 1) * Ns), (uint8_t)(kPrevBase + (dy + 1) * 3 + dx + 1)});
+              if (nb >= 0) row.push_back({(int32_t)(nb + (t - 1) * Ns), tcode(kPrevBase, kPrevWide, dx, dy)});
             }
         for (int dy = -2; dy <= 2; ++dy)
           for (int dx = -2; dx <= 2; ++dx) {
@@ -42,10 +46,10 @@ void build_structure(const Lattice& lat, Structure& s) {
             if (nb >= 0) row.push_back({(int32_t)(nb + t * Ns), (uint8_t)((dy + 2) * 5 + dx + 2)});
           }
         if (t + 1 < lat.T)
-          for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
+          for (int dy = -rt; dy <= rt; ++dy)
+            for (int dx = -rt; dx <= rt; ++dx) {
               int64_t nb = lat.nbr(i, j, dx, dy);
-              if (nb >= 0) row.push_back({(int32_t)(nb + (t + 1) * Ns), (uint8_t)(kNextBase + (dy + 1) * 3 + dx + 1)});
+              if (nb >= 0) row.push_back({(int32_t)(nb + (t + 1) * Ns), tcode(kNextBase, kNextWide, dx, dy)});
             }
         std::sort(row.begin(), row.end());
         for (size_t q = 1; q < row.size(); ++q)
@@ -68,6 +72,29 @@ void build_structure(const Lattice& lat, Structure& s) {
       if (it == en || *it != r) throw std::runtime_error("structural pattern not symmetric");
       s.mirror[e] = it - s.indices.begin();
     }
+}
+
+void check_wide_layout(const Lattice& lat, const Structure& s) {
+  const int64_t Ns = lat.Ns();
+  for (int64_t r = 0; r < lat.n(); ++r) {
+    const int t = (int)(r / Ns);
+    const int64_t b = s.indptr[r], len = s.indptr[r + 1] - b;
+    int64_t cnt = 0;
+    for (int64_t e = b; e < b + len; ++e) cnt += s.code[e] < kDiagSlots;
+    const int nblk = 1 + (t > 0) + (t + 1 < lat.T);
+    if (cnt * nblk != len) throw std::runtime_error("wide-time layout: unequal slice blocks");
+    const int64_t same = b + (t > 0 ? cnt : 0);
+    for (int64_t q = 0; q < cnt; ++q) {
+      const int64_t col = s.indices[same + q] - (int64_t)t * Ns;
+      const int cd = s.code[same + q];
+      if (cd >= kDiagSlots) throw std::runtime_error("wide-time layout: slice block order");
+      if (t > 0 && (s.indices[b + q] - (int64_t)(t - 1) * Ns != col || s.code[b + q] != kPrevWide + cd))
+        throw std::runtime_error("wide-time layout: previous slice differs");
+      if (t + 1 < lat.T &&
+          (s.indices[same + cnt + q] - (int64_t)(t + 1) * Ns != col || s.code[same + cnt + q] != kNextWide + cd))
+        throw std::runtime_error("wide-time layout: next slice differs");
+    }
+  }
 }
 
 namespace {

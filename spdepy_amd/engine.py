@@ -15,10 +15,10 @@ OUT_KAPPA, OUT_H, OUT_DGX, OUT_DGY, OUT_KAPPA0, OUT_H0, OUT_HC, OUT_WX, OUT_WY =
 OUT_DOTGQ, OUT_DOT00, OUT_TAUTR, OUT_WIDTH = 36, 37, 38, 40
 
 
-def get_plan(M, N, T, bc, leaf_max=128, nslots=2):
-    key = (M, N, T, bc, leaf_max, nslots)
+def get_plan(M, N, T, bc, leaf_max=128, nslots=2, wide=False):
+    key = (M, N, T, bc, leaf_max, nslots, wide)
     if key not in _PLANS:
-        _PLANS[key] = Plan(M, N, T, bc, leaf_max=leaf_max, nslots=nslots)
+        _PLANS[key] = Plan(M, N, T, bc, leaf_max=leaf_max, nslots=nslots, wide=wide)
     return _PLANS[key]
 
 
@@ -255,4 +255,134 @@ class Engine:
             parts["g_sigma"] -= 0.5 * r * (T - 1) * self.plan.shape[0] * self.plan.shape[1]
         parts["g_tau"] = m * r / 2 - 0.5 * r * colsum[OUT_TAUTR] - tau / 2 * rss
         del keep
+        return -like / (m * r), parts
+
+
+class KronEngine:
+    """Separable space-time models Q = Qt (x) Qs (seperable_spatial_temporal2D.py:
+    64-124): the space-time plan couples neighbouring slices through the 5x5
+    window (a wide-time plan); Qs lives on the one-slice plan of the same
+    lattice.  Only Q_c is factorised at full size: log|Q| = Ns log|Qt| +
+    T log|Qs|, and Q^-1 = Qt^-1 (x) Qs^-1 gives the log|Q| part of the
+    gradient from the selected inverse of Qs."""
+
+    def __init__(self, grid, bc, leaf_max=128):
+        M, N, T = grid.shape
+        self.grid, self.bc = grid, bc
+        self.plan = get_plan(M, N, T, bc, leaf_max, wide=True)
+        self.spatial = get_plan(M, N, 1, bc, leaf_max)
+        self.n, self.Ns, self.nnz, self.T = grid.n, grid.Ns, self.plan.nnz, T
+
+    def _qt_dev(self, qt):
+        torch = _torch()
+        return torch.as_tensor(np.ascontiguousarray(qt, dtype=np.float64)).cuda()
+
+    def assemble(self, qs, qt):
+        """Q values on the wide plan from Qs (one-slice values) and qt[2T-1]."""
+        torch = _torch()
+        q = torch.empty(self.nnz, dtype=torch.float64, device="cuda")
+        qt_d = self._qt_dev(qt)
+        nat.check(nat.lib().spde_assemble_kron(self.plan._h, self.spatial._h, _ptr(qs), _ptr(qt_d), _ptr(q),
+                                               _stream(torch)))
+        return q
+
+    to_scipy = Engine.to_scipy
+    _side_stream = Engine._side_stream
+
+    @staticmethod
+    def qt_dense(qt, T):
+        return np.diag(qt[:T]) + np.diag(qt[T:], 1) + np.diag(qt[T:], -1)
+
+    def loglik(self, qs, qt, dqts, spatial_coeffs, obs_cols, data, log_tau, grad=True, contract=None):
+        """(-loglik/(m r), parts) for Q = Qt (x) Qs with the exact trace.
+        dqts: the derivatives of qt (each 2T-1) of the temporal parameters;
+        parts["g_temporal"] their gradient entries, parts["cells"] (or
+        "coef_grad") the per-cell adjoints of the spatial operator A0 (the
+        variable Whittle-Matern layout), parts["g_tau"]."""
+        torch = _torch()
+        y = np.asarray(data, dtype=np.float64)
+        y2 = y.reshape(y.shape[0], -1)
+        m, r = y2.shape
+        tau = float(np.exp(log_tau))
+        n, Ns, T = self.n, self.Ns, self.T
+        sp = self.spatial
+        qval = self.assemble(qs, qt)
+        counts = np.bincount(obs_cols, minlength=n).astype(np.float64)
+        diag_add = torch.as_tensor(tau * counts).cuda()
+        b = np.zeros((n, r))
+        np.add.at(b, obs_cols, y2)
+        b *= tau
+        main = torch.cuda.current_stream()
+        side = self._side_stream(torch)
+        ready = torch.cuda.Event()
+        ready.record(main)
+        side.wait_event(ready)
+        if grad:
+            sig_s = torch.empty(sp.nnz, dtype=torch.float64, device="cuda")
+            sig_c = torch.empty(self.nnz, dtype=torch.float64, device="cuda")
+        with torch.cuda.stream(side):
+            sp.factor_async(0, qs, owner=self)
+            if grad:
+                sp.selinv(0, sig_s)
+        self.plan.factor_async(1, qval, diag_add, owner=self)
+        if grad:
+            factored = torch.cuda.Event()
+            factored.record(main)
+            side.wait_event(factored)
+            with torch.cuda.stream(side):
+                self.plan.selinv(1, sig_c)
+        mu = self.plan.solve(1, torch.as_tensor(b).cuda())
+        qmu = self.plan.spmv(qval, mu)
+        cols_t = torch.as_tensor(obs_cols).cuda()
+        resid = torch.as_tensor(y2).cuda() - mu[cols_t]
+        done = torch.cuda.Event()
+        done.record(side)
+        main.wait_event(done)
+        ld_qc = self.plan.factor_finish(1)
+        ld_s = sp.factor_finish(0)
+        Qt = self.qt_dense(np.asarray(qt, dtype=np.float64), T)
+        sign, ld_t = np.linalg.slogdet(Qt)
+        if sign <= 0:
+            raise nat.NotPositiveDefinite(nat.SPDE_ERR_NOT_SPD, "the temporal precision Qt is not positive definite")
+        ld_q = Ns * ld_t + T * ld_s
+        quad = float((mu * qmu).sum())
+        rss = float((resid * resid).sum())
+        like = 0.5 * ld_q * r + m * r * np.log(tau) / 2 - 0.5 * ld_qc * r - 0.5 * quad - tau / 2 * rss
+        parts = dict(m=m, r=r, tau=tau, rss=rss, ld_q=ld_q, ld_qc=ld_qc, qval=qval, mu=mu, like_raw=like)
+        if not grad:
+            return -like / (m * r), parts
+        mu_cm = mu.t().contiguous()
+        weights = torch.empty(sp.nnz, dtype=torch.float64, device="cuda")
+        blocks = torch.empty(9 * T, dtype=torch.float64, device="cuda")
+        qt_d = self._qt_dev(qt)
+        nat.check(nat.lib().spde_kron_reduce(self.plan._h, sp._h, _ptr(qs), _ptr(qt_d), _ptr(sig_s), 0.5 * r * T,
+                                             _ptr(sig_c), _ptr(mu_cm), r, _ptr(diag_add), _ptr(weights),
+                                             _ptr(blocks), _stream(torch)))
+        g = nat.GradArgs()
+        g.mode, g.r, g.nh1 = 0, r, 0
+        g.qval = qs.data_ptr()
+        g.weights = weights.data_ptr()
+        g.coeffs = spatial_coeffs.c
+        out = torch.empty(Ns * OUT_WIDTH, dtype=torch.float64, device="cuda")
+        nat.check(nat.lib().spde_grad_cells(sp._h, g, _ptr(out), _stream(torch)))
+        if contract is not None:
+            parts["coef_grad"], parts["colsum"] = contract(out)
+        else:
+            parts["cells"] = out.cpu().numpy().reshape(Ns, OUT_WIDTH)
+            parts["colsum"] = parts["cells"].sum(0)
+        B = blocks.cpu().numpy().reshape(T, 3, 3)
+        iQt = np.linalg.inv(Qt)
+        gt = []
+        for dqt in dqts:
+            dqt = np.asarray(dqt, dtype=np.float64)
+            dQt = self.qt_dense(dqt, T)
+            coef = np.zeros((T, 3))
+            coef[:, 1] = dqt[:T]
+            coef[1:, 0] = dqt[T:]
+            coef[:-1, 2] = dqt[T:]
+            gt.append(0.5 * r * Ns * float(np.sum(iQt * dQt)) - 0.5 * r * float(np.sum(coef * B[:, :, 0]))
+                      - 0.5 * float(np.sum(coef * B[:, :, 1])))
+        parts["g_temporal"] = gt
+        parts["g_tau"] = m * r / 2 - 0.5 * r * float(B[:, 1, 2].sum()) - tau / 2 * rss
+        del mu_cm, weights, blocks, sig_s, sig_c
         return -like / (m * r), parts

@@ -18,12 +18,15 @@ def _stream(torch):
 
 
 class Plan:
-    def __init__(self, M, N, T, bc, leaf_max=128, nslots=2, host_only=False):
+    def __init__(self, M, N, T, bc, leaf_max=128, nslots=2, host_only=False, wide=False):
+        """wide=True: neighbouring time slices coupled through the 5x5 window
+        (separable models, values from assemble_kron)."""
         self.shape = (M, N, T)
         self.bc = bc
+        self.wide = wide
         L = nat.lib()
         h = ctypes.c_void_p()
-        flags = nat.SPDE_PLAN_HOST_ONLY if host_only else 0
+        flags = (nat.SPDE_PLAN_HOST_ONLY if host_only else 0) | (nat.SPDE_PLAN_WIDE_TIME if wide else 0)
         nat.check(L.spde_plan_create(M, N, T, bc, leaf_max, nslots, flags, ctypes.byref(h)))
         self._h = h
         self.host_only = host_only

@@ -23,7 +23,7 @@ import numpy as np
 from scipy import sparse
 
 from . import _native as nat
-from .engine import (Coefficients, Engine, OUT_DGX, OUT_DGY, OUT_H, OUT_H0, OUT_HC, OUT_KAPPA, OUT_KAPPA0,
+from .engine import (Coefficients, Engine, KronEngine, OUT_DGX, OUT_DGY, OUT_H, OUT_H0, OUT_HC, OUT_KAPPA, OUT_KAPPA0,
                      OUT_WIDTH, OUT_WX, OUT_WY)
 from .plan import _ptr, _stream
 
@@ -510,20 +510,24 @@ class _VarWhittleMaternBase(_StencilMixin):
                                  probes=probes, logdet="factor", contract=self._device_contract())
         if not grad:
             return like
-        nf, Np = self._nfield(), self.Np
-        if "coef_grad" in parts:
-            g = list(parts["coef_grad"][:nf * Np])
-        else:
-            gr = self.grid
-            cells = parts["cells"]
-            kappa, H, d = self._hfields
-            Hb = cells[:, OUT_H:OUT_H + 16].reshape(-1, 4, 2, 2)
-            g = list(gr.bs.T @ (cells[:, OUT_KAPPA] * kappa))
-            for dH in d:
-                g += list(np.einsum("kfi,kf->i", gr.bsH, (Hb * dH).sum(axis=(2, 3))))
+        g = self._field_grad(parts)
         g.append(parts["g_tau"])
         m, r = parts["m"], parts["r"]
         return like, -np.array(g) / (m * r)
+
+    def _field_grad(self, parts):
+        """Contraction of the per-cell A0 adjoints onto the field coefficients."""
+        nf, Np = self._nfield(), self.Np
+        if "coef_grad" in parts:
+            return list(parts["coef_grad"][:nf * Np])
+        gr = self.grid
+        cells = parts["cells"]
+        kappa, H, d = self._hfields
+        Hb = cells[:, OUT_H:OUT_H + 16].reshape(-1, 4, 2, 2)
+        g = list(gr.bs.T @ (cells[:, OUT_KAPPA] * kappa))
+        for dH in d:
+            g += list(np.einsum("kfi,kf->i", gr.bsH, (Hb * dH).sum(axis=(2, 3))))
+        return g
 
     def print(self, par):
         Np = self.Np
@@ -1066,6 +1070,170 @@ class VarAdvectionVarHaDiffusion2D(_FieldAdvDiff):
     kind, dvar, gvar, name = "ha", True, True, "var-advection-var-diffusion-ha-2D"
 
 
+class _SeparableBase(_StencilMixin):
+    """Separable space-time models Q = Qt (x) Qs (seperable_spatial_temporal2D.py,
+    seperable_spatial_temporal_ha2D.py, seperable_spatial_temporal_idiffusion2D.py):
+    Qs is the variable Whittle-Matern precision of the same kind (B-spline
+    kappa, gamma (, v) fields; A = Dv Dk - A_H(H), Qs = A^T iDv A) and Qt the
+    tridiagonal temporal precision:
+      anisotropic: par = [c_k, c_g, c_vx, c_vy, log rho, log tau],
+                   Qt = tridiag(-rho; 1, 1+rho^2, .., 1+rho^2, 1) / (1 - rho^2);
+      isotropic:   par = [c_k, c_g, a, log sigma, log tau];
+      ha:          par = [c_k, c_g, c_vx, c_vy, a, log sigma, log tau],
+                   Qt = sigma tridiag(-a; 1, 1+a^2, .., 1+a^2, 1).
+    (The reference's isotropic and ha makeQt loop over range(10), i.e. are
+    written for T = 10; the form here holds for every T and equals theirs at
+    T = 10.)  Only Q_c is factorised at full size (log|Q| = Ns log|Qt| +
+    T log|Qs|)."""
+    kind = "ani"
+    label = "ani-"
+
+    def __init__(self, grid, par=None, bc=3):
+        from .grids import grid as make_grid
+        self.grid, self.bc = grid, bc
+        self.type = "seperable-spatial-temporal-%s2D-bc%d" % (self.label, bc)
+        self.Np = grid.Nbs2
+        self.Q = self.Q_fac = self.data = self.r = self.S = None
+        self._eng = None
+        g0 = make_grid(x=grid.x, y=grid.y, extend=grid.Ne if grid.Ne else None, Nbs=grid.Nbs)
+        wcls = {"iso": VarWhittleMatern2D, "ani": VarWhittleMaternAnisotropic2D, "ha": VarWhittleMaternHa2D}[self.kind]
+        self._wm = wcls(grid=g0, bc=bc)
+        if par is None:
+            self.setPars(self._default())
+        else:
+            self.setQ(par=par)
+
+    def _ns(self):
+        return self.Np * (2 if self.kind == "iso" else 4)
+
+    def _default(self):
+        Np = self.Np
+        v = [[-1.0] * Np, [-1.0] * Np] + ([[0.1] * Np, [0.1] * Np] if self.kind != "iso" else [])
+        v += [[-1.0]] if self.kind == "ani" else [[0.1, 1.0]]
+        return np.hstack(v + [[np.log(100)]]).astype(np.float64)
+
+    def getPars(self, *args, **kwargs):
+        return np.hstack([self.coef, [self.tau]]).astype(np.float64)
+
+    def setPars(self, par):
+        par = np.asarray(par, dtype=np.float64)
+        self.coef = par[:-1].copy()
+        Np = self.Np
+        self.kappa, self.gamma = par[:Np], par[Np:2 * Np]
+        if self.kind != "iso":
+            self.vx, self.vy = par[2 * Np:3 * Np], par[3 * Np:4 * Np]
+        tp = par[self._ns():-1]
+        if self.kind == "ani":
+            self.rho = tp[0]
+        else:
+            self.a, self.sigma = tp[0], tp[1]
+        self.tau = par[-1]
+
+    def initFit(self, data, **kwargs):
+        assert data.shape[0] <= self.grid.n
+        self.data = data
+        self.r = data.shape[1] if data.ndim == 2 else 1
+        self.S = self.grid.getS(idxs=kwargs.get("idx"))
+        return self._default()
+
+    _obs_cols = _SpaceTimeBase._obs_cols
+
+    def _engine(self):
+        if self._eng is None:
+            self._eng = KronEngine(self.grid, self.bc)
+        return self._eng
+
+    def qt(self, tp, diff=False):
+        """Diagonal | off-diagonal of Qt (makeQt), or of its derivative in
+        log rho (ani) / a (iso, ha)."""
+        T = self.grid.T
+        d, o = np.zeros(T), np.zeros(T - 1)
+        if self.kind == "ani":
+            rho = np.exp(tp[0])
+            den = 1.0 - rho ** 2
+            if diff:
+                d[:], o[:] = 4 * rho ** 2 / den ** 2, -rho * (1 + rho ** 2) / den ** 2
+                d[0] = d[-1] = 2 * rho ** 2 / den ** 2
+            else:
+                d[:], o[:] = (1 + rho ** 2) / den, -rho / den
+                d[0] = d[-1] = 1.0 / den
+        else:
+            a, sig = tp[0], np.exp(tp[1])
+            if diff:
+                d[:], o[:] = 2 * a * sig, -sig
+                d[0] = d[-1] = 0.0
+            else:
+                d[:], o[:] = (1 + a ** 2) * sig, -a * sig
+                d[0] = d[-1] = sig
+        return np.hstack([d, o])
+
+    def _spatial(self, par):
+        """Spatial coefficients and Qs (one-slice values) of par."""
+        coeffs = self._wm._coeffs(np.hstack([par[:self._ns()], [0.0]]))
+        qs = self._wm._engine().assemble(coeffs)
+        return coeffs, qs
+
+    def makeQ(self, par, grad=True):
+        par = np.asarray(par, dtype=np.float64)
+        eng = self._engine()
+        _, qs = self._spatial(par)
+        qval = eng.assemble(qs, self.qt(par[self._ns():-1]))
+        Q = eng.to_scipy(qval)
+        self._last_qval = qval
+        return Q, Factor(eng, 0, qval), (_LazyDQ(self, par) if grad else None)
+
+    def setQ(self, par=None, S=None):
+        if par is None:
+            par = self.getPars()
+        else:
+            self.setPars(par)
+        if S is not None:
+            self.S = S
+        self.Q, self.Q_fac, _ = self.makeQ(par=par, grad=False)
+        self.S = self.grid.getS()
+
+    def logLike(self, par, nh1=100, grad=True, trace="exact"):
+        """seperable_spatial_temporal2D.py:127-164: -loglik/(m r) and its
+        gradient with the exact trace (selected inverses of Q_c and Qs)."""
+        if trace != "exact":
+            raise NotImplementedError("separable models: the exact trace only")
+        par = np.asarray(par, dtype=np.float64)
+        eng = self._engine()
+        coeffs, qs = self._spatial(par)
+        tp = par[self._ns():-1]
+        dqts = [self.qt(tp, diff=True)]
+        if self.kind != "ani":
+            dqts.append(self.qt(tp))          # d/d log sigma: Qt itself
+        like, parts = eng.loglik(qs, self.qt(tp), dqts, coeffs, self._obs_cols(), self.data, par[-1], grad=grad,
+                                 contract=self._wm._device_contract())
+        if not grad:
+            return like
+        g = self._wm._field_grad(parts) + list(parts["g_temporal"]) + [parts["g_tau"]]
+        m, r = parts["m"], parts["r"]
+        return like, -np.array(g) / (m * r)
+
+    def print(self, par):
+        Np = self.Np
+        s = "| κ = %2.2f, γ = %2.2f" % (np.exp(par[:Np]).mean(), np.exp(par[Np:2 * Np]).mean())
+        if self.kind != "iso":
+            s += ", vx = %2.2f, vy = %2.2f" % (par[2 * Np:3 * Np].mean(), par[3 * Np:4 * Np].mean())
+        tp = par[self._ns():-1]
+        s += (", ρ = %2.2f" % np.exp(tp[0])) if self.kind == "ani" else (", a = %2.2f, σ = %2.2f" % (tp[0], np.exp(tp[1])))
+        return s + ", τ = %2.2f" % np.exp(par[-1])
+
+
+class SeperableSpatialTemporal2D(_SeparableBase):
+    kind, label = "ani", "ani-"
+
+
+class SeperableSpatialTemporalHa2D(_SeparableBase):
+    kind, label = "ha", "ha-"
+
+
+class SeperableSpatialTemporalIDiffusion2D(_SeparableBase):
+    kind, label = "iso", ""
+
+
 class _LazyDQ:
     """The reference's list of dQ_i, produced on demand (one matrix per
     parameter except tau).  Not needed by logLike, whose gradient contracts all
@@ -1104,4 +1272,8 @@ def spde_init(model, grid, parameters=None, ani=True, ha=True, bc=3, mod0=None):
         cls = (VarAdvectionVarHaDiffusion2D if ha else
                (VarAdvectionVarDiffusion2D if ani else VarAdvectionVarIDiffusion2D))
         return cls(grid=grid, mod0=mod0, par=parameters, bc=bc)
+    if model in ("seperable-spatial-temporal", 8):
+        cls = (SeperableSpatialTemporalHa2D if ha else
+               (SeperableSpatialTemporal2D if ani else SeperableSpatialTemporalIDiffusion2D))
+        return cls(grid=grid, par=parameters, bc=bc)
     raise NotImplementedError("model %r is outside the GMRF hot path built here" % (model,))

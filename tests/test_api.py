@@ -33,6 +33,12 @@ FAMILIES = {
         ("VarAdvectionVarDiffusion2D", np.hstack([[-1] * 9, [-1] * 9, [0.1] * 18, [0.1] * 18, 0, LOG100])),
     ("var-advection-var-diffusion", True, True):
         ("VarAdvectionVarHaDiffusion2D", np.hstack([[-1] * 9, [-1] * 9, [0.1] * 18, [0.1] * 18, 0, LOG100])),
+    ("seperable-spatial-temporal", False, False):
+        ("SeperableSpatialTemporalIDiffusion2D", np.hstack([[-1] * 9, [-1] * 9, 0.1, 1, LOG100])),
+    ("seperable-spatial-temporal", True, False):
+        ("SeperableSpatialTemporal2D", np.hstack([[-1] * 9, [-1] * 9, [0.1] * 9, [0.1] * 9, -1, LOG100])),
+    ("seperable-spatial-temporal", True, True):
+        ("SeperableSpatialTemporalHa2D", np.hstack([[-1] * 9, [-1] * 9, [0.1] * 9, [0.1] * 9, 0.1, 1, LOG100])),
 }
 
 
@@ -45,7 +51,10 @@ def test_dispatch_and_default_parameters(key):
     mod0 = spdes.spde_init("whittle-matern", g0, ani=False, ha=False, bc=3)
     m = spdes.spde_init(spde, g, ani=ani, ha=ha, bc=3, mod0=mod0)
     assert type(m).__name__ == name
-    if default is not None:
+    if default is not None and spde.startswith("seperable"):
+        assert np.array_equal(m.getPars(), default)
+        assert np.array_equal(m.initFit(np.zeros(10), idx=np.arange(10)), default)
+    elif default is not None:
         assert np.array_equal(m.getPars(), default)
         # initFit appends the initial-state parameters before log tau
         data = np.zeros(10)

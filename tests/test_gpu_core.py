@@ -346,3 +346,62 @@ def test_field_family_q_and_loglike(kind, dvar, gvar, bc):
     scale = np.abs(jac_o).max()
     assert jac.shape == jac_o.shape
     assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * scale), np.abs(jac - jac_o).max() / scale
+
+
+@pytest.mark.parametrize("kind,T,bc", [("ani", 5, 3), ("ani", 4, 1), ("iso", 10, 3), ("ha", 10, 1)])
+def test_separable_q_and_loglike(kind, T, bc):
+    """Separable models Q = Qt (x) Qs (seperable_spatial_temporal*2D.py) on the
+    wide-time plan: Q (pattern and values) and the exact-trace logLike value
+    and gradient against the oracle (dense inverses)."""
+    import spdepy_amd as sp
+    from spdepy_amd import spdes
+    from oracle import grid as OG
+    M, N = 8, 7
+    g = OG.STGrid(F.axis(M), F.axis(N), np.arange(float(T)), Nbs=3)
+    om = Mo.Separable(g, kind, bc=bc)
+    rng = np.random.default_rng(12)
+    tp = [np.log(0.6)] if kind == "ani" else [0.5, np.log(2.0)]
+    par = np.hstack([0.3 * rng.normal(size=om.ns), tp, [np.log(50.0)]])
+    Qo, dQo = om.makeQ(par)
+    L = np.linalg.cholesky(Qo.toarray())
+    x = np.linalg.solve(L.T, rng.normal(size=(g.n, 2)))
+    idx = np.sort(rng.choice(g.n, 60, replace=False))
+    y = x[g.n2e(idx)] + 0.1 * rng.normal(size=(60, 2))
+    like_o, jac_o, _ = Mo.loglike(Qo, dQo, g.getS(idx), y, par[-1], trace="exact")
+    pg = sp.grid(x=g.x, y=g.y, t=g.t, Nbs=3)
+    m = spdes.spde_init("seperable-spatial-temporal", pg, ani=kind != "iso", ha=kind == "ha", bc=bc)
+    step("separable Q %s T=%d bc=%d" % (kind, T, bc))
+    Qg, fac, _ = m.makeQ(par, grad=False)
+    Qo = Qo.tocsc()
+    Qo.sort_indices()
+    Qg.sort_indices()
+    assert np.array_equal(Qg.indptr, Qo.indptr) and np.array_equal(Qg.indices, Qo.indices)
+    assert np.allclose(Qg.data, Qo.data, rtol=1e-12, atol=1e-13 * np.abs(Qo.data).max())
+    step("separable factor %s" % kind)
+    ref = np.linalg.slogdet(Qo.toarray())[1]
+    assert abs(fac.logdet() - ref) <= 1e-11 * abs(ref)
+    m.initFit(y, idx=idx)
+    step("separable logLike %s T=%d bc=%d" % (kind, T, bc))
+    like, jac = m.logLike(par)
+    assert abs(like - like_o) <= 1e-9 * abs(like_o), (like, like_o)
+    scale = np.abs(jac_o).max()
+    assert jac.shape == jac_o.shape
+    assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * scale), np.abs(jac - jac_o).max() / scale
+
+
+def test_library_first_then_torch_in_a_fresh_process():
+    """A script whose first GPU work is a native plan (no torch call before
+    it) still gets a working torch device afterwards."""
+    import subprocess
+    import sys
+    code = ("import numpy as np\n"
+            "from spdepy_amd.plan import Plan\n"
+            "p = Plan(6, 5, 3, 3, nslots=1, wide=True)\n"
+            "import torch\n"
+            "x = torch.ones(4, device='cuda')\n"
+            "print(float(x.sum()))\n")
+    step("fresh process: plan before torch")
+    res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=180,
+                         cwd=os.path.dirname(os.path.dirname(__file__)))
+    assert res.returncode == 0, res.stderr[-2000:]
+    assert res.stdout.strip().endswith("4.0")

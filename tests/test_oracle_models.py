@@ -225,3 +225,33 @@ def test_field_family_derivatives(kind, dvar, gvar):
         pm[i] -= h
         fd = (m.makeQ(pp, False)[0] - m.makeQ(pm, False)[0]) / (2 * h)
         assert abs(fd - dQ[i]).max() <= 1e-6 * max(abs(dQ[i]).max(), 1e-3 * abs(Q).max()), i
+
+
+@pytest.mark.parametrize("kind", ["iso", "ani", "ha"])
+def test_separable_temporal_precision_and_derivatives(kind):
+    """Separable oracle: Qt^-1 is the AR(1) covariance (rho^|t-t'| for ani;
+    a^|t-t'| / (sigma (1 - a^2)) for iso/ha), Q = Qt (x) Qs with Qs the
+    variable Whittle-Matern precision, and every 3rd dQ_i (plus the temporal
+    ones) against central differences of Q."""
+    from oracle import grid as OG
+    g = OG.STGrid(F.axis(7), F.axis(6), np.arange(10.0), Nbs=3)
+    m = Mo.Separable(g, kind, bc=3)
+    rng = np.random.default_rng(3)
+    tp = [np.log(0.6)] if kind == "ani" else [0.7, np.log(2.0)]
+    par = np.hstack([0.3 * rng.normal(size=m.ns), tp, [np.log(100.0)]])
+    Qt = m.qt(np.array(tp), 10).toarray()
+    lag = np.abs(np.subtract.outer(np.arange(10), np.arange(10)))
+    cov = 0.6 ** lag if kind == "ani" else 0.7 ** lag / (2.0 * (1 - 0.49))
+    assert np.allclose(np.linalg.inv(Qt), cov, rtol=1e-12, atol=1e-12)
+    Q, dQ = m.makeQ(par)
+    Qs, _ = m.wm.makeQ(np.hstack([par[:m.ns], [0.0]]), grad=False)
+    assert abs(Q[:g.Ns, :g.Ns] - Qt[0, 0] * Qs).max() <= 1e-12 * abs(Qs).max()
+    assert abs(Q[g.Ns:2 * g.Ns, :g.Ns] - Qt[1, 0] * Qs).max() <= 1e-12 * abs(Qs).max()
+    assert len(dQ) == par.size - 1
+    for i in list(range(0, m.ns, 3)) + list(range(m.ns, par.size - 1)):
+        h = 1e-6
+        pp, pm = par.copy(), par.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (m.makeQ(pp, False)[0] - m.makeQ(pm, False)[0]) / (2 * h)
+        assert abs(fd - dQ[i]).max() <= 1e-6 * abs(dQ[i]).max(), i

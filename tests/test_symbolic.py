@@ -146,6 +146,42 @@ def test_one_slice_plan_reproduces_dense_factor_and_inverse(M, N, bc, leaf):
     assert np.allclose(flat[e["map"][m]], inv[rows[m], e["indices"][m]], rtol=1e-9, atol=1e-12 * np.abs(inv).max())
 
 
+@pytest.mark.parametrize("M,N,T,bc,leaf", [(7, 6, 4, 1, 16), (6, 6, 3, 2, 20), (6, 7, 5, 3, 1000)])
+def test_wide_time_plan_reproduces_dense_factor_and_inverse(M, N, T, bc, leaf):
+    """Wide-time lattices (separable models, Qt (x) Qs): every row repeats the
+    one-slice row in slices t-1, t, t+1 and the plan factors and inverts a
+    Kronecker matrix on that pattern."""
+    pl = Plan(M, N, T, bc, leaf_max=leaf, nslots=0, host_only=True, wide=True)
+    sp = Plan(M, N, 1, bc, leaf_max=leaf, nslots=0, host_only=True)
+    e, es = pl.export(), sp.export()
+    n, Ns = M * N * T, M * N
+    Ss = sparse.csr_matrix((np.ones(len(es["indices"])), es["indices"], es["indptr"]), shape=(Ns, Ns))
+    Qt = sparse.diags([np.full(T - 1, -0.4), np.full(T, 1.5), np.full(T - 1, -0.4)], [-1, 0, 1])
+    S = sparse.csr_matrix((np.ones(len(e["indices"])), e["indices"], e["indptr"]), shape=(n, n))
+    assert abs(sparse.kron(Qt, Ss).astype(bool).astype(float) - S).sum() == 0
+    R = sparse.random(Ns, Ns, density=1.0, random_state=5).multiply(Ss)
+    Qs = sparse.csr_matrix((R @ R.T).multiply(Ss) + sparse.eye(Ns) * 30.0)
+    A = sparse.csr_matrix(sparse.kron(Qt, Qs))
+    # Q's entry ((t,i),(t+d,j)) at indptr + (t>0) cnt_i + d cnt_i + q (q: position in the one-slice row)
+    for r in range(0, n, 7):
+        t, i = divmod(r, Ns)
+        cnt = es["indptr"][i + 1] - es["indptr"][i]
+        for d in (-1, 0, 1):
+            if 0 <= t + d < T:
+                b = e["indptr"][r] + (cnt if t > 0 else 0) + d * cnt
+                assert np.array_equal(e["indices"][b:b + cnt] - (t + d) * Ns,
+                                      es["indices"][es["indptr"][i]:es["indptr"][i + 1]])
+    logdet, Sig = _mf_factor(e, A)
+    dense = A.toarray()
+    sign, ref = np.linalg.slogdet(dense)
+    assert sign > 0 and abs(logdet - ref) <= 1e-10 * abs(ref)
+    inv = np.linalg.inv(dense)
+    flat = np.concatenate([Sig[k].ravel(order="F") for k in range(e["nodes"].shape[0])])
+    rows = np.repeat(np.arange(n), np.diff(e["indptr"]))
+    m = e["map"] >= 0
+    assert np.allclose(flat[e["map"][m]], inv[rows[m], e["indices"][m]], rtol=1e-9, atol=1e-12 * np.abs(inv).max())
+
+
 def test_config2_plan_statistics():
     """100x100x20 lattice: the analysis the GPU factorisation is sized from."""
     pl = Plan(100, 100, 20, 1, leaf_max=128, nslots=0, host_only=True)
