@@ -211,6 +211,12 @@ typedef struct spde_grad_args {
    * trq, trqc, probes are not read; the outputs are the adjoints of
    * <weights, dQ> (the separable models' spatial gradient, spde_kron_reduce). */
   const double* weights;
+  /* Optional advection direction adv_dir[Ns*4] (faces E,N,W,S): column 39 of
+   * out receives dt <Abar, Aw(adv_dir)>, the derivative along a scaling of the
+   * advection field taken with the stencil of adv_dir itself, as the
+   * cov-advection models form it (cov_advection_idiffusion2D.py:59,153-159:
+   * dA = Aw(ww) dt for advection lambda ww). */
+  const double* adv_dir;
 } spde_grad_args;
 
 int spde_grad_cells(spde_plan* plan, const spde_grad_args* args, double* out, void* stream);

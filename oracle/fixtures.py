@@ -80,6 +80,15 @@ def small(M=8, N=7, T=4, Nbs=3, bc=1, m=40, ne=None):
     return var_model(M, N, T, Nbs=Nbs, bc=bc, m=m, ne=ne)
 
 
+def currents(g):
+    """Smooth synthetic cell-centred currents (we, wn) on the interior cells
+    (a rotating cell pattern like the reference's advection example)."""
+    X, Y = np.meshgrid(g.x, g.y)
+    we = np.sin(3 * X) * np.cos(3 * Y) + 0.2
+    wn = -np.cos(3 * X) * np.sin(3 * Y) - 0.1
+    return we.ravel(), wn.ravel()
+
+
 def field_par(model, rng, sd=0.3):
     """kappa = 1, gamma = 0.5 plus field noise; random v and w."""
     nd, nh, nw = model.sizes()
@@ -90,15 +99,20 @@ def field_par(model, rng, sd=0.3):
     return np.hstack(blocks + [[0.0, np.log(100.0)]])
 
 
-def field_model(M=8, N=7, T=4, kind="iso", dvar=True, gvar=False, bc=3, m=40, seeds=(8, 9, 10, 11)):
+def field_model(M=8, N=7, T=4, kind="iso", dvar=True, gvar=False, bc=3, m=40, seeds=(8, 9, 10, 11), cov=None):
+    """cov=lambda: the cov-advection family with advection lambda * ww,
+    ww = assimilate_adv(currents)."""
     g = G.STGrid(axis(M), axis(N), np.arange(float(T)), Nbs=3)
     mod0 = mod0_iso(M, N, bc=bc)
-    model = Mo.FieldAdvDiff(g, mod0, kind=kind, dvar=dvar, gvar=gvar, bc=bc)
+    ww = None if cov is None else g.assimilate_adv(*currents(g))
+    model = Mo.FieldAdvDiff(g, mod0, kind=kind, dvar=dvar, gvar=gvar, bc=bc, ww=ww)
     par = field_par(model, np.random.default_rng(seeds[0]))
+    if cov is not None:
+        par[-3] = cov
     Q, _ = model.makeQ(par, grad=False)
     L = np.linalg.cholesky(Q.toarray())
     z = np.random.default_rng(seeds[2]).normal(size=(g.n, 1))
     x = np.linalg.solve(L.T, z)
     idx, y = observations(M * N * T, m, x, g, np.random.default_rng(seeds[1]),
                           np.random.default_rng(seeds[3]))
-    return dict(grid=g, mod0=mod0, model=model, par=par, idx=idx, y=y, x=x)
+    return dict(grid=g, mod0=mod0, model=model, par=par, idx=idx, y=y, x=x, ww=ww)

@@ -186,6 +186,22 @@ class STGrid(_Base):
     def shape(self):
         return [self.M + 2 * self.Ne, self.N + 2 * self.Ne, self.T]
 
+    def assimilate_adv(self, we, wn):
+        """Face velocities from cell-centred currents (``:277-344``): each face
+        takes the mean of its cell and the neighbour across it (the cell
+        itself at the lattice border); E, N, W, S from we, wn, we, wn; on
+        extended grids the border extrapolation of ``advBound``."""
+        M, N = self.M, self.N
+        we, wn = np.asarray(we, dtype=np.float64), np.asarray(wn, dtype=np.float64)
+        k = np.arange(M * N)
+        i, j = k % M, k // M
+        east = np.where(i + 1 == M, k, k + 1)
+        west = np.where(i == 0, k, k - 1)
+        north = np.where(j + 1 == N, k, k + M)
+        south = np.where(j == 0, k, k - M)
+        ww = np.stack([(we + we[east]) / 2, (wn + wn[north]) / 2, (we + we[west]) / 2, (wn + wn[south]) / 2], axis=1)
+        return self._ext_map() @ ww if self.Ne else ww
+
     def n2e(self, idx):
         idx = np.asarray(idx)
         MN = self.M * self.N

@@ -487,14 +487,22 @@ class FieldAdvDiff(_STBase):
     log tau]; a block is Np coefficients when its field is a B-spline field
     (2 Np for w), else one log-scalar (two scalars for w)."""
 
-    def __init__(self, grid, mod0, kind="iso", dvar=True, gvar=True, bc=3):
+    def __init__(self, grid, mod0, kind="iso", dvar=True, gvar=True, bc=3, ww=None):
+        """ww (Ns x 4 faces E,N,W,S): the cov-advection families
+        (cov_advection_{i,,ha_}diffusion2D.py, cov_advection_var_*): the w
+        block is one scalar lambda, the advection lambda * ww (variable
+        stencil), and its dQ uses the stencil of ww itself, dA = Aw(ww) dt
+        (cov_advection_idiffusion2D.py:59,153-159)."""
         self.grid, self.mod0, self.kind, self.bc = grid, mod0, kind, bc
-        self.dvar, self.gvar = dvar, gvar
+        self.dvar, self.gvar = dvar, gvar and ww is None
+        self.ww = None if ww is None else np.asarray(ww, dtype=np.float64)
         self.Np = grid.Nbs2
 
     def sizes(self):
         nd = self.Np if self.dvar else 1
         nh = 1 if self.kind == "iso" else 3
+        if self.ww is not None:
+            return nd, nh, 1
         return nd, nh, (2 * self.Np if self.gvar else 2)
 
     def split(self, par):
@@ -527,7 +535,9 @@ class FieldAdvDiff(_STBase):
             vx, vy = (blocks[2][0], blocks[3][0]) if self.kind != "iso" else (0.0, 0.0)
             Hc, dHc = h_map(self.kind, np.exp(blocks[1][0]), vx, vy)
             A_H = ah_const_csc(g, Hc, self.bc)
-        if self.gvar:
+        if self.ww is not None:
+            A_w = aw_var_csc(g, cw[0] * self.ww, self.bc)
+        elif self.gvar:
             ws = g.evalAdv(cw)
             A_w = aw_var_csc(g, ws, self.bc)
         else:
@@ -559,7 +569,9 @@ class FieldAdvDiff(_STBase):
             dkappa(kappa)
             for d in dHc:
                 dQ.append(d_blocks(A, Qs, iDv, T, -ah_const_csc(g, d, self.bc) * dt) / c)
-        if self.gvar:
+        if self.ww is not None:
+            dQ.append(d_blocks(A, Qs, iDv, T, aw_var_csc(g, self.ww, self.bc) * dt) / c)
+        elif self.gvar:
             for i in range(2 * self.Np):
                 e = np.zeros(2 * self.Np)
                 e[i] = 1.0

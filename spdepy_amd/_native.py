@@ -51,7 +51,7 @@ class GradArgs(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("r", ctypes.c_int), ("nh1", ctypes.c_int), ("qval", vp),
                 ("sigma_q", vp), ("sigma_qc", vp), ("trq", vp), ("trqc", vp), ("probes", vp),
                 ("mu", vp), ("diag_add", vp), ("coeffs", Coeffs),
-                ("sigma_gram", vp), ("sigma_q0", vp), ("spatial", vp), ("weights", vp)]
+                ("sigma_gram", vp), ("sigma_q0", vp), ("spatial", vp), ("weights", vp), ("adv_dir", vp)]
 
 
 _SIGS = {

@@ -49,6 +49,7 @@ struct GradKArgs {
   const double* sq0;
   const int32_t* pos25;
   const double* wts;  // one-slice weights in place of G (separable models)
+  const double* adv_dir;  // direction of a scaled advection field (cov-advection models)
 };
 
 constexpr int kWin = 60;  // Gd[25] | Gu[9] | G00[25] | Gq
@@ -240,6 +241,14 @@ __global__ void grad_cell_kernel(GradKArgs a, const double* __restrict__ win, do
       o[33 + diff] = a.dt * s;
     }
   }
+  double sdir = 0.0;
+  if (a.adv_dir) {
+    Cell5 c5;
+    advection_var(i, j, M, N, bc, a.hx, a.hy, a.adv_dir + 4 * (int64_t)m, nullptr, 3, c5);
+    canon5(c5, i, j, M, N, canv, true);
+    for (int q = 0; q < 9; ++q) sdir += abar[q] * canv[q];
+  }
+  o[39] = a.dt * sdir;
 }
 
 }  // namespace spde
@@ -323,6 +332,7 @@ extern "C" int spde_grad_cells(spde_plan* pl, const spde_grad_args* g, double* o
   a.sq0 = closed ? g->sigma_q0 : nullptr;
   a.pos25 = closed ? g->spatial->d_pos25 : nullptr;
   a.wts = g->weights;
+  a.adv_dir = g->adv_dir;
   double* win = pl->d_win;
   const unsigned blocks = (unsigned)((a.Ns + 127) / 128);
   hipLaunchKernelGGL(grad_reduce_kernel, dim3(blocks), dim3(128), 0, s, a, win, out);

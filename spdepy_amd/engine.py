@@ -12,7 +12,7 @@ _PLANS = {}
 
 # column layout of the per-cell adjoints written by spde_grad_cells
 OUT_KAPPA, OUT_H, OUT_DGX, OUT_DGY, OUT_KAPPA0, OUT_H0, OUT_HC, OUT_WX, OUT_WY = 0, 1, 17, 21, 25, 26, 30, 34, 35
-OUT_DOTGQ, OUT_DOT00, OUT_TAUTR, OUT_WIDTH = 36, 37, 38, 40
+OUT_DOTGQ, OUT_DOT00, OUT_TAUTR, OUT_ADVDIR, OUT_WIDTH = 36, 37, 38, 39, 40
 
 
 def get_plan(M, N, T, bc, leaf_max=128, nslots=2, wide=False):
@@ -124,7 +124,7 @@ class Engine:
 
     # ------------------------------------------------------- log-likelihood
     def loglik(self, coeffs, obs_cols, data, log_tau, grad=True, trace="exact", probes=None, logdet="closed",
-               contract=None):
+               contract=None, adv_dir=None):
         """Returns (like, parts) where like is the normalised -loglik/(m r) and
         parts holds the per-cell gradient adjoints and the scalar pieces.
 
@@ -137,7 +137,9 @@ class Engine:
 
         contract(cells_device) -> (coef_grad, column_sums): when given, the
         per-cell adjoints stay on the device and only the contraction onto
-        the model's coefficients and the 40 column sums come back."""
+        the model's coefficients and the 40 column sums come back.
+        adv_dir (Ns x 4 faces): column 39 of the cells receives the derivative
+        along the scaling of the advection field (cov-advection models)."""
         torch = _torch()
         y = np.asarray(data, dtype=np.float64)
         y2 = y.reshape(y.shape[0], -1)
@@ -221,6 +223,10 @@ class Engine:
         g.diag_add = diag_add.data_ptr()
         g.coeffs = coeffs.c
         keep = [mu_cm]
+        if adv_dir is not None:
+            adv = torch.as_tensor(np.ascontiguousarray(adv_dir, dtype=np.float64).reshape(-1)).cuda()
+            g.adv_dir = adv.data_ptr()
+            keep.append(adv)
         if exact:
             g.mode, g.nh1 = 0, 0
             g.sigma_qc = sqc.data_ptr()

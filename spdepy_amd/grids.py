@@ -217,6 +217,21 @@ class Grid(_Lattice2D):
         self.Ae = Ae.tocsr()
         return self.Ae
 
+    def assimilate_adv(self, we, wn):
+        """Face velocities (faces E, N, W, S) from cell-centred currents we, wn
+        on the interior M x N cells (``:277-344``): a face takes the mean of its
+        cell and the neighbour across it, the cell alone at the lattice border;
+        extended grids extrapolate to the border cells as ``advBound`` does."""
+        M, N = self.M, self.N
+        we = np.asarray(we, dtype=np.float64).reshape(M * N)
+        wn = np.asarray(wn, dtype=np.float64).reshape(M * N)
+        k = np.arange(M * N)
+        i, j = k % M, k // M
+        nb = [np.where(i + 1 == M, k, k + 1), np.where(j + 1 == N, k, k + M), np.where(i == 0, k, k - 1),
+              np.where(j == 0, k, k - M)]
+        ww = np.stack([(w + w[b]) / 2 for w, b in zip((we, wn, we, wn), nb)], axis=1)
+        return self.ext_map() @ ww if self.isExtended else ww
+
     def evalAdv(self, par, bs=None, d=None):
         if d is not None:
             par = np.zeros(np.shape(par))

@@ -24,7 +24,7 @@ from scipy import sparse
 
 from . import _native as nat
 from .engine import (Coefficients, Engine, KronEngine, OUT_DGX, OUT_DGY, OUT_H, OUT_H0, OUT_HC, OUT_KAPPA, OUT_KAPPA0,
-                     OUT_WIDTH, OUT_WX, OUT_WY)
+                     OUT_WIDTH, OUT_WX, OUT_WY, OUT_ADVDIR)
 from .plan import _ptr, _stream
 
 
@@ -619,7 +619,8 @@ class _SpaceTimeBase(_StencilMixin):
         if grad and trace == "hutchinson":
             probes = (2 * np.random.randint(1, 3, self.grid.n * nh1) - 3).reshape(self.grid.n, nh1)
         like, parts = eng.loglik(coeffs, self._obs_cols(), self.data, par[-1], grad=grad, trace=trace,
-                                 probes=probes, logdet=logdet, contract=self._device_contract())
+                                 probes=probes, logdet=logdet, contract=self._device_contract(),
+                                 adv_dir=self._adv_dir())
         if not grad:
             return like
         g = self._contract(par, coeffs, parts)
@@ -629,6 +630,11 @@ class _SpaceTimeBase(_StencilMixin):
     def _device_contract(self):
         """Contraction of the per-cell adjoints on the device, or None (the
         host contracts them)."""
+        return None
+
+    def _adv_dir(self):
+        """Advection direction whose scaling is a parameter (cov-advection
+        models), or None."""
         return None
 
     def _mod0_grad(self, par, parts, dH0, kappa0):
@@ -1234,6 +1240,111 @@ class SeperableSpatialTemporalIDiffusion2D(_SeparableBase):
     kind, label = "iso", ""
 
 
+class _CovAdvDiff(_FieldAdvDiff):
+    """Cov-advection families: the advection is lambda * ww, a given face
+    field (``grid.assimilate_adv`` of ocean currents), with constant
+    (cov_advection_{i,,ha_}diffusion2D.py) or B-spline
+    (cov_advection_var_{i,,ha_}diffusion2D.py) kappa and diffusion.
+    par = [kappa block, gamma block, (vx, vy blocks,) lambda, log sigma,
+    (mod0...), log tau].  As in the reference, the lambda derivative of the
+    advection operator is the stencil of ww itself (dA = Aw(ww) dt,
+    cov_advection_idiffusion2D.py:59,153-159), which is the derivative for
+    lambda > 0.  ww comes with the constructor or initFit(ww=...)."""
+    gvar = False
+    v0, w0 = 0.1, 0.1   # initial v and lambda
+
+    def __init__(self, grid, mod0, par=None, bc=3, ww=None):
+        self.grid, self.mod0, self.bc = grid, mod0, bc
+        self.type = "%s-bc%d" % (self.name, bc)
+        self.Np = grid.Nbs2
+        self.Q = self.Q_fac = self.data = self.r = self.S = None
+        self.fitQ0 = True
+        self._eng = None
+        self.ww = None if ww is None else np.asarray(ww, dtype=np.float64)
+        self.setPars(np.hstack([self._init_self(), [np.log(100)]]) if par is None else par)
+
+    def _sizes(self):
+        nd = self.Np if self.dvar else 1
+        return nd, (1 if self.kind == "iso" else 3), 1
+
+    def _init_self(self):
+        nd, nh, _ = self._sizes()
+        return np.hstack([[-1.0] * nd, [-1.0] * nd] + [[self.v0] * nd] * (nh - 1) + [[self.w0], [0.0]])
+
+    def initFit(self, data, **kwargs):
+        if kwargs.get("ww") is not None:
+            self.ww = np.asarray(kwargs["ww"], dtype=np.float64)
+        assert self.ww is not None, "the advection field ww is needed (initFit(ww=...))"
+        return super().initFit(data, **kwargs)
+
+    def _coeffs(self, par):
+        assert self.ww is not None, "the advection field ww is needed (initFit(ww=...))"
+        g = self.grid
+        blocks, lsig = self._split(par)
+        _, kappa0, H0, _ = self._mod0_split(par)
+        kw = dict(variable=True, sigma=np.exp(lsig), kappa0=kappa0, H0=H0, G=blocks[-1][0] * self.ww)
+        if self.dvar:
+            kw["kappa"] = np.exp(g.bs @ blocks[0])
+            kw["H"], dH = self._face_fields(blocks)
+            kw["mix"] = 0
+        else:
+            vx, vy = (blocks[2][0], blocks[3][0]) if self.kind != "iso" else (0.0, 0.0)
+            Hc, dH = h_tensor(self.kind, np.exp(blocks[1][0]), vx, vy)
+            kw.update(kappa=None, H=None, kappa_c=np.exp(blocks[0][0]), H_c=Hc, mix=2)
+        self._fields = (kw.get("kappa"), kw.get("kappa_c"), dH)
+        return Coefficients(g, **kw)
+
+    def _adv_dir(self):
+        return self.ww
+
+    def _contract(self, par, coeffs, parts):
+        g = self.grid
+        _, kappa0, H0, dH0 = self._mod0_split(par)
+        kappa, kappa_c, dH = self._fields
+        colsum = parts["colsum"]
+        out = []
+        if self.dvar:
+            cells = parts["cells"]
+            out.append(g.bs.T @ (cells[:, OUT_KAPPA] * kappa))
+            Hb = cells[:, OUT_H:OUT_H + 16].reshape(-1, 4, 2, 2)
+            for d in dH:
+                out.append(np.einsum("kfi,kf->i", g.bsH, (Hb * d).sum(axis=(2, 3))))
+        else:
+            out.append([kappa_c * colsum[OUT_KAPPA]])
+            hbar = colsum[OUT_HC:OUT_HC + 4]
+            out.append([float(np.dot(hbar, d.reshape(4))) for d in dH])
+        out.append([colsum[OUT_ADVDIR]])
+        out.append([parts["g_sigma"]])
+        out.append(self._mod0_grad(par, parts, dH0, kappa0))
+        out.append([parts["g_tau"]])
+        return np.hstack(out).astype(np.float64)
+
+
+class CovAdvectionIDiffusion2D(_CovAdvDiff):
+    kind, dvar, name = "iso", False, "cov-advection-idiffusion-2D"
+
+
+class CovAdvectionDiffusion2D(_CovAdvDiff):
+    kind, dvar, name = "ani", False, "cov-advection-diffusion-2D"
+
+
+class CovAdvectionHaDiffusion2D(_CovAdvDiff):
+    kind, dvar, name = "ha", False, "cov-advection-ha-diffusion-2D"
+    v0, w0 = 0.01, 0.01   # cov_advection_ha_diffusion2D.py:27
+
+
+class CovAdvectionVarIDiffusion2D(_CovAdvDiff):
+    kind, dvar, name = "iso", True, "cov-advection-var-idiffusion-2D"
+
+
+class CovAdvectionVarDiffusion2D(_CovAdvDiff):
+    kind, dvar, name = "ani", True, "cov-advection-var-diffusion-2D"
+
+
+class CovAdvectionVarHaDiffusion2D(_CovAdvDiff):
+    kind, dvar, name = "ha", True, "cov-advection-var-ha-diffusion-2D"
+
+
 class _LazyDQ:
     """The reference's list of dQ_i, produced on demand (one matrix per
     parameter except tau).  Not needed by logLike, whose gradient contracts all
@@ -1264,6 +1375,13 @@ def spde_init(model, grid, parameters=None, ani=True, ha=True, bc=3, mod0=None):
         return cls(grid=grid, mod0=mod0, par=parameters, bc=bc)
     if model in ("advection-var-diffusion", 3):
         cls = AdvectionVarHaDiffusion2D if ha else (AdvectionVarDiffusion2D if ani else AdvectionVarIDiffusion2D)
+        return cls(grid=grid, mod0=mod0, par=parameters, bc=bc)
+    if model in ("cov-advection-diffusion", 4):
+        cls = CovAdvectionHaDiffusion2D if ha else (CovAdvectionDiffusion2D if ani else CovAdvectionIDiffusion2D)
+        return cls(grid=grid, mod0=mod0, par=parameters, bc=bc)
+    if model in ("cov-advection-var-diffusion", 5):
+        cls = (CovAdvectionVarHaDiffusion2D if ha else
+               (CovAdvectionVarDiffusion2D if ani else CovAdvectionVarIDiffusion2D))
         return cls(grid=grid, mod0=mod0, par=parameters, bc=bc)
     if model in ("var-advection-diffusion", 6):
         cls = VarAdvectionHaDiffusion2D if ha else (VarAdvectionDiffusion2D if ani else VarAdvectionIDiffusion2D)

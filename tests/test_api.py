@@ -33,6 +33,17 @@ FAMILIES = {
         ("VarAdvectionVarDiffusion2D", np.hstack([[-1] * 9, [-1] * 9, [0.1] * 18, [0.1] * 18, 0, LOG100])),
     ("var-advection-var-diffusion", True, True):
         ("VarAdvectionVarHaDiffusion2D", np.hstack([[-1] * 9, [-1] * 9, [0.1] * 18, [0.1] * 18, 0, LOG100])),
+    ("cov-advection-diffusion", False, False): ("CovAdvectionIDiffusion2D", np.array([-1, -1, 0.1, 0, LOG100])),
+    ("cov-advection-diffusion", True, False):
+        ("CovAdvectionDiffusion2D", np.array([-1, -1, 0.1, 0.1, 0.1, 0, LOG100])),
+    ("cov-advection-diffusion", True, True):
+        ("CovAdvectionHaDiffusion2D", np.array([-1, -1, 0.01, 0.01, 0.01, 0, LOG100])),
+    ("cov-advection-var-diffusion", False, False):
+        ("CovAdvectionVarIDiffusion2D", np.hstack([[-1] * 9, [-1] * 9, 0.1, 0, LOG100])),
+    ("cov-advection-var-diffusion", True, False):
+        ("CovAdvectionVarDiffusion2D", np.hstack([[-1] * 9, [-1] * 9, [0.1] * 9, [0.1] * 9, 0.1, 0, LOG100])),
+    ("cov-advection-var-diffusion", True, True):
+        ("CovAdvectionVarHaDiffusion2D", np.hstack([[-1] * 9, [-1] * 9, [0.1] * 9, [0.1] * 9, 0.1, 0, LOG100])),
     ("seperable-spatial-temporal", False, False):
         ("SeperableSpatialTemporalIDiffusion2D", np.hstack([[-1] * 9, [-1] * 9, 0.1, 1, LOG100])),
     ("seperable-spatial-temporal", True, False):
@@ -58,7 +69,8 @@ def test_dispatch_and_default_parameters(key):
         assert np.array_equal(m.getPars(), default)
         # initFit appends the initial-state parameters before log tau
         data = np.zeros(10)
-        par = m.initFit(data, idx=np.arange(10))
+        kw = dict(ww=np.zeros((g.Ns, 4))) if spde.startswith("cov") else {}
+        par = m.initFit(data, idx=np.arange(10), **kw)
         assert par.size == default.size + mod0.getPars().size - 1
         assert np.array_equal(par[:default.size - 1], default[:-1]) and par[-1] == LOG100
 

@@ -405,3 +405,40 @@ def test_library_first_then_torch_in_a_fresh_process():
                          cwd=os.path.dirname(os.path.dirname(__file__)))
     assert res.returncode == 0, res.stderr[-2000:]
     assert res.stdout.strip().endswith("4.0")
+
+
+@pytest.mark.parametrize("kind,dvar,bc,lam", [("iso", False, 3, 0.8), ("ani", False, 1, 0.6), ("ha", False, 3, -0.5),
+                                              ("iso", True, 1, 0.7), ("ha", True, 3, 0.9)])
+def test_cov_family_q_and_loglike(kind, dvar, bc, lam):
+    """Cov-advection families (advection lambda * ww from assimilate_adv of
+    synthetic currents; SINMOD currents are not available): Q and the
+    exact-trace logLike value and gradient against the oracle, whose lambda
+    derivative uses the stencil of ww as the reference does (also for
+    lambda < 0, where it is not the derivative)."""
+    import spdepy_amd as sp
+    from spdepy_amd import spdes
+    c = F.field_model(9, 8, 4, kind=kind, dvar=dvar, gvar=False, bc=bc, m=45, cov=lam)
+    g, par = c["grid"], c["par"]
+    Qo, dQo = c["model"].makeQ(par, grad=True)
+    like_o, jac_o, _ = Mo.loglike(Qo, dQo, g.getS(c["idx"]), c["y"], par[-1], trace="exact")
+    pg = sp.grid(x=g.x, y=g.y, t=g.t, Nbs=3)
+    mod0 = spdes.WhittleMatern2D(grid=sp.grid(x=g.x, y=g.y, Nbs=3), par=np.array(c["mod0"].par), bc=bc)
+    spde = "cov-advection-var-diffusion" if dvar else "cov-advection-diffusion"
+    m = spdes.spde_init(spde, pg, ani=kind != "iso", ha=kind == "ha", bc=bc, mod0=mod0)
+    X, Y = np.meshgrid(g.x, g.y)
+    ww = pg.assimilate_adv(*F.currents(g))
+    assert np.array_equal(ww, c["ww"])
+    m.initFit(c["y"], idx=c["idx"], fitQ0=False, ww=ww)
+    step("cov family Q %s dvar=%d bc=%d" % (kind, dvar, bc))
+    Qg, _, _ = m.makeQ(par, grad=False)
+    Qo = Qo.tocsc()
+    Qo.sort_indices()
+    Qg.sort_indices()
+    assert np.array_equal(Qg.indptr, Qo.indptr) and np.array_equal(Qg.indices, Qo.indices)
+    assert np.allclose(Qg.data, Qo.data, rtol=1e-12, atol=1e-13 * np.abs(Qo.data).max())
+    step("cov family logLike %s dvar=%d bc=%d" % (kind, dvar, bc))
+    like, jac = m.logLike(par)
+    assert abs(like - like_o) <= 1e-9 * abs(like_o), (like, like_o)
+    scale = np.abs(jac_o).max()
+    assert jac.shape == jac_o.shape
+    assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * scale), np.abs(jac - jac_o).max() / scale

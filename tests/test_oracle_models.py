@@ -255,3 +255,48 @@ def test_separable_temporal_precision_and_derivatives(kind):
         pm[i] -= h
         fd = (m.makeQ(pp, False)[0] - m.makeQ(pm, False)[0]) / (2 * h)
         assert abs(fd - dQ[i]).max() <= 1e-6 * abs(dQ[i]).max(), i
+
+
+def test_assimilate_adv_faces():
+    """Face velocities from currents: a constant field stays constant on
+    every face (border faces use the cell alone), a linear one gives the face
+    mid-points inside and the cell value at the border; extended grids
+    extrapolate with the advection border map."""
+    from oracle import grid as OG
+    import spdepy_amd as sp
+    M, N = 6, 5
+    for ne in (None, 2):
+        og = OG.STGrid(F.axis(M), F.axis(N), np.arange(3.0), extend=ne, Nbs=3)
+        pg = sp.grid(x=F.axis(M), y=F.axis(N), t=np.arange(3.0), extend=ne, Nbs=3)
+        X, Y = np.meshgrid(F.axis(M), F.axis(N))
+        we, wn = X.ravel(), 2 * Y.ravel()
+        wo, wp = og.assimilate_adv(we, wn), pg.assimilate_adv(we, wn)
+        assert np.allclose(wo, wp, rtol=0, atol=1e-15)
+        if ne is None:
+            h = 1.0 / M
+            i = np.arange(M * N) % M
+            assert np.allclose(wo[:, 0], np.where(i + 1 < M, we + h / 2, we))
+            assert np.allclose(wo[:, 2], np.where(i > 0, we - h / 2, we))
+            c = og.assimilate_adv(np.full(M * N, 0.7), np.full(M * N, -0.3))
+            assert np.allclose(c[:, [0, 2]], 0.7) and np.allclose(c[:, [1, 3]], -0.3)
+        else:
+            assert wo.shape == (og.Ns, 4)
+            assert np.allclose(wo, og._ext_map() @ OG.STGrid(F.axis(M), F.axis(N), np.arange(3.0)).assimilate_adv(we, wn))
+
+
+@pytest.mark.parametrize("kind,dvar", [("iso", False), ("ani", True), ("ha", False)])
+def test_cov_family_derivatives(kind, dvar):
+    """Cov-advection oracle (advection lambda * ww): with lambda > 0 at bc=3
+    the reference's dA = Aw(ww) dt is the derivative, so every 3rd dQ_i and
+    the lambda one match central differences of Q."""
+    c = F.field_model(kind=kind, dvar=dvar, gvar=False, bc=3, cov=0.8)
+    m, par = c["model"], c["par"]
+    Q, dQ = m.makeQ(par, grad=True)
+    assert len(dQ) == par.size - 1
+    for i in list(range(0, par.size - 3, 3)) + [par.size - 3, par.size - 2]:
+        h = 1e-6
+        pp, pm = par.copy(), par.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (m.makeQ(pp, False)[0] - m.makeQ(pm, False)[0]) / (2 * h)
+        assert abs(fd - dQ[i]).max() <= 1e-6 * max(abs(dQ[i]).max(), 1e-3 * abs(Q).max()), i
