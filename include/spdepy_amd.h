@@ -78,6 +78,7 @@ typedef struct spde_plan_info {
   int64_t n, nnz_struct, nnz_L, front_doubles, z_doubles;
   int64_t nnodes, nlevels, max_front, max_p;
   double flops_factor, flops_selinv;
+  int64_t split_doubles;  /* per-slot split-K workspace (factor, selected inverse, last solve) */
 } spde_plan_info;
 
 int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int flags,

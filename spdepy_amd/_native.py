@@ -35,7 +35,7 @@ class PlanInfo(ctypes.Structure):
                 ("front_doubles", ctypes.c_int64), ("z_doubles", ctypes.c_int64),
                 ("nnodes", ctypes.c_int64), ("nlevels", ctypes.c_int64), ("max_front", ctypes.c_int64),
                 ("max_p", ctypes.c_int64), ("flops_factor", ctypes.c_double),
-                ("flops_selinv", ctypes.c_double)]
+                ("flops_selinv", ctypes.c_double), ("split_doubles", ctypes.c_int64)]
 
 
 class Coeffs(ctypes.Structure):

@@ -22,9 +22,13 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // 16 consecutive rows of one column of C: stores are 128-byte coalesced in
 // column-major memory.  Accumulator layout of v_mfma_f64_16x16x4f64
 // (measured): lane l, register v holds D[l/16 + 4v][l%16].
-template <bool TA, bool TB, int BM, int BN>
+// SPLIT: the block computes the chunk t.pad of `split` chunks of the tile's
+// K range (each a multiple of BK deep) and writes the raw 64x64 partial tile
+// to ws + blockIdx.x * 4096 (column-major, ld 64); split_reduce_kernel then
+// applies alpha, beta and the lower-triangle rule.
+template <bool TA, bool TB, int BM, int BN, bool SPLIT = false>
 __device__ __forceinline__ void gemm_body(const GemmProb* __restrict__ probs, const GemmTile* __restrict__ tiles,
-                                          const Bases& bases) {
+                                          const Bases& bases, int split = 1, double* __restrict__ ws = nullptr) {
   constexpr int BK = 16;
   // LDS layout follows each operand's memory order so that both the global
   // loads and the LDS stores are contiguous: an operand stored with m (n)
@@ -46,7 +50,13 @@ __device__ __forceinline__ void gemm_body(const GemmProb* __restrict__ probs, co
   const double* A = bases.p[P.a_base] + P.a_off;
   const double* __restrict__ B = bases.p[P.b_base] + P.b_off;
   double* C = bases.p[P.c_base] + P.c_off;
-  const int M = P.M, N = P.N, K = P.K;
+  const int M = P.M, N = P.N;
+  int kb = 0, K = P.K;  // this block's K range [kb, K)
+  if constexpr (SPLIT) {
+    const int kc = ((P.K + split - 1) / split + 15) / 16 * 16;
+    kb = min(P.K, t.pad * kc);
+    K = min(P.K, kb + kc);
+  }
   const int m0 = t.tm * BM, n0 = t.tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
@@ -65,6 +75,7 @@ __device__ __forceinline__ void gemm_body(const GemmProb* __restrict__ probs, co
     for (int y = 0; y < TM; ++y) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
   double ra[LA], rb[LB];
   auto load = [&](int k0) {
+    k0 += kb;
     const bool full = interior && (k0 + BK <= K);
 #pragma unroll
     for (int q = 0; q < LA; ++q) {
@@ -87,7 +98,7 @@ __device__ __forceinline__ void gemm_body(const GemmProb* __restrict__ probs, co
 #pragma unroll
     for (int q = 0; q < LB; ++q) Bs[buf][bidx(bk + q * BSTEP_K, bn + q * BSTEP_N)] = rb[q];
   };
-  const int nk = (K + BK - 1) / BK;
+  const int nk = (K - kb + BK - 1) / BK;
   if (nk > 0) {
     load(0);
     store(0);
@@ -119,6 +130,17 @@ __device__ __forceinline__ void gemm_body(const GemmProb* __restrict__ probs, co
     if (kt + 1 < nk) store(cur ^ 1);
     __syncthreads();
   }
+  if constexpr (SPLIT) {
+    double* w = ws + (int64_t)blockIdx.x * (BM * BN);
+#pragma unroll
+    for (int ni = 0; ni < TN; ++ni)
+#pragma unroll
+      for (int mi = 0; mi < TM; ++mi)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          w[(wm + mi * 16 + lr) + (wn + ni * 16 + lk + 4 * v) * BM] = acc[ni][mi][v];
+    return;
+  }
   const double alpha = P.alpha, beta = P.beta;
   // lower problems: the part of a diagonal tile above the diagonal is left
   // untouched (it is never read, and stays as initialised)
@@ -146,6 +168,37 @@ __global__ __launch_bounds__(256) void gemm64_kernel(const GemmProb* __restrict_
   gemm_body<TA, TB, 64, 64>(probs, tiles, bases);
 }
 
+// Split-K partial tiles (64x64) and their ordered sum.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm64_split_kernel(const GemmProb* __restrict__ probs,
+                                                           const GemmTile* __restrict__ tiles, Bases bases, int split,
+                                                           double* __restrict__ ws) {
+  gemm_body<TA, TB, 64, 64, true>(probs, tiles, bases, split, ws);
+}
+
+// Block b: C tile of tiles[b * split] = alpha * sum_q partial[b * split + q]
+// (q ascending) + beta C, inside the problem and, for lower problems, on or
+// below the diagonal.
+__global__ __launch_bounds__(256) void split_reduce_kernel(const GemmProb* __restrict__ probs,
+                                                           const GemmTile* __restrict__ tiles, Bases bases, int split,
+                                                           const double* __restrict__ ws) {
+  const GemmTile t = tiles[(int64_t)blockIdx.x * split];
+  const GemmProb P = probs[t.prob];
+  double* C = bases.p[P.c_base] + P.c_off;
+  const int m0 = t.tm * 64, n0 = t.tn * 64;
+  const bool diag_tile = P.lower && t.tm == t.tn;
+  const double* w = ws + (int64_t)blockIdx.x * split * 4096;
+  for (int e = threadIdx.x; e < 4096; e += 256) {
+    const int m = m0 + (e & 63), n = n0 + (e >> 6);
+    if (m >= P.M || n >= P.N || (diag_tile && m < n)) continue;
+    double acc = 0.0;
+    for (int q = 0; q < split; ++q) acc += w[(int64_t)q * 4096 + e];
+    double* c = C + m + (int64_t)n * P.ldc;
+    const double r = P.alpha * acc;
+    *c = (P.beta == 0.0) ? r : r + P.beta * (*c);
+  }
+}
+
 // 128x128 tiles (each wave 64x64: half the LDS reads per MFMA): two blocks
 // per CU (LDS 74 KB); the register budget is pinned to two waves per SIMD,
 // without it the compiler takes ~360 registers and one wave.
@@ -158,6 +211,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 void launch_gemm(const GemmLaunch& g, const Bases& b, hipStream_t s) {
   if (g.ntiles == 0) return;
   dim3 grid(g.ntiles), block(256);
+  if (g.split > 1) {
+    double* ws = b.ws + g.ws_off;
+    if (!g.ta && !g.tb)
+      hipLaunchKernelGGL((gemm64_split_kernel<false, false>), grid, block, 0, s, g.d_probs, g.d_tiles, b, g.split, ws);
+    else if (!g.ta && g.tb)
+      hipLaunchKernelGGL((gemm64_split_kernel<false, true>), grid, block, 0, s, g.d_probs, g.d_tiles, b, g.split, ws);
+    else if (g.ta && !g.tb)
+      hipLaunchKernelGGL((gemm64_split_kernel<true, false>), grid, block, 0, s, g.d_probs, g.d_tiles, b, g.split, ws);
+    else
+      hipLaunchKernelGGL((gemm64_split_kernel<true, true>), grid, block, 0, s, g.d_probs, g.d_tiles, b, g.split, ws);
+    hipLaunchKernelGGL(split_reduce_kernel, dim3(g.ntiles / g.split), block, 0, s, g.d_probs, g.d_tiles, b, g.split,
+                       ws);
+    return;
+  }
   if (g.tile == 128) {
     if (!g.ta && !g.tb)
       hipLaunchKernelGGL((gemm128_kernel<false, false>), grid, block, 0, s, g.d_probs, g.d_tiles, b);

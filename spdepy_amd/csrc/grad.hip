@@ -55,15 +55,20 @@ struct GradKArgs {
 constexpr int kWin = 60;  // Gd[25] | Gu[9] | G00[25] | Gq
 constexpr int kOut = 40;
 
-// Pass 1: per cell, reduce G over time into windows (thread per cell).
-__global__ void grad_reduce_kernel(GradKArgs a, double* __restrict__ win, double* __restrict__ out) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= a.Ns) return;
+constexpr int kPart = kWin + 3;  // window | dotGQ, dot00, tautr
+
+// Pass 1a: per cell and time slice, the slice's share of the windows (thread
+// per (cell, slice)); pass 1b sums the slices in order.
+__global__ void grad_reduce_kernel(GradKArgs a, double* __restrict__ part) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= a.Ns * a.T) return;
+  const int64_t i = idx % a.Ns;
+  const int t = (int)(idx / a.Ns);
   double w[kWin];
   for (int q = 0; q < kWin; ++q) w[q] = 0.0;
   double dotGQ = 0.0, dot00 = 0.0, tautr = 0.0;
   const double inh = a.nh1 > 0 ? 1.0 / a.nh1 : 0.0;
-  for (int t = 0; t < a.T; ++t) {
+  {
     const int64_t r = i + t * a.Ns;
     for (int64_t e = a.indptr[r]; e < a.indptr[r + 1]; ++e) {
       const int64_t col = a.indices[e];
@@ -107,10 +112,25 @@ __global__ void grad_reduce_kernel(GradKArgs a, double* __restrict__ win, double
       }
     }
   }
-  for (int q = 0; q < kWin; ++q) win[i * kWin + q] = w[q];
-  out[i * kOut + 36] = dotGQ;
-  out[i * kOut + 37] = dot00;
-  out[i * kOut + 38] = tautr;
+  double* p = part + idx * kPart;
+  for (int q = 0; q < kWin; ++q) p[q] = w[q];
+  p[kWin] = dotGQ;
+  p[kWin + 1] = dot00;
+  p[kWin + 2] = tautr;
+}
+
+__global__ void grad_sum_kernel(GradKArgs a, const double* __restrict__ part, double* __restrict__ win,
+                                double* __restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= a.Ns) return;
+  for (int q = 0; q < kPart; ++q) {
+    double acc = 0.0;
+    for (int t = 0; t < a.T; ++t) acc += part[(i + t * a.Ns) * kPart + q];
+    if (q < kWin)
+      win[i * kWin + q] = acc;
+    else
+      out[i * kOut + 36 + (q - kWin)] = acc;
+  }
 }
 
 __device__ inline int widx(int dx, int dy) { return (dy + 2) * 5 + dx + 2; }
@@ -334,8 +354,14 @@ extern "C" int spde_grad_cells(spde_plan* pl, const spde_grad_args* g, double* o
   a.wts = g->weights;
   a.adv_dir = g->adv_dir;
   double* win = pl->d_win;
+  if (!pl->d_gpart) {
+    SPDE_HIP(hipStreamSynchronize(s));
+    SPDE_HIP(hipMalloc((void**)&pl->d_gpart, sizeof(double) * a.Ns * a.T * kPart));
+  }
   const unsigned blocks = (unsigned)((a.Ns + 127) / 128);
-  hipLaunchKernelGGL(grad_reduce_kernel, dim3(blocks), dim3(128), 0, s, a, win, out);
+  hipLaunchKernelGGL(grad_reduce_kernel, dim3((unsigned)((a.Ns * a.T + 127) / 128)), dim3(128), 0, s, a,
+                     pl->d_gpart);
+  hipLaunchKernelGGL(grad_sum_kernel, dim3(blocks), dim3(128), 0, s, a, pl->d_gpart, win, out);
   hipLaunchKernelGGL(grad_cell_kernel, dim3(blocks), dim3(128), 0, s, a, win, out);
   SPDE_HIP(hipGetLastError());
   return SPDE_OK;

@@ -29,6 +29,7 @@ constexpr int kTmpl = 143;  // per-cell assembly template (doubles)
 enum Base : int8_t { kFront = 0, kZ = 1, kSigma = 2, kWork = 3, kVec = 4, kX = 5, kNumBases = 6 };
 struct Bases {
   double* p[kNumBases];
+  double* ws;  // split-K partial tiles of the slot
 };
 
 // One GEMM problem: C = alpha op(A) op(B) + beta C, column-major.
@@ -54,6 +55,11 @@ struct GemmLaunch {
   std::vector<GemmTile> h_tiles;
   double flops = 0;
   double bytes = 0;   // algorithmic HBM bytes: A, B, C read once (C only if beta != 0), C written once
+  // split-K (few tiles, deep K): each tile's K range is cut into `split`
+  // chunks computed by separate blocks into partial tiles at ws + ws_off (in
+  // the slot's split workspace), summed in chunk order by a second kernel
+  int split = 1;
+  int64_t ws_off = 0;
 };
 
 struct PotrfDesc {      // one diagonal block of one front
@@ -111,6 +117,8 @@ struct Slot {
   int64_t xy_cap = 0;
   hipStream_t stream = nullptr;  // stream of the last factorisation
   bool factored = false;
+  double* splitws = nullptr;  // split-K partial tiles: [factor + selected inverse | solve] regions
+  int64_t splitws_cap = 0;
 };
 
 struct NodeDev {            // per-node table on the device
@@ -146,6 +154,9 @@ struct spde_plan {
   double* d_win = nullptr;        // per-cell gradient windows
   double* d_red = nullptr;        // one device scalar (reductions)
   int32_t* d_pos25 = nullptr;     // one-slice plans: entry index of (cell, 5x5 code), -1 if absent
+  int64_t split_fs = 0;           // split-K workspace (doubles) of the factor and selected-inverse launches
+  int64_t split_solve = 0;        // ... of the solve launches (current nrhs)
+  double* d_gpart = nullptr;      // per (cell, slice) partial windows of the gradient reduction
   int32_t* d_sprow = nullptr;     // one-slice plans: row and column of each entry (Kronecker kernels)
   int32_t* d_spcol = nullptr;
   int64_t work_doubles = 0;       // per-slot workspace size

@@ -37,6 +37,12 @@ void launch_zero_lower(const NodeDev* nodes, const int4* tiles, int ntiles, doub
 static thread_local std::string g_err;
 void set_error(const std::string& m) { g_err = m; }
 
+// Clears a factorisation's status word (a kernel rather than a memset node,
+// so that a captured factorisation replays correctly).
+__global__ void clear_status_kernel(int* status) {
+  if (threadIdx.x == 0) status[threadIdx.x] = 0;
+}
+
 // Device-side per-level index tables that are not part of LevelPlan.
 struct LevelDev {
   int32_t* d_nodes = nullptr;
@@ -122,8 +128,32 @@ ArenaSizes arena_sizes(const spde_plan* pl, int nrhs) {
 
 void make_tiles(GemmLaunch& g, int force_tile);
 
-int finalize(GemmLaunch& g) {
+// Split-K for launches with few tiles and deep problems: the launch fills at
+// most a quarter of the chip's block slots and its deepest problem is at
+// least 512 deep.  S = min(512 / tiles, K_max / 256, 64) chunks, each at
+// least 256 deep; the partial tiles take S * tiles * 64 * 64 doubles of the
+// workspace counted in *ws.
+void plan_split(GemmLaunch& g, int64_t* ws) {
+  if (!ws || g.tile != 64 || g.h_tiles.empty()) return;
+  int kmax = 0;
+  for (const GemmProb& P : g.h_probs) kmax = std::max(kmax, P.K);
+  const int64_t nt = (int64_t)g.h_tiles.size();
+  if (nt >= 256 || kmax < 512) return;
+  const int S = (int)std::min<int64_t>(std::min<int64_t>(512 / nt, kmax / 256), 64);
+  if (S < 2) return;
+  std::vector<GemmTile> t;
+  t.reserve(nt * S);
+  for (const GemmTile& b : g.h_tiles)
+    for (int q = 0; q < S; ++q) t.push_back({b.prob, b.tm, b.tn, q});
+  g.h_tiles.swap(t);
+  g.split = S;
+  g.ws_off = *ws;
+  *ws += (int64_t)g.h_tiles.size() * 64 * 64;
+}
+
+int finalize(GemmLaunch& g, int64_t* ws) {
   make_tiles(g, 0);
+  plan_split(g, ws);
   g.nprobs = (int)g.h_probs.size();
   g.ntiles = (int)g.h_tiles.size();
   int rc = upload(&g.d_probs, g.h_probs);
@@ -194,6 +224,7 @@ GemmProb prob(int8_t ab, int64_t ao, int lda, int8_t bb, int64_t bo, int ldb, in
 
 int build_factor_plan(spde_plan* pl) {
   const Symbolic& sym = pl->sym;
+  pl->split_fs = 0;
   pl->levels.assign(sym.levels.size(), LevelPlan());
   for (size_t l = 0; l < sym.levels.size(); ++l) {
     LevelPlan& L = pl->levels[l];
@@ -256,7 +287,7 @@ int build_factor_plan(spde_plan* pl) {
         st.npotrf = (int)pd.size();
         int rc = upload(&st.d_potrf, pd);
         if (rc) return rc;
-        rc = finalize(st.panel);
+        rc = finalize(st.panel, &pl->split_fs);
         if (rc) return rc;
         for (const GemmProb& P : st.panel.h_probs)
           if (P.N > st.panel.tile) {  // the in-place solve needs one tile column
@@ -279,7 +310,7 @@ int build_factor_plan(spde_plan* pl) {
         add_prob(st.update, prob(kFront, ab, n, kFront, ab, n, kFront, nd.foff + m + (int64_t)m * n, n, M, N, K, -1.0, 1.0),
                  true);
       }
-      rc = finalize(st.update);
+      rc = finalize(st.update, &pl->split_fs);
       if (rc) return rc;
       L.steps.push_back(std::move(st));
       return rec(m, b);
@@ -300,7 +331,7 @@ int build_factor_plan(spde_plan* pl) {
                     b, b, nd.p, -1.0, 0.0),
                true);
     }
-    int rc = finalize(L.syrk);
+    int rc = finalize(L.syrk, &pl->split_fs);
     if (rc) return rc;
   }
   // inverse of L11 for every node: recursive halving, batched by height
@@ -338,9 +369,9 @@ int build_factor_plan(spde_plan* pl) {
       woff += (int64_t)r * c;
     }
     work = std::max(work, woff);
-    int rc = finalize(gx);
+    int rc = finalize(gx, &pl->split_fs);
     if (rc) return rc;
-    rc = finalize(gz);
+    rc = finalize(gz, &pl->split_fs);
     if (rc) return rc;
     pl->trtri.push_back(std::move(gx));
     pl->trtri.push_back(std::move(gz));
@@ -388,6 +419,9 @@ int build_factor_plan(spde_plan* pl) {
 int build_solve_plan(spde_plan* pl, int k) {
   const Symbolic& sym = pl->sym;
   const int n = (int)sym.n;
+  // solve launches take the split workspace after the factor and
+  // selected-inverse regions (fixed when the plan is made)
+  int64_t acc = pl->split_fs;
   for (auto& L : pl->levels) {
     for (GemmLaunch* g : {&L.fwd_y, &L.fwd_u, &L.bwd_t, &L.bwd_x}) {
       if (g->d_probs) hipFree(g->d_probs);
@@ -411,10 +445,11 @@ int build_solve_plan(spde_plan* pl, int k) {
     for (GemmLaunch* g : {&L.fwd_y, &L.fwd_u, &L.bwd_t, &L.bwd_x}) {
       int rc = validate(*g, as, "solve");
       if (rc) return rc;
-      rc = finalize(*g);
+      rc = finalize(*g, &acc);
       if (rc) return rc;
     }
   }
+  pl->split_solve = acc - pl->split_fs;
   pl->solve_nrhs = k;
   return SPDE_OK;
 }
@@ -453,7 +488,7 @@ int build_selinv_plan(spde_plan* pl) {
     for (GemmLaunch* g : {&L.si_x, &L.si_s21, &L.si_s11a, &L.si_s11b}) {
       int rc = validate(*g, as, "selected inverse");
       if (rc) return rc;
-      rc = finalize(*g);
+      rc = finalize(*g, &pl->split_fs);
       if (rc) return rc;
     }
     LevelDev& D = ex->lvl[l];
@@ -501,8 +536,25 @@ void run_gemm(spde_plan* pl, const GemmLaunch& g, const Bases& b, hipStream_t s)
   launch_gemm(g, b, s);
 }
 
+// The slot's split-K workspace covers the factor, selected-inverse and
+// current solve regions (grows only when the solve plan grows, after the
+// device has drained).
+int ensure_splitws(spde_plan* pl, int slot) {
+  Slot& sl = pl->slots[slot];
+  const int64_t need = pl->split_fs + pl->split_solve;
+  if (need <= sl.splitws_cap) return SPDE_OK;
+  SPDE_HIP(hipDeviceSynchronize());
+  hipFree(sl.splitws);
+  sl.splitws = nullptr;
+  sl.splitws_cap = 0;
+  SPDE_HIP(hipMalloc((void**)&sl.splitws, sizeof(double) * need));
+  sl.splitws_cap = need;
+  return SPDE_OK;
+}
+
 Bases make_bases(spde_plan* pl, int slot, double* x) {
   Bases b;
+  b.ws = pl->slots[slot].splitws;
   b.p[kFront] = pl->slots[slot].front;
   b.p[kZ] = pl->slots[slot].z;
   b.p[kSigma] = pl->slots[slot].sigma;
@@ -561,8 +613,8 @@ int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int 
     g_host_only_build = true;
     ex->lvl.resize(pl->sym.levels.size());
     int rc = build_factor_plan(pl);
-    if (!rc) rc = build_solve_plan(pl, 3);
     if (!rc) rc = build_selinv_plan(pl);
+    if (!rc) rc = build_solve_plan(pl, 3);
     g_host_only_build = false;
     if (rc) {
       spde_plan_destroy(pl);
@@ -627,6 +679,7 @@ int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int 
     TRY(upload(&ex->lvl[l].d_nodes, sym.levels[l]));
   }
   TRY(build_factor_plan(pl));
+  TRY(build_selinv_plan(pl));  // fixes the split workspace of factor and selected inverse
   const int64_t Ns = pl->lat.Ns();
   SPDE_HIP(hipMalloc((void**)&pl->d_tmpl, sizeof(double) * Ns * kTmpl));
   SPDE_HIP(hipMalloc((void**)&pl->d_red, sizeof(double)));
@@ -675,7 +728,7 @@ void spde_plan_destroy(spde_plan* pl) {
                     (void*)pl->d_map, (void*)pl->d_mapsym, (void*)pl->d_perm, (void*)pl->d_relpos,
                     (void*)pl->d_bnd, (void*)pl->d_nodes, (void*)pl->d_tmpl, (void*)pl->d_acan,
                     (void*)pl->d_a0can, (void*)pl->d_qdiag, (void*)pl->d_win, (void*)pl->d_red, (void*)pl->d_pos25,
-                    (void*)pl->d_sprow, (void*)pl->d_spcol})
+                    (void*)pl->d_sprow, (void*)pl->d_spcol, (void*)pl->d_gpart})
       hipFree(p);
     for (auto& s : pl->slots) {
       hipFree(s.front);
@@ -687,6 +740,7 @@ void spde_plan_destroy(spde_plan* pl) {
       hipFree(s.status);
       hipFree(s.vec);
       hipFree(s.xy);
+      hipFree(s.splitws);
     }
     for (auto& L : pl->levels) {
       for (int s = 0; s < 2; ++s) {
@@ -739,6 +793,7 @@ int spde_plan_get_info(const spde_plan* pl, spde_plan_info* info) {
   info->max_p = pl->sym.max_p;
   info->flops_factor = pl->sym.flops_factor;
   info->flops_selinv = pl->sym.flops_selinv;
+  info->split_doubles = pl->split_fs + pl->split_solve;
   return SPDE_OK;
 }
 
@@ -788,7 +843,9 @@ int spde_factor_async(spde_plan* pl, int slot, const double* qval, const double*
   Slot& sl = pl->slots[slot];
   const Symbolic& sym = pl->sym;
   PlanExtra* ex = extra(pl);
-  SPDE_HIP(hipMemsetAsync(sl.status, 0, sizeof(int), s));
+  rc = ensure_splitws(pl, slot);
+  if (rc) return rc;
+  hipLaunchKernelGGL(clear_status_kernel, dim3(1), dim3(64), 0, s, sl.status);
   launch_zero_lower(pl->d_nodes, ex->d_ztiles, ex->nztiles, sl.front, s);
   launch_scatter(qval, pl->d_map, (int64_t)pl->st.indices.size(), sl.front, s);
   if (diag_add) launch_diag_add(diag_add, pl->d_diag, pl->d_map, sym.n, sl.front, s);
@@ -876,6 +933,8 @@ int spde_solve(spde_plan* pl, int slot, int mode, double* B, int nrhs, int64_t l
       if (rc) return rc;
     }
   }
+  rc = ensure_splitws(pl, slot);
+  if (rc) return rc;
   Bases b = make_bases(pl, slot, sl.xy);
   const int nl = (int)pl->levels.size();
   if (mode == SPDE_SOLVE_LT) {
@@ -1007,6 +1066,8 @@ int spde_selinv(spde_plan* pl, int slot, double* sigma, void* stream) {
     rc = build_selinv_plan(pl);
     if (rc) return rc;
   }
+  rc = ensure_splitws(pl, slot);
+  if (rc) return rc;
   Bases b = make_bases(pl, slot, nullptr);
   for (int l = (int)pl->levels.size() - 1; l >= 0; --l) {
     LevelPlan& L = pl->levels[l];

@@ -159,9 +159,6 @@ class Engine:
         # the data terms run on the current one.
         main = torch.cuda.current_stream()
         side = self._side_stream(torch)
-        ready = torch.cuda.Event()
-        ready.record(main)
-        side.wait_event(ready)
         sel = grad and exact
         sp = self.spatial if closed else None
         if sel:
@@ -171,6 +168,15 @@ class Engine:
                 s_gram = torch.empty(sp.nnz, dtype=torch.float64, device="cuda")
             else:
                 sq = torch.empty(self.plan.nnz, dtype=torch.float64, device="cuda")
+        # The critical path (factorisation of Q_c, then its selected inverse)
+        # is enqueued before the log|Q| side work, so that the host's time
+        # spent launching the latter does not delay it.
+        ready = torch.cuda.Event()
+        ready.record(main)
+        self.plan.factor_async(1, qval, diag_add)
+        factored = torch.cuda.Event()
+        factored.record(main)
+        side.wait_event(ready)
         with torch.cuda.stream(side):
             if closed:
                 q0 = sp.assemble_spatial(self.plan, 0)
@@ -184,12 +190,9 @@ class Engine:
                 self.plan.factor_async(0, qval)
                 if sel:
                     self.plan.selinv(0, sq)
-        self.plan.factor_async(1, qval, diag_add)
         if sel:
             # the selected inverse of Q_c (large GEMMs) runs on the side stream
             # while the solve for mu_c (latency-bound) runs here
-            factored = torch.cuda.Event()
-            factored.record(main)
             side.wait_event(factored)
             with torch.cuda.stream(side):
                 self.plan.selinv(1, sqc)
@@ -320,20 +323,20 @@ class KronEngine:
         b *= tau
         main = torch.cuda.current_stream()
         side = self._side_stream(torch)
-        ready = torch.cuda.Event()
-        ready.record(main)
-        side.wait_event(ready)
         if grad:
             sig_s = torch.empty(sp.nnz, dtype=torch.float64, device="cuda")
             sig_c = torch.empty(self.nnz, dtype=torch.float64, device="cuda")
+        ready = torch.cuda.Event()
+        ready.record(main)
+        self.plan.factor_async(1, qval, diag_add, owner=self)
+        factored = torch.cuda.Event()
+        factored.record(main)
+        side.wait_event(ready)
         with torch.cuda.stream(side):
             sp.factor_async(0, qs, owner=self)
             if grad:
                 sp.selinv(0, sig_s)
-        self.plan.factor_async(1, qval, diag_add, owner=self)
         if grad:
-            factored = torch.cuda.Event()
-            factored.record(main)
             side.wait_event(factored)
             with torch.cuda.stream(side):
                 self.plan.selinv(1, sig_c)
