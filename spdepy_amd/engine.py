@@ -154,9 +154,10 @@ class Engine:
         b = np.zeros((n, r))
         np.add.at(b, obs_cols, y2)
         b *= tau
-        # The log|Q| side (full factorisation of Q, or the spatial matrices)
-        # is independent of Q_c: it runs on a side stream while Q_c, mu_c and
-        # the data terms run on the current one.
+        # Two streams: the selected inverse of Q_c (the longest piece) runs on
+        # a side stream while mu_c, the data terms and the log|Q| pieces run
+        # on the current one.  A cross-stream wait is enqueued right after the
+        # event it waits on is recorded.
         main = torch.cuda.current_stream()
         side = self._side_stream(torch)
         sel = grad and exact
@@ -168,36 +169,46 @@ class Engine:
                 s_gram = torch.empty(sp.nnz, dtype=torch.float64, device="cuda")
             else:
                 sq = torch.empty(self.plan.nnz, dtype=torch.float64, device="cuda")
-        # The critical path (factorisation of Q_c, then its selected inverse)
-        # is enqueued before the log|Q| side work, so that the host's time
-        # spent launching the latter does not delay it.
-        ready = torch.cuda.Event()
-        ready.record(main)
-        self.plan.factor_async(1, qval, diag_add)
-        factored = torch.cuda.Event()
-        factored.record(main)
-        side.wait_event(ready)
-        with torch.cuda.stream(side):
-            if closed:
-                q0 = sp.assemble_spatial(self.plan, 0)
-                gram = sp.assemble_spatial(self.plan, 1)
-                sp.factor_async(0, q0, owner=self)
-                sp.factor_async(1, gram, owner=self)
-                if sel:
-                    sp.selinv(0, s_q0)
-                    sp.selinv(1, s_gram)
-            else:
+        b_dev = torch.as_tensor(b).cuda()
+        if closed:
+            # Critical path: factorisation of Q_c on this stream, then its
+            # selected inverse on the side stream (waiting on an event
+            # recorded just before).  The solve for mu_c and the one-slice
+            # log|Q| work follow here, concurrent with the selected inverse.
+            self.plan.factor_async(1, qval, diag_add)
+            if sel:
+                factored = torch.cuda.Event()
+                factored.record(main)
+                side.wait_event(factored)
+                with torch.cuda.stream(side):
+                    self.plan.selinv(1, sqc)
+            mu = self.plan.solve(1, b_dev)
+            qmu = self.plan.spmv(qval, mu)
+            q0 = sp.assemble_spatial(self.plan, 0)
+            gram = sp.assemble_spatial(self.plan, 1)
+            sp.factor_async(0, q0, owner=self)
+            sp.factor_async(1, gram, owner=self)
+            if sel:
+                sp.selinv(0, s_q0)
+                sp.selinv(1, s_gram)
+        else:
+            # the reference's route: Q factorised (and inverted) on the side stream
+            ready = torch.cuda.Event()
+            ready.record(main)
+            side.wait_event(ready)
+            self.plan.factor_async(1, qval, diag_add)
+            factored = torch.cuda.Event()
+            factored.record(main)
+            with torch.cuda.stream(side):
                 self.plan.factor_async(0, qval)
                 if sel:
                     self.plan.selinv(0, sq)
-        if sel:
-            # the selected inverse of Q_c (large GEMMs) runs on the side stream
-            # while the solve for mu_c (latency-bound) runs here
-            side.wait_event(factored)
-            with torch.cuda.stream(side):
-                self.plan.selinv(1, sqc)
-        mu = self.plan.solve(1, torch.as_tensor(b).cuda())
-        qmu = self.plan.spmv(qval, mu)
+            if sel:
+                side.wait_event(factored)
+                with torch.cuda.stream(side):
+                    self.plan.selinv(1, sqc)
+            mu = self.plan.solve(1, b_dev)
+            qmu = self.plan.spmv(qval, mu)
         cols_t = torch.as_tensor(obs_cols).cuda()
         resid = torch.as_tensor(y2).cuda() - mu[cols_t]
         done = torch.cuda.Event()
@@ -328,10 +339,10 @@ class KronEngine:
             sig_c = torch.empty(self.nnz, dtype=torch.float64, device="cuda")
         ready = torch.cuda.Event()
         ready.record(main)
+        side.wait_event(ready)
         self.plan.factor_async(1, qval, diag_add, owner=self)
         factored = torch.cuda.Event()
         factored.record(main)
-        side.wait_event(ready)
         with torch.cuda.stream(side):
             sp.factor_async(0, qs, owner=self)
             if grad:
