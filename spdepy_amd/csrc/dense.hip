@@ -208,9 +208,80 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   gemm_body<TA, TB, 128, 128>(probs, tiles, bases);
 }
 
+// y = alpha op(A) x + beta y for N <= kGemvMaxN columns (solves with few
+// right-hand sides).  Bandwidth-bound: A is read once, coalesced.
+// A not transposed (M x K): block = 64 rows (lane = row), the four waves take
+// quarters of K, partial sums combined in LDS in wave order.
+// A transposed (K x M stored): one wave per output row m (a column of A),
+// lanes stride over k, butterfly sum across the wave.
+template <bool TA>
+__global__ __launch_bounds__(256) void gemv_kernel(const GemmProb* __restrict__ probs,
+                                                   const GemmTile* __restrict__ tiles, Bases bases) {
+  const GemmTile t = tiles[blockIdx.x];
+  const GemmProb P = probs[t.prob];
+  const double* __restrict__ A = bases.p[P.a_base] + P.a_off;
+  const double* __restrict__ X = bases.p[P.b_base] + P.b_off;
+  double* __restrict__ Y = bases.p[P.c_base] + P.c_off;
+  const int N = P.N, K = P.K;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double acc[kGemvMaxN];
+#pragma unroll
+  for (int c = 0; c < kGemvMaxN; ++c) acc[c] = 0.0;
+  if constexpr (!TA) {
+    __shared__ double red[4][kGemvMaxN][64];
+    const int m = t.tm * 64 + lane;
+    const int q = (K + 3) / 4, k0 = wave * q, k1 = min(K, k0 + q);
+    if (m < P.M)
+#pragma unroll 8
+      for (int k = k0; k < k1; ++k) {
+        const double a = A[m + (int64_t)k * P.lda];
+#pragma unroll
+        for (int c = 0; c < kGemvMaxN; ++c)
+          if (c < N) acc[c] += a * X[k + (int64_t)c * P.ldb];
+      }
+#pragma unroll
+    for (int c = 0; c < kGemvMaxN; ++c) red[wave][c][lane] = acc[c];
+    __syncthreads();
+    if (wave == 0 && m < P.M)
+      for (int c = 0; c < N; ++c) {
+        const double v = ((red[0][c][lane] + red[1][c][lane]) + red[2][c][lane]) + red[3][c][lane];
+        double* y = Y + m + (int64_t)c * P.ldc;
+        const double r = P.alpha * v;
+        *y = (P.beta == 0.0) ? r : r + P.beta * (*y);
+      }
+  } else {
+    const int m = t.tm * 4 + wave;
+    if (m >= P.M) return;
+    const double* a = A + (int64_t)m * P.lda;
+#pragma unroll 8
+    for (int k = lane; k < K; k += 64) {
+      const double av = a[k];
+#pragma unroll
+      for (int c = 0; c < kGemvMaxN; ++c)
+        if (c < N) acc[c] += av * X[k + (int64_t)c * P.ldb];
+    }
+#pragma unroll
+    for (int c = 0; c < kGemvMaxN; ++c)
+      for (int o = 32; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, 64);
+    if (lane == 0)
+      for (int c = 0; c < N; ++c) {
+        double* y = Y + m + (int64_t)c * P.ldc;
+        const double r = P.alpha * acc[c];
+        *y = (P.beta == 0.0) ? r : r + P.beta * (*y);
+      }
+  }
+}
+
 void launch_gemm(const GemmLaunch& g, const Bases& b, hipStream_t s) {
   if (g.ntiles == 0) return;
   dim3 grid(g.ntiles), block(256);
+  if (g.gemv) {
+    if (g.ta)
+      hipLaunchKernelGGL(gemv_kernel<true>, grid, block, 0, s, g.d_probs, g.d_tiles, b);
+    else
+      hipLaunchKernelGGL(gemv_kernel<false>, grid, block, 0, s, g.d_probs, g.d_tiles, b);
+    return;
+  }
   if (g.split > 1) {
     double* ws = b.ws + g.ws_off;
     if (!g.ta && !g.tb)

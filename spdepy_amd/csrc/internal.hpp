@@ -60,7 +60,11 @@ struct GemmLaunch {
   // the slot's split workspace), summed in chunk order by a second kernel
   int split = 1;
   int64_t ws_off = 0;
+  // few right-hand sides (N <= kGemvMaxN, op(B) = B, general C): a
+  // bandwidth-bound kernel reading A once instead of 64-wide MFMA tiles
+  bool gemv = false;
 };
+constexpr int kGemvMaxN = 8;
 
 struct PotrfDesc {      // one diagonal block of one front
   int64_t foff;         // front offset

@@ -134,7 +134,7 @@ void make_tiles(GemmLaunch& g, int force_tile);
 // least 256 deep; the partial tiles take S * tiles * 64 * 64 doubles of the
 // workspace counted in *ws.
 void plan_split(GemmLaunch& g, int64_t* ws) {
-  if (!ws || g.tile != 64 || g.h_tiles.empty()) return;
+  if (!ws || g.gemv || g.tile != 64 || g.h_tiles.empty()) return;
   int kmax = 0;
   for (const GemmProb& P : g.h_probs) kmax = std::max(kmax, P.K);
   const int64_t nt = (int64_t)g.h_tiles.size();
@@ -180,6 +180,19 @@ void add_prob(GemmLaunch& g, const GemmProb& P, bool lower, double flop_scale = 
 // Then the tile list (tiles strictly above the diagonal of a lower problem
 // are skipped).
 void make_tiles(GemmLaunch& g, int force_tile) {
+  // few right-hand sides: row blocks of 64 (A not transposed) or groups of 4
+  // outputs (A transposed) for the bandwidth-bound kernel
+  bool narrow = !g.h_probs.empty() && !g.tb && !force_tile;
+  for (const GemmProb& P : g.h_probs) narrow = narrow && P.N <= kGemvMaxN && !P.lower;
+  g.gemv = narrow;
+  if (narrow) {
+    g.tile = 0;
+    g.h_tiles.clear();
+    const int rows = g.ta ? 4 : 64;
+    for (size_t i = 0; i < g.h_probs.size(); ++i)
+      for (int a = 0; a < (g.h_probs[i].M + rows - 1) / rows; ++a) g.h_tiles.push_back({(int32_t)i, a, 0, 0});
+    return;
+  }
   int64_t t128 = 0;
   double ksum = 0.0;
   bool any_lower = false;
