@@ -59,13 +59,18 @@ constexpr int kPart = kWin + 3;  // window | dotGQ, dot00, tautr
 
 // Pass 1a: per cell and time slice, the slice's share of the windows (thread
 // per (cell, slice)); pass 1b sums the slices in order.
-__global__ void grad_reduce_kernel(GradKArgs a, double* __restrict__ part) {
+// The windows live in LDS (indexed by the entry's code; a register array
+// indexed at run time would go to scratch memory): thread x of the block owns
+// wl[q * 128 + x].  part is [slice][q][cell] so that pass 1b reads it
+// coalesced.
+__global__ __launch_bounds__(128) void grad_reduce_kernel(GradKArgs a, double* __restrict__ part) {
+  __shared__ double wl[kWin * 128];
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double* w = wl + threadIdx.x;  // w[q * 128]
+  for (int q = 0; q < kWin; ++q) w[q * 128] = 0.0;
   if (idx >= a.Ns * a.T) return;
   const int64_t i = idx % a.Ns;
   const int t = (int)(idx / a.Ns);
-  double w[kWin];
-  for (int q = 0; q < kWin; ++q) w[q] = 0.0;
   double dotGQ = 0.0, dot00 = 0.0, tautr = 0.0;
   const double inh = a.nh1 > 0 ? 1.0 / a.nh1 : 0.0;
   {
@@ -98,25 +103,26 @@ __global__ void grad_reduce_kernel(GradKArgs a, double* __restrict__ part) {
       const double qv = a.qval[e];
       dotGQ += Ge * qv;
       if (cd < 25) {
-        if (t >= 1) w[cd] += Ge;
+        if (t >= 1) w[cd * 128] += Ge;
         if (t == 0) {
-          w[34 + cd] += Ge;
+          w[(34 + cd) * 128] += Ge;
           dot00 += Ge * ((cd == 12) ? qv - a.c * a.qdiag[i] : qv);
         }
         if (cd == 12) {
-          if (t <= a.T - 2) w[59] += Ge;
+          if (t <= a.T - 2) w[59 * 128] += Ge;
           if (a.diag_add) tautr += a.diag_add[r] * tr_c;
         }
       } else if (cd >= 34 && t <= a.T - 2) {
-        w[25 + cd - 34] += Ge;
+        w[(25 + cd - 34) * 128] += Ge;
       }
     }
   }
-  double* p = part + idx * kPart;
-  for (int q = 0; q < kWin; ++q) p[q] = w[q];
-  p[kWin] = dotGQ;
-  p[kWin + 1] = dot00;
-  p[kWin + 2] = tautr;
+  // part[(t * kPart + q) * Ns + i]
+  double* p = part + (int64_t)t * kPart * a.Ns + i;
+  for (int q = 0; q < kWin; ++q) p[(int64_t)q * a.Ns] = w[q * 128];
+  p[(int64_t)kWin * a.Ns] = dotGQ;
+  p[(int64_t)(kWin + 1) * a.Ns] = dot00;
+  p[(int64_t)(kWin + 2) * a.Ns] = tautr;
 }
 
 __global__ void grad_sum_kernel(GradKArgs a, const double* __restrict__ part, double* __restrict__ win,
@@ -125,7 +131,7 @@ __global__ void grad_sum_kernel(GradKArgs a, const double* __restrict__ part, do
   if (i >= a.Ns) return;
   for (int q = 0; q < kPart; ++q) {
     double acc = 0.0;
-    for (int t = 0; t < a.T; ++t) acc += part[(i + t * a.Ns) * kPart + q];
+    for (int t = 0; t < a.T; ++t) acc += part[((int64_t)t * kPart + q) * a.Ns + i];
     if (q < kWin)
       win[i * kWin + q] = acc;
     else
