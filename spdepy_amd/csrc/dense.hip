@@ -51,13 +51,20 @@ __device__ __forceinline__ void gemm_body(const GemmProb* __restrict__ probs, co
   const double* __restrict__ B = bases.p[P.b_base] + P.b_off;
   double* C = bases.p[P.c_base] + P.c_off;
   const int M = P.M, N = P.N;
-  int kb = 0, K = P.K;  // this block's K range [kb, K)
-  if constexpr (SPLIT) {
-    const int kc = ((P.K + split - 1) / split + 15) / 16 * 16;
-    kb = min(P.K, t.pad * kc);
-    K = min(P.K, kb + kc);
-  }
   const int m0 = t.tm * BM, n0 = t.tn * BN;
+  int kb = 0, K = P.K;  // this block's K range [kb, K): the zeros of a triangular operand are skipped
+  if (P.kmode == 1)
+    kb = min(K, n0);
+  else if (P.kmode == 2)
+    kb = min(K, m0);
+  else if (P.kmode == 3)
+    K = min(K, m0 + BM);
+  if constexpr (SPLIT) {
+    const int kc = ((K - kb + split - 1) / split + 15) / 16 * 16;
+    const int k0 = min(K, kb + t.pad * kc);
+    K = min(K, k0 + kc);
+    kb = k0;
+  }
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
   const int lr = lane & 15, lk = lane >> 4;

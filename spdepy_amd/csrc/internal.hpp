@@ -38,6 +38,9 @@ struct GemmProb {
   int32_t lda, ldb, ldc;
   int32_t M, N, K;
   int8_t a_base, b_base, c_base, lower;
+  // triangular operand, zeros skipped: 1 op(B) lower (k >= first column of
+  // the tile), 2 op(A) upper (k >= first row), 3 op(A) lower (k < last row + 1)
+  int8_t kmode;
   double alpha, beta;
 };
 struct GemmTile {

@@ -214,6 +214,12 @@ void make_tiles(GemmLaunch& g, int force_tile) {
   }
 }
 
+// A problem with a triangular operand whose zeros the kernel skips (kmode).
+GemmProb tri(GemmProb P, int kmode) {
+  P.kmode = (int8_t)kmode;
+  return P;
+}
+
 GemmProb prob(int8_t ab, int64_t ao, int lda, int8_t bb, int64_t bo, int ldb, int8_t cb, int64_t co, int ldc, int M,
               int N, int K, double alpha, double beta) {
   GemmProb P;
@@ -230,6 +236,7 @@ GemmProb prob(int8_t ab, int64_t ao, int lda, int8_t bb, int64_t bo, int ldb, in
   P.b_base = bb;
   P.c_base = cb;
   P.lower = 0;
+  P.kmode = 0;
   P.alpha = alpha;
   P.beta = beta;
   return P;
@@ -373,12 +380,15 @@ int build_factor_plan(spde_plan* pl) {
       const Node& nd = sym.nodes[o.node];
       const int n = nd.n(), p = nd.p;
       const int r = o.b - o.m, c = o.m - o.a;
-      add_prob(gx, prob(kFront, nd.foff + o.m + (int64_t)o.a * n, n, kZ, nd.zoff + o.a + (int64_t)o.a * p, p, kWork,
-                        woff, r, r, c, c, 1.0, 0.0),
-               false);
-      add_prob(gz, prob(kZ, nd.zoff + o.m + (int64_t)o.m * p, p, kWork, woff, r, kZ, nd.zoff + o.m + (int64_t)o.a * p,
-                        p, r, c, r, -1.0, 0.0),
-               false);
+      // X = L_ma Z_aa (Z_aa lower), Z_ma = -Z_mm X (Z_mm lower)
+      add_prob(gx, tri(prob(kFront, nd.foff + o.m + (int64_t)o.a * n, n, kZ, nd.zoff + o.a + (int64_t)o.a * p, p,
+                            kWork, woff, r, r, c, c, 1.0, 0.0),
+                       1),
+               false, 0.5);
+      add_prob(gz, tri(prob(kZ, nd.zoff + o.m + (int64_t)o.m * p, p, kWork, woff, r, kZ,
+                            nd.zoff + o.m + (int64_t)o.a * p, p, r, c, r, -1.0, 0.0),
+                       3),
+               false, 0.5);
       woff += (int64_t)r * c;
     }
     work = std::max(work, woff);
@@ -484,11 +494,13 @@ int build_selinv_plan(spde_plan* pl) {
       if (b > 0 && nd.parent >= 0)
         for (int ti = 0; ti < (b + 63) / 64; ++ti)
           for (int tj = 0; tj < (b + 63) / 64; ++tj) gt.push_back(make_int4((int)q, ti, tj, 0));
-      add_prob(L.si_x, prob(kFront, nd.foff + p, nf, kZ, nd.zoff, p, kWork, woff, b, b, p, p, 1.0, 0.0), false);
+      add_prob(L.si_x, tri(prob(kFront, nd.foff + p, nf, kZ, nd.zoff, p, kWork, woff, b, b, p, p, 1.0, 0.0), 1), false,
+               0.5);
       add_prob(L.si_s21, prob(kSigma, nd.foff + p + (int64_t)p * nf, nf, kWork, woff, b, kSigma, nd.foff + p, nf, b, p,
                               b, -1.0, 0.0),
                false);
-      add_prob(L.si_s11a, prob(kZ, nd.zoff, p, kZ, nd.zoff, p, kSigma, nd.foff, nf, p, p, p, 1.0, 0.0), true);
+      add_prob(L.si_s11a, tri(prob(kZ, nd.zoff, p, kZ, nd.zoff, p, kSigma, nd.foff, nf, p, p, p, 1.0, 0.0), 2), true,
+               1.0 / 3.0);
       add_prob(L.si_s11b, prob(kWork, woff, b, kSigma, nd.foff + p, nf, kSigma, nd.foff, nf, p, p, b, -1.0, 1.0), true);
       woff += (int64_t)b * p;
     }
