@@ -196,12 +196,18 @@ void make_tiles(GemmLaunch& g, int force_tile) {
   int64_t t128 = 0;
   double ksum = 0.0;
   bool any_lower = false;
+  int64_t t128_lower = 0;  // tiles of 128 actually launched (on or below the diagonal of lower problems)
   for (const GemmProb& P : g.h_probs) {
-    t128 += (int64_t)((P.M + 127) / 128) * ((P.N + 127) / 128);
+    const int64_t tm = (P.M + 127) / 128, tn = (P.N + 127) / 128;
+    t128 += tm * tn;
+    t128_lower += P.lower ? (tm <= tn ? tm * (tm + 1) / 2 : tm * tn - tn * (tn - 1) / 2) : tm * tn;
     ksum += P.K;
     any_lower = any_lower || P.lower;
   }
-  const bool big = !any_lower && !g.h_probs.empty() && ksum / g.h_probs.size() >= 256.0 && t128 >= 512;
+  // lower problems: 128 only when the launch still has two full rounds of
+  // 128-tiles (the diagonal tiles do redundant work above the diagonal)
+  const bool deep = !g.h_probs.empty() && ksum / g.h_probs.size() >= 256.0;
+  const bool big = deep && (any_lower ? t128_lower >= 1024 : t128 >= 512);
   g.tile = force_tile ? force_tile : (big ? 128 : 64);
   g.h_tiles.clear();
   const int T = g.tile;
