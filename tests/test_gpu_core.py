@@ -442,3 +442,50 @@ def test_cov_family_q_and_loglike(kind, dvar, bc, lam):
     scale = np.abs(jac_o).max()
     assert jac.shape == jac_o.shape
     assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * scale), np.abs(jac - jac_o).max() / scale
+
+
+@pytest.mark.parametrize("family", ["separable", "field", "cov"])
+def test_model_sample_and_update_other_families(family):
+    """Model.setModel/sample/update (model.py:73-127) on the separable, field
+    and cov-advection families: L^T P x = z for the samples and
+    mu = tau Q_c^-1 S^T y against dense algebra on the oracle's Q."""
+    import spdepy_amd as sp
+    from spdepy_amd import spdes
+    from oracle import grid as OG
+    rng = np.random.default_rng(21)
+    if family == "separable":
+        g = OG.STGrid(F.axis(8), F.axis(7), np.arange(5.0), Nbs=3)
+        om = Mo.Separable(g, "ani", bc=3)
+        par = np.hstack([0.3 * rng.normal(size=om.ns), [np.log(0.6)], [np.log(50.0)]])
+        pg = sp.grid(x=g.x, y=g.y, t=g.t, Nbs=3)
+        m = spdes.SeperableSpatialTemporal2D(grid=pg, par=par.copy(), bc=3)
+        Qd = om.makeQ(par, grad=False)[0].toarray()
+    else:
+        c = F.field_model(8, 7, 4, kind="ani", dvar=True, gvar=family == "field", bc=3, m=30,
+                          cov=0.7 if family == "cov" else None)
+        g, par = c["grid"], c["par"]
+        pg = sp.grid(x=g.x, y=g.y, t=g.t, Nbs=3)
+        mod0 = spdes.WhittleMatern2D(grid=sp.grid(x=g.x, y=g.y, Nbs=3), par=np.array(c["mod0"].par), bc=3)
+        spde = "cov-advection-var-diffusion" if family == "cov" else "var-advection-var-diffusion"
+        m = spdes.spde_init(spde, pg, ani=True, ha=False, bc=3, mod0=mod0)
+        if family == "cov":
+            m.ww = c["ww"]
+        m.setQ(par=par.copy())
+        Qd = c["model"].makeQ(par, grad=False)[0].toarray()
+    M = sp.Model.__new__(sp.Model)
+    M.grid, M.mod, M.useCov = pg, m, False
+    M.mu = np.zeros(g.n)
+    M.setModel()
+    step("Model.sample " + family)
+    x = M.sample(n=2, simple=True, seed=3)
+    fac = M.Q_fac
+    back = fac.solve_Lt(fac.solve_L(Qd @ x))
+    assert np.allclose(np.asarray(back), x, rtol=1e-8, atol=1e-10)
+    step("Model.update " + family)
+    idx = np.sort(rng.choice(g.n, 40, replace=False))
+    y = rng.normal(size=40)
+    tau = np.exp(par[-1])
+    M.update(y, idx, tau=tau)
+    S = g.getS(idx)
+    mu_ref = np.linalg.solve(Qd + tau * (S.T @ S).toarray(), tau * (S.T @ y))
+    assert np.allclose(M.mu, mu_ref, rtol=1e-9, atol=1e-12)
