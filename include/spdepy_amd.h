@@ -276,6 +276,10 @@ int spde_fields_contract(int64_t Ns, int nb, const double* bs, const double* bsH
  * launches on different streams overlap, so busy <= summed time) and the
  * algorithmic bytes (A, B and C read once, C written once). */
 int spde_prof_begin(spde_plan* plan);
+/* The same for the GEMM launches of the given kinds only (bit k of kind_mask
+ * for kind k, the code spde_prof_dump reports), so that timing one kernel
+ * adds no events around the others (no reference counterpart). */
+int spde_prof_begin_kinds(spde_plan* plan, int kind_mask);
 int spde_prof_end(spde_plan* plan, double* gemm_ms, double* gemm_flops, int64_t* launches,
                   double* busy_ms, double* gemm_bytes);
 /* Per-launch view of the timed launches so far (host arrays of length cap,

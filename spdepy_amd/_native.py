@@ -81,6 +81,7 @@ _SIGS = {
     "spde_solve": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int64, vp]),
     "spde_selinv": (ctypes.c_int, [vp, ctypes.c_int, vp, vp]),
     "spde_prof_begin": (ctypes.c_int, [vp]),
+    "spde_prof_begin_kinds": (ctypes.c_int, [vp, ctypes.c_int]),
     "spde_prof_end": (ctypes.c_int, [vp, c_dp, c_dp, c_i64p, c_dp, c_dp]),
     "spde_prof_dump": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, ctypes.c_int64, c_i64p]),
     "spde_debug_mfma_probe": (ctypes.c_int, [ctypes.c_int, vp, vp]),

@@ -175,6 +175,7 @@ struct spde_plan {
   bool selinv_built = false;
   // optional event timing of the GEMM launches (measurement only)
   bool prof_on = false;
+  int prof_mask = 0xff;           // kinds of GEMM launch timed (bit = kind code)
   std::vector<hipEvent_t> prof_ev;
   hipEvent_t prof_base = nullptr;
   std::vector<double> prof_flops;

@@ -541,7 +541,8 @@ void free_launch(GemmLaunch& g) {
 // GEMM launch, bracketed by HIP events when the plan's timing is enabled.
 void run_gemm(spde_plan* pl, const GemmLaunch& g, const Bases& b, hipStream_t s) {
   if (g.ntiles == 0) return;
-  if (pl->prof_on) {
+  const int kind = (g.tile == 128 ? 4 : 0) + (g.ta ? 2 : 0) + (g.tb ? 1 : 0);
+  if (pl->prof_on && ((pl->prof_mask >> kind) & 1)) {
     while (pl->prof_ev.size() < 2 * (pl->prof_n + 1)) {
       hipEvent_t e;
       hipEventCreate(&e);
@@ -559,7 +560,7 @@ void run_gemm(spde_plan* pl, const GemmLaunch& g, const Bases& b, hipStream_t s)
     pl->prof_flops[pl->prof_n] = g.flops;
     pl->prof_bytes[pl->prof_n] = g.bytes;
     pl->prof_tiles[pl->prof_n] = g.ntiles;
-    pl->prof_kind[pl->prof_n] = (g.tile == 128 ? 4 : 0) + (g.ta ? 2 : 0) + (g.tb ? 1 : 0);
+    pl->prof_kind[pl->prof_n] = kind;
     pl->prof_n++;
     pl->prof_stream = s;
     return;
@@ -1000,11 +1001,14 @@ int spde_solve(spde_plan* pl, int slot, int mode, double* B, int nrhs, int64_t l
   return SPDE_OK;
 }
 
-int spde_prof_begin(spde_plan* pl) {
-  if (!pl || pl->host_only) {
-    set_error("spde_prof_begin: no device plan");
+int spde_prof_begin(spde_plan* pl) { return spde_prof_begin_kinds(pl, 0xff); }
+
+int spde_prof_begin_kinds(spde_plan* pl, int kind_mask) {
+  if (!pl || pl->host_only || kind_mask <= 0 || kind_mask > 0xff) {
+    set_error("spde_prof_begin: no device plan or empty kind mask");
     return SPDE_ERR_ARG;
   }
+  pl->prof_mask = kind_mask;
   if (!pl->prof_base) SPDE_HIP(hipEventCreate(&pl->prof_base));
   SPDE_HIP(hipDeviceSynchronize());
   SPDE_HIP(hipEventRecord(pl->prof_base, nullptr));
