@@ -218,6 +218,18 @@ void make_tiles(GemmLaunch& g, int force_tile) {
       for (int b = 0; b < tn; ++b)
         if (!P.lower || a >= b) g.h_tiles.push_back({(int32_t)i, a, b, 0});
   }
+  // deepest tiles first: the blocks dispatched last are the shortest, which
+  // shortens the launch's final, partly filled round
+  auto depth = [&](const GemmTile& t) {
+    const GemmProb& P = g.h_probs[t.prob];
+    int kb = 0, ke = P.K;
+    if (P.kmode == 1) kb = std::min(P.K, t.tn * T);
+    if (P.kmode == 2) kb = std::min(P.K, t.tm * T);
+    if (P.kmode == 3) ke = std::min(P.K, (t.tm + 1) * T);
+    return ke - kb;
+  };
+  std::stable_sort(g.h_tiles.begin(), g.h_tiles.end(),
+                   [&](const GemmTile& x, const GemmTile& y) { return depth(x) > depth(y); });
 }
 
 // A problem with a triangular operand whose zeros the kernel skips (kmode).
