@@ -58,3 +58,18 @@ def test_polyak_average_and_fixed_indices():
     assert opt.opt_steps == 6
     assert np.allclose(np.array(opt.histX)[:, 1], 2.0)              # index 1 stays fixed
     assert np.allclose(res["x"], np.array(opt.histX)[-3:].mean(0))  # Polyak over the last 3
+
+
+def test_fit_writes_the_result_vector(tmp_path):
+    """Optimize.fit(end=...) saves the Polyak-averaged vector as <end>.npy, the
+    format of the reference's fits/*.npy files (optim/__init__.py:55-62)."""
+    from spdepy_amd.optim import Optimize
+    target = np.array([1.0, -2.0, 0.5])
+
+    def fun(x):
+        return float(((x - target) ** 2).sum()), 2 * (x - target)
+    opt = Optimize(fun)
+    stem = str(tmp_path / "fit_bc1")
+    res = opt.fit(x0=np.zeros(3), lr=0.1, max_steps=50, pol=5, end=stem)
+    saved = np.load(stem + ".npy", allow_pickle=False)
+    assert saved.dtype == np.float64 and np.array_equal(saved, res["x"])
