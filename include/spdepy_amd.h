@@ -222,6 +222,31 @@ typedef struct spde_grad_args {
 
 int spde_grad_cells(spde_plan* plan, const spde_grad_args* args, double* out, void* stream);
 
+/* ---------------------------------------------------------------- fused evaluation
+ * One exact-trace log-likelihood + gradient evaluation in one call (the body of
+ * logLike, advection_idiffusion2D.py:167-207, var_advection_var_diffusion2D.py:
+ * 201-237), with log|Q| in closed form from the one-slice plan:
+ *   like_raw = 1/2 r log|Q| + 1/2 m r log tau - 1/2 r log|Q_c| - 1/2 mu^T Q mu
+ *              - tau/2 ||y - S mu||^2,   Q_c = Q + tau S^T S,  mu = Q_c^-1 tau S^T y.
+ * plan: space-time plan with >= 2 slots (slot 1 is used); spatial: its one-slice
+ * plan (spde_plan_create with T = 1, same M, N, bc) with >= 2 slots; coeffs as
+ * for spde_assemble.  obs_cols[m] (host): lattice index of each observation
+ * (repeats allowed); y[m * r] (host, column-major m x r).  cells[Ns * 40]
+ * (device) receives the per-cell adjoints of spde_grad_cells (closed form), from
+ * which the model's gradient is contracted with its bases; out (host) the
+ * scalars.  Blocks until the evaluation is done; uses a side stream of the plan
+ * for the selected inverse of Q_c. */
+typedef struct spde_loglik_out {
+  double like;      /* -like_raw / (m r), the value logLike returns */
+  double like_raw;
+  double logdet_q, logdet_qc;
+  double g_sigma;   /* d like_raw / d log sigma */
+  double g_tau;     /* d like_raw / d log tau */
+} spde_loglik_out;
+int spde_loglik_grad(spde_plan* plan, spde_plan* spatial, const spde_coeffs* coeffs, const int64_t* obs_cols,
+                     const double* y, int64_t m, int r, double log_tau, double* cells, spde_loglik_out* out,
+                     void* stream);
+
 /* ---------------------------------------------------------------- separable models
  * Q = Qt (x) Qs (seperable_spatial_temporal2D.py:80-81, _ha2D.py, _idiffusion2D.py:
  * sparse.kron(Qt, Qs)) on a plan created with SPDE_PLAN_WIDE_TIME; `spatial` is

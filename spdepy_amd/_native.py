@@ -47,6 +47,11 @@ class Coeffs(ctypes.Structure):
                 ("mix", ctypes.c_int)]
 
 
+class LoglikOut(ctypes.Structure):
+    _fields_ = [("like", ctypes.c_double), ("like_raw", ctypes.c_double), ("logdet_q", ctypes.c_double),
+                ("logdet_qc", ctypes.c_double), ("g_sigma", ctypes.c_double), ("g_tau", ctypes.c_double)]
+
+
 class GradArgs(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("r", ctypes.c_int), ("nh1", ctypes.c_int), ("qval", vp),
                 ("sigma_q", vp), ("sigma_qc", vp), ("trq", vp), ("trqc", vp), ("probes", vp),
@@ -68,6 +73,8 @@ _SIGS = {
     "spde_assemble_spatial": (ctypes.c_int, [vp, vp, ctypes.c_int, vp, vp]),
     "spde_logq_sum": (ctypes.c_int, [vp, ctypes.c_double, c_dp, vp]),
     "spde_assemble_kron": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+    "spde_loglik_grad": (ctypes.c_int, [vp, vp, ctypes.POINTER(Coeffs), vp, vp, ctypes.c_int64, ctypes.c_int,
+                                        ctypes.c_double, vp, ctypes.POINTER(LoglikOut), vp]),
     "spde_kron_reduce": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_double, vp, vp, ctypes.c_int, vp, vp, vp,
                                         vp]),
     "spde_fields_eval": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int] + [vp] * 8 + [vp]),

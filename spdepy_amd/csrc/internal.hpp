@@ -164,6 +164,7 @@ struct spde_plan {
   int64_t split_fs = 0;           // split-K workspace (doubles) of the factor and selected-inverse launches
   int64_t split_solve = 0;        // ... of the solve launches (current nrhs)
   double* d_gpart = nullptr;      // per (cell, slice) partial windows of the gradient reduction
+  void* llws = nullptr;           // buffers and side stream of spde_loglik_grad (loglik.hip)
   int32_t* d_sprow = nullptr;     // one-slice plans: row and column of each entry (Kronecker kernels)
   int32_t* d_spcol = nullptr;
   int64_t work_doubles = 0;       // per-slot workspace size

@@ -33,6 +33,7 @@ void launch_sigma_gather(const int32_t* lvl_nodes, const NodeDev* nodes, const i
                          const int4* tiles, int ntiles, hipStream_t s);
 void launch_gather_pattern(const double* S, const int64_t* mapsym, int64_t nnz, double* out, hipStream_t s);
 void launch_zero_lower(const NodeDev* nodes, const int4* tiles, int ntiles, double* F, hipStream_t s);
+void loglik_release(spde_plan* pl);  // loglik.hip
 
 static thread_local std::string g_err;
 void set_error(const std::string& m) { g_err = m; }
@@ -759,6 +760,7 @@ int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int 
 
 void spde_plan_destroy(spde_plan* pl) {
   if (!pl) return;
+  if (!pl->host_only) spde::loglik_release(pl);
   if (pl->host_only) {
     for (size_t i = 0; i < g_extra.size(); ++i)
       if (g_extra[i].first == pl) {

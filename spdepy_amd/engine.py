@@ -3,6 +3,8 @@ Cholesky factorisations, log-determinants, mu_c, the log-likelihood and the
 per-cell gradient adjoints (``logLike``, advection_idiffusion2D.py:167-207 and
 var_advection_var_diffusion2D.py:201-237).  Everything runs in the native
 library on the current HIP stream; torch supplies device memory only."""
+import ctypes
+
 import numpy as np
 
 from . import _native as nat
@@ -276,6 +278,26 @@ class Engine:
         parts["g_tau"] = m * r / 2 - 0.5 * r * colsum[OUT_TAUTR] - tau / 2 * rss
         del keep
         return -like / (m * r), parts
+
+
+def loglik_fused(engine, coeffs, obs_cols, data, log_tau):
+    """The exact-trace evaluation through the single C entry point
+    spde_loglik_grad (closed-form log|Q|); returns (like, parts) like
+    Engine.loglik(..., trace="exact", logdet="closed")."""
+    torch = _torch()
+    y = np.ascontiguousarray(np.asarray(data, dtype=np.float64).reshape(len(obs_cols), -1).T)  # r x m = m x r col-major
+    m = len(obs_cols)
+    r = y.shape[0]
+    cols = np.ascontiguousarray(obs_cols, dtype=np.int64)
+    cells = torch.empty(engine.Ns * OUT_WIDTH, dtype=torch.float64, device="cuda")
+    out = nat.LoglikOut()
+    nat.check(nat.lib().spde_loglik_grad(engine.plan._h, engine.spatial._h, ctypes.byref(coeffs.c),
+                                         cols.ctypes.data, y.ctypes.data, m, r, float(log_tau), _ptr(cells),
+                                         ctypes.byref(out), _stream(torch)))
+    h = cells.cpu().numpy().reshape(engine.Ns, OUT_WIDTH)
+    parts = dict(m=m, r=r, tau=float(np.exp(log_tau)), ld_q=out.logdet_q, ld_qc=out.logdet_qc, like_raw=out.like_raw,
+                 cells=h, colsum=h.sum(0), g_sigma=out.g_sigma, g_tau=out.g_tau)
+    return out.like, parts
 
 
 class KronEngine:

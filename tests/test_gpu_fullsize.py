@@ -113,3 +113,19 @@ def test_sampler_and_selected_inverse_identities(workload):
     # FP64 round-off of the factor and of the recursion grows with the
     # condition number of Q (a few 1e-9 here); a wrong term is an O(1) error
     assert err < 1e-8
+
+
+def test_fused_entry_point_at_full_size(workload):
+    """spde_loglik_grad at 100x100x20 gives the engine path's value and
+    per-cell adjoints."""
+    from spdepy_amd import engine as E
+    mod, par = workload
+    eng = mod._engine()
+    coeffs = mod._coeffs(par)
+    step("engine path at 100x100x20")
+    like_e, parts_e = eng.loglik(coeffs, mod._obs_cols(), mod.data, par[-1], grad=True)
+    step("fused entry point at 100x100x20")
+    like_f, parts_f = E.loglik_fused(eng, coeffs, mod._obs_cols(), mod.data, par[-1])
+    assert abs(like_f - like_e) <= 1e-12 * abs(like_e)
+    scale = np.abs(parts_e["cells"]).max()
+    assert np.allclose(parts_f["cells"], parts_e["cells"], rtol=1e-11, atol=1e-12 * scale)
