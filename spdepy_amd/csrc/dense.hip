@@ -26,6 +26,18 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // K range (each a multiple of BK deep) and writes the raw 64x64 partial tile
 // to ws + blockIdx.x * 4096 (column-major, ld 64); split_reduce_kernel then
 // applies alpha, beta and the lower-triangle rule.
+// List entry of block b: the dispatcher hands blocks to the eight XCDs in
+// turn (b and b + 8 share an XCD and its L2).  Inside every full group of 64
+// blocks, XCD x takes the eight consecutive entries 8x .. 8x + 7, which share
+// operand panels (a tile row of a problem is consecutive in the list); the
+// list order, deepest tiles first, holds at the granularity of a group.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int g = b >> 6;
+  if ((g + 1) * 64 > nb) return b;
+  const int r = b & 63;
+  return (g << 6) + ((r & 7) << 3) + (r >> 3);
+}
+
 template <bool TA, bool TB, int BM, int BN, bool SPLIT = false>
 __device__ __forceinline__ void gemm_body(const GemmProb* __restrict__ probs, const GemmTile* __restrict__ tiles,
                                           const Bases& bases, int split = 1, double* __restrict__ ws = nullptr) {
@@ -42,7 +54,7 @@ __device__ __forceinline__ void gemm_body(const GemmProb* __restrict__ probs, co
   constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
   __shared__ double As[2][SA];
   __shared__ double Bs[2][SB];
-  const GemmTile t = tiles[blockIdx.x];
+  const GemmTile t = tiles[SPLIT ? (int)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x)];
   const GemmProb P = probs[t.prob];
   // A and C may be the same memory (the in-place panel solve X = S W^T,
   // one tile column): every tile reads all of its rows of A before its

@@ -164,11 +164,13 @@ __global__ __launch_bounds__(256) void bwd_gather_kernel(const int32_t* __restri
 
 // --------------------------------------------------------- selected inverse
 // Sigma22 of a node = symmetric gather from the lower triangle of the
-// parent's Sigma front.  One block per 64x64 tile (node, ti, tj) of the b x b
-// block; entries below the parent's diagonal are read with consecutive
-// threads on consecutive rows, mirrored ones with consecutive threads on
-// consecutive columns, both through an LDS tile so every global access is
-// coalesced.
+// parent's Sigma front.  One block per 64x64 tile (node, ti, tj), ti >= tj,
+// of the b x b block: relpos increases, so such a tile lies wholly in the
+// parent's lower triangle (diagonal tiles: their lower half).  The block
+// reads it once, consecutive threads on consecutive rows, writes it in place
+// and writes its mirror (tj, ti) through an LDS transpose, again with
+// consecutive threads on consecutive rows.  All sixteen reads of a thread
+// are issued before the first store.
 __global__ __launch_bounds__(256) void sigma_gather_kernel(const int32_t* __restrict__ lvl_nodes, const NodeDev* __restrict__ nodes,
                                                            const int32_t* __restrict__ relpos, double* __restrict__ S,
                                                            const int4* __restrict__ tiles) {
@@ -181,6 +183,7 @@ __global__ __launch_bounds__(256) void sigma_gather_kernel(const int32_t* __rest
   const int32_t* rp = relpos + nd.relpos_off;
   const int b = nd.nb;
   const int i0 = tt.y * 64, j0 = tt.z * 64;
+  const bool diag = tt.y == tt.z;
   const double* src = S + pa.foff;
   const int64_t pn = pa.n;
   const int tid = threadIdx.x, u = tid & 63, w0 = tid >> 6;
@@ -188,34 +191,29 @@ __global__ __launch_bounds__(256) void sigma_gather_kernel(const int32_t* __rest
   if (tid < 64) rpi[tid] = (i0 + tid < b) ? rp[i0 + tid] : -1;
   else if (tid < 128) rpj[tid - 64] = (j0 + tid - 64 < b) ? rp[j0 + tid - 64] : -1;
   __syncthreads();
-  // relpos increases, so a tile below the diagonal (ti > tj) lies wholly in
-  // the parent's lower triangle, one above it wholly in the upper (mirrored)
-  // part; only diagonal tiles need both passes
-  if (tt.y >= tt.z) {  // rows fastest: entries with rp[i] >= rp[j]
-    const int pi = rpi[u];
-#pragma unroll 4
-    for (int jj = w0; jj < 64; jj += 4) {
-      const int pj = rpj[jj];
-      if (pi >= 0 && pj >= 0 && pi >= pj) tile[u][jj] = src[pi + (int64_t)pj * pn];
-    }
+  const int pi = rpi[u];
+  double v[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int jj = w0 + 4 * t, pj = rpj[jj];
+    v[t] = (pi >= 0 && pj >= 0 && (!diag || u >= jj)) ? src[pi + (int64_t)pj * pn] : 0.0;
   }
-  if (tt.y <= tt.z) {  // columns fastest: entries with rp[i] < rp[j]
-    const int pj = rpj[u];
-#pragma unroll 4
-    for (int ii = w0; ii < 64; ii += 4) {
-      const int pi = rpi[ii];
-      if (pi >= 0 && pj >= 0 && pi < pj) tile[ii][u] = src[pj + (int64_t)pi * pn];
-    }
-  }
-  __syncthreads();
   double* dst = S + nd.foff + nd.p + (int64_t)nd.p * nd.n;
   const int i = i0 + u;
-  if (i < b)
-#pragma unroll 4
-    for (int jj = w0; jj < 64; jj += 4) {
-      const int j = j0 + jj;
-      if (j < b) dst[i + (int64_t)j * nd.n] = tile[u][jj];
-    }
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int jj = w0 + 4 * t, j = j0 + jj;
+    if (i < b && j < b && (!diag || u >= jj)) dst[i + (int64_t)j * nd.n] = v[t];
+    tile[u][jj] = v[t];
+  }
+  __syncthreads();
+  // mirror: entry (j0 + u, i0 + c) is the lower entry (i0 + c, j0 + u)
+  const int row = j0 + u;
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int c = w0 + 4 * t, col = i0 + c;
+    if (row < b && col < b && (!diag || u < c)) dst[row + (int64_t)col * nd.n] = tile[c][u];
+  }
 }
 
 // out[e] = S[mapsym[e]]

@@ -63,6 +63,11 @@ struct PlanExtra {            // kept beside the plan (not in the public struct)
   int32_t* d_children = nullptr;
   int4* d_ztiles = nullptr;    // lower 64x64 tiles of every front (node, ti, tj)
   int nztiles = 0;
+  // selected inverse: second stream for the products that need only the
+  // factor, and its events (created at the first selected inverse)
+  hipStream_t pre = nullptr;
+  hipEvent_t ev_start = nullptr;
+  std::vector<hipEvent_t> ev_pre;
 };
 
 std::vector<std::pair<spde_plan*, PlanExtra*>> g_extra;
@@ -499,20 +504,22 @@ int build_solve_plan(spde_plan* pl, int k) {
 int build_selinv_plan(spde_plan* pl) {
   const Symbolic& sym = pl->sym;
   PlanExtra* ex = extra(pl);
+  // X = L21 Z of every level keeps its own region of the workspace: the X
+  // products run ahead of the level chain on a second stream
+  int64_t woff = 0;
   for (size_t l = 0; l < pl->levels.size(); ++l) {
     LevelPlan& L = pl->levels[l];
     L.si_x.ta = false, L.si_x.tb = false;
     L.si_s21.ta = false, L.si_s21.tb = false;
     L.si_s11a.ta = true, L.si_s11a.tb = false;
     L.si_s11b.ta = true, L.si_s11b.tb = false;
-    int64_t woff = 0;
     std::vector<int4> gt;
     for (size_t q = 0; q < L.nodes.size(); ++q) {
       const Node& nd = sym.nodes[L.nodes[q]];
       const int nf = nd.n(), p = nd.p, b = (int)nd.B.size();
       if (b > 0 && nd.parent >= 0)
         for (int ti = 0; ti < (b + 63) / 64; ++ti)
-          for (int tj = 0; tj < (b + 63) / 64; ++tj) gt.push_back(make_int4((int)q, ti, tj, 0));
+          for (int tj = 0; tj <= ti; ++tj) gt.push_back(make_int4((int)q, ti, tj, 0));
       add_prob(L.si_x, tri(prob(kFront, nd.foff + p, nf, kZ, nd.zoff, p, kWork, woff, b, b, p, p, 1.0, 0.0), 1), false,
                0.5);
       add_prob(L.si_s21, prob(kSigma, nd.foff + p + (int64_t)p * nf, nf, kWork, woff, b, kSigma, nd.foff + p, nf, b, p,
@@ -525,8 +532,11 @@ int build_selinv_plan(spde_plan* pl) {
     }
     L.si_work = woff;
     if (woff > pl->work_doubles) {
-      set_error("selected-inverse workspace too small");
-      return SPDE_ERR_STATE;
+      if (!pl->slots.empty()) {  // the slots' workspaces are allocated after this plan is built
+        set_error("selected-inverse workspace too small");
+        return SPDE_ERR_STATE;
+      }
+      pl->work_doubles = woff;
     }
     const ArenaSizes as = arena_sizes(pl, 1);
     for (GemmLaunch* g : {&L.si_x, &L.si_s21, &L.si_s11a, &L.si_s11b}) {
@@ -815,6 +825,9 @@ void spde_plan_destroy(spde_plan* pl) {
         }
         hipFree(ex->d_children);
         hipFree(ex->d_ztiles);
+        for (auto e : ex->ev_pre) hipEventDestroy(e);
+        if (ex->ev_start) hipEventDestroy(ex->ev_start);
+        if (ex->pre) hipStreamDestroy(ex->pre);
         delete ex;
         g_extra.erase(g_extra.begin() + i);
         break;
@@ -1117,14 +1130,29 @@ int spde_selinv(spde_plan* pl, int slot, double* sigma, void* stream) {
   }
   rc = ensure_splitws(pl, slot);
   if (rc) return rc;
+  if (!ex->pre) {
+    SPDE_HIP(hipStreamCreateWithFlags(&ex->pre, hipStreamNonBlocking));  // default priority
+    SPDE_HIP(hipEventCreateWithFlags(&ex->ev_start, hipEventDisableTiming));
+    ex->ev_pre.resize(pl->levels.size());
+    for (auto& e : ex->ev_pre) SPDE_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
   Bases b = make_bases(pl, slot, nullptr);
+  // Top-down over the levels.  X = L21 Z and Sigma11 = Z^T Z need only the
+  // factor: they run on the second stream, one level ahead of the chain
+  // gather -> Sigma21 = -Sigma22 X -> Sigma11 -= X^T Sigma21 on this one,
+  // which waits for its level's event.  Every wait is enqueued right after
+  // the record it waits on.
+  SPDE_HIP(hipEventRecord(ex->ev_start, s));
+  SPDE_HIP(hipStreamWaitEvent(ex->pre, ex->ev_start, 0));
   for (int l = (int)pl->levels.size() - 1; l >= 0; --l) {
     LevelPlan& L = pl->levels[l];
     LevelDev& D = ex->lvl[l];
+    run_gemm(pl, L.si_x, b, ex->pre);
+    run_gemm(pl, L.si_s11a, b, ex->pre);
+    SPDE_HIP(hipEventRecord(ex->ev_pre[l], ex->pre));
     launch_sigma_gather(D.d_nodes, pl->d_nodes, pl->d_relpos, sl.sigma, D.d_gtiles, D.ngtiles, s);
-    run_gemm(pl, L.si_x, b, s);
+    SPDE_HIP(hipStreamWaitEvent(s, ex->ev_pre[l], 0));
     run_gemm(pl, L.si_s21, b, s);
-    run_gemm(pl, L.si_s11a, b, s);
     run_gemm(pl, L.si_s11b, b, s);
   }
   if (sigma) launch_gather_pattern(sl.sigma, pl->d_mapsym, (int64_t)pl->st.indices.size(), sigma, s);
