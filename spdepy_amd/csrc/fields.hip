@@ -113,6 +113,9 @@ __global__ __launch_bounds__(256) void contract_kernel(int64_t Ns, int nb, int j
     partial[(int64_t)blockIdx.x * (6 * 64 + kCells) + t] = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
 }
 
+// Blocks of the contraction: chunks of at least 64 cells, at most 240 blocks.
+inline int64_t contract_blocks(int64_t Ns) { return (Ns + 63) / 64 < 240 ? (Ns + 63) / 64 : 240; }
+
 // out[t] = sum over blocks of partial[b][t] in block order (deterministic).
 __global__ void contract_sum_kernel(int nblk, const double* __restrict__ partial, double* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -146,7 +149,7 @@ int spde_fields_contract_workspace(int64_t Ns, int nb, int64_t* doubles) {
     set_error("spde_fields_contract_workspace: invalid argument");
     return SPDE_ERR_ARG;
   }
-  const int64_t nblk = (Ns + 255) / 256 < 240 ? (Ns + 255) / 256 : 240;
+  const int64_t nblk = contract_blocks(Ns);
   *doubles = nblk * (6 * 64 + kCells) + (6 * 64 + kCells);
   return SPDE_OK;
 }
@@ -159,7 +162,7 @@ int spde_fields_contract(int64_t Ns, int nb, const double* bs, const double* bsH
     return SPDE_ERR_ARG;
   }
   hipStream_t s = (hipStream_t)stream;
-  const int64_t nblk = (Ns + 255) / 256 < 240 ? (Ns + 255) / 256 : 240;
+  const int64_t nblk = contract_blocks(Ns);
   const int64_t chunk = (Ns + nblk - 1) / nblk;
   double* partial = work;
   double* sums = work + nblk * (6 * 64 + kCells);

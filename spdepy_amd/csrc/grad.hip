@@ -125,18 +125,20 @@ __global__ __launch_bounds__(128) void grad_reduce_kernel(GradKArgs a, double* _
   p[(int64_t)(kWin + 2) * a.Ns] = tautr;
 }
 
+// Thread per (cell i, entry q): the sum over time slices in slice order
+// (deterministic); consecutive threads take consecutive cells of one entry.
 __global__ void grad_sum_kernel(GradKArgs a, const double* __restrict__ part, double* __restrict__ win,
                                 double* __restrict__ out) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= a.Ns) return;
-  for (int q = 0; q < kPart; ++q) {
-    double acc = 0.0;
-    for (int t = 0; t < a.T; ++t) acc += part[((int64_t)t * kPart + q) * a.Ns + i];
-    if (q < kWin)
-      win[i * kWin + q] = acc;
-    else
-      out[i * kOut + 36 + (q - kWin)] = acc;
-  }
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= a.Ns * kPart) return;
+  const int64_t i = g % a.Ns;
+  const int q = (int)(g / a.Ns);
+  double acc = 0.0;
+  for (int t = 0; t < a.T; ++t) acc += part[((int64_t)t * kPart + q) * a.Ns + i];
+  if (q < kWin)
+    win[i * kWin + q] = acc;
+  else
+    out[i * kOut + 36 + (q - kWin)] = acc;
 }
 
 __device__ inline int widx(int dx, int dy) { return (dy + 2) * 5 + dx + 2; }
@@ -367,7 +369,8 @@ extern "C" int spde_grad_cells(spde_plan* pl, const spde_grad_args* g, double* o
   const unsigned blocks = (unsigned)((a.Ns + 127) / 128);
   hipLaunchKernelGGL(grad_reduce_kernel, dim3((unsigned)((a.Ns * a.T + 127) / 128)), dim3(128), 0, s, a,
                      pl->d_gpart);
-  hipLaunchKernelGGL(grad_sum_kernel, dim3(blocks), dim3(128), 0, s, a, pl->d_gpart, win, out);
+  hipLaunchKernelGGL(grad_sum_kernel, dim3((unsigned)((a.Ns * kPart + 255) / 256)), dim3(256), 0, s, a,
+                     pl->d_gpart, win, out);
   hipLaunchKernelGGL(grad_cell_kernel, dim3(blocks), dim3(128), 0, s, a, win, out);
   SPDE_HIP(hipGetLastError());
   return SPDE_OK;

@@ -130,22 +130,9 @@ extern "C" int spde_loglik_grad(spde_plan* pl, spde_plan* sp, const spde_coeffs*
   SPDE_HIP(hipMemcpyAsync(qmu.data(), w->qmu, sizeof(double) * n * r, hipMemcpyDeviceToHost, s));
   SPDE_HIP(hipEventRecord(w->done, w->side));
   SPDE_HIP(hipStreamWaitEvent(s, w->done, 0));
-  double ld_qc = 0.0, ld_q0 = 0.0, ld_g = 0.0, logq = 0.0;
-  if ((rc = spde_factor_finish(pl, 1, &ld_qc))) return rc;
-  if ((rc = spde_factor_finish(sp, 0, &ld_q0))) return rc;
-  if ((rc = spde_factor_finish(sp, 1, &ld_g))) return rc;
-  if ((rc = spde_logq_sum(pl, 1.0 / (c->dt * c->sigma), &logq, s))) return rc;
-  const int T = pl->lat.T;
-  const double ld_q = ld_q0 + (T - 1) * (ld_g + logq);
-  double quad = 0.0, rss = 0.0;
-  for (size_t i = 0; i < mu.size(); ++i) quad += mu[i] * qmu[i];
-  for (int q = 0; q < r; ++q)
-    for (int64_t i = 0; i < m; ++i) {
-      const double e = y[i + (int64_t)q * m] - mu[obs_cols[i] + (int64_t)q * n];
-      rss += e * e;
-    }
-  const double like = 0.5 * ld_q * r + 0.5 * (double)m * r * log_tau - 0.5 * ld_qc * r - 0.5 * quad - 0.5 * tau * rss;
-  // adjoint pass (closed-form log|Q| terms from the one-slice selected inverses)
+  // adjoint pass (closed-form log|Q| terms from the one-slice selected
+  // inverses): it needs only device data, so it is enqueued, with the copy of
+  // its result, before the host waits for the scalar pieces
   spde_grad_args g{};
   g.mode = 0;
   g.r = r;
@@ -162,6 +149,21 @@ extern "C" int spde_loglik_grad(spde_plan* pl, spde_plan* sp, const spde_coeffs*
   if ((rc = spde_grad_cells(pl, &g, cells, s))) return rc;
   std::vector<double> hc((size_t)Ns * 40);
   SPDE_HIP(hipMemcpyAsync(hc.data(), cells, sizeof(double) * Ns * 40, hipMemcpyDeviceToHost, s));
+  double ld_qc = 0.0, ld_q0 = 0.0, ld_g = 0.0, logq = 0.0;
+  if ((rc = spde_factor_finish(pl, 1, &ld_qc))) return rc;
+  if ((rc = spde_factor_finish(sp, 0, &ld_q0))) return rc;
+  if ((rc = spde_factor_finish(sp, 1, &ld_g))) return rc;
+  if ((rc = spde_logq_sum(pl, 1.0 / (c->dt * c->sigma), &logq, s))) return rc;
+  const int T = pl->lat.T;
+  const double ld_q = ld_q0 + (T - 1) * (ld_g + logq);
+  double quad = 0.0, rss = 0.0;
+  for (size_t i = 0; i < mu.size(); ++i) quad += mu[i] * qmu[i];
+  for (int q = 0; q < r; ++q)
+    for (int64_t i = 0; i < m; ++i) {
+      const double e = y[i + (int64_t)q * m] - mu[obs_cols[i] + (int64_t)q * n];
+      rss += e * e;
+    }
+  const double like = 0.5 * ld_q * r + 0.5 * (double)m * r * log_tau - 0.5 * ld_qc * r - 0.5 * quad - 0.5 * tau * rss;
   SPDE_HIP(hipStreamSynchronize(s));
   double dotgq = 0.0, dot00 = 0.0, tautr = 0.0;
   for (int64_t k = 0; k < Ns; ++k) {

@@ -216,8 +216,46 @@ class Engine:
         done = torch.cuda.Event()
         done.record(side)
         main.wait_event(done)
-        ld_qc = self.plan.factor_finish(1)
         T = self.plan.shape[2]
+        keep = []
+        if grad:
+            # the per-cell adjoints need only device data: they are enqueued
+            # before the host reads the scalar pieces back, so the GPU works
+            # on them meanwhile
+            g = nat.GradArgs()
+            g.r = r
+            g.qval = qval.data_ptr()
+            mu_cm = mu.t().contiguous()            # n x r column-major
+            g.mu = mu_cm.data_ptr()
+            g.diag_add = diag_add.data_ptr()
+            g.coeffs = coeffs.c
+            keep.append(mu_cm)
+            if adv_dir is not None:
+                adv = torch.as_tensor(np.ascontiguousarray(adv_dir, dtype=np.float64).reshape(-1)).cuda()
+                g.adv_dir = adv.data_ptr()
+                keep.append(adv)
+            if exact:
+                g.mode, g.nh1 = 0, 0
+                g.sigma_qc = sqc.data_ptr()
+                keep.append(sqc)
+                if closed:
+                    g.sigma_q = None
+                    g.sigma_gram, g.sigma_q0, g.spatial = s_gram.data_ptr(), s_q0.data_ptr(), sp._h
+                    keep += [s_gram, s_q0]
+                else:
+                    g.sigma_q = sq.data_ptr()
+                    keep.append(sq)
+            else:
+                V = torch.as_tensor(np.ascontiguousarray(probes, dtype=np.float64)).cuda()
+                trq = self.plan.solve(0, V).t().contiguous()
+                trqc = self.plan.solve(1, V).t().contiguous()
+                Vt = V.t().contiguous()
+                g.mode, g.nh1 = 1, V.shape[1]
+                g.trq, g.trqc, g.probes = trq.data_ptr(), trqc.data_ptr(), Vt.data_ptr()
+                keep += [trq, trqc, Vt]
+            out = torch.empty(self.Ns * OUT_WIDTH, dtype=torch.float64, device="cuda")
+            nat.check(nat.lib().spde_grad_cells(self.plan._h, g, _ptr(out), _stream(torch)))
+        ld_qc = self.plan.factor_finish(1)
         if closed:
             ld_q0 = sp.factor_finish(0)
             ld_gram = sp.factor_finish(1)
@@ -231,39 +269,6 @@ class Engine:
         parts = dict(m=m, r=r, tau=tau, rss=rss, ld_q=ld_q, ld_qc=ld_qc, qval=qval, mu=mu, like_raw=like)
         if not grad:
             return -like / (m * r), parts
-        g = nat.GradArgs()
-        g.r = r
-        g.qval = qval.data_ptr()
-        mu_cm = mu.t().contiguous()            # n x r column-major
-        g.mu = mu_cm.data_ptr()
-        g.diag_add = diag_add.data_ptr()
-        g.coeffs = coeffs.c
-        keep = [mu_cm]
-        if adv_dir is not None:
-            adv = torch.as_tensor(np.ascontiguousarray(adv_dir, dtype=np.float64).reshape(-1)).cuda()
-            g.adv_dir = adv.data_ptr()
-            keep.append(adv)
-        if exact:
-            g.mode, g.nh1 = 0, 0
-            g.sigma_qc = sqc.data_ptr()
-            keep.append(sqc)
-            if closed:
-                g.sigma_q = None
-                g.sigma_gram, g.sigma_q0, g.spatial = s_gram.data_ptr(), s_q0.data_ptr(), sp._h
-                keep += [s_gram, s_q0]
-            else:
-                g.sigma_q = sq.data_ptr()
-                keep.append(sq)
-        else:
-            V = torch.as_tensor(np.ascontiguousarray(probes, dtype=np.float64)).cuda()
-            trq = self.plan.solve(0, V).t().contiguous()
-            trqc = self.plan.solve(1, V).t().contiguous()
-            Vt = V.t().contiguous()
-            g.mode, g.nh1 = 1, V.shape[1]
-            g.trq, g.trqc, g.probes = trq.data_ptr(), trqc.data_ptr(), Vt.data_ptr()
-            keep += [trq, trqc, Vt]
-        out = torch.empty(self.Ns * OUT_WIDTH, dtype=torch.float64, device="cuda")
-        nat.check(nat.lib().spde_grad_cells(self.plan._h, g, _ptr(out), _stream(torch)))
         if contract is not None:
             parts["coef_grad"], colsum = contract(out)
         else:
