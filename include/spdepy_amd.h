@@ -173,7 +173,11 @@ int spde_solve(spde_plan* plan, int slot, int mode, double* B, int nrhs, int64_t
 
 /* Takahashi selected inverse of the factor in `slot`: Sigma = Q^-1 on the
  * structural pattern (sigma[nnz_struct], device).  Each slot has its own
- * arenas, so selected inverses of different slots may run concurrently. */
+ * arenas, so selected inverses of different slots may run concurrently.  The
+ * products that need only the factor (X = L21 L11^-1, L11^-T L11^-1) run on a
+ * second stream of the plan, ordered after `stream` by an event; the level
+ * chain on `stream` waits for them, so the call's results are ordered on
+ * `stream` alone. */
 int spde_selinv(spde_plan* plan, int slot, double* sigma, void* stream);
 
 /* ---------------------------------------------------------------- gradient
