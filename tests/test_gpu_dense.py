@@ -32,8 +32,12 @@ def test_mfma_f64_lane_layout():
 
 @pytest.mark.parametrize("tile", [64, 128])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-@pytest.mark.parametrize("M,N,K", [(64, 64, 16), (100, 37, 70), (5, 130, 3), (129, 65, 200), (300, 260, 33)])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 16), (100, 37, 70), (5, 130, 3), (129, 65, 200), (300, 260, 33),
+                                   (1100, 1060, 40)])
 def test_gemm_matches_numpy(ta, tb, M, N, K, tile):
+    """The last shape has more than 64 tiles and a partial last group of 64
+    (306 tiles of 64, 81 of 128): the XCD-grouped block-to-tile mapping must
+    cover every tile once."""
     import torch
     nat = _lib()
     rng = np.random.default_rng(M * 7 + N * 3 + K)
