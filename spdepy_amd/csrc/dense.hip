@@ -347,7 +347,7 @@ void launch_gemm(const GemmLaunch& g, const Bases& b, hipStream_t s) {
 // columns right of the pivot block with the finished rows read back from Y
 // (broadcast reads, no recomputation).  Entries right of the diagonal are
 // never read, so the update runs unconditionally over them.  W = L^-1 is
-// then built per wave without barriers (below).  Pivots use v_rsq_f64 with
+// then built by recursive blocking in LDS (below).  Pivots use v_rsq_f64 with
 // two Newton steps (relative error below 2^-52) instead of a square root and
 // a division.  Blocks narrower than 64 are padded with the identity.  A
 // non-positive pivot is recorded in *status as (global elimination column
@@ -360,13 +360,6 @@ struct Piv4 {
 // compiler defers the rank-4 updates of later columns to their first use and
 // keeps every step's operands live (hundreds of registers, spilled).
 __device__ inline void pin(double& v) { asm volatile("" : "+v"(v)); }
-
-// Value of v in lane `lane` of the wave (lane is a compile-time constant).
-__device__ inline double lane_value(double v, int lane) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
-}
 
 // 1/sqrt(x) for x > 0 (hardware estimate refined twice), x for the factor.
 __device__ inline double pivot_rsq(double x, bool& bad) {
@@ -466,44 +459,88 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(const PotrfDesc* __rest
       if (c < kb && c <= r) Fb[r + (int64_t)c * d.ld] = a[q];
     }
   }
-  // W = L^-1, four rows per step; w[q] = W(r, 4q + g).  Column 4q + g of W
-  // lives in wave g alone (lane = row), so the sweep needs no barrier: the
-  // four residual entries of the pivot rows are read across lanes, the 4x4
-  // solve is done on them (uniform values) and every row is updated; the
-  // pivot rows solve for their own entry in the same expression (their
-  // diagonal term left out, scaled by 1/L(r, r)), rows above read zeros
-  // from Y and keep their values.
-  double w[16];
+  // W = L^-1 by recursive blocking: the four 16x16 diagonal blocks are
+  // inverted by column-oriented forward substitution (wave g, lane j < 16:
+  // column j of block g), then W21 = -W22 (L21 W11) for the two 32x32 and
+  // the 64x64 block rows, the products spread over the block.  Wl is
+  // row-major with zeros above the diagonal; blocks beyond kb hold the
+  // identity padding of L, so W is the identity there too.
+  constexpr int WP = 65;
+  __shared__ double Wl[64 * WP];
+  __shared__ double Xs[32 * 33];
+  for (int e = tid; e < 64 * WP; e += 256) Wl[e] = 0.0;
+  __syncthreads();
+  if (r < 16) {
+    const int o = 16 * g, j = r;
+    double x[16];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) w[q] = (r == 4 * q + g) ? 1.0 : 0.0;
+    for (int i = 0; i < 16; ++i) x[i] = (i == j) ? 1.0 : 0.0;
 #pragma unroll
-  for (int s = 0; s < 16; ++s) {
-    const int k0 = 4 * s;
-    const double* L0 = &Y[k0 * YP + k0];
-    const double e0 = idg[k0], e1 = idg[k0 + 1], e2 = idg[k0 + 2], e3 = idg[k0 + 3];
-    const double m10 = L0[YP], m20 = L0[2 * YP], m21 = L0[2 * YP + 1];
-    const double m30 = L0[3 * YP], m31 = L0[3 * YP + 1], m32 = L0[3 * YP + 2];
-    const double* Lr = &Y[r * YP + k0];
-    const int k = r - k0;
-    const double g0 = (k == 0) ? 0.0 : Lr[0], g1 = (k == 1) ? 0.0 : Lr[1];
-    const double g2 = (k == 2) ? 0.0 : Lr[2], g3 = (k == 3) ? 0.0 : Lr[3];
-    const double h = (k >= 0 && k < 4) ? idg[r] : 1.0;
+    for (int kk = 0; kk < 16; ++kk) {
+      x[kk] *= idg[o + kk];
 #pragma unroll
-    for (int q = 0; q <= s; ++q) {
-      const double z0 = lane_value(w[q], k0) * e0;
-      const double z1 = (lane_value(w[q], k0 + 1) - m10 * z0) * e1;
-      const double z2 = (lane_value(w[q], k0 + 2) - m20 * z0 - m21 * z1) * e2;
-      const double z3 = (lane_value(w[q], k0 + 3) - m30 * z0 - m31 * z1 - m32 * z2) * e3;
-      w[q] = (w[q] - (g0 * z0 + g1 * z1 + g2 * z2 + g3 * z3)) * h;
-      pin(w[q]);
+      for (int i = kk + 1; i < 16; ++i) x[i] -= Y[(o + i) * YP + o + kk] * x[kk];
     }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) Wl[(o + i) * WP + o + j] = x[i];
   }
+  __syncthreads();
+  {
+    // 32 level, pair P = rows/cols 32P .. 32P + 31: X = L(o+16.., o..) W00,
+    // W10 = -W11 X; thread: pair tid >> 7, row i, columns j0, j0 + 1
+    const int P = tid >> 7, t = tid & 127, i = t >> 3, j0 = (t & 7) * 2, o = 32 * P;
+    double x0 = 0.0, x1 = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      const double l = Y[(o + 16 + i) * YP + o + kk];
+      x0 += l * Wl[(o + kk) * WP + o + j0];
+      x1 += l * Wl[(o + kk) * WP + o + j0 + 1];
+    }
+    Xs[(P * 16 + i) * 33 + j0] = x0;
+    Xs[(P * 16 + i) * 33 + j0 + 1] = x1;
+    __syncthreads();
+    double w0 = 0.0, w1 = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      const double v = Wl[(o + 16 + i) * WP + o + 16 + kk];
+      w0 += v * Xs[(P * 16 + kk) * 33 + j0];
+      w1 += v * Xs[(P * 16 + kk) * 33 + j0 + 1];
+    }
+    Wl[(o + 16 + i) * WP + o + j0] = -w0;
+    Wl[(o + 16 + i) * WP + o + j0 + 1] = -w1;
+  }
+  __syncthreads();
+  {
+    // 64 level: X = L(32.., 0..31) W_aa, W_ba = -W_bb X; thread: column
+    // j = tid & 31, rows i0 + 8u
+    const int j = tid & 31, i0 = tid >> 5;
+    double x[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < 32; ++kk) {
+      const double wv = Wl[kk * WP + j];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) x[u] += Y[(32 + i0 + 8 * u) * YP + kk] * wv;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) Xs[(i0 + 8 * u) * 33 + j] = x[u];
+    __syncthreads();
+    double w[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < 32; ++kk) {
+      const double xv = Xs[kk * 33 + j];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) w[u] += Wl[(32 + i0 + 8 * u) * WP + 32 + kk] * xv;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) Wl[(32 + i0 + 8 * u) * WP + j] = -w[u];
+  }
+  __syncthreads();
   double* Zb = Z + d.zoff + d.k0 + (int64_t)d.k0 * d.zld;
   if (r < kb) {
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int c = 4 * q + g;
-      if (c < kb) Zb[r + (int64_t)c * d.zld] = (c <= r) ? w[q] : 0.0;
+      if (c < kb) Zb[r + (int64_t)c * d.zld] = Wl[r * WP + c];
     }
   }
 }
