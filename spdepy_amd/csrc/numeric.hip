@@ -134,18 +134,22 @@ ArenaSizes arena_sizes(const spde_plan* pl, int nrhs) {
 
 void make_tiles(GemmLaunch& g, int force_tile);
 
-// Split-K for launches with few tiles and deep problems: the launch fills at
-// most a quarter of the chip's block slots and its deepest problem is at
-// least 512 deep.  S = min(512 / tiles, K_max / 256, 64) chunks, each at
-// least 256 deep; the partial tiles take S * tiles * 64 * 64 doubles of the
-// workspace counted in *ws.
+// Split-K for launches of 64-tiles that leave the chip's 1,024 block slots
+// (four per CU) under-filled with deep problems: under 256 tiles and at least
+// 512 deep, S = min(512 / tiles, K_max / 256, 64) chunks; under 1,024 tiles
+// and at least 1,024 deep, S = min(round(2048 / tiles), K_max / 256) chunks
+// (about two rounds of the chip).  Chunks are at least 256 deep; the partial
+// tiles take S * tiles * 64 * 64 doubles of the workspace counted in *ws.
 void plan_split(GemmLaunch& g, int64_t* ws) {
   if (!ws || g.gemv || g.tile != 64 || g.h_tiles.empty()) return;
   int kmax = 0;
   for (const GemmProb& P : g.h_probs) kmax = std::max(kmax, P.K);
   const int64_t nt = (int64_t)g.h_tiles.size();
-  if (nt >= 256 || kmax < 512) return;
-  const int S = (int)std::min<int64_t>(std::min<int64_t>(512 / nt, kmax / 256), 64);
+  int S = 0;
+  if (nt < 256 && kmax >= 512)
+    S = (int)std::min<int64_t>(std::min<int64_t>(512 / nt, kmax / 256), 64);
+  else if (nt < 1024 && kmax >= 1024)
+    S = (int)std::min<int64_t>((2048 + nt / 2) / nt, kmax / 256);
   if (S < 2) return;
   std::vector<GemmTile> t;
   t.reserve(nt * S);
