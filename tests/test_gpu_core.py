@@ -119,6 +119,27 @@ def test_loglike_exact_gradient(case):
     assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * scale), np.abs(jac - jac_o).max() / scale
 
 
+@pytest.mark.parametrize("case", ["var_b1", "iso_b1", "ha_b1"])
+def test_loglike_exact_gradient_several_realisations(case):
+    """Data with r = 3 columns (independent realisations at the same
+    locations, the reference's 2-D data): mu_c and the data terms take three
+    right-hand sides (the narrow solve path), the trace term is scaled by r."""
+    c = CASES[case]()
+    g = c["grid"]
+    S = g.getS(c["idx"])
+    rng = np.random.default_rng(21)
+    y3 = np.column_stack([c["y"], c["y"][::-1] * 0.7, 0.2 * rng.normal(size=c["y"].shape[0])])
+    Qo, dQo = c["model"].makeQ(c["par"], grad=True)
+    like_o, jac_o, _ = Mo.loglike(Qo, dQo, S, y3, c["par"][-1], trace="exact")
+    m = product_case(c)
+    m.initFit(y3, idx=c["idx"], fitQ0=False)
+    step("logLike exact r=3 " + case)
+    like, jac = m.logLike(c["par"])
+    assert abs(like - like_o) <= 1e-9 * abs(like_o)
+    scale = np.abs(jac_o).max()
+    assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * scale), np.abs(jac - jac_o).max() / scale
+
+
 @pytest.mark.parametrize("case", ["var_b1", "ani_b3", "iso_b1"])
 def test_closed_form_and_factorised_logdet_agree(case):
     """The two routes to log|Q| and its gradient (spatial matrices of

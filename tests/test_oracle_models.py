@@ -82,6 +82,26 @@ def test_loglike_is_gaussian_marginal(case):
     assert abs(-like * m - logp) <= 1e-9 * abs(logp)
 
 
+@pytest.mark.parametrize("case", ["iso_b1", "var_b3"])
+def test_loglike_several_realisations_is_sum_of_marginals(case):
+    """r independent realisations at the same locations: -like*m*r is the sum
+    of the r Gaussian log-densities (brute force), so the r-column terms
+    (r log|Q|, r log|Q_c|, the summed quadratic forms) are pinned."""
+    c = CASES[case]()
+    g = c["grid"]
+    Q, _ = c["model"].makeQ(c["par"], grad=False)
+    S = g.getS(c["idx"])
+    rng = np.random.default_rng(5)
+    m = S.shape[0]
+    Y = np.column_stack([c["y"], 0.5 * c["y"][::-1], 0.2 * rng.normal(size=m)])
+    like, _, _ = Mo.loglike(Q, [], S, Y, c["par"][-1])
+    tau = np.exp(c["par"][-1])
+    C = S.toarray() @ np.linalg.solve(Q.toarray(), S.toarray().T) + np.eye(m) / tau
+    sgn, ld = np.linalg.slogdet(C)
+    logp = sum(-0.5 * ld - 0.5 * y @ np.linalg.solve(C, y) for y in Y.T)
+    assert abs(-like * m * Y.shape[1] - logp) <= 1e-9 * abs(logp)
+
+
 def _fd_check(c, which, h=1e-5, rtol=2e-5):
     g = c["grid"]
     S = g.getS(c["idx"])
