@@ -50,6 +50,8 @@ CASES = {
     "iso_b1": lambda: F.stationary(10, 9, 4, bc=1, m=40),
     "ani_b3": lambda: F.stationary(9, 10, 3, bc=3, kind="ani", m=40),
     "ha_b1": lambda: F.stationary(8, 9, 4, bc=1, kind="ha", m=40),
+    "ani_b2": lambda: F.stationary(9, 10, 3, bc=2, kind="ani", m=40),
+    "var_b2": lambda: F.small(8, 9, 3, bc=2, m=40),
 }
 
 
