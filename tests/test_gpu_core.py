@@ -142,6 +142,28 @@ def test_loglike_exact_gradient_several_realisations(case):
     assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * scale), np.abs(jac - jac_o).max() / scale
 
 
+@pytest.mark.parametrize("case", ["var_b1", "ani_b3"])
+def test_loglike_repeated_locations(case):
+    """Several observations of one location (repeated rows of S): Q_c gets
+    tau times the count on the diagonal and S^T y sums the repeats."""
+    c = CASES[case]()
+    g = c["grid"]
+    rng = np.random.default_rng(33)
+    idx = np.sort(np.concatenate([c["idx"], c["idx"][::4]]))
+    y = rng.normal(size=idx.shape[0])
+    S = g.getS(idx)
+    assert S.shape[0] > len(set(idx.tolist()))
+    Qo, dQo = c["model"].makeQ(c["par"], grad=True)
+    like_o, jac_o, _ = Mo.loglike(Qo, dQo, S, y, c["par"][-1], trace="exact")
+    m = product_case(c)
+    m.initFit(y, idx=idx, fitQ0=False)
+    step("logLike exact repeated locations " + case)
+    like, jac = m.logLike(c["par"])
+    assert abs(like - like_o) <= 1e-9 * abs(like_o)
+    scale = np.abs(jac_o).max()
+    assert np.allclose(jac, jac_o, rtol=1e-9, atol=1e-9 * scale), np.abs(jac - jac_o).max() / scale
+
+
 @pytest.mark.parametrize("case", ["var_b1", "ani_b3", "iso_b1"])
 def test_closed_form_and_factorised_logdet_agree(case):
     """The two routes to log|Q| and its gradient (spatial matrices of
