@@ -96,7 +96,9 @@ def test_sampler_and_selected_inverse_identities(workload):
     qval = eng.assemble(mod._coeffs(par))
     step("factor + sample + selinv")
     eng.plan.factor(0, qval)
-    z = torch.as_tensor(np.random.default_rng(0).normal(size=(eng.n, 3))).cuda()
+    # 16 right-hand sides, as the configuration's 16 conditional samples (the
+    # tile-GEMM solve path; up to 8 take the narrow one)
+    z = torch.as_tensor(np.random.default_rng(0).normal(size=(eng.n, 16))).cuda()
     x = eng.plan.solve(0, z, 2)
     qx = eng.plan.spmv(qval, x)
     lhs = (x * qx).sum(0).cpu().numpy()

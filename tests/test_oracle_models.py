@@ -320,3 +320,17 @@ def test_cov_family_derivatives(kind, dvar):
         pm[i] -= h
         fd = (m.makeQ(pp, False)[0] - m.makeQ(pm, False)[0]) / (2 * h)
         assert abs(fd - dQ[i]).max() <= 1e-6 * max(abs(dQ[i]).max(), 1e-3 * abs(Q).max()), i
+
+
+def test_config1_golden_is_the_oracle():
+    """tests/golden/config1.npz (the GPU parity target at 30 x 30 x 10) is what
+    the oracle computes now (dense inverses at n = 9,000: ~15 s)."""
+    import os
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config1.npz"))
+    c = F.stationary()
+    assert np.array_equal(c["idx"], gold["idx"]) and np.array_equal(c["y"], gold["y"])
+    Q, dQ = c["model"].makeQ(c["par"], grad=True)
+    S = c["grid"].getS(c["idx"])
+    like, jac, _ = Mo.loglike(Q.tocsc(), dQ, S, c["y"], c["par"][-1], trace="exact")
+    assert abs(like - float(gold["like"])) <= 1e-12 * abs(like)
+    assert np.allclose(jac, gold["jac"], rtol=1e-10, atol=1e-12 * np.abs(jac).max())
