@@ -343,9 +343,9 @@ void launch_gemm(const GemmLaunch& g, const Bases& b, hipStream_t s) {
 // thread.  Step: the threads publish their pivot-column entry (one LDS write
 // each, double buffered), every thread forms the 4x4 pivot block's factor
 // itself, finishes its row's four entries, writes its own one to the L copy
-// Y in LDS, and after one more barrier applies the rank-4 update to its
-// columns right of the pivot block with the finished rows read back from Y
-// (broadcast reads, no recomputation).  Entries right of the diagonal are
+// Y in LDS, and the rank-4 update of its columns right of the pivot block,
+// with the finished rows read back from Y (broadcast reads, no
+// recomputation), follows in the next step.  Entries right of the diagonal are
 // never read, so the update runs unconditionally over them.  W = L^-1 is
 // then built by recursive blocking in LDS (below).  Pivots use v_rsq_f64 with
 // two Newton steps (relative error below 2^-52) instead of a square root and
@@ -419,15 +419,30 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(const PotrfDesc* __rest
     const double v = Fb[rr + (int64_t)cc * d.ld];  // clamped: always inside the block
     a[q] = (r < kb && c < kb) ? (c <= r ? v : 0.0) : (r == c ? 1.0 : 0.0);
   }
+  // The rank-4 update of step s - 1 is applied to the next pivot column
+  // first and to the later columns during step s, beside its serial 4x4
+  // factor and row solve (same operations in the same order per entry).
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
-    const int j0 = 4 * s, buf = s & 1;
+    const int j0 = 4 * s, buf = s & 1, jp = j0 - 4;
+    if (s > 0) {
+      const double* yc = &Y[(4 * s + g) * YP + jp];
+      a[s] -= p0 * yc[0] + p1 * yc[1] + p2 * yc[2] + p3 * yc[3];
+    }
     pub[buf][4 * r + g] = a[s];
     __syncthreads();
     Piv4 c;
     int bad;
     chol4<4>(&pub[buf][4 * j0], c, bad);
     if (bad >= 0 && tid == 0 && j0 + bad < kb) atomicCAS(status, 0, d.colbase + j0 + bad + 1);
+    if (s > 0) {
+#pragma unroll
+      for (int q = s + 1; q < 16; ++q) {
+        const double* yc = &Y[(4 * q + g) * YP + jp];
+        a[q] -= p0 * yc[0] + p1 * yc[1] + p2 * yc[2] + p3 * yc[3];
+      }
+    }
     // row r's four entries; on the pivot rows this yields the factor's
     // entries left of and on the diagonal, the rest (and all of the rows
     // above) are cleared
@@ -445,12 +460,12 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(const PotrfDesc* __rest
     Y[r * YP + j0 + g] = mine;
     if (tid < 4) idg[j0 + tid] = (tid == 0) ? c.i0 : (tid == 1 ? c.i1 : (tid == 2 ? c.i2 : c.i3));
     __syncthreads();
+    p0 = y0;
+    p1 = y1;
+    p2 = y2;
+    p3 = y3;
 #pragma unroll
-    for (int q = s + 1; q < 16; ++q) {
-      const double* yc = &Y[(4 * q + g) * YP + j0];
-      a[q] -= y0 * yc[0] + y1 * yc[1] + y2 * yc[2] + y3 * yc[3];
-      pin(a[q]);
-    }
+    for (int q = s + 1; q < 16; ++q) pin(a[q]);
   }
   if (r < kb) {
 #pragma unroll
