@@ -233,9 +233,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // quarters of K, partial sums combined in LDS in wave order.
 // A transposed (K x M stored): one wave per output row m (a column of A),
 // lanes stride over k, butterfly sum across the wave.
-template <bool TA>
-__global__ __launch_bounds__(256) void gemv_kernel(const GemmProb* __restrict__ probs,
-                                                   const GemmTile* __restrict__ tiles, Bases bases) {
+template <bool TA, int NW = 4>
+__global__ __launch_bounds__(NW * 64) void gemv_kernel(const GemmProb* __restrict__ probs,
+                                                       const GemmTile* __restrict__ tiles, Bases bases) {
   const GemmTile t = tiles[blockIdx.x];
   const GemmProb P = probs[t.prob];
   const double* __restrict__ A = bases.p[P.a_base] + P.a_off;
@@ -247,9 +247,9 @@ __global__ __launch_bounds__(256) void gemv_kernel(const GemmProb* __restrict__ 
 #pragma unroll
   for (int c = 0; c < kGemvMaxN; ++c) acc[c] = 0.0;
   if constexpr (!TA) {
-    __shared__ double red[4][kGemvMaxN][64];
+    __shared__ double red[NW][kGemvMaxN][64];
     const int m = t.tm * 64 + lane;
-    const int q = (K + 3) / 4, k0 = wave * q, k1 = min(K, k0 + q);
+    const int q = (K + NW - 1) / NW, k0 = wave * q, k1 = min(K, k0 + q);
     if (m < P.M)
 #pragma unroll 8
       for (int k = k0; k < k1; ++k) {
@@ -263,7 +263,9 @@ __global__ __launch_bounds__(256) void gemv_kernel(const GemmProb* __restrict__ 
     __syncthreads();
     if (wave == 0 && m < P.M)
       for (int c = 0; c < N; ++c) {
-        const double v = ((red[0][c][lane] + red[1][c][lane]) + red[2][c][lane]) + red[3][c][lane];
+        double v = red[0][c][lane];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) v += red[w][c][lane];
         double* y = Y + m + (int64_t)c * P.ldc;
         const double r = P.alpha * v;
         *y = (P.beta == 0.0) ? r : r + P.beta * (*y);
@@ -297,6 +299,8 @@ void launch_gemm(const GemmLaunch& g, const Bases& b, hipStream_t s) {
   if (g.gemv) {
     if (g.ta)
       hipLaunchKernelGGL(gemv_kernel<true>, grid, block, 0, s, g.d_probs, g.d_tiles, b);
+    else if (g.gemv_wide)
+      hipLaunchKernelGGL((gemv_kernel<false, 16>), grid, dim3(1024), 0, s, g.d_probs, g.d_tiles, b);
     else
       hipLaunchKernelGGL(gemv_kernel<false>, grid, block, 0, s, g.d_probs, g.d_tiles, b);
     return;

@@ -66,6 +66,7 @@ struct GemmLaunch {
   // few right-hand sides (N <= kGemvMaxN, op(B) = B, general C): a
   // bandwidth-bound kernel reading A once instead of 64-wide MFMA tiles
   bool gemv = false;
+  bool gemv_wide = false;        // narrow launch with deep K: sixteen waves share each row block's K range
 };
 constexpr int kGemvMaxN = 8;
 

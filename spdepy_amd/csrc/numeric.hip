@@ -198,6 +198,9 @@ void make_tiles(GemmLaunch& g, int force_tile) {
   if (narrow) {
     g.tile = 0;
     g.h_tiles.clear();
+    int kmax = 0;
+    for (const GemmProb& P : g.h_probs) kmax = std::max(kmax, P.K);
+    g.gemv_wide = !g.ta && kmax >= 1024;
     const int rows = g.ta ? 4 : 64;
     for (size_t i = 0; i < g.h_probs.size(); ++i)
       for (int a = 0; a < (g.h_probs[i].M + rows - 1) / rows; ++a) g.h_tiles.push_back({(int32_t)i, a, 0, 0});
