@@ -285,14 +285,21 @@ __global__ void prune_write_kernel(int64_t n, const int64_t* __restrict__ indptr
     }
 }
 
-__global__ void spmv_kernel(int64_t n, int nrhs, const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
-                            const double* __restrict__ q, const double* __restrict__ x, double* __restrict__ y) {
-  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (r >= n) return;
+// Sixteen lanes per row: lane l takes entries l, l + 16, ... of the row
+// (consecutive lanes read consecutive entries), then a fixed butterfly over
+// the sixteen lanes (deterministic).
+__global__ __launch_bounds__(256) void spmv_kernel(int64_t n, int nrhs, const int64_t* __restrict__ indptr,
+                                                   const int32_t* __restrict__ indices, const double* __restrict__ q,
+                                                   const double* __restrict__ x, double* __restrict__ y) {
+  const int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 4;
+  const int l = threadIdx.x & 15;
+  if (r >= n) return;  // uniform over the sixteen lanes of a row
+  const int64_t e0 = indptr[r], e1 = indptr[r + 1];
   for (int c = 0; c < nrhs; ++c) {
     double s = 0.0;
-    for (int64_t e = indptr[r]; e < indptr[r + 1]; ++e) s += q[e] * x[indices[e] + (int64_t)c * n];
-    y[r + (int64_t)c * n] = s;
+    for (int64_t e = e0 + l; e < e1; e += 16) s += q[e] * x[indices[e] + (int64_t)c * n];
+    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+    if (l == 0) y[r + (int64_t)c * n] = s;
   }
 }
 
@@ -438,7 +445,7 @@ int spde_spmv(spde_plan* pl, const double* qval, const double* x, double* y, int
     return SPDE_ERR_ARG;
   }
   const int64_t n = pl->sym.n;
-  hipLaunchKernelGGL(spmv_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, n, nrhs,
+  hipLaunchKernelGGL(spmv_kernel, dim3(blocks_for(n * 16, 256)), dim3(256), 0, (hipStream_t)stream, n, nrhs,
                      pl->d_indptr, pl->d_indices, qval, x, y);
   SPDE_HIP(hipGetLastError());
   return SPDE_OK;
