@@ -68,7 +68,11 @@ struct GemmLaunch {
   bool gemv = false;
   bool gemv_wide = false;        // narrow launch with deep K: sixteen waves share each row block's K range
 };
-constexpr int kGemvMaxN = 8;
+// One column only: at 100x100x20 a full solve takes 2.5 ms with the
+// bandwidth-bound kernel for one right-hand side, but from two right-hand
+// sides on the MFMA tiles are faster (3.2 ms for two, 4.9 ms for eight,
+// against 3.8 and 10.6 ms with the per-column kernel).
+constexpr int kGemvMaxN = 1;
 
 struct PotrfDesc {      // one diagonal block of one front
   int64_t foff;         // front offset
