@@ -615,7 +615,7 @@ extern "C" int spde_debug_gemm(int ta, int tb, int M, int N, int K, double alpha
     set_error("spde_debug_gemm: invalid argument");
     return SPDE_ERR_ARG;
   }
-  GemmProb P;
+  GemmProb P{};  // every field zero (kmode 0: no triangular operand)
   P.a_off = P.b_off = P.c_off = 0;
   P.lda = lda;
   P.ldb = ldb;

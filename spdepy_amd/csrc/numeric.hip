@@ -253,7 +253,7 @@ GemmProb tri(GemmProb P, int kmode) {
 
 GemmProb prob(int8_t ab, int64_t ao, int lda, int8_t bb, int64_t bo, int ldb, int8_t cb, int64_t co, int ldc, int M,
               int N, int K, double alpha, double beta) {
-  GemmProb P;
+  GemmProb P{};
   P.a_off = ao;
   P.b_off = bo;
   P.c_off = co;

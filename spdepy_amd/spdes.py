@@ -99,6 +99,19 @@ def _p(t):
 
 
 class _StencilMixin:
+    def _obs_cols(self):
+        """The observed cell of each row of S (one entry per row), kept while
+        S is the same object."""
+        cached = getattr(self, "_obs_cache", None)
+        if cached is not None and cached[0] is self.S:
+            return cached[1]
+        S = self.S.tocsr()
+        if np.any(np.diff(S.indptr) != 1):
+            raise ValueError("observation matrix must have one entry per row")
+        cols = S.indices.astype(np.int64)
+        self._obs_cache = (self.S, cols)
+        return cols
+
     def Ah(self, Hs):
         shp = self.grid.shape
         n = int(shp[0] * shp[1])
@@ -276,11 +289,6 @@ class _WhittleMaternBase(_StencilMixin):
                          kappa0=kappa, H0=H)
         return c
 
-    def _obs_cols(self):
-        S = self.S.tocsr()
-        if np.any(np.diff(S.indptr) != 1):
-            raise ValueError("observation matrix must have one entry per row")
-        return S.indices.astype(np.int64)
 
     def logLike(self, par, nh1=100, grad=True, trace="exact"):
         """whittle_matern2D.py:90-126: -loglik/(m r) and its gradient for
@@ -373,11 +381,6 @@ class _VarWhittleMaternBase(_StencilMixin):
         self.S = self.grid.getS(idxs=kwargs.get("idx"))
         return self._default()
 
-    def _obs_cols(self):
-        S = self.S.tocsr()
-        if np.any(np.diff(S.indptr) != 1):
-            raise ValueError("observation matrix must have one entry per row")
-        return S.indices.astype(np.int64)
 
     def _engine(self):
         if self._eng is None:
@@ -600,11 +603,6 @@ class _SpaceTimeBase(_StencilMixin):
             return Q, fac, None
         return Q, fac, _LazyDQ(self, par)
 
-    def _obs_cols(self):
-        S = self.S.tocsr()
-        if np.any(np.diff(S.indptr) != 1):
-            raise ValueError("observation matrix must have one entry per row")
-        return S.indices.astype(np.int64)
 
     def logLike(self, par, nh1=100, grad=True, trace="exact", logdet="closed"):
         """The reference's logLike (advection_idiffusion2D.py:167-207,
