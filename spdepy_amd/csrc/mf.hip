@@ -69,14 +69,18 @@ __global__ __launch_bounds__(256) void zero_lower_kernel(const NodeDev* __restri
   }
 }
 
-// partial[k] = sum_j log L_jj over the own columns of node k (one block per node).
-__global__ __launch_bounds__(256) void logdet_partial_kernel(const NodeDev* __restrict__ nodes, const double* __restrict__ F,
+// partial[b] = sum_j log L_jj over own columns j0 .. j0 + 255 of one node
+// (block b takes chunk (node, j0) = chunks[b]: one diagonal entry per thread,
+// so the front columns' scattered reads are all in flight at once).
+__global__ __launch_bounds__(256) void logdet_partial_kernel(const NodeDev* __restrict__ nodes,
+                                                             const int2* __restrict__ chunks,
+                                                             const double* __restrict__ F,
                                                              double* __restrict__ partial) {
   __shared__ double red[256];
-  const NodeDev nd = nodes[blockIdx.x];
-  double s = 0.0;
-  for (int j = threadIdx.x; j < nd.p; j += 256) s += log(F[nd.foff + j + (int64_t)j * nd.n]);
-  red[threadIdx.x] = s;
+  const int2 ch = chunks[blockIdx.x];
+  const NodeDev nd = nodes[ch.x];
+  const int j = ch.y + (int)threadIdx.x;
+  red[threadIdx.x] = j < nd.p ? log(F[nd.foff + j + (int64_t)j * nd.n]) : 0.0;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
@@ -240,9 +244,10 @@ void launch_extend_add(const ExtAddDesc* d, const ExtAddTile* t, int ntiles, con
                        hipStream_t s) {
   if (ntiles > 0) hipLaunchKernelGGL(extend_add_kernel, dim3(ntiles), dim3(256), 0, s, d, t, relpos, F);
 }
-void launch_logdet(const NodeDev* nodes, int nnodes, const double* F, double* partial, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(logdet_partial_kernel, dim3(nnodes), dim3(256), 0, s, nodes, F, partial);
-  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, partial, nnodes, 2.0, out);
+void launch_logdet(const NodeDev* nodes, const int2* chunks, int nchunks, const double* F, double* partial,
+                   double* out, hipStream_t s) {
+  if (nchunks > 0) hipLaunchKernelGGL(logdet_partial_kernel, dim3(nchunks), dim3(256), 0, s, nodes, chunks, F, partial);
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, partial, nchunks, 2.0, out);
 }
 void launch_permute_in(const double* B, int64_t ldb, const int32_t* perm, int64_t n, int nrhs, double* xy,
                        hipStream_t s) {
