@@ -120,7 +120,7 @@ struct Slot {
   double* z = nullptr;      // inverses of the diagonal blocks of L
   double* sigma = nullptr;  // selected-inverse fronts (allocated on first use)
   double* work = nullptr;   // inverse-of-L11 and selected-inverse temporaries
-  double* partial = nullptr;  // log-det partial sums per chunk of 256 own columns
+  double* partial = nullptr;  // log-det partial sums per 256 variables
   double* ld = nullptr;     // device log-det
   int* status = nullptr;    // first non-positive pivot (elimination column + 1)
   double* vec = nullptr;    // solve front vectors
@@ -173,8 +173,6 @@ struct spde_plan {
   int32_t* d_sprow = nullptr;     // one-slice plans: row and column of each entry (Kronecker kernels)
   int32_t* d_spcol = nullptr;
   int64_t work_doubles = 0;       // per-slot workspace size
-  int2* d_ldchunks = nullptr;     // log-det partial sums: (node, first own column) per 256 columns
-  int nldchunks = 0;
   std::vector<spde::Slot> slots;
   std::vector<spde::LevelPlan> levels;
   std::vector<spde::GemmLaunch> trtri;  // per recursion height: pairs (X, Z2)

@@ -69,18 +69,27 @@ __global__ __launch_bounds__(256) void zero_lower_kernel(const NodeDev* __restri
   }
 }
 
-// partial[b] = sum_j log L_jj over own columns j0 .. j0 + 255 of one node
-// (block b takes chunk (node, j0) = chunks[b]: one diagonal entry per thread,
-// so the front columns' scattered reads are all in flight at once).
-__global__ __launch_bounds__(256) void logdet_partial_kernel(const NodeDev* __restrict__ nodes,
-                                                             const int2* __restrict__ chunks,
+// partial[b] = sum of log L_ii over the variables i of block b's range of
+// kLogdetPer (the diagonal of L sits where the diagonal of Q was scattered,
+// F[map[diag[i]]]); each thread takes kLogdetPer / 256 entries, all loads in
+// flight at once.
+constexpr int kLogdetPer = 256;
+__global__ __launch_bounds__(256) void logdet_partial_kernel(const int64_t* __restrict__ diag,
+                                                             const int64_t* __restrict__ map, int64_t n,
                                                              const double* __restrict__ F,
                                                              double* __restrict__ partial) {
   __shared__ double red[256];
-  const int2 ch = chunks[blockIdx.x];
-  const NodeDev nd = nodes[ch.x];
-  const int j = ch.y + (int)threadIdx.x;
-  red[threadIdx.x] = j < nd.p ? log(F[nd.foff + j + (int64_t)j * nd.n]) : 0.0;
+  const int64_t i0 = blockIdx.x * (int64_t)kLogdetPer + threadIdx.x;
+  double v[kLogdetPer / 256];
+#pragma unroll
+  for (int u = 0; u < kLogdetPer / 256; ++u) {
+    const int64_t i = i0 + 256 * u;
+    v[u] = i < n ? F[map[diag[i]]] : 1.0;
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int u = 0; u < kLogdetPer / 256; ++u) s += log(v[u]);
+  red[threadIdx.x] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
@@ -244,10 +253,12 @@ void launch_extend_add(const ExtAddDesc* d, const ExtAddTile* t, int ntiles, con
                        hipStream_t s) {
   if (ntiles > 0) hipLaunchKernelGGL(extend_add_kernel, dim3(ntiles), dim3(256), 0, s, d, t, relpos, F);
 }
-void launch_logdet(const NodeDev* nodes, const int2* chunks, int nchunks, const double* F, double* partial,
-                   double* out, hipStream_t s) {
-  if (nchunks > 0) hipLaunchKernelGGL(logdet_partial_kernel, dim3(nchunks), dim3(256), 0, s, nodes, chunks, F, partial);
-  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, partial, nchunks, 2.0, out);
+int logdet_parts(int64_t n) { return (int)((n + kLogdetPer - 1) / kLogdetPer); }
+void launch_logdet(const int64_t* diag, const int64_t* map, int64_t n, const double* F, double* partial, double* out,
+                   hipStream_t s) {
+  const int parts = logdet_parts(n);
+  if (parts > 0) hipLaunchKernelGGL(logdet_partial_kernel, dim3(parts), dim3(256), 0, s, diag, map, n, F, partial);
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, partial, parts, 2.0, out);
 }
 void launch_permute_in(const double* B, int64_t ldb, const int32_t* perm, int64_t n, int nrhs, double* xy,
                        hipStream_t s) {

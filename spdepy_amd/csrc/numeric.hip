@@ -20,8 +20,9 @@ void launch_scatter(const double* q, const int64_t* map, int64_t nnz, double* F,
 void launch_diag_add(const double* add, const int64_t* diag, const int64_t* map, int64_t n, double* F, hipStream_t s);
 void launch_extend_add(const ExtAddDesc* d, const ExtAddTile* t, int ntiles, const int32_t* relpos, double* F,
                        hipStream_t s);
-void launch_logdet(const NodeDev* nodes, const int2* chunks, int nchunks, const double* F, double* partial,
-                   double* out, hipStream_t s);
+int logdet_parts(int64_t n);
+void launch_logdet(const int64_t* diag, const int64_t* map, int64_t n, const double* F, double* partial, double* out,
+                   hipStream_t s);
 void launch_permute_in(const double* B, int64_t ldb, const int32_t* perm, int64_t n, int nrhs, double* xy,
                        hipStream_t s);
 void launch_permute_out(const double* xy, const int32_t* perm, int64_t n, int nrhs, double* B, int64_t ldb,
@@ -699,6 +700,13 @@ int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int 
   TRY(upload(&pl->d_indptr, pl->st.indptr));
   TRY(upload(&pl->d_indices, pl->st.indices));
   TRY(upload(&pl->d_code, pl->st.code));
+  // the diagonal add and the log-determinant read F[map[diag[i]]] for every variable
+  if ((int64_t)pl->st.diag.size() != sym.n) TRY((set_error("diagonal positions do not match the order"), SPDE_ERR_STATE));
+  for (int64_t i = 0; i < sym.n; ++i) {
+    const int64_t e = pl->st.diag[i];
+    if (e < 0 || e >= (int64_t)pl->map.size() || pl->map[e] < 0 || pl->map[e] >= sym.front_doubles)
+      TRY((set_error("diagonal entry outside the fronts"), SPDE_ERR_STATE));
+  }
   TRY(upload(&pl->d_diag, pl->st.diag));
   TRY(upload(&pl->d_map, pl->map));
   TRY(upload(&pl->d_mapsym, pl->mapsym));
@@ -756,14 +764,6 @@ int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int 
   SPDE_HIP(hipMalloc((void**)&pl->d_acan, sizeof(double) * Ns * 9));
   SPDE_HIP(hipMalloc((void**)&pl->d_a0can, sizeof(double) * Ns * 9));
   SPDE_HIP(hipMalloc((void**)&pl->d_qdiag, sizeof(double) * Ns));
-  {
-    // log-det chunks: 256 own columns of one node each, in node order
-    std::vector<int2> ch;
-    for (size_t k = 0; k < sym.nodes.size(); ++k)
-      for (int j0 = 0; j0 < sym.nodes[k].p; j0 += 256) ch.push_back(make_int2((int)k, j0));
-    pl->nldchunks = (int)ch.size();
-    TRY(upload(&pl->d_ldchunks, ch));
-  }
   pl->slots.resize(nslots);
   for (int s = 0; s < nslots; ++s) {
     SPDE_HIP(hipMalloc((void**)&pl->slots[s].front, sizeof(double) * std::max<int64_t>(sym.front_doubles, 1)));
@@ -773,7 +773,7 @@ int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int 
     SPDE_HIP(hipMalloc((void**)&pl->slots[s].z, sizeof(double) * std::max<int64_t>(sym.z_doubles, 1)));
     SPDE_HIP(hipMemset(pl->slots[s].z, 0, sizeof(double) * std::max<int64_t>(sym.z_doubles, 1)));
     SPDE_HIP(hipMalloc((void**)&pl->slots[s].work, sizeof(double) * pl->work_doubles));
-    SPDE_HIP(hipMalloc((void**)&pl->slots[s].partial, sizeof(double) * std::max(pl->nldchunks, 1)));
+    SPDE_HIP(hipMalloc((void**)&pl->slots[s].partial, sizeof(double) * std::max(logdet_parts(sym.n), 1)));
     SPDE_HIP(hipMalloc((void**)&pl->slots[s].ld, sizeof(double)));
     SPDE_HIP(hipMalloc((void**)&pl->slots[s].status, sizeof(int)));
     SPDE_HIP(hipMemset(pl->slots[s].status, 0, sizeof(int)));
@@ -800,7 +800,7 @@ void spde_plan_destroy(spde_plan* pl) {
                     (void*)pl->d_map, (void*)pl->d_mapsym, (void*)pl->d_perm, (void*)pl->d_relpos,
                     (void*)pl->d_bnd, (void*)pl->d_nodes, (void*)pl->d_tmpl, (void*)pl->d_acan,
                     (void*)pl->d_a0can, (void*)pl->d_qdiag, (void*)pl->d_win, (void*)pl->d_red, (void*)pl->d_pos25,
-                    (void*)pl->d_sprow, (void*)pl->d_spcol, (void*)pl->d_gpart, (void*)pl->d_ldchunks})
+                    (void*)pl->d_sprow, (void*)pl->d_spcol, (void*)pl->d_gpart})
       hipFree(p);
     for (auto& s : pl->slots) {
       hipFree(s.front);
@@ -937,7 +937,7 @@ int spde_factor_async(spde_plan* pl, int slot, const double* qval, const double*
     for (int c = 0; c < 2; ++c) launch_extend_add(L.d_ext[c], L.d_exttile[1][c], L.next[1][c], pl->d_relpos, sl.front, s);
   }
   for (auto& g : pl->trtri) run_gemm(pl, g, b, s);
-  launch_logdet(pl->d_nodes, pl->d_ldchunks, pl->nldchunks, sl.front, sl.partial, sl.ld, s);
+  launch_logdet(pl->d_diag, pl->d_map, sym.n, sl.front, sl.partial, sl.ld, s);
   SPDE_HIP(hipGetLastError());
   sl.stream = s;
   sl.factored = true;  // a non-positive pivot is reported by spde_factor_finish
