@@ -295,6 +295,16 @@ int spde_fields_contract_workspace(int64_t Ns, int nb, int64_t* doubles);
 int spde_fields_contract(int64_t Ns, int nb, const double* bs, const double* bsH, const double* bsA,
                          const double* cells, const double* kappa, const double* fgv, double* work,
                          double* out, void* stream);
+/* No counterpart in the reference: spde_fields_contract for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops), with
+ * no `work` argument.  Its workspace is a static device array of
+ * (240 + 1) * 424 doubles (0.8 MB) that each device holds until the process
+ * ends -- the largest the contraction can use, so every (Ns, nb) fits; a larger
+ * need would return SPDE_ERR_ARG.  Calls allocate nothing.  All calls on one
+ * device share the array: calls that overlap on one device race. */
+int spde_fields_contract_noalloc(int64_t Ns, int nb, const double* bs, const double* bsH,
+                                 const double* bsA, const double* cells, const double* kappa,
+                                 const double* fgv, double* out, void* stream);
 
 /* ---------------------------------------------------------------- measurement
  * Between spde_prof_begin and spde_prof_end every FP64 GEMM launch of the

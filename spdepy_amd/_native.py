@@ -80,6 +80,7 @@ _SIGS = {
     "spde_fields_eval": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int] + [vp] * 8 + [vp]),
     "spde_fields_contract_workspace": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, c_i64p]),
     "spde_fields_contract": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int] + [vp] * 8 + [vp]),
+    "spde_fields_contract_noalloc": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int] + [vp] * 7 + [vp]),
     "spde_prune": (ctypes.c_int, [vp, vp, vp, vp, vp, c_i64p, vp]),
     "spde_spmv": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int, vp]),
     "spde_factor": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, c_dp, vp]),

@@ -12,7 +12,20 @@
 // Bases are dense row-major: bs[Ns][nb], bsH[Ns][4][nb], bsA[Ns][4][nb].
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+#include <vector>
+
 #include "internal.hpp"
+
+// Static scratch of spde_fields_contract_noalloc: the largest workspace the
+// contraction can ask for (at most 240 block partial rows of 6 * 64 + 40
+// doubles, plus their sum), so no call allocates.  Each device holds its own
+// copy until the process ends; all calls on one device share it (calls that
+// overlap on one device race).  Host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device.
+constexpr int64_t kContractScratch = (240 + 1) * (6 * 64 + 40);
+static __device__ __align__(256) double scratch[kContractScratch];
 
 namespace spde {
 
@@ -70,6 +83,7 @@ __global__ __launch_bounds__(256) void fields_kernel(int64_t Ns, int nb, const d
 }
 
 constexpr int kCells = 40;  // columns of the per-cell adjoints
+constexpr int kContractWidth = 6 * 64 + kCells;  // one block's partial row (doubles)
 
 // Block partials of the contraction over a contiguous chunk of cells.
 // Thread (part, j): basis column j0 + j, every 4th cell of the chunk.
@@ -125,6 +139,54 @@ __global__ void contract_sum_kernel(int nblk, const double* __restrict__ partial
   out[t] = s;
 }
 
+
+// Contraction shared by both entry points.  work: partials of up to 240
+// blocks, then the summed row (kContractWidth doubles each).
+int contract_impl(int64_t Ns, int nb, const double* bs, const double* bsH, const double* bsA, const double* cells,
+                  const double* kappa, const double* fgv, double* work, double* out, hipStream_t s) {
+  const int64_t nblk = contract_blocks(Ns);
+  const int64_t chunk = (Ns + nblk - 1) / nblk;
+  double* partial = work;
+  double* sums = work + nblk * kContractWidth;
+  const int width = kContractWidth;
+  for (int j0 = 0; j0 < nb; j0 += 64) {
+    hipLaunchKernelGGL(contract_kernel, dim3((unsigned)nblk), dim3(256), 0, s, Ns, nb, j0, chunk, bs, bsH, bsA,
+                       cells, kappa, fgv, partial);
+    hipLaunchKernelGGL(contract_sum_kernel, dim3((width + 255) / 256), dim3(256), 0, s, (int)nblk, partial, sums);
+    // out layout: [kappa(nb) | gamma(nb) | vx(nb) | vy(nb) | wx(nb) | wy(nb) | cell column sums(40)]
+    const int w = (nb - j0) < 64 ? (nb - j0) : 64;
+    for (int q = 0; q < 6; ++q)
+      SPDE_HIP(hipMemcpyAsync(out + q * nb + j0, sums + q * 64, sizeof(double) * w, hipMemcpyDeviceToDevice, s));
+    if (j0 == 0)
+      SPDE_HIP(hipMemcpyAsync(out + 6 * nb, sums + 6 * 64, sizeof(double) * kCells, hipMemcpyDeviceToDevice, s));
+  }
+  SPDE_HIP(hipGetLastError());
+  return SPDE_OK;
+}
+
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;  // device address of `scratch`, per device
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.
+int contract_scratch(double** ptr) {
+  int dev = 0;
+  SPDE_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    SPDE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      set_error("spde_fields_contract_noalloc: no device address for the scratch array");
+      return SPDE_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return SPDE_OK;
+}
+
 }  // namespace spde
 
 using namespace spde;
@@ -150,7 +212,8 @@ int spde_fields_contract_workspace(int64_t Ns, int nb, int64_t* doubles) {
     return SPDE_ERR_ARG;
   }
   const int64_t nblk = contract_blocks(Ns);
-  *doubles = nblk * (6 * 64 + kCells) + (6 * 64 + kCells);
+  static_assert(kContractScratch == (240 + 1) * kContractWidth, "scratch holds the largest workspace");
+  *doubles = nblk * kContractWidth + kContractWidth;
   return SPDE_OK;
 }
 
@@ -161,25 +224,26 @@ int spde_fields_contract(int64_t Ns, int nb, const double* bs, const double* bsH
     set_error("spde_fields_contract: invalid argument");
     return SPDE_ERR_ARG;
   }
-  hipStream_t s = (hipStream_t)stream;
-  const int64_t nblk = contract_blocks(Ns);
-  const int64_t chunk = (Ns + nblk - 1) / nblk;
-  double* partial = work;
-  double* sums = work + nblk * (6 * 64 + kCells);
-  const int width = 6 * 64 + kCells;
-  for (int j0 = 0; j0 < nb; j0 += 64) {
-    hipLaunchKernelGGL(contract_kernel, dim3((unsigned)nblk), dim3(256), 0, s, Ns, nb, j0, chunk, bs, bsH, bsA,
-                       cells, kappa, fgv, partial);
-    hipLaunchKernelGGL(contract_sum_kernel, dim3((width + 255) / 256), dim3(256), 0, s, (int)nblk, partial, sums);
-    // out layout: [kappa(nb) | gamma(nb) | vx(nb) | vy(nb) | wx(nb) | wy(nb) | cell column sums(40)]
-    const int w = (nb - j0) < 64 ? (nb - j0) : 64;
-    for (int q = 0; q < 6; ++q)
-      SPDE_HIP(hipMemcpyAsync(out + q * nb + j0, sums + q * 64, sizeof(double) * w, hipMemcpyDeviceToDevice, s));
-    if (j0 == 0)
-      SPDE_HIP(hipMemcpyAsync(out + 6 * nb, sums + 6 * 64, sizeof(double) * kCells, hipMemcpyDeviceToDevice, s));
+  return contract_impl(Ns, nb, bs, bsH, bsA, cells, kappa, fgv, work, out, (hipStream_t)stream);
+}
+
+int spde_fields_contract_noalloc(int64_t Ns, int nb, const double* bs, const double* bsH, const double* bsA,
+                                 const double* cells, const double* kappa, const double* fgv, double* out,
+                                 void* stream) {
+  if (Ns <= 0 || nb <= 0 || !bs || !bsH || !bsA || !cells || !kappa || !fgv || !out) {
+    set_error("spde_fields_contract_noalloc: invalid argument");
+    return SPDE_ERR_ARG;
   }
-  SPDE_HIP(hipGetLastError());
-  return SPDE_OK;
+  const int64_t need = contract_blocks(Ns) * kContractWidth + kContractWidth;
+  if (need > kContractScratch) {
+    set_error("spde_fields_contract_noalloc: the contraction needs " + std::to_string(need) +
+              " doubles of scratch, more than the static maximum of " + std::to_string(kContractScratch));
+    return SPDE_ERR_ARG;
+  }
+  double* work = nullptr;
+  const int rc = contract_scratch(&work);
+  if (rc != SPDE_OK) return rc;
+  return contract_impl(Ns, nb, bs, bsH, bsA, cells, kappa, fgv, work, out, (hipStream_t)stream);
 }
 
 }  // extern "C"
