@@ -386,6 +386,12 @@ class VarAdvVarDiff(_STBase):
                 + vv[:, :, :, None] * vv[:, :, None, :])
 
     def makeQ(self, par, grad=True):
+        Q = self._operators(par)[0]
+        if not grad:
+            return Q, None
+        return Q, list(self.dQ_iter(par))
+
+    def _operators(self, par):
         g, Np = self.grid, self.Np
         par = np.asarray(par, dtype=np.float64)
         dt, T, Ns, Dv, iDv = g.dt, g.T, g.Ns, g.Dv, g.iDv
@@ -397,21 +403,27 @@ class VarAdvVarDiff(_STBase):
         Qs = As.transpose() @ iDv @ As
         A = Dv + (Dv @ Dk - ah_var_csc(g, Hs, self.bc) + aw_var_csc(g, ws, self.bc)) * dt
         p0 = par[Np * 6 + 1:] if par.size > Np * 6 + 2 else None
-        Q0, dQ0 = self._q0(p0, grad)
+        Q0 = self._q0(p0, False)[0]
         Q = assemble_blocks(A, Qs, Q0, iDv, T, sigma, dt).tocsc() / (dt * sigma)
-        if not grad:
-            return Q, None
+        return Q, A, As, Qs, Q0, kappa, gamma, vx, vy, ws, sigma, p0
+
+    def dQ_iter(self, par):
+        """The per-parameter derivatives dQ_i one at a time, in the parameter
+        order (``var_advection_var_diffusion2D.py:118-197``)."""
+        g, Np = self.grid, self.Np
+        par = np.asarray(par, dtype=np.float64)
+        dt, T, Ns, Dv, iDv = g.dt, g.T, g.Ns, g.Dv, g.iDv
+        Q, A, As, Qs, Q0, kappa, gamma, vx, vy, ws, sigma, p0 = self._operators(par)
         c = dt * sigma
-        dQ = []
         for i in range(Np):
             dk = sparse.diags(g.bs[:, i] * kappa).tocsc()
             dA = dt * Dv @ dk
             dAs = Dv @ dk
             dQs = dAs.T @ iDv @ As + As.T @ iDv @ dAs
-            dQ.append(d_blocks(A, Qs, iDv, T, dA, dQs) / c)
+            yield d_blocks(A, Qs, iDv, T, dA, dQs) / c
         for i in range(Np):
             dHs = np.eye(2) * (np.stack([g.bsH[:, :, i] * gamma] * 2, axis=2)[:, :, :, None])
-            dQ.append(d_blocks(A, Qs, iDv, T, -ah_var_csc(g, dHs, self.bc) * dt) / c)
+            yield d_blocks(A, Qs, iDv, T, -ah_var_csc(g, dHs, self.bc) * dt) / c
         vv = np.stack([vx, vy], axis=2)
         for comp in (0, 1):
             for i in range(Np):
@@ -421,18 +433,18 @@ class VarAdvVarDiff(_STBase):
                 z = np.zeros_like(b)
                 dv = np.stack([b, z] if comp == 0 else [z, b], axis=2)
                 dHs = vv[:, :, :, None] * dv[:, :, None, :] + dv[:, :, :, None] * vv[:, :, None, :]
-                dQ.append(d_blocks(A, Qs, iDv, T, -ah_var_csc(g, dHs, self.bc) * dt) / c)
+                yield d_blocks(A, Qs, iDv, T, -ah_var_csc(g, dHs, self.bc) * dt) / c
         for i in range(2 * Np):
             e = np.zeros(2 * Np)
             e[i] = 1.0
             dws = g.evalAdv(e)
             diff = 1 if i < Np else 2
-            dQ.append(d_blocks(A, Qs, iDv, T, aw_var_csc(g, ws, self.bc, diff, dws) * dt) / c)
+            yield d_blocks(A, Qs, iDv, T, aw_var_csc(g, ws, self.bc, diff, dws) * dt) / c
         full = assemble_blocks(A, Qs, Q0, iDv, T, sigma, dt, diag0=Qs)
-        dQ.append(-full.tocsc() / c)
-        if dQ0 is not None:
-            dQ += [pad_q0(d, Ns, T) for d in dQ0]
-        return Q, dQ
+        yield -full.tocsc() / c
+        if p0 is not None:
+            for d in self._q0(p0, True)[1]:
+                yield pad_q0(d, Ns, T)
 
     def spatial_operator(self, par):
         """A (Ns x Ns) and q = diag(Qs) for the factorised form."""
@@ -693,6 +705,137 @@ def loglike(Q, dQ, S, data, log_tau, trace="exact", probes=None, dense_inverse=N
             g[i] = 0.5 * ((TrQ - TrQc) * (d @ Vp)).sum() * r / nh1 - 0.5 * (mu * (d @ mu)).sum()
         g[-1] = (m * r / 2 - 0.5 * (TrQc * (StS * tau @ Vp)).sum() * r / nh1
                  - tau / 2 * (resid ** 2).sum())
+    return -like / (m * r), -g / (m * r), mu
+
+
+def qinv_columns(A, q, A0, V, T, sigma, dt, cols):
+    """W = Q^{-1} S^T for the unit columns S^T e_k = e_{cols[k]}, through the
+    factorised form Q = F^T D F (``sample_fform``'s F and D):
+        F^T u = e:   u_t = F_tt^{-T} (e_t + u_{t+1})   (t = T-1 .. 0),
+        F w = D^{-1} u:  w_0 = F_00^{-1} u_0,  w_t = F_tt^{-1} (D_t^{-1} u_t + w_{t-1}),
+    with F_00 = V^{-1/2} A0, F_tt = A / V, D_0 = I, D_t = q / (dt sigma), using
+    dense inverses of the two spatial operators (Ns x Ns).  Returns W (n x m)."""
+    Ns = A.shape[0]
+    n = Ns * T
+    cols = np.asarray(cols)
+    m = cols.size
+    iA = np.linalg.inv(A.toarray())
+    iA0 = np.linalg.inv(A0.toarray())
+    iF = [np.sqrt(V) * iA0] + [V * iA] * (T - 1)
+    W = np.zeros((n, m))
+    cell, slc = cols % Ns, cols // Ns
+    carry = np.zeros((Ns, m))
+    for t in range(T - 1, -1, -1):
+        e = carry
+        sel = np.nonzero(slc == t)[0]
+        e[cell[sel], sel] += 1.0
+        carry = iF[t].T @ e
+        W[t * Ns:(t + 1) * Ns] = carry
+    dinv = np.concatenate([np.ones(Ns), np.tile(dt * sigma / q, T - 1)])
+    prev = np.zeros((Ns, m))
+    for t in range(T):
+        blk = W[t * Ns:(t + 1) * Ns] * dinv[t * Ns:(t + 1) * Ns, None]
+        prev = iF[t] @ (blk + (prev if t > 0 else 0.0))
+        W[t * Ns:(t + 1) * Ns] = prev
+    return W
+
+
+def pattern_product(E, M, N, T, rows_block=2048):
+    """G = E E^T on the 43-point space-time pattern of Q: a dict mapping each
+    lattice offset (dx, dy, dt) (|dx|, |dy| <= 2 in the same slice, <= 1 in
+    the neighbouring slices) to an array g[a] = (E E^T)[a, a + offset] (NaN
+    where a + offset leaves the lattice).  Blocked dense products over
+    consecutive lattice rows."""
+    n, m = E.shape
+    Ns = M * N
+    offs = [(dx, dy, 0) for dy in range(-2, 3) for dx in range(-2, 3)]
+    offs += [(dx, dy, d) for d in (-1, 1) for dy in range(-1, 2) for dx in range(-1, 2)]
+    out = {o: np.full(n, np.nan) for o in offs}
+    a_all = np.arange(n)
+    ia, ja, ta = a_all % M, (a_all // M) % N, a_all // Ns
+    reach = 2 * M + 2
+    for a0 in range(0, n, rows_block):
+        a1 = min(n, a0 + rows_block)
+        Ea = E[a0:a1]
+        for d in (-1, 0, 1):
+            b0 = max(0, a0 + d * Ns - reach)
+            b1 = min(n, a1 + d * Ns + reach)
+            if b0 >= b1:
+                continue
+            P = Ea @ E[b0:b1].T
+            for o in offs:
+                if o[2] != d:
+                    continue
+                dx, dy, dtt = o
+                a = a_all[a0:a1]
+                ok = ((ia[a0:a1] + dx >= 0) & (ia[a0:a1] + dx < M) & (ja[a0:a1] + dy >= 0)
+                      & (ja[a0:a1] + dy < N) & (ta[a0:a1] + dtt >= 0) & (ta[a0:a1] + dtt < T))
+                b = a + dx + dy * M + dtt * Ns
+                vals = np.full(a1 - a0, np.nan)
+                vals[ok] = P[np.nonzero(ok)[0], b[ok] - b0]
+                out[o][a0:a1] = vals
+    return out
+
+
+def pattern_inner(G, D, M, N):
+    """<D, G> for a sparse D whose entries lie on the 43-point pattern, G from
+    ``pattern_product``."""
+    D = D.tocoo()
+    Ns = M * N
+    a, b = D.row.astype(np.int64), D.col.astype(np.int64)
+    dx = (b % M) - (a % M)
+    dy = ((b // M) % N) - ((a // M) % N)
+    dtt = b // Ns - a // Ns
+    tot = 0.0
+    keys = dx + 5 * dy + 25 * dtt
+    for key in np.unique(keys):
+        sel = keys == key
+        o = (int(dx[sel][0]), int(dy[sel][0]), int(dtt[sel][0]))
+        g = G[o][a[sel]]
+        if np.isnan(g).any():
+            raise ValueError("entry outside the lattice pattern")
+        tot += float(np.dot(D.data[sel], g))
+    return tot
+
+
+def loglike_woodbury(Q, dQ, A, q, A0, V, T, sigma, dt, M, N, cols, data, log_tau):
+    """The exact log-likelihood and gradient of ``loglike`` (trace="exact") by
+    the Woodbury identity instead of factorising Q_c (no reference
+    counterpart; it reproduces ``var_advection_var_diffusion2D.py:201-237``
+    with the exact trace):
+        W = Q^{-1} S^T,  K = I_m / tau + S W,   Q_c^{-1} = Q^{-1} - W K^{-1} W^T,
+        log|Q_c| = log|Q| + m log tau + log|K|,   mu_c = W K^{-1} y,
+        tr((Q^{-1} - Q_c^{-1}) dQ_i) = <dQ_i, W K^{-1} W^T>,
+        tr(Q_c^{-1} tau S^T S) = tau sum_k [(S W)_kk - (W K^{-1} W^T)_{c_k c_k}],
+    log|Q| from the closed form.  dQ may be a generator.  Returns
+    (-like/(m r), -g/(m r), mu)."""
+    from scipy import linalg as sla
+    tau = np.exp(log_tau)
+    y = np.asarray(data, dtype=np.float64).reshape(len(cols), -1)
+    m, r = y.shape
+    cols = np.asarray(cols)
+    ld_q = logdet_closed_form(A, q, A0, V, T, sigma, dt)
+    W = qinv_columns(A, q, A0, V, T, sigma, dt, cols)
+    SW = W[cols]
+    K = 0.5 * (SW + SW.T) + np.eye(m) / tau
+    LK = np.linalg.cholesky(K)
+    ld_qc = ld_q + m * np.log(tau) + 2.0 * np.log(np.diag(LK)).sum()
+    mu = W @ sla.cho_solve((LK, True), y)
+    resid = y - mu[cols]
+    like = (0.5 * ld_q * r + m * r * np.log(tau) / 2 - 0.5 * ld_qc * r
+            - 0.5 * (mu * (Q @ mu)).sum() - tau / 2 * (resid ** 2).sum())
+    # E = W L_K^{-T}: E E^T = W K^{-1} W^T
+    iLK = sla.solve_triangular(LK, np.eye(m), lower=True)
+    for a0 in range(0, W.shape[0], 8192):
+        W[a0:a0 + 8192] = W[a0:a0 + 8192] @ iLK.T
+    G = pattern_product(W, M, N, T)
+    g = []
+    for d in dQ:
+        g.append(0.5 * r * pattern_inner(G, d, M, N) - 0.5 * (mu * (d @ mu)).sum())
+    diag_g = G[(0, 0, 0)][cols]
+    trc = tau * float((np.diag(SW) - diag_g).sum())
+    g.append(m * r / 2 - 0.5 * trc * r - tau / 2 * (resid ** 2).sum())
+    g = np.asarray(g)
     return -like / (m * r), -g / (m * r), mu
 
 

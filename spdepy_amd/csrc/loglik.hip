@@ -175,7 +175,10 @@ extern "C" int spde_loglik_grad(spde_plan* pl, spde_plan* sp, const spde_coeffs*
   out->like = -like / ((double)m * r);
   out->logdet_q = ld_q;
   out->logdet_qc = ld_qc;
-  out->g_sigma = -(dotgq - dot00) - 0.5 * r * (T - 1) * (double)Ns;
+  // <G, Q> without its cancellation: tr(Q_c^-1 Q) = n - tr(Q_c^-1 tau S^T S)
+  (void)dotgq;
+  const double dot_gq = -0.5 * r * ((double)n - tautr) - 0.5 * quad;
+  out->g_sigma = -(dot_gq - dot00) - 0.5 * r * (T - 1) * (double)Ns;
   out->g_tau = 0.5 * (double)m * r - 0.5 * r * tautr - 0.5 * tau * rss;
   return SPDE_OK;
 }

@@ -135,6 +135,13 @@ class Engine:
         # Q is symmetric: its CSR arrays are the CSC arrays of Q^T = Q
         return sparse.csc_matrix((data.cpu().numpy(), indices.cpu().numpy(), indptr.cpu().numpy()), shape=(n, n))
 
+    def iterative(self):
+        """The IC(0) / PCG context of the space-time plan (created once)."""
+        if getattr(self, "_iter", None) is None:
+            from .krylov import IterativeSolver
+            self._iter = IterativeSolver(self.plan)
+        return self._iter
+
     def _side_stream(self, torch):
         dev = torch.cuda.current_device()
         if getattr(self, "_side", None) is None or self._side_dev != dev:
@@ -290,7 +297,14 @@ class Engine:
             parts["cells"] = cells
             colsum = cells.sum(0)
         parts["colsum"] = colsum
-        parts["g_sigma"] = -(colsum[OUT_DOTGQ] - colsum[OUT_DOT00])
+        dot_gq = colsum[OUT_DOTGQ]
+        if exact:
+            # <G, Q> without its cancellation: tr(Q_c^-1 Q) = n - tr(Q_c^-1 tau S^T S)
+            # (the trace term falls from ~n to O(m) in the sum over all entries)
+            dot_gq = -0.5 * r * (n - colsum[OUT_TAUTR]) - 0.5 * quad
+            if not closed:
+                dot_gq += 0.5 * r * n      # + 1/2 r tr(Q^-1 Q)
+        parts["g_sigma"] = -(dot_gq - colsum[OUT_DOT00])
         if exact and closed:
             # 1/2 r d log|Q| / d log sigma = -1/2 r (T-1) Ns
             parts["g_sigma"] -= 0.5 * r * (T - 1) * self.plan.shape[0] * self.plan.shape[1]

@@ -4,7 +4,8 @@ advection-diffusion, 30 x 30 cells x 10 steps, n = 9,000, bc = 1,
 par = [log 1, log 0.5, 0.3, -0.2, log 1, log 100], mod0 = isotropic
 Whittle-Matern [0, log 0.5, 0], one seeded sample, 200 observations with
 noise sd 0.1: ``oracle.fixtures.stationary()``), with the observation
-indices and values, and checksums of Q.  The exact trace uses dense inverses
+indices and values, Q's pattern (indptr, indices of the sorted CSR) and
+checksums of its values.  The exact trace uses dense inverses
 of Q and Q_c (n = 9,000: about a minute on 8 cores).  Run in the CPU
 container:
 
@@ -30,7 +31,8 @@ def main():
     like, jac, _ = Mo.loglike(Q.tocsc(), dQ, S, c["y"], c["par"][-1], trace="exact")
     np.savez_compressed(os.path.join(HERE, "config1.npz"), par=c["par"], idx=c["idx"], y=c["y"],
                         like=np.array(like), jac=jac, q_nnz=np.array(Q.nnz),
-                        q_abs_sum=np.array(np.abs(Q.data).sum()), q_diag=Q.diagonal())
+                        q_abs_sum=np.array(np.abs(Q.data).sum()), q_diag=Q.diagonal(),
+                        q_indptr=Q.indptr.astype(np.int32), q_indices=Q.indices.astype(np.int32))
     print("like", like, "jac", jac)
 
 

@@ -26,6 +26,9 @@ def test_config1_q_loglike_gradient():
     Q = Q.tocsr()
     Q.sort_indices()
     assert Q.nnz == int(GOLD["q_nnz"])
+    # pattern bit-exact: the reference's scipy pattern (exact zeros dropped)
+    assert np.array_equal(Q.indptr, GOLD["q_indptr"])
+    assert np.array_equal(Q.indices, GOLD["q_indices"])
     assert np.allclose(Q.diagonal(), GOLD["q_diag"], rtol=1e-12, atol=0)
     qs = float(GOLD["q_abs_sum"])
     assert abs(np.abs(Q.data).sum() - qs) <= 1e-12 * qs
