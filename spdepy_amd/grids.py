@@ -104,6 +104,53 @@ class _Lattice2D:
         self.Dv = self.V * sparse.eye(self.Ns)
         self.iDv = sparse.eye(self.Ns) / self.V
 
+    # ------------------------------------------------ covariates in S
+    def addCov(self, cov, inter=True, scale=False):
+        """Dense covariate columns appended to S (``spat2Dtemp_regular_mesh.py:
+        60-64,35-51``): with an intercept [1, cov] (or [1, cov / cov.max()]
+        when scale), without one [cov].  cov holds one value per interior
+        cell (in the interior index order of getS)."""
+        self.inter = bool(inter)
+        self.scale = bool(scale)
+        self.cov = np.asarray(cov, dtype=np.float64).reshape(-1)
+        self.S = None
+
+    def addInt(self):
+        """An intercept column appended to S (``:66-68``)."""
+        self.inter = True
+        self.S = None
+
+    def cov_columns(self):
+        """The dense columns getS appends (n_interior x k), or None."""
+        cov = getattr(self, "cov", None)
+        inter = getattr(self, "inter", False)
+        if cov is not None:
+            if inter:
+                return np.stack([np.ones(cov.size), cov / cov.max() if getattr(self, "scale", False) else cov], 1)
+            return cov.reshape(-1, 1)
+        if inter:
+            return np.ones((self._n_interior(), 1))
+        return None
+
+    def _with_cov(self, S, rows):
+        X = self.cov_columns()
+        if X is None:
+            return S
+        return sparse.hstack([S, sparse.csc_matrix(X[rows])]).tocsc()
+
+    def extend(self, extend=1):
+        """Border extension by `extend` cells on every side (``:102-117``):
+        extended coordinates, lattice sizes and the centre and diffusion-face
+        bases (the advection faces keep the interior support, as in the
+        reference).  The volume matrices follow the new size."""
+        self._extend(extend)
+        self._set_sizes()
+        self.basisN()
+        self.basisH()
+        self.Ae = None
+        self.S = None
+        self.setDv()
+
 
 class SpatialGrid(_Lattice2D):
     """2-D grid (``spatial2D_regular_mesh.py``); used for the initial-state model."""
@@ -111,6 +158,14 @@ class SpatialGrid(_Lattice2D):
 
     def __init__(self):
         self.S = None
+        self.Ae = None
+        self.cov, self.inter, self.scale = None, False, False
+
+    def _n_interior(self):
+        return self.M * self.N
+
+    def _set_sizes(self):
+        self.Ns = self.n = (self.M + 2 * self.Ne) * (self.N + 2 * self.Ne)
 
     def setGrid(self, x, y, extend=None, Nbs=3):
         self._space(x, y, Nbs)
@@ -124,6 +179,13 @@ class SpatialGrid(_Lattice2D):
             self.basisH()
         self.setDv()
 
+    def getIdx(self, pos, extend=True):
+        """Lattice index of the cell (idx x, idx y) (``spatial2D_regular_mesh.py:64-76``):
+        on the extended lattice when extend, else on the interior grid."""
+        if extend:
+            return (pos[0] + self.Ne) + (pos[1] + self.Ne) * (self.M + 2 * self.Ne)
+        return pos[0] + pos[1] * self.M
+
     @property
     def shape(self):
         return [self.M + 2 * self.Ne, self.N + 2 * self.Ne]
@@ -134,8 +196,9 @@ class SpatialGrid(_Lattice2D):
 
     def getS(self, idxs=None):
         rows = np.arange(self.M * self.N) if idxs is None else np.asarray(idxs)
-        return sparse.csc_matrix((np.ones(rows.size), (np.arange(rows.size), self.n2e(rows))),
-                                 shape=(rows.size, int(np.prod(self.shape))))
+        S = sparse.csc_matrix((np.ones(rows.size), (np.arange(rows.size), self.n2e(rows))),
+                              shape=(rows.size, int(np.prod(self.shape))))
+        return self._with_cov(S, rows)
 
 
 class Grid(_Lattice2D):
@@ -145,6 +208,14 @@ class Grid(_Lattice2D):
     def __init__(self):
         self.S = None
         self.Ae = None
+        self.cov, self.inter, self.scale = None, False, False
+
+    def _n_interior(self):
+        return self.M * self.N * self.T
+
+    def _set_sizes(self):
+        self.Ns = (self.M + 2 * self.Ne) * (self.N + 2 * self.Ne)
+        self.n = self.Ns * self.T
 
     def setGrid(self, x, y, t, extend=None, Nbs=3):
         self._space(x, y, Nbs)
@@ -152,6 +223,9 @@ class Grid(_Lattice2D):
         self.T = self.t.size
         self.Tdur = self.t.max() - self.t.min()
         self.dt = self.Tdur / (self.T - 1)
+        # interior cell coordinates of index i + j*M + t*M*N (``:137-141``)
+        iy, it, ix = np.meshgrid(np.arange(self.N), np.arange(self.T), np.arange(self.M))
+        self.ix, self.iy, self.it = ix.ravel(), iy.ravel(), it.ravel()
         self.basisN()
         self.basisH()
         self.basisA()   # advection faces keep the interior support (not rebuilt on extension)
@@ -178,9 +252,20 @@ class Grid(_Lattice2D):
         return (rem % self.M + self.Ne) + (rem // self.M + self.Ne) * Me + t * Me * Nee
 
     def getS(self, idxs=None):
+        """Observation operator of the interior indices idxs (all when None),
+        with the covariate columns of addCov / addInt appended
+        (``spat2Dtemp_regular_mesh.py:35-58``)."""
         rows = np.arange(self.M * self.N * self.T) if idxs is None else np.asarray(idxs)
-        return sparse.csc_matrix((np.ones(rows.size), (np.arange(rows.size), self.n2e(rows))),
-                                 shape=(rows.size, self.n))
+        S = sparse.csc_matrix((np.ones(rows.size), (np.arange(rows.size), self.n2e(rows))),
+                              shape=(rows.size, self.n))
+        return self._with_cov(S, rows)
+
+    def getIdx(self, pos, extend=True):
+        """Lattice index of the cell (idx x, idx y, idx t) (``:73-84``): on the
+        extended lattice when extend, else the interior index."""
+        if extend:
+            return (pos[0] + self.Ne) + (pos[1] + self.Ne) * (self.M + 2 * self.Ne) + pos[2] * self.Ns
+        return pos[0] + pos[1] * self.M + pos[2] * self.M * self.N
 
     def ext_map(self):
         """Border extrapolation of advection faces (``advBound :347-391``):

@@ -104,10 +104,11 @@ class Plan:
 
     def factor(self, slot, qval, diag_add=None, owner=None):
         import torch
-        self.slot_owner[slot] = owner
         ld = ctypes.c_double()
+        self.slot_owner.pop(slot, None)   # the slot's contents change whatever the outcome
         nat.check(nat.lib().spde_factor(self._h, slot, _ptr(qval), _ptr(diag_add), ctypes.byref(ld),
                                         _stream(torch)))
+        self.slot_owner[slot] = owner     # only a successful factorisation has an owner
         return ld.value
 
     def assemble_spatial(self, st_plan, which, out=None):
@@ -130,12 +131,16 @@ class Plan:
         """Enqueues the factorisation on the current stream; factor_finish
         waits for it and returns the log-determinant."""
         import torch
-        self.slot_owner[slot] = owner
+        self.slot_owner.pop(slot, None)
         nat.check(nat.lib().spde_factor_async(self._h, slot, _ptr(qval), _ptr(diag_add), _stream(torch)))
+        self._pending_owner = getattr(self, "_pending_owner", {})
+        self._pending_owner[slot] = owner
 
     def factor_finish(self, slot):
         ld = ctypes.c_double()
+        owner = getattr(self, "_pending_owner", {}).pop(slot, None)
         nat.check(nat.lib().spde_factor_finish(self._h, slot, ctypes.byref(ld)))
+        self.slot_owner[slot] = owner
         return ld.value
 
     def solve(self, slot, B, mode=nat.SOLVE_A):
