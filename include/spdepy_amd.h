@@ -306,6 +306,120 @@ int spde_fields_contract_noalloc(int64_t Ns, int nb, const double* bs, const dou
                                  const double* bsA, const double* cells, const double* kappa,
                                  const double* fgv, double* out, void* stream);
 
+/* ---------------------------------------------------------------- iterative path
+ * Mixed-precision solves with Q_c = Q + diag(diag_add) on a space-time plan
+ * (the pattern of the advection-diffusion models), replacing the factorisation
+ * of Q_c that the reference hands to CHOLMOD (advection_idiffusion2D.py:173-189,
+ * var_advection_var_diffusion2D.py:205-221): IC(0) in an 18-colour ordering
+ * ((i mod 3) + 3 (j mod 3) + 9 (t mod 2)), FP32 preconditioned conjugate
+ * gradients, FP64 iterative refinement, Lanczos coefficients for stochastic
+ * log-determinants, Rademacher probes and the Hutchinson estimate of Q_c^-1
+ * on the pattern.  Vectors with k right-hand sides are row-major n x k floats
+ * (k a power of two <= 64). */
+typedef struct spde_ic spde_ic;
+int spde_ic_create(spde_plan* plan, spde_ic** ic);
+void spde_ic_destroy(spde_ic* ic);
+/* nlow: entries of the strictly lower factor (one per off-diagonal pair). */
+int spde_ic_info(const spde_ic* ic, int64_t* nlow, int32_t* ncolours);
+/* Host copies (any may be NULL): colour[n]; lower lists lptr[n+1], lcol[nlow]
+ * (row i: columns of lower colour, by (colour, index)); lch[nlow] = cell * 22
+ * + channel of each lower entry in the correction layout (-1 none); upper
+ * lists uptr[n+1], ucol[nlow]. */
+int spde_ic_export(const spde_ic* ic, int32_t* colour, int64_t* lptr, int32_t* lcol, int64_t* lch,
+                   int64_t* uptr, int32_t* ucol);
+/* IC(0) of Q_c with its diagonal scaled by (1 + shift); *logdet_m (host) =
+ * log|L L^T|.  qval, diag_add (may be NULL) device, kept by reference until the
+ * next factorisation.  SPDE_ERR_NOT_SPD names the lattice index of a
+ * non-positive pivot. */
+int spde_ic_factor(spde_ic* ic, const double* qval, const double* diag_add, double shift, double* logdet_m,
+                   void* stream);
+/* Correction of the factor in the Jacobi-scaled space (D = diag(Q_c)):
+ * L~ = D^-1/2 L; L~_ij += corr[cell * 22 + ch] for the pair channel of each
+ * lower entry, L~_ii *= exp(corr[i * 22]).  corr (device float, n * 22) is
+ * the learned preconditioner's output; call once after spde_ic_factor
+ * (NULL: refresh the FP32 copies only).  *logdet_m = log|L L^T|. */
+int spde_ic_correct(spde_ic* ic, const float* corr, double* logdet_m, void* stream);
+/* Device copies of the FP64 factor: lower[nlow], diag[n] (L_ii), adiag[n]
+ * (the diagonal of Q_c); any may be NULL. */
+int spde_ic_values(spde_ic* ic, double* lower, double* diag, double* adiag, void* stream);
+/* z = (L L^T)^-1 r (FP32); which = 1: out = L^-1 in, which = 2: out = L^-T in. */
+int spde_ic_apply(spde_ic* ic, const float* r, float* z, int k, void* stream);
+int spde_ic_sweep(spde_ic* ic, int which, const float* in, float* out, int k, void* stream);
+/* y = Q_c x in FP32. */
+int spde_ic_spmm(spde_ic* ic, const float* x, float* y, int k, void* stream);
+/* FP32 PCG for Q_c X = B from X = 0, each right-hand side until
+ * ||r|| <= tol ||b|| or maxit iterations; iters[k] (host) per right-hand side. */
+int spde_pcg_solve(spde_ic* ic, const float* B, float* X, int k, double tol, int maxit, int32_t* iters,
+                   void* stream);
+/* FP64 x = Q_c^-1 b (one right-hand side, device) by iterative refinement:
+ * r = b - Q_c x in FP64, an FP32 PCG solve of Q_c d = r / ||r|| to inner_tol,
+ * x += ||r|| d, until ||r|| <= tol ||b|| or max_rounds; *iters total inner
+ * iterations, *rounds, *relres = ||b - Q_c x|| / ||b|| (host). */
+int spde_pcg_refine(spde_ic* ic, const double* b, double* x, double tol, double inner_tol, int max_rounds,
+                    int maxit, int32_t* iters, int32_t* rounds, double* relres, void* stream);
+/* V[u] = +-scale for u = i * k + r: the sign is the top bit of
+ * splitmix64(seed ^ splitmix64(u)) (1: negative). */
+int spde_rademacher(int64_t n, int k, uint64_t seed, float scale, float* V, void* stream);
+/* Lanczos on L^-1 Q_c L^-T from k Rademacher probes (spde_rademacher with
+ * seed, normalised): alpha[r * steps + j], beta[r * steps + j] (host) for
+ * probe r; FP32 vectors, FP64 coefficients. */
+int spde_slq(spde_ic* ic, int k, int steps, uint64_t seed, double* alpha, double* beta, void* stream);
+/* sigma[e] = (1/k) sum_r (U[i][r] V[j][r] + U[j][r] V[i][r]) / 2 for every
+ * structural entry e = (i, j) (device; U = Q_c^-1 V gives the Hutchinson
+ * estimate of Q_c^-1 on the pattern), written as sigma = beta sigma + alpha
+ * (estimate) so that batches of probes accumulate. */
+int spde_hutch_pattern(spde_ic* ic, const float* U, const float* V, int k, double alpha, double beta,
+                       double* sigma, void* stream);
+
+/* ---------------------------------------------------------------- learned preconditioner
+ * No reference counterpart: a correction of the IC(0) factor predicted by a
+ * 4-layer 3x3 convolutional network over the time slices (8 input channels,
+ * 32 hidden, 32 output of which 22 are used), trained on the GPU by the
+ * gradient of KL(N(0, Q_c^-1) || N(0, (L' L'^T)^-1)) over samples of the
+ * exact Cholesky path.  Activations pixel-major (pixel = lattice index, the
+ * channels contiguous, FP32); layer weights W[k][co] FP32 with k = tap * cin +
+ * ci, tap = 3 (dy + 1) + (dx + 1), packed as BF16 matrix-core fragments.
+ *
+ * feat[n * 8] (device float): per lattice entry [tau count / A_ii, A~ to the
+ * E, W, N, S neighbours, A~ to the same cell in the next and previous slice,
+ * 0] with A = Q_c, A~_ij = A_ij / sqrt(A_ii A_jj), from the last
+ * spde_ic_factor. */
+int spde_ic_features(spde_ic* ic, float* feat, void* stream);
+/* dout[n * 22] (device float) = scale * d KL / d corr at the current
+ * (corrected) factor from k exact samples x (n x k row-major FP64, lattice
+ * order, x ~ N(0, Q_c^-1)); *loss (host, may be NULL) = 1/2 mean ||L'^T x||^2
+ * - sum log L'_ii. */
+int spde_ic_kl_grad(spde_ic* ic, const double* x, int k, double scale, float* dout, double* loss, void* stream);
+/* The training loss: dout[n * 22] = scale * dJ / d corr of the symmetrised
+ * divergence J = KL(p||q) + KL(q||p) = 1/2 E||L'^T x||^2 + 1/2 E[y^T Q_c y] - n,
+ * p = N(0, Q_c^-1), q = N(0, (L' L'^T)^-1), over k exact samples x (n x k
+ * row-major FP64) and y = L'^-T z for k probes z (n x k row-major float);
+ * *loss (host, may be NULL) = J.  Both ends of the spectrum of L'^-1 Q_c L'^-T
+ * are penalised (sum of lambda + 1/lambda - 2). */
+int spde_ic_jeffreys_grad(spde_ic* ic, const double* x, const float* z, int k, double scale, float* dout,
+                          double* loss, void* stream);
+/* Packed BF16 fragment size (in halves) of a layer's operand; transposed = 1
+ * for the backward-data operand (flipped taps, channels swapped). */
+int spde_conv_packed_size(int cin, int transposed, int64_t* halves);
+int spde_conv_pack(int cin, int transposed, const float* W, unsigned short* frag, void* stream);
+/* out[P x 32] = act(conv3x3(in * (mask > 0)) + bias) over the T slices of an
+ * M x N lattice (zero padding); cin 8 or 32 input channels; mask, bias may
+ * be NULL; relu 0/1.  With transposed fragments and mask = the layer's
+ * output this is the backward-data pass through the layer. */
+int spde_conv_forward(int M, int N, int T, int cin, const float* in, const float* mask,
+                      const unsigned short* frag, const float* bias, int relu, float* out, void* stream);
+/* dW[9 cin x 32] and db[32] of a layer from its input X (P x cin) and the
+ * gradient G (P x 32) of its output, masked by Y > 0 (the ReLU) when Y is not
+ * NULL; operands rounded to BF16 as in the forward pass.  work: device
+ * floats, count from spde_conv_wgrad_workspace. */
+int spde_conv_wgrad_workspace(int64_t P, int cin, int64_t* floats);
+int spde_conv_backward(int M, int N, int T, int cin, const float* X, const float* G, const float* Y, float* dW,
+                       float* db, float* work, void* stream);
+/* Adam step (step >= 1) on n FP32 parameters; mask (device bytes, may be NULL):
+ * entries with 0 stay fixed. */
+int spde_adam(int64_t n, const float* g, const unsigned char* mask, float lr, float b1, float b2, float eps,
+              int step, float* theta, float* m, float* v, void* stream);
+
 /* ---------------------------------------------------------------- measurement
  * Between spde_prof_begin and spde_prof_end every FP64 GEMM launch of the
  * plan (factorisation, inverse of L11, solves, selected inverse) is bracketed
@@ -336,6 +450,9 @@ int spde_prof_dump(spde_plan* plan, double* ms, double* flops, double* bytes, in
 int spde_debug_mfma_probe(int which, double* out, void* stream);
 int spde_debug_gemm(int ta, int tb, int M, int N, int K, double alpha, const double* A, int lda,
                     const double* B, int ldb, double beta, double* C, int ldc, int tile, void* stream);
+/* The 64 x 4 accumulator lanes of one BF16 16x16x16 matrix instruction with
+ * A[i][k] = 16 i + k and B = I (out: 256 floats, device). */
+int spde_debug_bf16_probe(float* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -96,6 +96,33 @@ _SIGS = {
     "spde_debug_gemm": (ctypes.c_int, [ctypes.c_int] * 5 + [ctypes.c_double, vp, ctypes.c_int, vp, ctypes.c_int,
                                                             ctypes.c_double, vp, ctypes.c_int, ctypes.c_int, vp]),
     "spde_grad_cells": (ctypes.c_int, [vp, ctypes.POINTER(GradArgs), vp, vp]),
+    "spde_ic_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
+    "spde_ic_destroy": (None, [vp]),
+    "spde_ic_info": (ctypes.c_int, [vp, c_i64p, c_i32p]),
+    "spde_ic_export": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    "spde_ic_factor": (ctypes.c_int, [vp, vp, vp, ctypes.c_double, c_dp, vp]),
+    "spde_ic_correct": (ctypes.c_int, [vp, vp, c_dp, vp]),
+    "spde_ic_values": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+    "spde_ic_apply": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, vp]),
+    "spde_ic_sweep": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, ctypes.c_int, vp]),
+    "spde_ic_spmm": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, vp]),
+    "spde_pcg_solve": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, vp, vp]),
+    "spde_pcg_refine": (ctypes.c_int, [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                       c_i32p, c_i32p, c_dp, vp]),
+    "spde_rademacher": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_uint64, ctypes.c_float, vp, vp]),
+    "spde_slq": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, vp, vp, vp]),
+    "spde_hutch_pattern": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp, vp]),
+    "spde_ic_features": (ctypes.c_int, [vp, vp, vp]),
+    "spde_ic_kl_grad": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_double, vp, c_dp, vp]),
+    "spde_ic_jeffreys_grad": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_double, vp, c_dp, vp]),
+    "spde_conv_packed_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, c_i64p]),
+    "spde_conv_pack": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, vp, vp, vp]),
+    "spde_conv_forward": (ctypes.c_int, [ctypes.c_int] * 4 + [vp, vp, vp, vp, ctypes.c_int, vp, vp]),
+    "spde_conv_wgrad_workspace": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, c_i64p]),
+    "spde_conv_backward": (ctypes.c_int, [ctypes.c_int] * 4 + [vp] * 7),
+    "spde_adam": (ctypes.c_int, [ctypes.c_int64, vp, vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                 ctypes.c_float, ctypes.c_int, vp, vp, vp, vp]),
+    "spde_debug_bf16_probe": (ctypes.c_int, [vp, vp]),
 }
 
 _lib = None

@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libspdepy_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-HIP_SOURCES = ["dense.hip", "mf.hip", "numeric.hip", "assemble.hip", "grad.hip", "fields.hip", "kron.hip", "loglik.hip"]
+HIP_SOURCES = ["dense.hip", "mf.hip", "numeric.hip", "assemble.hip", "grad.hip", "fields.hip", "kron.hip", "loglik.hip", "pcg.hip", "learned.hip"]
 CXX_SOURCES = ["symbolic.cpp"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-value", "-Wno-unused-result"]
 # packed FP32 instructions are disabled for every device compile

@@ -628,6 +628,34 @@ Bases make_bases(spde_plan* pl, int slot, double* x) {
   return b;
 }
 
+// Device arenas of a factor slot, allocated on its first factorisation.  The
+// fronts and Z are cleared once: later factorisations clear only the lower
+// triangles, and nothing writes above a front's diagonal.
+int ensure_slot(spde_plan* pl, int slot) {
+  Slot& sl = pl->slots[slot];
+  if (sl.front) return SPDE_OK;
+  const Symbolic& sym = pl->sym;
+  const size_t fb = sizeof(double) * std::max<int64_t>(sym.front_doubles, 1);
+  const size_t zb = sizeof(double) * std::max<int64_t>(sym.z_doubles, 1);
+  if (hipMalloc((void**)&sl.front, fb) != hipSuccess || hipMalloc((void**)&sl.z, zb) != hipSuccess ||
+      hipMalloc((void**)&sl.work, sizeof(double) * pl->work_doubles) != hipSuccess ||
+      hipMalloc((void**)&sl.partial, sizeof(double) * std::max(logdet_parts(sym.n), 1)) != hipSuccess ||
+      hipMalloc((void**)&sl.ld, sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&sl.status, sizeof(int)) != hipSuccess) {
+    for (void* p : {(void*)sl.front, (void*)sl.z, (void*)sl.work, (void*)sl.partial, (void*)sl.ld, (void*)sl.status})
+      if (p) hipFree(p);
+    sl.front = sl.z = sl.work = sl.partial = sl.ld = nullptr;
+    sl.status = nullptr;
+    set_error("factor slot " + std::to_string(slot) + ": device allocation of " +
+              std::to_string((fb + zb) >> 20) + " MiB failed");
+    return SPDE_ERR_ALLOC;
+  }
+  SPDE_HIP(hipMemset(sl.front, 0, fb));
+  SPDE_HIP(hipMemset(sl.z, 0, zb));
+  SPDE_HIP(hipMemset(sl.status, 0, sizeof(int)));
+  return SPDE_OK;
+}
+
 int check_slot(spde_plan* pl, int slot) {
   if (!pl || pl->host_only) {
     set_error("plan has no device state");
@@ -764,20 +792,9 @@ int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int 
   SPDE_HIP(hipMalloc((void**)&pl->d_acan, sizeof(double) * Ns * 9));
   SPDE_HIP(hipMalloc((void**)&pl->d_a0can, sizeof(double) * Ns * 9));
   SPDE_HIP(hipMalloc((void**)&pl->d_qdiag, sizeof(double) * Ns));
+  // slot arenas are allocated by the slot's first factorisation (ensure_slot):
+  // a plan with two slots of which one is used holds one slot's memory
   pl->slots.resize(nslots);
-  for (int s = 0; s < nslots; ++s) {
-    SPDE_HIP(hipMalloc((void**)&pl->slots[s].front, sizeof(double) * std::max<int64_t>(sym.front_doubles, 1)));
-    // cleared once: later factorisations clear only the lower triangles,
-    // and nothing writes above a front's diagonal
-    SPDE_HIP(hipMemset(pl->slots[s].front, 0, sizeof(double) * std::max<int64_t>(sym.front_doubles, 1)));
-    SPDE_HIP(hipMalloc((void**)&pl->slots[s].z, sizeof(double) * std::max<int64_t>(sym.z_doubles, 1)));
-    SPDE_HIP(hipMemset(pl->slots[s].z, 0, sizeof(double) * std::max<int64_t>(sym.z_doubles, 1)));
-    SPDE_HIP(hipMalloc((void**)&pl->slots[s].work, sizeof(double) * pl->work_doubles));
-    SPDE_HIP(hipMalloc((void**)&pl->slots[s].partial, sizeof(double) * std::max(logdet_parts(sym.n), 1)));
-    SPDE_HIP(hipMalloc((void**)&pl->slots[s].ld, sizeof(double)));
-    SPDE_HIP(hipMalloc((void**)&pl->slots[s].status, sizeof(int)));
-    SPDE_HIP(hipMemset(pl->slots[s].status, 0, sizeof(int)));
-  }
   SPDE_HIP(hipMalloc((void**)&pl->d_win, sizeof(double) * Ns * 60));
   *out = pl;
   return SPDE_OK;
@@ -915,6 +932,8 @@ int spde_factor_async(spde_plan* pl, int slot, const double* qval, const double*
     return SPDE_ERR_ARG;
   }
   hipStream_t s = (hipStream_t)stream;
+  rc = ensure_slot(pl, slot);
+  if (rc) return rc;
   Slot& sl = pl->slots[slot];
   const Symbolic& sym = pl->sym;
   PlanExtra* ex = extra(pl);
@@ -948,6 +967,10 @@ int spde_factor_finish(spde_plan* pl, int slot, double* logdet) {
   int rc = check_slot(pl, slot);
   if (rc) return rc;
   Slot& sl = pl->slots[slot];
+  if (!sl.front) {  // (the stream may be the null stream)
+    set_error("spde_factor_finish: no factorisation was enqueued in this slot");
+    return SPDE_ERR_STATE;
+  }
   int status = 0;
   double ld = 0.0;
   SPDE_HIP(hipMemcpyAsync(&status, sl.status, sizeof(int), hipMemcpyDeviceToHost, sl.stream));

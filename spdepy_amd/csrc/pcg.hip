@@ -1,0 +1,1232 @@
+// Mixed-precision iterative path for Q_c = Q + diag(diag_add) on a space-time
+// lattice: an incomplete Cholesky factor IC(0) on Q's structural pattern in a
+// multi-colour ordering, FP32 preconditioned conjugate gradients with k
+// right-hand sides, FP64 iterative refinement, Lanczos for stochastic
+// log-determinants, Rademacher probes and the Hutchinson estimate of Q_c^-1
+// on the pattern.
+//
+// The reference evaluates log|Q_c| and tr(Q_c^-1 dQ) with CHOLMOD and a
+// Hutchinson estimator (advection_idiffusion2D.py:173-189,
+// var_advection_var_diffusion2D.py:205-221); this path replaces the
+// factorisation of Q_c by PCG solves.
+//
+// Colouring.  Row (i, j, t) of Q couples to the 5x5 spatial window of its own
+// slice and the 3x3 windows of the neighbouring slices, so the colour
+// (i mod 3) + 3 (j mod 3) + 9 (t mod 2) (18 colours) never repeats inside a
+// row: rows of one colour are mutually independent, and IC(0) in colour order
+// is computed, and its triangular sweeps applied, one colour per launch.
+//
+// Factor storage.  Per row i, the lower list holds the entries (i, j) with
+// colour(j) < colour(i), sorted by (colour(j), j), with L_ij; the upper list
+// the entries with colour(j) > colour(i), with L_ji (gathered from row j's
+// lower list).  Vectors with k right-hand sides are row-major n x k (the k
+// values of a row contiguous), k a power of two <= 64: the factor and matrix
+// values of a row are read once for all right-hand sides.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+struct spde_ic {
+  spde_plan* pl = nullptr;
+  int64_t n = 0, nnz = 0, nlow = 0;
+  int M = 0, N = 0, T = 0;
+  std::vector<int64_t> coff;        // 19 offsets into crow
+  std::vector<int32_t> h_colour;    // colour per lattice index
+  std::vector<int64_t> h_lptr, h_uptr, h_lsrc, h_lch;
+  std::vector<int32_t> h_lcol, h_ucol;
+  std::vector<int64_t> h_ulow;
+  int32_t* d_crow = nullptr;
+  int64_t *d_lptr = nullptr, *d_uptr = nullptr, *d_lsrc = nullptr, *d_ulow = nullptr, *d_lch = nullptr;
+  int32_t *d_lcol = nullptr, *d_ucol = nullptr;
+  double *d_l64 = nullptr, *d_dg64 = nullptr, *d_adiag = nullptr;  // factor (FP64), Q_c diagonal
+  float *d_l32 = nullptr, *d_u32 = nullptr, *d_dinv = nullptr, *d_qc32 = nullptr;
+  const double* qval = nullptr;      // Q values of the last factorisation (structural order)
+  const double* diag_add = nullptr;
+  int* d_status = nullptr;
+  double* d_part = nullptr;          // reduction partials [kRedBlocks * 64 * 2]
+  double* d_scal = nullptr;          // per right-hand side scalars [64 * 8]
+  int* d_active = nullptr;           // [64]
+  int* d_iters = nullptr;            // [64]
+  float* d_work = nullptr;           // 4 vectors n x 64 (PCG: r, z, p, q; Lanczos: q_prev, q_cur, w, u)
+  double* d_r64 = nullptr;           // refinement residual and solution (n each)
+  float *d_b32 = nullptr, *d_x32 = nullptr;
+  double* d_lanczos = nullptr;       // alpha, beta [2 * steps * 64]
+  int64_t lanczos_cap = 0;
+  double* d_klw = nullptr;           // L'^T x of the KL loss (n x K)
+  int64_t kl_cap = 0;
+  bool factored = false;
+};
+
+namespace spde {
+
+constexpr int kColours = 18;
+constexpr int kRedBlocks = 512;      // blocks of the reduction kernels
+constexpr int kPairs = 22;           // correction channels per cell: diagonal + 21 forward-half pairs
+
+// Scalars per right-hand side in d_scal[s * 64 + r].
+enum : int { S_RZ = 0, S_PQ = 1, S_ALPHA = 2, S_BETA = 3, S_RR = 4, S_BN = 5, S_TOL2 = 6, S_X = 7 };
+
+__device__ inline int lat_colour(int64_t idx, int M, int N) {
+  const int64_t Ns = (int64_t)M * N;
+  const int64_t t = idx / Ns, rem = idx - t * Ns;
+  const int j = (int)(rem / M), i = (int)(rem - (int64_t)j * M);
+  return (i % 3) + 3 * (j % 3) + 9 * (int)(t & 1);
+}
+
+// ------------------------------------------------------------ IC(0)
+// One thread per row of colour c.  L_ij = (A_ij - sum_k L_ik L_jk) / L_jj over
+// the k adjacent to both with colour(k) < colour(j), in (colour, index) order
+// of the row's lower list; L_ii = sqrt((1 + shift) A_ii - sum_j L_ij^2).
+__global__ __launch_bounds__(256) void ic_factor_kernel(int64_t r0, int64_t r1, const int32_t* __restrict__ crow,
+                                                        const int64_t* __restrict__ lptr,
+                                                        const int32_t* __restrict__ lcol,
+                                                        const int64_t* __restrict__ lsrc,
+                                                        const double* __restrict__ qval,
+                                                        const double* __restrict__ adiag, double shift,
+                                                        double* __restrict__ l64, double* __restrict__ dg64,
+                                                        int* __restrict__ status) {
+  const int64_t rr = r0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (rr >= r1) return;
+  const int i = crow[rr];
+  const int64_t lo = lptr[i], hi = lptr[i + 1];
+  double ssq = 0.0;
+  for (int64_t q = lo; q < hi; ++q) {
+    const int j = lcol[q];
+    double a = qval[lsrc[q]];
+    for (int64_t p = lptr[j]; p < lptr[j + 1]; ++p) {
+      const int kk = lcol[p];
+      for (int64_t s = lo; s < q; ++s)
+        if (lcol[s] == kk) {
+          a -= l64[s] * l64[p];
+          break;
+        }
+    }
+    const double v = a / dg64[j];
+    l64[q] = v;
+    ssq += v * v;
+  }
+  const double d = (1.0 + shift) * adiag[i] - ssq;
+  if (!(d > 0.0)) {
+    atomicMax(status, i + 1);
+    dg64[i] = 1.0;
+  } else {
+    dg64[i] = sqrt(d);
+  }
+}
+
+// Q_c diagonal (FP64) per row, and Q_c values in FP32 on the structural CSR.
+__global__ void qc_values_kernel(int64_t n, const int64_t* __restrict__ indptr, const int64_t* __restrict__ diagpos,
+                                 const double* __restrict__ qval, const double* __restrict__ dadd,
+                                 double* __restrict__ adiag, float* __restrict__ qc32) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t dp = diagpos[i];
+  for (int64_t e = indptr[i]; e < indptr[i + 1]; ++e) qc32[e] = (float)(qval[e] + (e == dp && dadd ? dadd[i] : 0.0));
+  adiag[i] = qval[dp] + (dadd ? dadd[i] : 0.0);
+}
+
+// FP32 copies of the factor: lower values, upper values (mirrors), 1 / L_ii.
+__global__ void ic_cast_lower_kernel(int64_t nlow, const double* __restrict__ l64, float* __restrict__ l32) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q < nlow) l32[q] = (float)l64[q];
+}
+__global__ void ic_cast_upper_kernel(int64_t nlow, const int64_t* __restrict__ ulow, const double* __restrict__ l64,
+                                     float* __restrict__ u32) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q < nlow) u32[q] = (float)l64[ulow[q]];
+}
+__global__ void ic_cast_diag_kernel(int64_t n, const double* __restrict__ dg64, float* __restrict__ dinv) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) dinv[i] = (float)(1.0 / dg64[i]);
+}
+
+// Learned correction in the Jacobi-scaled space (A~ = D^-1/2 A D^-1/2,
+// L = D^1/2 L~ row-scaled by sqrt(A_ii) of the row): L~_ij += corr, and
+// L~_ii *= exp(corr) for the diagonal channel.  corr[cell * 22 + ch] with the
+// channel of each lower entry in lch (-1: no channel).
+__global__ void ic_correct_lower_kernel(int64_t n, const int64_t* __restrict__ lptr, const int64_t* __restrict__ lch,
+                                        const double* __restrict__ adiag, const float* __restrict__ corr,
+                                        double* __restrict__ l64) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double s = sqrt(adiag[i]);
+  for (int64_t q = lptr[i]; q < lptr[i + 1]; ++q) {
+    const int64_t c = lch[q];
+    if (c >= 0) l64[q] += s * (double)corr[c];
+  }
+}
+__global__ void ic_correct_diag_kernel(int64_t n, const float* __restrict__ corr, double* __restrict__ dg64) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) dg64[i] *= exp((double)corr[i * kPairs]);
+}
+
+// ------------------------------------------------------------ reductions
+// partial[b][r] over threads of block b, for r = thread & (k-1).
+template <int NV>
+__device__ inline void block_reduce_store(double (&v)[NV], int k, double* __restrict__ part) {
+  __shared__ double red[NV][256];
+#pragma unroll
+  for (int a = 0; a < NV; ++a) red[a][threadIdx.x] = v[a];
+  __syncthreads();
+  // threads with the same (thread & (k-1)) share a right-hand side
+  for (int stride = 128; stride >= k; stride >>= 1) {
+    if ((int)threadIdx.x < stride)
+#pragma unroll
+      for (int a = 0; a < NV; ++a) red[a][threadIdx.x] += red[a][threadIdx.x + stride];
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < k)
+#pragma unroll
+    for (int a = 0; a < NV; ++a) part[((int64_t)a * kRedBlocks + blockIdx.x) * 64 + threadIdx.x] = red[a][threadIdx.x];
+}
+
+// ------------------------------------------------------------ SpMM, sweeps
+// y = A x for A on the structural CSR (FP32), k right-hand sides; when part is
+// set, the block partials of x^T y per right-hand side.
+__global__ __launch_bounds__(256) void spmm32_kernel(int64_t n, int logk, const int64_t* __restrict__ indptr,
+                                                     const int32_t* __restrict__ indices, const float* __restrict__ a,
+                                                     const float* __restrict__ x, float* __restrict__ y,
+                                                     double* __restrict__ part) {
+  const int k = 1 << logk;
+  const int64_t units = n << logk;
+  double acc[1] = {0.0};
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = u >> logk;
+    const int r = (int)(u & (k - 1));
+    float s = 0.f;
+    for (int64_t e = indptr[i]; e < indptr[i + 1]; ++e) s += a[e] * x[((int64_t)indices[e] << logk) + r];
+    y[u] = s;
+    acc[0] += (double)s * (double)x[u];
+  }
+  if (part) block_reduce_store<1>(acc, k, part);
+}
+
+// FP64 residual r = b - (Q + diag(dadd)) x for one right-hand side.
+__global__ __launch_bounds__(256) void resid64_kernel(int64_t n, const int64_t* __restrict__ indptr,
+                                                      const int32_t* __restrict__ indices,
+                                                      const int64_t* __restrict__ diagpos,
+                                                      const double* __restrict__ qval, const double* __restrict__ dadd,
+                                                      const double* __restrict__ b, const double* __restrict__ x,
+                                                      double* __restrict__ r, double* __restrict__ part) {
+  double acc[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int64_t e = indptr[i]; e < indptr[i + 1]; ++e) s += qval[e] * x[indices[e]];
+    if (dadd) s += dadd[i] * x[i];
+    const double v = b[i] - s;
+    r[i] = v;
+    acc[0] += v * v;
+  }
+  block_reduce_store<1>(acc, 1, part);
+}
+
+// Forward sweep of one colour: z_i = (r_i - sum_lower L_ij z_j) / L_ii.
+__global__ __launch_bounds__(256) void fwd_kernel(int64_t r0, int64_t r1, int logk, const int32_t* __restrict__ crow,
+                                                  const int64_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
+                                                  const float* __restrict__ l32, const float* __restrict__ dinv,
+                                                  const float* r, float* z) {
+  const int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t rr = r0 + (u >> logk);
+  if (rr >= r1) return;
+  const int rhs = (int)(u & ((1 << logk) - 1));
+  const int i = crow[rr];
+  float s = r[((int64_t)i << logk) + rhs];
+  for (int64_t p = lptr[i]; p < lptr[i + 1]; ++p) s -= l32[p] * z[((int64_t)lcol[p] << logk) + rhs];
+  z[((int64_t)i << logk) + rhs] = s * dinv[i];
+}
+
+// Backward sweep of one colour: x_i = (y_i - sum_upper L_ji x_j) / L_ii
+// (y and x may be the same array).
+__global__ __launch_bounds__(256) void bwd_kernel(int64_t r0, int64_t r1, int logk, const int32_t* __restrict__ crow,
+                                                  const int64_t* __restrict__ uptr, const int32_t* __restrict__ ucol,
+                                                  const float* __restrict__ u32, const float* __restrict__ dinv,
+                                                  const float* y, float* x) {
+  const int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t rr = r0 + (u >> logk);
+  if (rr >= r1) return;
+  const int rhs = (int)(u & ((1 << logk) - 1));
+  const int i = crow[rr];
+  float s = y[((int64_t)i << logk) + rhs];
+  for (int64_t p = uptr[i]; p < uptr[i + 1]; ++p) s -= u32[p] * x[((int64_t)ucol[p] << logk) + rhs];
+  x[((int64_t)i << logk) + rhs] = s * dinv[i];
+}
+
+// ------------------------------------------------------------ PCG steps
+// Finalise per right-hand side sums of the partials: out[a][r] = sum_b part[a][b][r].
+__device__ inline double sum_parts(const double* __restrict__ part, int a, int r, int nblk) {
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[((int64_t)a * kRedBlocks + b) * 64 + r];
+  return s;
+}
+
+// x = 0, p = z, rz = r^T z, bn = b^T b (r = b), active = bn > 0.
+__global__ void pcg_init_scalars_kernel(int k, int nblk, const double* __restrict__ part, double tol2,
+                                        double* __restrict__ sc, int* __restrict__ active, int* __restrict__ iters) {
+  const int r = threadIdx.x;
+  if (r >= k) return;
+  const double rz = sum_parts(part, 0, r, nblk), bn = sum_parts(part, 1, r, nblk);
+  sc[S_RZ * 64 + r] = rz;
+  sc[S_BN * 64 + r] = bn;
+  sc[S_TOL2 * 64 + r] = tol2;
+  sc[S_RR * 64 + r] = bn;
+  active[r] = bn > 0.0 ? 1 : 0;
+  iters[r] = 0;
+}
+
+__global__ void pcg_alpha_kernel(int k, int nblk, const double* __restrict__ part, double* __restrict__ sc,
+                                 const int* __restrict__ active) {
+  const int r = threadIdx.x;
+  if (r >= k) return;
+  const double pq = sum_parts(part, 0, r, nblk);
+  sc[S_PQ * 64 + r] = pq;
+  sc[S_ALPHA * 64 + r] = (active[r] && pq > 0.0) ? sc[S_RZ * 64 + r] / pq : 0.0;
+}
+
+__global__ __launch_bounds__(256) void pcg_update_kernel(int64_t units, int logk, const double* __restrict__ sc,
+                                                         const float* __restrict__ p, const float* __restrict__ q,
+                                                         float* __restrict__ x, float* __restrict__ r) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
+    const float a = (float)sc[S_ALPHA * 64 + (int)(u & ((1 << logk) - 1))];
+    x[u] += a * p[u];
+    r[u] -= a * q[u];
+  }
+}
+
+// Partials of r^T z and r^T r (two sums per right-hand side).
+__global__ __launch_bounds__(256) void dot2_kernel(int64_t units, int logk, const float* __restrict__ r,
+                                                   const float* __restrict__ z, double* __restrict__ part) {
+  double acc[2] = {0.0, 0.0};
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
+    const double rv = r[u];
+    acc[0] += rv * (double)z[u];
+    acc[1] += rv * rv;
+  }
+  block_reduce_store<2>(acc, 1 << logk, part);
+}
+
+// beta = rz_new / rz; convergence: ||r||^2 <= tol^2 ||b||^2 stops a right-hand side.
+__global__ void pcg_beta_kernel(int k, int nblk, const double* __restrict__ part, double* __restrict__ sc,
+                                int* __restrict__ active, int* __restrict__ iters) {
+  const int r = threadIdx.x;
+  if (r >= k) return;
+  const double rz = sum_parts(part, 0, r, nblk), rr = sum_parts(part, 1, r, nblk);
+  if (!active[r]) {
+    sc[S_BETA * 64 + r] = 0.0;
+    return;
+  }
+  iters[r] += 1;
+  sc[S_RR * 64 + r] = rr;
+  const double old = sc[S_RZ * 64 + r];
+  sc[S_BETA * 64 + r] = old != 0.0 ? rz / old : 0.0;
+  sc[S_RZ * 64 + r] = rz;
+  if (rr <= sc[S_TOL2 * 64 + r] * sc[S_BN * 64 + r] || !(rz > 0.0)) active[r] = 0;
+}
+
+__global__ __launch_bounds__(256) void pcg_p_kernel(int64_t units, int logk, const double* __restrict__ sc,
+                                                    const int* __restrict__ active, const float* __restrict__ z,
+                                                    float* __restrict__ p) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(u & ((1 << logk) - 1));
+    if (active[r]) p[u] = z[u] + (float)sc[S_BETA * 64 + r] * p[u];
+  }
+}
+
+__global__ __launch_bounds__(256) void fill_kernel(int64_t units, float v, float* __restrict__ x) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x)
+    x[u] = v;
+}
+__global__ __launch_bounds__(256) void copy_kernel(int64_t units, const float* __restrict__ a, float* __restrict__ b) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x)
+    b[u] = a[u];
+}
+
+// ------------------------------------------------------------ refinement
+// b32 = r64 / s (s = ||r64|| on the device), for the FP32 inner solve.
+__global__ __launch_bounds__(256) void scale_down_kernel(int64_t n, int nblk, const double* __restrict__ part,
+                                                         const double* __restrict__ r64, float* __restrict__ b32) {
+  const double s = sqrt(sum_parts(part, 0, 0, nblk));
+  const double inv = s > 0.0 ? 1.0 / s : 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b32[i] = (float)(r64[i] * inv);
+}
+// x64 += s * d32.
+__global__ __launch_bounds__(256) void correct_kernel(int64_t n, int nblk, const double* __restrict__ part,
+                                                      const float* __restrict__ d32, double* __restrict__ x64) {
+  const double s = sqrt(sum_parts(part, 0, 0, nblk));
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x64[i] += s * (double)d32[i];
+}
+__global__ void sum_one_kernel(int nblk, const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = sum_parts(part, 0, 0, nblk);
+}
+
+// ------------------------------------------------------------ Lanczos
+// alpha_j = q^T w per right-hand side (partials), then w -= alpha q + beta_prev q_prev
+// with the partials of ||w||^2; then q_prev = q, q = w / beta_j.
+__global__ void lanczos_alpha_kernel(int k, int nblk, const double* __restrict__ part, double* __restrict__ sc,
+                                     double* __restrict__ alpha_out) {
+  const int r = threadIdx.x;
+  if (r >= k) return;
+  const double a = sum_parts(part, 0, r, nblk);
+  sc[S_ALPHA * 64 + r] = a;
+  alpha_out[r] = a;
+}
+__global__ __launch_bounds__(256) void lanczos_orth_kernel(int64_t units, int logk, const double* __restrict__ sc,
+                                                           const float* __restrict__ q, const float* __restrict__ qp,
+                                                           float* __restrict__ w, double* __restrict__ part) {
+  double acc[1] = {0.0};
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(u & ((1 << logk) - 1));
+    const float v = w[u] - (float)sc[S_ALPHA * 64 + r] * q[u] - (float)sc[S_BETA * 64 + r] * qp[u];
+    w[u] = v;
+    acc[0] += (double)v * (double)v;
+  }
+  block_reduce_store<1>(acc, 1 << logk, part);
+}
+__global__ void lanczos_beta_kernel(int k, int nblk, const double* __restrict__ part, double* __restrict__ sc,
+                                    double* __restrict__ beta_out) {
+  const int r = threadIdx.x;
+  if (r >= k) return;
+  const double b = sqrt(sum_parts(part, 0, r, nblk));
+  sc[S_BETA * 64 + r] = b;
+  beta_out[r] = b;
+}
+// q_prev = q; q = w / beta (beta 0: q = 0).
+__global__ __launch_bounds__(256) void lanczos_next_kernel(int64_t units, int logk, const double* __restrict__ sc,
+                                                           float* __restrict__ q, float* __restrict__ qp,
+                                                           const float* __restrict__ w) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
+    const double b = sc[S_BETA * 64 + (int)(u & ((1 << logk) - 1))];
+    qp[u] = q[u];
+    q[u] = b > 0.0 ? (float)(w[u] / b) : 0.f;
+  }
+}
+
+// ------------------------------------------------------------ probes
+__device__ __host__ inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+// V[i][r] = +-scale from the top bit of splitmix64(seed * 2^32 + i * k + r)... see
+// spde_rademacher for the exact counter.
+__global__ __launch_bounds__(256) void rademacher_kernel(int64_t units, uint64_t seed, float scale,
+                                                         float* __restrict__ v) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t h = splitmix64(seed ^ splitmix64((uint64_t)u));
+    v[u] = (h >> 63) ? -scale : scale;
+  }
+}
+
+// sigma[e] = (1/k) sum_r (U[i][r] V[j][r] + U[j][r] V[i][r]) / 2 for every
+// structural entry e = (i, j); one wave per row, lanes = right-hand sides.
+__global__ __launch_bounds__(256) void hutch_pattern_kernel(int64_t n, int logk, const int64_t* __restrict__ indptr,
+                                                            const int32_t* __restrict__ indices,
+                                                            const float* __restrict__ U, const float* __restrict__ V,
+                                                            double alpha, double beta, double* __restrict__ sigma) {
+  const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const int k = 1 << logk;
+  const bool on = lane < k;
+  const double ui = on ? (double)U[(i << logk) + lane] : 0.0, vi = on ? (double)V[(i << logk) + lane] : 0.0;
+  for (int64_t e = indptr[i]; e < indptr[i + 1]; ++e) {
+    const int64_t j = indices[e];
+    double s = on ? 0.5 * (ui * (double)V[(j << logk) + lane] + (double)U[(j << logk) + lane] * vi) : 0.0;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) sigma[e] = (beta != 0.0 ? beta * sigma[e] : 0.0) + alpha * (s / k);
+  }
+}
+
+__global__ void logdet_diag_kernel(int64_t n, const double* __restrict__ dg64, double* __restrict__ part) {
+  double acc[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc[0] += log(dg64[i]);
+  block_reduce_store<1>(acc, 1, part);
+}
+
+
+// ------------------------------------------------------------ learned correction: inputs and loss
+// Seven Jacobi-scaled stencil coefficients of Q_c per lattice entry (one time
+// slice of the network's input): [tau count / A_ii, A~ to E, W, N, S in the
+// slice, A~ to the same cell in the next and the previous slice, 0], with
+// A~_ij = A_ij / sqrt(A_ii A_jj) (0 where the neighbour is outside).
+__global__ __launch_bounds__(256) void features_kernel(int64_t n, const int64_t* __restrict__ indptr,
+                                                       const int32_t* __restrict__ indices,
+                                                       const uint8_t* __restrict__ code,
+                                                       const double* __restrict__ qval,
+                                                       const double* __restrict__ dadd,
+                                                       const double* __restrict__ adiag, float* __restrict__ feat) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const double ai = adiag[i];
+  f[0] = (float)((dadd ? dadd[i] : 0.0) / ai);
+  for (int64_t e = indptr[i]; e < indptr[i + 1]; ++e) {
+    int slot = -1;
+    switch (code[e]) {
+      case 13: slot = 1; break;  // (dy, dx) = (0, +1): E
+      case 11: slot = 2; break;  // (0, -1): W
+      case 17: slot = 3; break;  // (+1, 0): N
+      case 7: slot = 4; break;   // (-1, 0): S
+      case 38: slot = 5; break;  // next slice, same cell
+      case 29: slot = 6; break;  // previous slice, same cell
+      default: break;
+    }
+    if (slot > 0) f[slot] = (float)(qval[e] / sqrt(ai * adiag[indices[e]]));
+  }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) feat[i * 8 + c] = f[c];
+}
+
+// w = L'^T x for K samples (row-major n x K doubles), FP64 factor, and the
+// block partials of sum ||w||^2.
+__global__ __launch_bounds__(256) void kl_w_kernel(int64_t n, int logk, const int64_t* __restrict__ uptr,
+                                                   const int32_t* __restrict__ ucol, const int64_t* __restrict__ ulow,
+                                                   const double* __restrict__ l64, const double* __restrict__ dg64,
+                                                   const double* __restrict__ x, double* __restrict__ w,
+                                                   double* __restrict__ part) {
+  double acc[1] = {0.0};
+  const int64_t units = n << logk;
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = u >> logk;
+    const int s = (int)(u & ((1 << logk) - 1));
+    double v = dg64[j] * x[u];
+    for (int64_t q = uptr[j]; q < uptr[j + 1]; ++q) v += l64[ulow[q]] * x[((int64_t)ucol[q] << logk) + s];
+    w[u] = v;
+    acc[0] += v * v;
+  }
+  block_reduce_store<1>(acc, 1, part);
+}
+
+// Gradient of KL(N(0, A^-1) || N(0, (L' L'^T)^-1)) = 1/2 E||L'^T x||^2 - sum log L'_ii + const
+// with respect to the correction channels (scaled by `scale`): for the lower
+// entry (i, j) of channel c, d/dcorr = sqrt(A_ii) E[x_i w_j]; for the diagonal
+// channel, L'_ii E[x_i w_i] - 1.  One wave per row, lanes = samples.
+__global__ __launch_bounds__(256) void kl_grad_kernel(int64_t n, int logk, const int64_t* __restrict__ lptr,
+                                                      const int32_t* __restrict__ lcol,
+                                                      const int64_t* __restrict__ lch,
+                                                      const double* __restrict__ dg64,
+                                                      const double* __restrict__ adiag,
+                                                      const double* __restrict__ x, const double* __restrict__ w,
+                                                      double scale, float* __restrict__ dout) {
+  const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const int k = 1 << logk;
+  const bool on = lane < k;
+  const double xi = on ? x[(i << logk) + lane] : 0.0;
+  const double inv = 1.0 / k;
+  const double si = sqrt(adiag[i]);
+  for (int64_t q = lptr[i]; q < lptr[i + 1]; ++q) {
+    double g = on ? xi * w[((int64_t)lcol[q] << logk) + lane] : 0.0;
+    for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o, 64);
+    if (lane == 0 && lch[q] >= 0) dout[lch[q]] = (float)(scale * si * g * inv);
+  }
+  double g = on ? xi * w[(i << logk) + lane] : 0.0;
+  for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o, 64);
+  if (lane == 0) dout[i * kPairs] = (float)(scale * (dg64[i] * g * inv - 1.0));
+}
+
+
+// Gradient of the symmetrised divergence J = 1/2 E||L'^T x||^2 + 1/2 E[y^T A y] - n
+// (y = L'^-T z, z Rademacher; E over samples) = KL(p||q) + KL(q||p) of
+// p = N(0, A^-1), q = N(0, (L' L'^T)^-1): for the pair (i, j) of a lower
+// entry, dJ/dL'_ij = E[x_i w_j] - E[y_i u_j] with w = L'^T x, u = L'^-1 A y;
+// the correction channels follow as in kl_grad_kernel (sqrt(A_ii) for pairs,
+// L'_ii for the diagonal's log-scale).  One wave per row, lanes = samples.
+__global__ __launch_bounds__(256) void jeffreys_grad_kernel(int64_t n, int logk, const int64_t* __restrict__ lptr,
+                                                            const int32_t* __restrict__ lcol,
+                                                            const int64_t* __restrict__ lch,
+                                                            const double* __restrict__ dg64,
+                                                            const double* __restrict__ adiag,
+                                                            const double* __restrict__ x, const double* __restrict__ w,
+                                                            const float* __restrict__ y, const float* __restrict__ u,
+                                                            double scale, float* __restrict__ dout) {
+  const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const int k = 1 << logk;
+  const bool on = lane < k;
+  const double xi = on ? x[(i << logk) + lane] : 0.0;
+  const double yi = on ? (double)y[(i << logk) + lane] : 0.0;
+  const double inv = 1.0 / k;
+  const double si = sqrt(adiag[i]);
+  for (int64_t q = lptr[i]; q < lptr[i + 1]; ++q) {
+    const int64_t j = lcol[q];
+    double g = on ? xi * w[(j << logk) + lane] - yi * (double)u[(j << logk) + lane] : 0.0;
+    for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o, 64);
+    if (lane == 0 && lch[q] >= 0) dout[lch[q]] = (float)(scale * si * g * inv);
+  }
+  double g = on ? xi * w[(i << logk) + lane] - yi * (double)u[(i << logk) + lane] : 0.0;
+  for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o, 64);
+  if (lane == 0) dout[i * kPairs] = (float)(scale * dg64[i] * g * inv);
+}
+
+inline unsigned grid_for(int64_t units) {
+  const int64_t b = (units + 255) / 256;
+  return (unsigned)(b < kRedBlocks ? (b < 1 ? 1 : b) : kRedBlocks);
+}
+inline unsigned blocks_of(int64_t units) { return (unsigned)((units + 255) / 256); }
+
+template <typename T>
+int upload_vec(T** dst, const std::vector<T>& src) {
+  const size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
+  SPDE_HIP(hipMalloc((void**)dst, bytes));
+  if (!src.empty()) SPDE_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return SPDE_OK;
+}
+
+int log2_of(int k) {
+  int l = 0;
+  while ((1 << l) < k) ++l;
+  return (1 << l) == k && k <= 64 ? l : -1;
+}
+
+// Sweeps of M^-1 = (L L^T)^-1: forward from r into z, then backward in place.
+int ic_apply(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s) {
+  for (int c = 0; c < kColours; ++c) {
+    const int64_t r0 = ic->coff[c], r1 = ic->coff[c + 1];
+    if (r1 > r0)
+      hipLaunchKernelGGL(fwd_kernel, dim3(blocks_of((r1 - r0) << logk)), dim3(256), 0, s, r0, r1, logk, ic->d_crow,
+                         ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dinv, r, z);
+  }
+  for (int c = kColours - 1; c >= 0; --c) {
+    const int64_t r0 = ic->coff[c], r1 = ic->coff[c + 1];
+    if (r1 > r0)
+      hipLaunchKernelGGL(bwd_kernel, dim3(blocks_of((r1 - r0) << logk)), dim3(256), 0, s, r0, r1, logk, ic->d_crow,
+                         ic->d_uptr, ic->d_ucol, ic->d_u32, ic->d_dinv, z, z);
+  }
+  return SPDE_OK;
+}
+int ic_fwd(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s) {
+  for (int c = 0; c < kColours; ++c) {
+    const int64_t r0 = ic->coff[c], r1 = ic->coff[c + 1];
+    if (r1 > r0)
+      hipLaunchKernelGGL(fwd_kernel, dim3(blocks_of((r1 - r0) << logk)), dim3(256), 0, s, r0, r1, logk, ic->d_crow,
+                         ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dinv, r, z);
+  }
+  return SPDE_OK;
+}
+int ic_bwd(spde_ic* ic, const float* y, float* x, int logk, hipStream_t s) {
+  for (int c = kColours - 1; c >= 0; --c) {
+    const int64_t r0 = ic->coff[c], r1 = ic->coff[c + 1];
+    if (r1 > r0)
+      hipLaunchKernelGGL(bwd_kernel, dim3(blocks_of((r1 - r0) << logk)), dim3(256), 0, s, r0, r1, logk, ic->d_crow,
+                         ic->d_uptr, ic->d_ucol, ic->d_u32, ic->d_dinv, y, x);
+  }
+  return SPDE_OK;
+}
+
+int ensure_work(spde_ic* ic) {
+  if (ic->d_work) return SPDE_OK;
+  SPDE_HIP(hipMalloc((void**)&ic->d_work, sizeof(float) * ic->n * 64 * 4));
+  SPDE_HIP(hipMalloc((void**)&ic->d_r64, sizeof(double) * ic->n));
+  SPDE_HIP(hipMalloc((void**)&ic->d_b32, sizeof(float) * ic->n));
+  SPDE_HIP(hipMalloc((void**)&ic->d_x32, sizeof(float) * ic->n));
+  return SPDE_OK;
+}
+
+// FP32 PCG with k = 2^logk right-hand sides; X (initial guess 0) and B in
+// row-major n x k.  Host polls the active flags every `poll` iterations.
+int pcg_run(spde_ic* ic, const float* B, float* X, int logk, double tol, int maxit, int32_t* iters, hipStream_t s) {
+  const int k = 1 << logk;
+  const int64_t n = ic->n, units = n << logk;
+  int rc = ensure_work(ic);
+  if (rc) return rc;
+  float* R = ic->d_work;
+  float* Z = R + n * 64;
+  float* P = Z + n * 64;
+  float* Qv = P + n * 64;
+  spde_plan* pl = ic->pl;
+  const unsigned g = grid_for(units);
+  hipLaunchKernelGGL(fill_kernel, dim3(g), dim3(256), 0, s, units, 0.f, X);
+  hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, s, units, B, R);
+  ic_apply(ic, R, Z, logk, s);
+  hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, R, Z, ic->d_part);
+  hipLaunchKernelGGL(pcg_init_scalars_kernel, dim3(1), dim3(64), 0, s, k, (int)g, ic->d_part, tol * tol, ic->d_scal,
+                     ic->d_active, ic->d_iters);
+  hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, s, units, Z, P);
+  std::vector<int> act(k);
+  const int poll = 16;
+  for (int it = 0; it < maxit; ++it) {
+    hipLaunchKernelGGL(spmm32_kernel, dim3(g), dim3(256), 0, s, n, logk, pl->d_indptr, pl->d_indices, ic->d_qc32, P,
+                       Qv, ic->d_part);
+    hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(64), 0, s, k, (int)g, ic->d_part, ic->d_scal, ic->d_active);
+    hipLaunchKernelGGL(pcg_update_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, P, Qv, X, R);
+    ic_apply(ic, R, Z, logk, s);
+    hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, R, Z, ic->d_part);
+    hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(64), 0, s, k, (int)g, ic->d_part, ic->d_scal, ic->d_active,
+                       ic->d_iters);
+    hipLaunchKernelGGL(pcg_p_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, ic->d_active, Z, P);
+    if ((it + 1) % poll == 0 || it + 1 == maxit) {
+      SPDE_HIP(hipMemcpyAsync(act.data(), ic->d_active, sizeof(int) * k, hipMemcpyDeviceToHost, s));
+      SPDE_HIP(hipStreamSynchronize(s));
+      bool any = false;
+      for (int r = 0; r < k; ++r) any = any || act[r];
+      if (!any) break;
+    }
+  }
+  SPDE_HIP(hipGetLastError());
+  if (iters) {
+    SPDE_HIP(hipMemcpyAsync(iters, ic->d_iters, sizeof(int) * k, hipMemcpyDeviceToHost, s));
+    SPDE_HIP(hipStreamSynchronize(s));
+  }
+  return SPDE_OK;
+}
+
+}  // namespace spde
+
+using namespace spde;
+
+extern "C" {
+
+int spde_ic_create(spde_plan* pl, spde_ic** out) {
+  if (!pl || !out || pl->host_only || pl->lat.wide) {
+    set_error("spde_ic_create: invalid argument (a device space-time plan of the advection-diffusion pattern)");
+    return SPDE_ERR_ARG;
+  }
+  const Lattice& L = pl->lat;
+  if (L.bc == 2 && (L.M % 3 || L.N % 3 || (L.T > 1 && L.T % 2))) {
+    set_error("spde_ic_create: periodic lattices need M, N divisible by 3 and T even for the 18-colour ordering");
+    return SPDE_ERR_ARG;
+  }
+  *out = nullptr;
+  spde_ic* ic = new spde_ic();
+  ic->pl = pl;
+  ic->M = L.M;
+  ic->N = L.N;
+  ic->T = L.T;
+  const int64_t n = L.n(), Ns = L.Ns();
+  ic->n = n;
+  ic->nnz = (int64_t)pl->st.indices.size();
+  const auto& ip = pl->st.indptr;
+  const auto& ix = pl->st.indices;
+  auto colour = [&](int64_t v) {
+    const int64_t t = v / Ns, rem = v % Ns;
+    const int j = (int)(rem / L.M), i = (int)(rem % L.M);
+    return (i % 3) + 3 * (j % 3) + 9 * (int)(t & 1);
+  };
+  ic->h_colour.resize(n);
+  for (int64_t v = 0; v < n; ++v) ic->h_colour[v] = colour(v);
+  // rows grouped by colour
+  std::vector<int32_t> crow;
+  crow.reserve(n);
+  ic->coff.assign(kColours + 1, 0);
+  for (int c = 0; c < kColours; ++c) {
+    ic->coff[c] = (int64_t)crow.size();
+    for (int64_t v = 0; v < n; ++v)
+      if (ic->h_colour[v] == c) crow.push_back((int32_t)v);
+  }
+  ic->coff[kColours] = (int64_t)crow.size();
+  // lower lists sorted by (colour, column); channel of each pair
+  ic->h_lptr.assign(n + 1, 0);
+  ic->h_uptr.assign(n + 1, 0);
+  for (int64_t v = 0; v < n; ++v) {
+    for (int64_t e = ip[v]; e < ip[v + 1]; ++e) {
+      const int64_t w = ix[e];
+      if (w == v) continue;
+      if (ic->h_colour[w] == ic->h_colour[v]) {
+        set_error("spde_ic_create: colouring is not an independent-set partition of the pattern");
+        delete ic;
+        return SPDE_ERR_STATE;
+      }
+      if (ic->h_colour[w] < ic->h_colour[v]) ++ic->h_lptr[v + 1];
+      else ++ic->h_uptr[v + 1];
+    }
+  }
+  for (int64_t v = 0; v < n; ++v) {
+    ic->h_lptr[v + 1] += ic->h_lptr[v];
+    ic->h_uptr[v + 1] += ic->h_uptr[v];
+  }
+  ic->nlow = ic->h_lptr[n];
+  if (ic->h_uptr[n] != ic->nlow) {
+    set_error("spde_ic_create: pattern is not symmetric");
+    delete ic;
+    return SPDE_ERR_STATE;
+  }
+  ic->h_lcol.resize(ic->nlow);
+  ic->h_lsrc.resize(ic->nlow);
+  ic->h_lch.assign(ic->nlow, -1);
+  ic->h_ucol.resize(ic->nlow);
+  ic->h_ulow.resize(ic->nlow);
+  // forward-half pair channels: same slice (dy > 0, or dy == 0 and dx > 0),
+  // then the next slice's 3x3 window; channel 0 is the diagonal
+  auto pair_channel = [&](int dx, int dy, int dt) -> int {
+    if (dt == 0) {
+      if (!(dy > 0 || (dy == 0 && dx > 0))) return -1;
+      const int lin = (dy + 2) * 5 + (dx + 2);  // 13..24 for the forward half
+      return 1 + (lin - 13);                    // 1..12
+    }
+    if (dt == 1) return 13 + (dy + 1) * 3 + (dx + 1);  // 13..21
+    return -1;
+  };
+  for (int64_t v = 0; v < n; ++v) {
+    std::vector<std::pair<int, int64_t>> lo;  // (colour, structural position)
+    for (int64_t e = ip[v]; e < ip[v + 1]; ++e) {
+      const int64_t w = ix[e];
+      if (w != v && ic->h_colour[w] < ic->h_colour[v]) lo.push_back({ic->h_colour[w] * (int)n + 0, e});
+    }
+    std::sort(lo.begin(), lo.end(), [&](const std::pair<int, int64_t>& a, const std::pair<int, int64_t>& b) {
+      const int ca = ic->h_colour[ix[a.second]], cb = ic->h_colour[ix[b.second]];
+      return ca != cb ? ca < cb : ix[a.second] < ix[b.second];
+    });
+    int64_t q = ic->h_lptr[v];
+    for (auto& pr : lo) {
+      const int64_t e = pr.second, w = ix[e];
+      ic->h_lcol[q] = (int32_t)w;
+      ic->h_lsrc[q] = e;
+      // lattice offset of the pair (v, w), wrapped for periodic lattices
+      const int64_t tv = v / Ns, tw = w / Ns;
+      int dx = (int)(w % Ns % L.M) - (int)(v % Ns % L.M);
+      int dy = (int)(w % Ns / L.M) - (int)(v % Ns / L.M);
+      int dt = (int)(tw - tv);
+      if (L.bc == 2) {
+        if (dx > 2) dx -= L.M;
+        if (dx < -2) dx += L.M;
+        if (dy > 2) dy -= L.N;
+        if (dy < -2) dy += L.N;
+      }
+      int ch = pair_channel(dx, dy, dt);
+      int64_t cell = v;
+      if (ch < 0) {
+        ch = pair_channel(-dx, -dy, -dt);
+        cell = w;
+      }
+      if (ch >= 1) ic->h_lch[q] = cell * kPairs + ch;
+      ++q;
+    }
+  }
+  // upper lists: entries (v, w) with colour(w) > colour(v); value L_wv in row w's lower list
+  for (int64_t v = 0; v < n; ++v) {
+    int64_t q = ic->h_uptr[v];
+    for (int64_t e = ip[v]; e < ip[v + 1]; ++e) {
+      const int64_t w = ix[e];
+      if (w == v || ic->h_colour[w] < ic->h_colour[v]) continue;
+      int64_t pos = -1;
+      for (int64_t p = ic->h_lptr[w]; p < ic->h_lptr[w + 1]; ++p)
+        if (ic->h_lcol[p] == (int32_t)v) pos = p;
+      if (pos < 0) {
+        set_error("spde_ic_create: mirror entry missing");
+        delete ic;
+        return SPDE_ERR_STATE;
+      }
+      ic->h_ucol[q] = (int32_t)w;
+      ic->h_ulow[q] = pos;
+      ++q;
+    }
+  }
+  int rc = SPDE_OK;
+  auto fail = [&](int code) {
+    spde_ic_destroy(ic);
+    return code;
+  };
+  if ((rc = upload_vec(&ic->d_crow, crow))) return fail(rc);
+  if ((rc = upload_vec(&ic->d_lptr, ic->h_lptr))) return fail(rc);
+  if ((rc = upload_vec(&ic->d_uptr, ic->h_uptr))) return fail(rc);
+  if ((rc = upload_vec(&ic->d_lcol, ic->h_lcol))) return fail(rc);
+  if ((rc = upload_vec(&ic->d_lsrc, ic->h_lsrc))) return fail(rc);
+  if ((rc = upload_vec(&ic->d_lch, ic->h_lch))) return fail(rc);
+  if ((rc = upload_vec(&ic->d_ucol, ic->h_ucol))) return fail(rc);
+  if ((rc = upload_vec(&ic->d_ulow, ic->h_ulow))) return fail(rc);
+  const int64_t nl = std::max<int64_t>(ic->nlow, 1);
+  if (hipMalloc((void**)&ic->d_l64, sizeof(double) * nl) != hipSuccess ||
+      hipMalloc((void**)&ic->d_dg64, sizeof(double) * n) != hipSuccess ||
+      hipMalloc((void**)&ic->d_adiag, sizeof(double) * n) != hipSuccess ||
+      hipMalloc((void**)&ic->d_l32, sizeof(float) * nl) != hipSuccess ||
+      hipMalloc((void**)&ic->d_u32, sizeof(float) * nl) != hipSuccess ||
+      hipMalloc((void**)&ic->d_dinv, sizeof(float) * n) != hipSuccess ||
+      hipMalloc((void**)&ic->d_qc32, sizeof(float) * ic->nnz) != hipSuccess ||
+      hipMalloc((void**)&ic->d_status, sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&ic->d_part, sizeof(double) * kRedBlocks * 64 * 2) != hipSuccess ||
+      hipMalloc((void**)&ic->d_scal, sizeof(double) * 64 * 8) != hipSuccess ||
+      hipMalloc((void**)&ic->d_active, sizeof(int) * 64) != hipSuccess ||
+      hipMalloc((void**)&ic->d_iters, sizeof(int) * 64) != hipSuccess) {
+    set_error("spde_ic_create: device allocation failed");
+    return fail(SPDE_ERR_ALLOC);
+  }
+  *out = ic;
+  return SPDE_OK;
+}
+
+void spde_ic_destroy(spde_ic* ic) {
+  if (!ic) return;
+  hipDeviceSynchronize();
+  for (void* p : {(void*)ic->d_crow, (void*)ic->d_lptr, (void*)ic->d_uptr, (void*)ic->d_lsrc, (void*)ic->d_ulow,
+                  (void*)ic->d_lch, (void*)ic->d_lcol, (void*)ic->d_ucol, (void*)ic->d_l64, (void*)ic->d_dg64,
+                  (void*)ic->d_adiag, (void*)ic->d_l32, (void*)ic->d_u32, (void*)ic->d_dinv, (void*)ic->d_qc32,
+                  (void*)ic->d_status, (void*)ic->d_part, (void*)ic->d_scal, (void*)ic->d_active,
+                  (void*)ic->d_iters, (void*)ic->d_work, (void*)ic->d_r64, (void*)ic->d_b32, (void*)ic->d_x32,
+                  (void*)ic->d_lanczos, (void*)ic->d_klw})
+    if (p) hipFree(p);
+  delete ic;
+}
+
+int spde_ic_info(const spde_ic* ic, int64_t* nlow, int32_t* ncolours) {
+  if (!ic) {
+    set_error("spde_ic_info: invalid argument");
+    return SPDE_ERR_ARG;
+  }
+  if (nlow) *nlow = ic->nlow;
+  if (ncolours) *ncolours = kColours;
+  return SPDE_OK;
+}
+
+int spde_ic_export(const spde_ic* ic, int32_t* colour, int64_t* lptr, int32_t* lcol, int64_t* lch, int64_t* uptr,
+                   int32_t* ucol) {
+  if (!ic) {
+    set_error("spde_ic_export: invalid argument");
+    return SPDE_ERR_ARG;
+  }
+  if (colour) std::memcpy(colour, ic->h_colour.data(), sizeof(int32_t) * ic->n);
+  if (lptr) std::memcpy(lptr, ic->h_lptr.data(), sizeof(int64_t) * (ic->n + 1));
+  if (lcol) std::memcpy(lcol, ic->h_lcol.data(), sizeof(int32_t) * ic->nlow);
+  if (lch) std::memcpy(lch, ic->h_lch.data(), sizeof(int64_t) * ic->nlow);
+  if (uptr) std::memcpy(uptr, ic->h_uptr.data(), sizeof(int64_t) * (ic->n + 1));
+  if (ucol) std::memcpy(ucol, ic->h_ucol.data(), sizeof(int32_t) * ic->nlow);
+  return SPDE_OK;
+}
+
+int spde_ic_factor(spde_ic* ic, const double* qval, const double* diag_add, double shift, double* logdet_m,
+                   void* stream) {
+  if (!ic || !qval || shift < 0.0) {
+    set_error("spde_ic_factor: invalid argument");
+    return SPDE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  spde_plan* pl = ic->pl;
+  const int64_t n = ic->n;
+  ic->qval = qval;
+  ic->diag_add = diag_add;
+  ic->factored = false;
+  hipLaunchKernelGGL(qc_values_kernel, dim3(blocks_of(n)), dim3(256), 0, s, n, pl->d_indptr, pl->d_diag, qval,
+                     diag_add, ic->d_adiag, ic->d_qc32);
+  SPDE_HIP(hipMemsetAsync(ic->d_status, 0, sizeof(int), s));
+  for (int c = 0; c < kColours; ++c) {
+    const int64_t r0 = ic->coff[c], r1 = ic->coff[c + 1];
+    if (r1 > r0)
+      hipLaunchKernelGGL(ic_factor_kernel, dim3(blocks_of(r1 - r0)), dim3(256), 0, s, r0, r1, ic->d_crow, ic->d_lptr,
+                         ic->d_lcol, ic->d_lsrc, qval, ic->d_adiag, shift, ic->d_l64, ic->d_dg64, ic->d_status);
+  }
+  SPDE_HIP(hipGetLastError());
+  int st = 0;
+  SPDE_HIP(hipMemcpyAsync(&st, ic->d_status, sizeof(int), hipMemcpyDeviceToHost, s));
+  SPDE_HIP(hipStreamSynchronize(s));
+  if (st) {
+    set_error("spde_ic_factor: non-positive pivot at lattice index " + std::to_string(st - 1) +
+              " (raise the diagonal shift)");
+    return SPDE_ERR_NOT_SPD;
+  }
+  return spde_ic_correct(ic, nullptr, logdet_m, stream);
+}
+
+int spde_ic_correct(spde_ic* ic, const float* corr, double* logdet_m, void* stream) {
+  if (!ic || !ic->qval) {
+    set_error("spde_ic_correct: no factorisation");
+    return SPDE_ERR_STATE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n = ic->n;
+  if (corr) {
+    hipLaunchKernelGGL(ic_correct_lower_kernel, dim3(blocks_of(n)), dim3(256), 0, s, n, ic->d_lptr, ic->d_lch,
+                       ic->d_adiag, corr, ic->d_l64);
+    hipLaunchKernelGGL(ic_correct_diag_kernel, dim3(blocks_of(n)), dim3(256), 0, s, n, corr, ic->d_dg64);
+  }
+  hipLaunchKernelGGL(ic_cast_lower_kernel, dim3(blocks_of(ic->nlow)), dim3(256), 0, s, ic->nlow, ic->d_l64, ic->d_l32);
+  hipLaunchKernelGGL(ic_cast_upper_kernel, dim3(blocks_of(ic->nlow)), dim3(256), 0, s, ic->nlow, ic->d_ulow,
+                     ic->d_l64, ic->d_u32);
+  hipLaunchKernelGGL(ic_cast_diag_kernel, dim3(blocks_of(n)), dim3(256), 0, s, n, ic->d_dg64, ic->d_dinv);
+  const unsigned g = grid_for(n);
+  hipLaunchKernelGGL(logdet_diag_kernel, dim3(g), dim3(256), 0, s, n, ic->d_dg64, ic->d_part);
+  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(64), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64);
+  SPDE_HIP(hipGetLastError());
+  double ld = 0.0;
+  SPDE_HIP(hipMemcpyAsync(&ld, ic->d_scal + S_X * 64, sizeof(double), hipMemcpyDeviceToHost, s));
+  SPDE_HIP(hipStreamSynchronize(s));
+  if (logdet_m) *logdet_m = 2.0 * ld;
+  ic->factored = true;
+  return SPDE_OK;
+}
+
+int spde_ic_values(spde_ic* ic, double* lower, double* diag, double* adiag, void* stream) {
+  if (!ic || !ic->factored) {
+    set_error("spde_ic_values: no factorisation");
+    return SPDE_ERR_STATE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (lower) SPDE_HIP(hipMemcpyAsync(lower, ic->d_l64, sizeof(double) * ic->nlow, hipMemcpyDeviceToDevice, s));
+  if (diag) SPDE_HIP(hipMemcpyAsync(diag, ic->d_dg64, sizeof(double) * ic->n, hipMemcpyDeviceToDevice, s));
+  if (adiag) SPDE_HIP(hipMemcpyAsync(adiag, ic->d_adiag, sizeof(double) * ic->n, hipMemcpyDeviceToDevice, s));
+  return SPDE_OK;
+}
+
+int spde_ic_apply(spde_ic* ic, const float* r, float* z, int k, void* stream) {
+  const int logk = log2_of(k);
+  if (!ic || !ic->factored || !r || !z || logk < 0) {
+    set_error("spde_ic_apply: invalid argument (k a power of two <= 64, after spde_ic_factor)");
+    return SPDE_ERR_ARG;
+  }
+  ic_apply(ic, r, z, logk, (hipStream_t)stream);
+  SPDE_HIP(hipGetLastError());
+  return SPDE_OK;
+}
+
+int spde_ic_sweep(spde_ic* ic, int which, const float* in, float* out, int k, void* stream) {
+  const int logk = log2_of(k);
+  if (!ic || !ic->factored || !in || !out || logk < 0 || (which != 1 && which != 2)) {
+    set_error("spde_ic_sweep: invalid argument");
+    return SPDE_ERR_ARG;
+  }
+  if (which == 1) ic_fwd(ic, in, out, logk, (hipStream_t)stream);
+  else ic_bwd(ic, in, out, logk, (hipStream_t)stream);
+  SPDE_HIP(hipGetLastError());
+  return SPDE_OK;
+}
+
+int spde_ic_spmm(spde_ic* ic, const float* x, float* y, int k, void* stream) {
+  const int logk = log2_of(k);
+  if (!ic || !ic->factored || !x || !y || logk < 0) {
+    set_error("spde_ic_spmm: invalid argument");
+    return SPDE_ERR_ARG;
+  }
+  hipLaunchKernelGGL(spmm32_kernel, dim3(grid_for(ic->n << logk)), dim3(256), 0, (hipStream_t)stream, ic->n, logk,
+                     ic->pl->d_indptr, ic->pl->d_indices, ic->d_qc32, x, y, nullptr);
+  SPDE_HIP(hipGetLastError());
+  return SPDE_OK;
+}
+
+int spde_pcg_solve(spde_ic* ic, const float* B, float* X, int k, double tol, int maxit, int32_t* iters,
+                   void* stream) {
+  const int logk = log2_of(k);
+  if (!ic || !ic->factored || !B || !X || logk < 0 || !(tol > 0.0) || maxit < 1) {
+    set_error("spde_pcg_solve: invalid argument (k a power of two <= 64, after spde_ic_factor)");
+    return SPDE_ERR_ARG;
+  }
+  return pcg_run(ic, B, X, logk, tol, maxit, iters, (hipStream_t)stream);
+}
+
+int spde_pcg_refine(spde_ic* ic, const double* b, double* x, double tol, double inner_tol, int max_rounds, int maxit,
+                    int32_t* iters, int32_t* rounds, double* relres, void* stream) {
+  if (!ic || !ic->factored || !b || !x || !(tol > 0.0) || !(inner_tol > 0.0) || max_rounds < 1 || maxit < 1) {
+    set_error("spde_pcg_refine: invalid argument");
+    return SPDE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  int rc = ensure_work(ic);
+  if (rc) return rc;
+  spde_plan* pl = ic->pl;
+  const int64_t n = ic->n;
+  const unsigned g = grid_for(n);
+  // ||b||^2 through the residual of x = 0
+  SPDE_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+  hipLaunchKernelGGL(resid64_kernel, dim3(g), dim3(256), 0, s, n, pl->d_indptr, pl->d_indices, pl->d_diag, ic->qval,
+                     ic->diag_add, b, x, ic->d_r64, ic->d_part);
+  double bn = 0.0, rn = 0.0;
+  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(64), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64);
+  SPDE_HIP(hipMemcpyAsync(&bn, ic->d_scal + S_X * 64, sizeof(double), hipMemcpyDeviceToHost, s));
+  SPDE_HIP(hipStreamSynchronize(s));
+  rn = bn;
+  int total = 0, r = 0;
+  while (r < max_rounds && rn > tol * tol * bn && bn > 0.0) {
+    hipLaunchKernelGGL(scale_down_kernel, dim3(g), dim3(256), 0, s, n, (int)g, ic->d_part, ic->d_r64, ic->d_b32);
+    int32_t it = 0;
+    rc = pcg_run(ic, ic->d_b32, ic->d_x32, 0, inner_tol, maxit, &it, s);
+    if (rc) return rc;
+    total += it;
+    // the partials still hold ||r||^2 of this round (pcg_run reuses d_part):
+    // recompute them from r64 before scaling the correction
+    hipLaunchKernelGGL(resid64_kernel, dim3(g), dim3(256), 0, s, n, pl->d_indptr, pl->d_indices, pl->d_diag,
+                       ic->qval, ic->diag_add, b, x, ic->d_r64, ic->d_part);
+    hipLaunchKernelGGL(correct_kernel, dim3(g), dim3(256), 0, s, n, (int)g, ic->d_part, ic->d_x32, x);
+    hipLaunchKernelGGL(resid64_kernel, dim3(g), dim3(256), 0, s, n, pl->d_indptr, pl->d_indices, pl->d_diag,
+                       ic->qval, ic->diag_add, b, x, ic->d_r64, ic->d_part);
+    hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(64), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64);
+    SPDE_HIP(hipMemcpyAsync(&rn, ic->d_scal + S_X * 64, sizeof(double), hipMemcpyDeviceToHost, s));
+    SPDE_HIP(hipStreamSynchronize(s));
+    ++r;
+  }
+  SPDE_HIP(hipGetLastError());
+  if (iters) *iters = total;
+  if (rounds) *rounds = r;
+  if (relres) *relres = bn > 0.0 ? std::sqrt(rn / bn) : 0.0;
+  return SPDE_OK;
+}
+
+int spde_rademacher(int64_t n, int k, uint64_t seed, float scale, float* V, void* stream) {
+  if (n < 1 || k < 1 || !V) {
+    set_error("spde_rademacher: invalid argument");
+    return SPDE_ERR_ARG;
+  }
+  const int64_t units = n * k;
+  hipLaunchKernelGGL(rademacher_kernel, dim3(grid_for(units)), dim3(256), 0, (hipStream_t)stream, units, seed, scale,
+                     V);
+  SPDE_HIP(hipGetLastError());
+  return SPDE_OK;
+}
+
+int spde_slq(spde_ic* ic, int k, int steps, uint64_t seed, double* alpha, double* beta, void* stream) {
+  const int logk = log2_of(k);
+  if (!ic || !ic->factored || logk < 0 || steps < 1 || !alpha || !beta) {
+    set_error("spde_slq: invalid argument (k a power of two <= 64, after spde_ic_factor)");
+    return SPDE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  int rc = ensure_work(ic);
+  if (rc) return rc;
+  if (ic->lanczos_cap < (int64_t)steps * 64 * 2) {
+    if (ic->d_lanczos) SPDE_HIP(hipFree(ic->d_lanczos));
+    ic->d_lanczos = nullptr;
+    SPDE_HIP(hipMalloc((void**)&ic->d_lanczos, sizeof(double) * steps * 64 * 2));
+    ic->lanczos_cap = (int64_t)steps * 64 * 2;
+  }
+  spde_plan* pl = ic->pl;
+  const int64_t n = ic->n, units = n << logk;
+  float* Qp = ic->d_work;
+  float* Qc = Qp + n * 64;
+  float* W = Qc + n * 64;
+  float* U = W + n * 64;
+  const unsigned g = grid_for(units);
+  // q_0 = v / sqrt(n) for Rademacher v; q_-1 = 0; beta_-1 = 0
+  hipLaunchKernelGGL(rademacher_kernel, dim3(g), dim3(256), 0, s, units, seed, (float)(1.0 / std::sqrt((double)n)),
+                     Qc);
+  hipLaunchKernelGGL(fill_kernel, dim3(g), dim3(256), 0, s, units, 0.f, Qp);
+  SPDE_HIP(hipMemsetAsync(ic->d_scal + S_BETA * 64, 0, sizeof(double) * 64, s));
+  double* da = ic->d_lanczos;
+  double* db = ic->d_lanczos + (int64_t)steps * 64;
+  for (int j = 0; j < steps; ++j) {
+    ic_bwd(ic, Qc, W, logk, s);                                                  // W = L^-T q
+    hipLaunchKernelGGL(spmm32_kernel, dim3(g), dim3(256), 0, s, n, logk, pl->d_indptr, pl->d_indices, ic->d_qc32, W,
+                       U, nullptr);                                              // U = Q_c W
+    ic_fwd(ic, U, W, logk, s);                                                   // W = L^-1 U
+    hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, Qc, W, ic->d_part);
+    hipLaunchKernelGGL(lanczos_alpha_kernel, dim3(1), dim3(64), 0, s, k, (int)g, ic->d_part, ic->d_scal,
+                       da + (int64_t)j * 64);
+    hipLaunchKernelGGL(lanczos_orth_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, Qc, Qp, W, ic->d_part);
+    hipLaunchKernelGGL(lanczos_beta_kernel, dim3(1), dim3(64), 0, s, k, (int)g, ic->d_part, ic->d_scal,
+                       db + (int64_t)j * 64);
+    hipLaunchKernelGGL(lanczos_next_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, Qc, Qp, W);
+  }
+  SPDE_HIP(hipGetLastError());
+  std::vector<double> ha((size_t)steps * 64), hb((size_t)steps * 64);
+  SPDE_HIP(hipMemcpyAsync(ha.data(), da, sizeof(double) * steps * 64, hipMemcpyDeviceToHost, s));
+  SPDE_HIP(hipMemcpyAsync(hb.data(), db, sizeof(double) * steps * 64, hipMemcpyDeviceToHost, s));
+  SPDE_HIP(hipStreamSynchronize(s));
+  for (int j = 0; j < steps; ++j)
+    for (int r = 0; r < k; ++r) {
+      alpha[(int64_t)r * steps + j] = ha[(size_t)j * 64 + r];
+      beta[(int64_t)r * steps + j] = hb[(size_t)j * 64 + r];
+    }
+  return SPDE_OK;
+}
+
+
+int spde_ic_features(spde_ic* ic, float* feat, void* stream) {
+  if (!ic || !ic->qval || !feat) {
+    set_error("spde_ic_features: invalid argument (after spde_ic_factor)");
+    return SPDE_ERR_ARG;
+  }
+  spde_plan* pl = ic->pl;
+  hipLaunchKernelGGL(features_kernel, dim3(blocks_of(ic->n)), dim3(256), 0, (hipStream_t)stream, ic->n, pl->d_indptr,
+                     pl->d_indices, pl->d_code, ic->qval, ic->diag_add, ic->d_adiag, feat);
+  SPDE_HIP(hipGetLastError());
+  return SPDE_OK;
+}
+
+int spde_ic_kl_grad(spde_ic* ic, const double* x, int k, double scale, float* dout, double* loss, void* stream) {
+  const int logk = log2_of(k);
+  if (!ic || !ic->factored || !x || !dout || logk < 0) {
+    set_error("spde_ic_kl_grad: invalid argument (k a power of two <= 64, after spde_ic_factor)");
+    return SPDE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n = ic->n;
+  if (!ic->kl_cap || ic->kl_cap < (n << logk)) {
+    if (ic->d_klw) SPDE_HIP(hipFree(ic->d_klw));
+    ic->d_klw = nullptr;
+    SPDE_HIP(hipMalloc((void**)&ic->d_klw, sizeof(double) * (n << logk)));
+    ic->kl_cap = n << logk;
+  }
+  SPDE_HIP(hipMemsetAsync(dout, 0, sizeof(float) * n * kPairs, s));
+  const unsigned g = grid_for(n << logk);
+  hipLaunchKernelGGL(kl_w_kernel, dim3(g), dim3(256), 0, s, n, logk, ic->d_uptr, ic->d_ucol, ic->d_ulow, ic->d_l64,
+                     ic->d_dg64, x, ic->d_klw, ic->d_part);
+  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(64), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64 + 1);
+  const unsigned g2 = grid_for(n);
+  hipLaunchKernelGGL(kl_grad_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, logk, ic->d_lptr, ic->d_lcol,
+                     ic->d_lch, ic->d_dg64, ic->d_adiag, x, ic->d_klw, scale, dout);
+  hipLaunchKernelGGL(logdet_diag_kernel, dim3(g2), dim3(256), 0, s, n, ic->d_dg64, ic->d_part);
+  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(64), 0, s, (int)g2, ic->d_part, ic->d_scal + S_X * 64 + 2);
+  SPDE_HIP(hipGetLastError());
+  if (loss) {
+    double h[2];
+    SPDE_HIP(hipMemcpyAsync(h, ic->d_scal + S_X * 64 + 1, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+    SPDE_HIP(hipStreamSynchronize(s));
+    *loss = 0.5 * h[0] / k - h[1];
+  }
+  return SPDE_OK;
+}
+
+
+int spde_ic_jeffreys_grad(spde_ic* ic, const double* x, const float* z, int k, double scale, float* dout,
+                          double* loss, void* stream) {
+  const int logk = log2_of(k);
+  if (!ic || !ic->factored || !x || !z || !dout || logk < 0) {
+    set_error("spde_ic_jeffreys_grad: invalid argument (k a power of two <= 64, after spde_ic_factor)");
+    return SPDE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n = ic->n;
+  int rc = ensure_work(ic);
+  if (rc) return rc;
+  if (!ic->kl_cap || ic->kl_cap < (n << logk)) {
+    if (ic->d_klw) SPDE_HIP(hipFree(ic->d_klw));
+    ic->d_klw = nullptr;
+    SPDE_HIP(hipMalloc((void**)&ic->d_klw, sizeof(double) * (n << logk)));
+    ic->kl_cap = n << logk;
+  }
+  float* Y = ic->d_work;
+  float* V = Y + n * 64;
+  float* U = V + n * 64;
+  SPDE_HIP(hipMemsetAsync(dout, 0, sizeof(float) * n * kPairs, s));
+  const unsigned g = grid_for(n << logk);
+  hipLaunchKernelGGL(kl_w_kernel, dim3(g), dim3(256), 0, s, n, logk, ic->d_uptr, ic->d_ucol, ic->d_ulow, ic->d_l64,
+                     ic->d_dg64, x, ic->d_klw, ic->d_part);
+  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(64), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64 + 1);
+  ic_bwd(ic, z, Y, logk, s);                                                     // y = L^-T z
+  hipLaunchKernelGGL(spmm32_kernel, dim3(g), dim3(256), 0, s, n, logk, ic->pl->d_indptr, ic->pl->d_indices,
+                     ic->d_qc32, Y, V, ic->d_part);                               // v = A y, partials of y^T A y
+  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(64), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64 + 2);
+  ic_fwd(ic, V, U, logk, s);                                                     // u = L^-1 A y
+  hipLaunchKernelGGL(jeffreys_grad_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, logk, ic->d_lptr,
+                     ic->d_lcol, ic->d_lch, ic->d_dg64, ic->d_adiag, x, ic->d_klw, Y, U, scale, dout);
+  SPDE_HIP(hipGetLastError());
+  if (loss) {
+    double h[2];
+    SPDE_HIP(hipMemcpyAsync(h, ic->d_scal + S_X * 64 + 1, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+    SPDE_HIP(hipStreamSynchronize(s));
+    *loss = 0.5 * h[0] / k + 0.5 * h[1] / k - (double)n;
+  }
+  return SPDE_OK;
+}
+
+int spde_hutch_pattern(spde_ic* ic, const float* U, const float* V, int k, double alpha, double beta, double* sigma,
+                       void* stream) {
+  const int logk = log2_of(k);
+  if (!ic || !U || !V || !sigma || logk < 0) {
+    set_error("spde_hutch_pattern: invalid argument");
+    return SPDE_ERR_ARG;
+  }
+  hipLaunchKernelGGL(hutch_pattern_kernel, dim3((unsigned)((ic->n + 3) / 4)), dim3(256), 0, (hipStream_t)stream, ic->n,
+                     logk, ic->pl->d_indptr, ic->pl->d_indices, U, V, alpha, beta, sigma);
+  SPDE_HIP(hipGetLastError());
+  return SPDE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,286 @@
+"""Mixed-precision iterative evaluation of the log-likelihood and its gradient
+(the body of ``logLike``, advection_idiffusion2D.py:167-207 and
+var_advection_var_diffusion2D.py:201-237) without factorising Q_c:
+
+  * log|Q| and its gradient exactly, from the spatial matrices of
+    Q = F^T D F on the one-slice plan (as the exact path);
+  * mu_c = Q_c^-1 tau S^T y by FP32 PCG with an IC(0) preconditioner in an
+    18-colour ordering and FP64 iterative refinement to ||r|| <= tol ||b||;
+  * log|Q_c| = log|L L^T| + tr log(L^-1 Q_c L^-T), the trace by stochastic
+    Lanczos quadrature over Rademacher probes (its standard error reported);
+  * tr(Q_c^-1 dQ_i) by the Hutchinson estimate of Q_c^-1 on Q's pattern from
+    PCG solves with Rademacher probes (the reference's estimator, :182-189),
+    contracted through the same adjoint kernels as the exact path.
+
+The preconditioner may carry a learned correction (``spdepy_amd.learned``).
+All arithmetic runs in the native library; torch provides device buffers."""
+import ctypes
+
+import numpy as np
+
+from . import _native as nat
+from .plan import _ptr, _stream
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("spdepy_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def slq_quadrature(alpha, beta, n):
+    """Gauss quadrature of log over each probe's Lanczos tridiagonal:
+    n * sum_i (e_1^T y_i)^2 log(theta_i); returns (mean, standard error,
+    per-probe values).  A probe's tridiagonal stops at its first zero beta."""
+    vals = []
+    for a, b in zip(alpha, beta):
+        m = len(a)
+        for j in range(m - 1):
+            if not b[j] > 0:
+                m = j + 1
+                break
+        T = np.diag(a[:m]) + np.diag(b[:m - 1], 1) + np.diag(b[:m - 1], -1)
+        th, Y = np.linalg.eigh(T)
+        vals.append(n * float(np.sum(Y[0] ** 2 * np.log(np.maximum(th, 1e-300)))))
+    vals = np.asarray(vals)
+    se = float(vals.std(ddof=1) / np.sqrt(len(vals))) if len(vals) > 1 else float("inf")
+    return float(vals.mean()), se, vals
+
+
+class IterativeSolver:
+    """The native IC(0) / PCG context of one space-time plan."""
+
+    def __init__(self, plan):
+        self.plan = plan
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().spde_ic_create(plan._h, ctypes.byref(h)))
+        self._h = h
+        nl, nc = ctypes.c_int64(), ctypes.c_int32()
+        nat.check(nat.lib().spde_ic_info(h, ctypes.byref(nl), ctypes.byref(nc)))
+        self.nlow, self.ncolours = int(nl.value), int(nc.value)
+        self.n = plan.n
+        self.shift = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            nat.lib().spde_ic_destroy(h)
+            self._h = None
+
+    def export(self):
+        n, nl = self.n, self.nlow
+        out = dict(colour=np.zeros(n, np.int32), lptr=np.zeros(n + 1, np.int64), lcol=np.zeros(max(nl, 1), np.int32),
+                   lch=np.zeros(max(nl, 1), np.int64), uptr=np.zeros(n + 1, np.int64),
+                   ucol=np.zeros(max(nl, 1), np.int32))
+        nat.check(nat.lib().spde_ic_export(self._h, *[out[k].ctypes.data for k in
+                                                      ("colour", "lptr", "lcol", "lch", "uptr", "ucol")]))
+        return out
+
+    def factor(self, qval, diag_add=None, shift=0.1, max_shift=10.0):
+        """IC(0) of Q_c with the diagonal scaled by (1 + shift); on a
+        non-positive pivot the shift is tripled (up to max_shift).  Returns
+        log|L L^T|; the shift used is kept in self.shift."""
+        torch = _torch()
+        ld = ctypes.c_double()
+        while True:
+            try:
+                nat.check(nat.lib().spde_ic_factor(self._h, _ptr(qval), _ptr(diag_add), float(shift),
+                                                   ctypes.byref(ld), _stream(torch)))
+                break
+            except nat.NotPositiveDefinite:
+                if shift * 3 > max_shift:
+                    raise
+                shift *= 3
+        self.shift = shift
+        self._keep = (qval, diag_add)
+        return ld.value
+
+    def correct(self, corr):
+        """Applies a learned correction (device float tensor, n x 22) to the
+        current factor; returns the new log|L L^T|."""
+        torch = _torch()
+        ld = ctypes.c_double()
+        nat.check(nat.lib().spde_ic_correct(self._h, _ptr(corr), ctypes.byref(ld), _stream(torch)))
+        return ld.value
+
+    def values(self):
+        torch = _torch()
+        lo = torch.empty(max(self.nlow, 1), dtype=torch.float64, device="cuda")
+        dg = torch.empty(self.n, dtype=torch.float64, device="cuda")
+        ad = torch.empty(self.n, dtype=torch.float64, device="cuda")
+        nat.check(nat.lib().spde_ic_values(self._h, _ptr(lo), _ptr(dg), _ptr(ad), _stream(torch)))
+        return lo[:self.nlow], dg, ad
+
+    @staticmethod
+    def _k(X):
+        k = 1 if X.dim() == 1 else X.shape[1]
+        if k & (k - 1) or k > 64:
+            raise ValueError("right-hand side count must be a power of two <= 64")
+        return k
+
+    def apply(self, R):
+        torch = _torch()
+        Z = torch.empty_like(R)
+        nat.check(nat.lib().spde_ic_apply(self._h, _ptr(R), _ptr(Z), self._k(R), _stream(torch)))
+        return Z
+
+    def sweep(self, which, X):
+        """which = 1: L^-1 X; which = 2: L^-T X."""
+        torch = _torch()
+        Y = torch.empty_like(X)
+        nat.check(nat.lib().spde_ic_sweep(self._h, which, _ptr(X), _ptr(Y), self._k(X), _stream(torch)))
+        return Y
+
+    def spmm(self, X):
+        torch = _torch()
+        Y = torch.empty_like(X)
+        nat.check(nat.lib().spde_ic_spmm(self._h, _ptr(X), _ptr(Y), self._k(X), _stream(torch)))
+        return Y
+
+    def solve(self, B, tol=1e-6, maxit=5000):
+        """FP32 PCG: X (n x k float32, row-major) and the iterations per column."""
+        torch = _torch()
+        B = B.contiguous()
+        k = self._k(B)
+        X = torch.empty_like(B)
+        it = np.zeros(k, np.int32)
+        nat.check(nat.lib().spde_pcg_solve(self._h, _ptr(B), _ptr(X), k, float(tol), int(maxit), it.ctypes.data,
+                                           _stream(torch)))
+        return X, it
+
+    def refine(self, b, tol=1e-10, inner_tol=1e-5, max_rounds=10, maxit=5000):
+        """FP64 solution of Q_c x = b by iterative refinement around FP32 PCG."""
+        torch = _torch()
+        b = b.contiguous()
+        x = torch.empty_like(b)
+        it, rounds, rel = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        nat.check(nat.lib().spde_pcg_refine(self._h, _ptr(b), _ptr(x), float(tol), float(inner_tol),
+                                            int(max_rounds), int(maxit), ctypes.byref(it), ctypes.byref(rounds),
+                                            ctypes.byref(rel), _stream(torch)))
+        return x, dict(iters=int(it.value), rounds=int(rounds.value), relres=float(rel.value))
+
+    def rademacher(self, k, seed, scale=1.0):
+        torch = _torch()
+        V = torch.empty((self.n, k), dtype=torch.float32, device="cuda")
+        nat.check(nat.lib().spde_rademacher(self.n, k, ctypes.c_uint64(seed), ctypes.c_float(scale), _ptr(V),
+                                            _stream(torch)))
+        return V
+
+    def slq(self, k, steps, seed):
+        torch = _torch()
+        a = np.zeros((k, steps))
+        b = np.zeros((k, steps))
+        nat.check(nat.lib().spde_slq(self._h, k, steps, ctypes.c_uint64(seed), a.ctypes.data, b.ctypes.data,
+                                     _stream(torch)))
+        return a, b
+
+    def hutch_pattern(self, U, V, out=None, alpha=1.0, beta=0.0):
+        """out = beta out + alpha (estimate of Q_c^-1 on the pattern from U = Q_c^-1 V)."""
+        torch = _torch()
+        sig = torch.empty(self.plan.nnz, dtype=torch.float64, device="cuda") if out is None else out
+        nat.check(nat.lib().spde_hutch_pattern(self._h, _ptr(U), _ptr(V), self._k(U), float(alpha), float(beta),
+                                               _ptr(sig), _stream(torch)))
+        return sig
+
+
+def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32, slq_steps=50, hutch_probes=64,
+               seed=0, tol=1e-10, probe_tol=1e-6, shift=0.1, correction=None, contract=None, adv_dir=None):
+    """(-loglik/(m r), parts) as ``Engine.loglik`` returns them, with log|Q_c|
+    from stochastic Lanczos quadrature and the trace part of the gradient from
+    the Hutchinson estimate.  correction(engine, coeffs, qval, diag_add) ->
+    device float tensor (n x 22) applies a learned preconditioner correction.
+    parts gains "logdet_qc_se" (standard error of the SLQ estimate),
+    "pcg" (iteration counts) and "ic_shift"."""
+    torch = _torch()
+    from .engine import OUT_DOT00, OUT_TAUTR, OUT_WIDTH, _obs_device
+    if engine.is_spatial:
+        raise ValueError("the iterative path is for space-time lattices")
+    y = np.asarray(data, dtype=np.float64)
+    y2 = y.reshape(y.shape[0], -1)
+    m, r = y2.shape
+    tau = float(np.exp(log_tau))
+    n, T = engine.n, engine.plan.shape[2]
+    it = engine.iterative()
+    qval = engine.assemble(coeffs)
+    obs = _obs_device(engine, torch, n, np.asarray(obs_cols), y2)
+    diag_add = obs["counts"] * tau
+    sp = engine.spatial
+    q0 = sp.assemble_spatial(engine.plan, 0)
+    gram = sp.assemble_spatial(engine.plan, 1)
+    sp.factor_async(0, q0, owner=engine)
+    sp.factor_async(1, gram, owner=engine)
+    if grad:
+        s_q0 = torch.empty(sp.nnz, dtype=torch.float64, device="cuda")
+        s_gram = torch.empty(sp.nnz, dtype=torch.float64, device="cuda")
+        sp.selinv(0, s_q0)
+        sp.selinv(1, s_gram)
+    ld_m = it.factor(qval, diag_add, shift=shift)
+    if correction is not None:
+        ld_m = it.correct(correction(engine, coeffs, qval, diag_add))
+    b_dev = obs["sty"] * tau
+    mus, info = [], []
+    for c in range(r):
+        x, inf = it.refine(b_dev[:, c].contiguous(), tol=tol)
+        mus.append(x)
+        info.append(inf)
+    mu = torch.stack(mus, 1)
+    qmu = engine.plan.spmv(qval, mu)
+    alpha, beta = it.slq(slq_probes, slq_steps, seed)
+    trlog, se, _ = slq_quadrature(alpha, beta, n)
+    ld_qc = ld_m + trlog
+    resid = obs["y"] - mu[obs["cols"]]
+    ld_q0 = sp.factor_finish(0)
+    ld_gram = sp.factor_finish(1)
+    logq = engine.plan.logq_sum(1.0 / (coeffs.c.dt * coeffs.c.sigma))
+    ld_q = ld_q0 + (T - 1) * (ld_gram + logq)
+    quad = float((mu * qmu).sum())
+    rss = float((resid * resid).sum())
+    like = 0.5 * ld_q * r + m * r * np.log(tau) / 2 - 0.5 * ld_qc * r - 0.5 * quad - tau / 2 * rss
+    parts = dict(m=m, r=r, tau=tau, rss=rss, ld_q=ld_q, ld_qc=ld_qc, qval=qval, mu=mu, like_raw=like,
+                 logdet_qc_se=se, logdet_m=ld_m, ic_shift=it.shift, pcg=dict(mu=info))
+    if not grad:
+        return -like / (m * r), parts
+    # Hutchinson estimate of Q_c^-1 on the pattern, in batches of up to 64 probes
+    sqc, iters, done = None, [], 0
+    while done < hutch_probes:
+        k = min(64, hutch_probes - done)
+        k = 1 << (k.bit_length() - 1)
+        V = it.rademacher(k, seed + 1 + done)
+        U, its = it.solve(V, tol=probe_tol)
+        iters += list(its)
+        sqc = it.hutch_pattern(U, V, out=sqc, alpha=k / hutch_probes, beta=0.0 if sqc is None else 1.0)
+        done += k
+    parts["pcg"]["probes"] = np.asarray(iters)
+    g = nat.GradArgs()
+    g.mode, g.nh1, g.r = 0, 0, r
+    g.qval = qval.data_ptr()
+    mu_cm = mu.t().contiguous()
+    g.mu = mu_cm.data_ptr()
+    g.diag_add = diag_add.data_ptr()
+    g.coeffs = coeffs.c
+    g.sigma_qc = sqc.data_ptr()
+    g.sigma_q = None
+    g.sigma_gram, g.sigma_q0, g.spatial = s_gram.data_ptr(), s_q0.data_ptr(), sp._h
+    keep = [mu_cm, sqc, s_gram, s_q0]
+    if adv_dir is not None:
+        adv = torch.as_tensor(np.ascontiguousarray(adv_dir, dtype=np.float64).reshape(-1)).cuda()
+        g.adv_dir = adv.data_ptr()
+        keep.append(adv)
+    out = torch.empty(engine.Ns * OUT_WIDTH, dtype=torch.float64, device="cuda")
+    nat.check(nat.lib().spde_grad_cells(engine.plan._h, g, _ptr(out), _stream(torch)))
+    if contract is not None:
+        parts["coef_grad"], colsum = contract(out)
+    else:
+        cells = out.cpu().numpy().reshape(engine.Ns, OUT_WIDTH)
+        parts["cells"] = cells
+        colsum = cells.sum(0)
+    parts["colsum"] = colsum
+    # <G, Q> through tr(Q_c^-1 Q) = n - tr(Q_c^-1 tau S^T S): the estimate then
+    # needs only the observed diagonal of Q_c^-1
+    dot_gq = -0.5 * r * (n - colsum[OUT_TAUTR]) - 0.5 * quad
+    parts["g_sigma"] = -(dot_gq - colsum[OUT_DOT00]) - 0.5 * r * (T - 1) * engine.plan.shape[0] * \
+        engine.plan.shape[1]
+    parts["g_tau"] = m * r / 2 - 0.5 * r * colsum[OUT_TAUTR] - tau / 2 * rss
+    del keep
+    return -like / (m * r), parts

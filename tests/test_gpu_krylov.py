@@ -1,0 +1,187 @@
+"""GPU checks of the iterative path (pcg.hip through the C ABI) against the
+oracle (oracle/krylov.py) on the same inputs: IC(0) values, triangular sweeps,
+FP32 PCG residuals, FP64 refinement, Rademacher probes, Lanczos coefficients
+and the SLQ log-determinant, the Hutchinson pattern estimate, and the
+iterative log-likelihood against the exact path."""
+import numpy as np
+import pytest
+from scipy import sparse
+from scipy.sparse import linalg as spla
+
+from oracle import fixtures as F, krylov as K
+from tests.gpu_util import product_case, step
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(M=12, N=10, T=6, m=60, bc=1):
+    import torch
+    c = F.small(M, N, T, bc=bc, m=m)
+    mod = product_case(c)
+    mod.initFit(c["y"], idx=c["idx"], fitQ0=False)
+    par = c["par"]
+    eng = mod._engine()
+    coeffs = mod._coeffs(par)
+    qval = eng.assemble(coeffs)
+    cols = mod._obs_cols()
+    tau = np.exp(par[-1])
+    dadd = np.bincount(cols, minlength=eng.n).astype(np.float64) * tau
+    dadd_d = torch.as_tensor(dadd).cuda()
+    # Q_c on the structural pattern with its explicit zeros kept (scipy's
+    # sparse addition would drop them, and IC(0) depends on the pattern)
+    e = eng.plan.export()
+    data = qval.cpu().numpy().copy()
+    rows = np.repeat(np.arange(eng.n), np.diff(e["indptr"]))
+    data[np.nonzero(rows == e["indices"])[0]] += dadd
+    A = sparse.csr_matrix((data, e["indices"], e["indptr"]), shape=(eng.n, eng.n))
+    return dict(c=c, mod=mod, eng=eng, coeffs=coeffs, qval=qval, dadd=dadd_d, A=A, col=K.colours(M, N, T),
+                coeffs_keep=coeffs)
+
+
+@pytest.fixture(scope="module")
+def case():
+    step("iterative path: setup 12x10x6")
+    return _setup()
+
+
+def test_ic0_values_match_oracle(case):
+    eng = case["eng"]
+    it = eng.iterative()
+    step("spde_ic_factor 12x10x6")
+    it.factor(case["qval"], case["dadd"], shift=0.1)
+    assert it.shift == 0.1
+    lo, dg, ad = it.values()
+    ex = it.export()
+    L, d = K.ic0(case["A"], case["col"], 0.1)
+    lo = lo.cpu().numpy()
+    rows = np.repeat(np.arange(eng.n), np.diff(ex["lptr"]))
+    ref = np.asarray(L[rows, ex["lcol"][:it.nlow]]).ravel()
+    scale = np.abs(ref).max()
+    assert np.allclose(lo, ref, rtol=1e-11, atol=1e-12 * scale)
+    assert np.allclose(dg.cpu().numpy(), d, rtol=1e-12)
+    assert np.allclose(ad.cpu().numpy(), case["A"].diagonal(), rtol=1e-15)
+    assert np.array_equal(ex["colour"], case["col"])
+
+
+def test_sweeps_apply_and_spmm(case):
+    import torch
+    eng = case["eng"]
+    it = eng.iterative()
+    it.factor(case["qval"], case["dadd"], shift=0.1)
+    L, d = K.ic0(case["A"], case["col"], 0.1)
+    Fm = K.full_factor(L, d).toarray()   # lower in colour order, indexed by lattice indices
+    R = torch.randn(eng.n, 4, dtype=torch.float32, device="cuda")
+    Rh = R.cpu().numpy().astype(np.float64)
+    order = np.lexsort((np.arange(eng.n), case["col"]))
+    Fo = Fm[np.ix_(order, order)]
+    step("sweeps")
+    y = it.sweep(1, R).cpu().numpy()
+    yo = np.empty_like(Rh)
+    yo[order] = np.linalg.solve(Fo, Rh[order])
+    assert np.allclose(y, yo, rtol=1e-4, atol=1e-4 * np.abs(yo).max())
+    x = it.sweep(2, R).cpu().numpy()
+    xo = np.empty_like(Rh)
+    xo[order] = np.linalg.solve(Fo.T, Rh[order])
+    assert np.allclose(x, xo, rtol=1e-4, atol=1e-4 * np.abs(xo).max())
+    z = it.apply(R).cpu().numpy()
+    zo = np.empty_like(Rh)
+    zo[order] = np.linalg.solve(Fo.T, np.linalg.solve(Fo, Rh[order]))
+    assert np.allclose(z, zo, rtol=1e-4, atol=1e-4 * np.abs(zo).max())
+    Y = it.spmm(R).cpu().numpy()
+    assert np.allclose(Y, case["A"] @ Rh, rtol=1e-5, atol=1e-5 * np.abs(case["A"] @ Rh).max())
+
+
+@pytest.mark.parametrize("k", [1, 8, 64])
+def test_pcg_solve_residual(case, k):
+    import torch
+    eng = case["eng"]
+    it = eng.iterative()
+    it.factor(case["qval"], case["dadd"], shift=0.1)
+    B = torch.randn(eng.n, k, dtype=torch.float32, device="cuda")
+    step("pcg k=%d" % k)
+    X, iters = it.solve(B, tol=1e-6, maxit=3000)
+    Bh = B.cpu().numpy().astype(np.float64)
+    Xh = X.cpu().numpy().astype(np.float64)
+    res = np.linalg.norm(Bh - case["A"] @ Xh, axis=0) / np.linalg.norm(Bh, axis=0)
+    print("  iterations", iters.min(), iters.max(), "max relres", res.max())
+    assert np.all(iters > 0) and np.all(iters < 3000)
+    assert res.max() < 5e-5
+
+
+def test_refinement_reaches_fp64(case):
+    import torch
+    eng = case["eng"]
+    it = eng.iterative()
+    it.factor(case["qval"], case["dadd"], shift=0.1)
+    b = torch.randn(eng.n, dtype=torch.float64, device="cuda")
+    step("refine")
+    x, info = it.refine(b, tol=1e-10)
+    bh, xh = b.cpu().numpy(), x.cpu().numpy()
+    rel = np.linalg.norm(bh - case["A"] @ xh) / np.linalg.norm(bh)
+    print("  refinement", info, "host relres %.2e" % rel)
+    assert info["relres"] <= 1e-10 and rel <= 2e-10
+    xd = spla.spsolve(case["A"].tocsc(), bh)
+    assert np.allclose(xh, xd, rtol=1e-7, atol=1e-8 * np.abs(xd).max())
+
+
+def test_rademacher_and_hutch_pattern(case):
+    import torch
+    eng = case["eng"]
+    it = eng.iterative()
+    step("rademacher + hutch pattern")
+    V = it.rademacher(8, seed=123)
+    assert np.array_equal(V.cpu().numpy(), K.rademacher(eng.n, 8, 123).astype(np.float32))
+    U = torch.randn(eng.n, 8, dtype=torch.float32, device="cuda")
+    sig = it.hutch_pattern(U, V).cpu().numpy()
+    e = eng.plan.export()
+    Uh, Vh = U.cpu().numpy().astype(np.float64), V.cpu().numpy().astype(np.float64)
+    rows = np.repeat(np.arange(eng.n), np.diff(e["indptr"]))
+    cols = e["indices"]
+    ref = 0.5 * ((Uh[rows] * Vh[cols]).sum(1) + (Uh[cols] * Vh[rows]).sum(1)) / 8
+    assert np.allclose(sig, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_lanczos_and_slq(case):
+    eng = case["eng"]
+    it = eng.iterative()
+    ld_m = it.factor(case["qval"], case["dadd"], shift=0.1)
+    L, d = K.ic0(case["A"], case["col"], 0.1)
+    assert abs(ld_m - 2 * np.log(d).sum()) <= 1e-11 * abs(ld_m)
+    step("slq 32 x 40")
+    a, b = it.slq(32, 40, seed=5)
+    n = eng.n
+    order = np.lexsort((np.arange(n), case["col"]))
+    Fo = K.full_factor(L, d).toarray()[np.ix_(order, order)]
+    Ao = case["A"].toarray()[np.ix_(order, order)]
+    Bm = np.linalg.solve(Fo, np.linalg.solve(Fo, Ao).T)
+    V = K.rademacher(n, 32, 5)[order] / np.sqrt(n)
+    a0, b0 = K.lanczos(lambda x: Bm @ x, V[:, 0], 6)
+    assert np.allclose(a[0, :6], a0, rtol=2e-4)
+    assert np.allclose(b[0, :5], b0[:5], rtol=2e-3)
+    est, se, _ = K.slq_logdet(a, b, n)
+    exact = np.linalg.slogdet(Ao)[1] - ld_m
+    print("  SLQ %.6f +- %.6f exact %.6f" % (est, se, exact))
+    assert abs(est - exact) <= 4 * se + 1e-5 * abs(np.linalg.slogdet(Ao)[1])
+
+
+def test_pcg_loglike_against_exact(case):
+    mod, par = case["mod"], case["c"]["par"]
+    step("logLike exact vs pcg 12x10x6")
+    l_ex, j_ex = mod.logLike(par)
+    l_it, j_it = mod.logLike(par, method="pcg", seed=3, hutch_probes=512)
+    parts = mod.last_parts
+    m, r = parts["m"], parts["r"]
+    se_like = 0.5 * r * parts["logdet_qc_se"] / (m * r)
+    rel_g = np.linalg.norm(j_it - j_ex) / np.linalg.norm(j_ex)
+    print("  like exact %.10f pcg %.10f (se %.2e)  grad rel err %.3f  pcg %s" % (l_ex, l_it, se_like, rel_g,
+                                                                              parts["pcg"]["mu"]))
+    assert abs(l_it - l_ex) <= 4 * se_like + 1e-9 * abs(l_ex)
+    assert parts["pcg"]["mu"][0]["relres"] <= 1e-10
+    assert rel_g < 0.3     # 512 Hutchinson probes on n = 720
+    # mu_c from the refined PCG solve equals the exact path's
+    mu_it = parts["mu"].cpu().numpy()
+    mod.logLike(par, grad=False)
+    eng = mod._engine()
+    like_ex, p_ex = eng.loglik(mod._coeffs(par), mod._obs_cols(), mod.data, par[-1], grad=False)
+    mu_ex = p_ex["mu"].cpu().numpy()
+    assert np.allclose(mu_it, mu_ex, rtol=1e-8, atol=1e-9 * np.abs(mu_ex).max())
