@@ -123,6 +123,42 @@ typedef struct spde_coeffs {
 /* Values of Q on the structural pattern (qval[nnz_struct], device). */
 int spde_assemble(spde_plan* plan, const spde_coeffs* c, double* qval, void* stream);
 
+/* out = F^T D^(1/2) z with Q = F^T D F the precision of the last
+ * spde_assemble on the plan (same c): F has V^(-1/2) A0 on the first time
+ * block, A / V on the others and -I below the diagonal, D = blockdiag(I,
+ * diag(q) / (dt sigma), ...).  For z ~ N(0, I) the result has covariance Q:
+ * with tau S^T (y + eps / sqrt(tau)) added (spde_add_rows) it is the
+ * right-hand side whose solve with Q_c is a conditional sample of x | y
+ * (in place of the reference's P^T L^-T z + mu, model.py:73-87, for
+ * solvers without a factor).  z, out: device FP64, row-major n x K. */
+int spde_fform_apply_t(spde_plan* plan, const spde_coeffs* c, const double* z, double* out, int K, void* stream);
+/* out[rows[a] * K + r] += w[a * K + r] (device; rows distinct, in [0, n)). */
+int spde_add_rows(int64_t n, int K, int64_t nrows, const int64_t* rows, const double* w, double* out, void* stream);
+
+/* Direction of one entry of the reference's dQ list (makeQ(par, grad=True),
+ * var_advection_var_diffusion2D.py:118-197, advection_idiffusion2D.py:118-163):
+ * kind 0 perturbs the coefficient fields -- dkappa (per cell, device, or the
+ * constant dkappa_c when NULL), dH (has_dH: per-cell faces W,E,S,N x 2 x 2,
+ * device, or the constant dH_c), the advection derivative stencil adv_diff
+ * (1 d/dwx, 2 d/dwy; with dG per-cell faces E,N,W,S for variable advection);
+ * kind 1 is log sigma; kind 2 the initial-state operator (dkappa0, dH0). */
+typedef struct spde_direction {
+  int kind;
+  const double* dkappa;
+  double dkappa_c;
+  const double* dH;
+  double dH_c[4];
+  int has_dH;
+  const double* dG;
+  int adv_diff;
+  double dkappa0, dH0[4];
+} spde_direction;
+/* dqval[nnz_struct] (device) = dQ along the direction at the coefficients of
+ * the last spde_assemble on the plan (the same `c`): the product rule through
+ * the block formula, exact to rounding (one-slice plans: dQ0). */
+int spde_assemble_tangent(spde_plan* plan, const spde_coeffs* c, const spde_direction* dir, double* dqval,
+                          void* stream);
+
 /* Spatial matrices of the closed-form log-determinant, on a one-slice plan
  * (spde_plan_create with T = 1, same M, N, bc): which = 0 writes
  * Q0 = A0^T A0 / V, which = 1 writes B^T B with B = A / V, both from the last
@@ -162,12 +198,20 @@ int spde_factor_async(spde_plan* plan, int slot, const double* qval, const doubl
                       void* stream);
 int spde_factor_finish(spde_plan* plan, int slot, double* logdet);
 
-/* Solves with the factor in `slot`.  mode: 0 Q^-1 B, 1 L^-1 P B,
- * 2 P^T L^-T B (the reference's apply_Pt(solve_Lt(.))).  B device,
- * n x nrhs column-major with leading dimension ldb, overwritten. */
+/* Solves with the factor in `slot` (P Q P^T = L L^T, P this library's
+ * nested-dissection ordering).  mode: 0 Q^-1 B, 1 L^-1 P B, 2 P^T L^-T B
+ * (the reference's apply_Pt(solve_Lt(.)) as one call, model.py:79-80),
+ * 3 P B and 4 P^T B (CHOLMOD's apply_P / apply_Pt; these need no factor),
+ * 5 L^-1 B and 6 L^-T B (CHOLMOD's solve_L / solve_Lt with
+ * use_LDLt_decomposition=False: B and the result in elimination order).
+ * B device, n x nrhs column-major with leading dimension ldb, overwritten. */
 #define SPDE_SOLVE_A 0
 #define SPDE_SOLVE_L 1
 #define SPDE_SOLVE_LT 2
+#define SPDE_SOLVE_P 3
+#define SPDE_SOLVE_PT 4
+#define SPDE_SOLVE_L_NOPERM 5
+#define SPDE_SOLVE_LT_NOPERM 6
 int spde_solve(spde_plan* plan, int slot, int mode, double* B, int nrhs, int64_t ldb,
                void* stream);
 
@@ -250,6 +294,29 @@ typedef struct spde_loglik_out {
 int spde_loglik_grad(spde_plan* plan, spde_plan* spatial, const spde_coeffs* coeffs, const int64_t* obs_cols,
                      const double* y, int64_t m, int r, double log_tau, double* cells, spde_loglik_out* out,
                      void* stream);
+
+/* The same evaluation in two halves: spde_loglik_enqueue returns once the
+ * work is enqueued (every scalar it needs is reduced on the device and copied
+ * to pinned host memory); spde_loglik_finish waits for it and fills `out`
+ * (SPDE_ERR_NOT_SPD names the lattice index of a failed pivot).  One
+ * evaluation per plan may be in flight. */
+int spde_loglik_enqueue(spde_plan* plan, spde_plan* spatial, const spde_coeffs* coeffs, const int64_t* obs_cols,
+                        const double* y, int64_t m, int r, double log_tau, double* cells, void* stream);
+int spde_loglik_finish(spde_plan* plan, spde_loglik_out* out);
+/* mu_c of the last evaluation on the plan (device, n x r column-major). */
+int spde_loglik_mu(spde_plan* plan, double* mu, int r, void* stream);
+/* Many models in one call (the loop of the reference's kriging drivers,
+ * examples/server/get_results.py:6-51, as one batch): model i is
+ * (plans[i], spatials[i], coeffs[i], obs_cols[i], y[i], m[i], log_taus[i]),
+ * its per-cell adjoints go to cells[i] and its scalars to outs[i].  Models are
+ * grouped into waves whose plans are distinct; a wave's models are enqueued
+ * on streams[i mod nstreams] before any is finished, so small models share
+ * the chip.  status[i] (host) = each model's return code; the call returns
+ * the first non-zero one. */
+int spde_loglik_batch(int B, spde_plan* const* plans, spde_plan* const* spatials, const spde_coeffs* coeffs,
+                      const int64_t* const* obs_cols, const double* const* y, const int64_t* m, int r,
+                      const double* log_taus, double* const* cells, spde_loglik_out* outs,
+                      void* const* streams, int nstreams, int32_t* status);
 
 /* ---------------------------------------------------------------- separable models
  * Q = Qt (x) Qs (seperable_spatial_temporal2D.py:80-81, _ha2D.py, _idiffusion2D.py:
@@ -357,6 +424,20 @@ int spde_pcg_solve(spde_ic* ic, const float* B, float* X, int k, double tol, int
  * iterations, *rounds, *relres = ||b - Q_c x|| / ||b|| (host). */
 int spde_pcg_refine(spde_ic* ic, const double* b, double* x, double tol, double inner_tol, int max_rounds,
                     int maxit, int32_t* iters, int32_t* rounds, double* relres, void* stream);
+/* X = Q_c^-1 B for k right-hand sides (k a power of two <= 64; B, X device
+ * FP64, row-major n x k, entry i * k + r) by iterative refinement around the
+ * FP32 block PCG: per column r = b - Q_c x in FP64, an inner solve of
+ * Q_c d = r / ||r|| to inner_tol, x += ||r|| d, until ||r|| <= tol ||b|| for
+ * every column or max_rounds; a column within tol takes no further inner
+ * iterations.  iters[k], relres[k] (host): inner iterations and
+ * ||b - Q_c x|| / ||b|| per column; *rounds (host). */
+int spde_pcg_refine_block(spde_ic* ic, const double* B, double* X, int k, double tol, double inner_tol,
+                          int max_rounds, int maxit, int32_t* iters, int32_t* rounds, double* relres, void* stream);
+/* Z[u] ~ N(0, 1) for u < n k (device FP64): Box-Muller on two 53-bit uniforms,
+ * U1 = ((h1 >> 11) + 0.5) 2^-53, U2 = (h2 >> 11) 2^-53 with
+ * h1 = splitmix64(seed ^ splitmix64(2u)), h2 = splitmix64(seed ^ splitmix64(2u + 1)),
+ * Z[u] = sqrt(-2 log U1) cos(2 pi U2). */
+int spde_gaussian(int64_t n, int k, uint64_t seed, double* Z, void* stream);
 /* V[u] = +-scale for u = i * k + r: the sign is the top bit of
  * splitmix64(seed ^ splitmix64(u)) (1: negative). */
 int spde_rademacher(int64_t n, int k, uint64_t seed, float scale, float* V, void* stream);

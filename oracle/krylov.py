@@ -13,6 +13,12 @@ its pieces in numpy so the kernels can be checked on the same inputs:
   * ``splitmix64``    -- the probe hash (pinned by the published first outputs
                          of Vigna's splitmix64 generator);
   * ``rademacher``    -- the probe matrix the GPU draws for a seed;
+  * ``gaussian``      -- the normal draws of the conditional samplers (Box-Muller
+                         on splitmix64 uniforms), pinned by moments and the
+                         Box-Muller identity;
+  * ``fform_t``       -- F^T D^(1/2) z for Q = F^T D F (the right-hand side of a
+                         factor-free conditional sample), pinned by
+                         (F^T D^(1/2)) (F^T D^(1/2))^T = Q against makeQ;
   * ``slq_logdet``    -- Gauss quadrature of log over Lanczos tridiagonals
                          (Ubaru, Chen, Saad 2017), pinned by exact log-determinants;
   * ``pcg``           -- textbook preconditioned conjugate gradients.
@@ -89,6 +95,42 @@ def rademacher(n, k, seed, scale=1.0):
     h = splitmix64(np.uint64(seed) ^ splitmix64(u))
     s = np.where((h >> np.uint64(63)) != 0, -scale, scale)
     return s.reshape(n, k)
+
+
+def uniforms(n, k, seed):
+    """The two 53-bit uniforms of unit u = i k + r: U1 = ((h1 >> 11) + 0.5) 2^-53,
+    U2 = (h2 >> 11) 2^-53 with h1, h2 = splitmix64(seed ^ splitmix64(2u)), (2u + 1)."""
+    u = np.arange(n * k, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        h1 = splitmix64(np.uint64(seed) ^ splitmix64(np.uint64(2) * u))
+        h2 = splitmix64(np.uint64(seed) ^ splitmix64(np.uint64(2) * u + np.uint64(1)))
+    u1 = ((h1 >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0 ** -53
+    u2 = (h2 >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    return u1.reshape(n, k), u2.reshape(n, k)
+
+
+def gaussian(n, k, seed):
+    """n x k standard normals: Box-Muller sqrt(-2 log U1) cos(2 pi U2)."""
+    u1, u2 = uniforms(n, k, seed)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def fform_t(A, q, A0, V, T, sigma, dt, z):
+    """F^T D^(1/2) z for Q = F^T D F with F_00 = V^(-1/2) A0, F_tt = A / V
+    (t >= 1), F_(t+1,t) = -I and D = blockdiag(I, q / (dt sigma), ...)
+    (the factorisation ``models.sample_fform`` inverts): with w_0 = z_0 and
+    w_t = sqrt(q / (dt sigma)) z_t,  (F^T w)_t = F_tt^T w_t - w_(t+1)."""
+    Ns = A.shape[0]
+    z = np.asarray(z, dtype=np.float64).reshape(T, Ns, -1)
+    s = np.sqrt(q / (dt * sigma))[:, None]
+    w = z.copy()
+    w[1:] *= s
+    out = np.empty_like(w)
+    out[0] = (A0.T @ w[0]) / np.sqrt(V)
+    for t in range(1, T):
+        out[t] = (A.T @ w[t]) / V
+    out[:-1] -= w[1:]
+    return out.reshape(T * Ns, -1)
 
 
 def slq_logdet(alpha, beta, n):

@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(_HERE, "libspdepy_amd.so")
 SPDE_OK, SPDE_ERR_ARG, SPDE_ERR_HIP, SPDE_ERR_NOT_SPD, SPDE_ERR_ALLOC, SPDE_ERR_STATE = range(6)
 SPDE_PLAN_HOST_ONLY = 1
 SPDE_PLAN_WIDE_TIME = 2
-SOLVE_A, SOLVE_L, SOLVE_LT = 0, 1, 2
+SOLVE_A, SOLVE_L, SOLVE_LT, SOLVE_P, SOLVE_PT, SOLVE_L_NOPERM, SOLVE_LT_NOPERM = 0, 1, 2, 3, 4, 5, 6
 
 c_i64p = ctypes.POINTER(ctypes.c_int64)
 c_i32p = ctypes.POINTER(ctypes.c_int32)
@@ -47,6 +47,12 @@ class Coeffs(ctypes.Structure):
                 ("mix", ctypes.c_int)]
 
 
+class Direction(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("dkappa", vp), ("dkappa_c", ctypes.c_double), ("dH", vp),
+                ("dH_c", ctypes.c_double * 4), ("has_dH", ctypes.c_int), ("dG", vp), ("adv_diff", ctypes.c_int),
+                ("dkappa0", ctypes.c_double), ("dH0", ctypes.c_double * 4)]
+
+
 class LoglikOut(ctypes.Structure):
     _fields_ = [("like", ctypes.c_double), ("like_raw", ctypes.c_double), ("logdet_q", ctypes.c_double),
                 ("logdet_qc", ctypes.c_double), ("g_sigma", ctypes.c_double), ("g_tau", ctypes.c_double)]
@@ -71,10 +77,17 @@ _SIGS = {
     "spde_plan_export": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "spde_assemble": (ctypes.c_int, [vp, ctypes.POINTER(Coeffs), vp, vp]),
     "spde_assemble_spatial": (ctypes.c_int, [vp, vp, ctypes.c_int, vp, vp]),
+    "spde_assemble_tangent": (ctypes.c_int, [vp, ctypes.POINTER(Coeffs), ctypes.POINTER(Direction), vp, vp]),
     "spde_logq_sum": (ctypes.c_int, [vp, ctypes.c_double, c_dp, vp]),
     "spde_assemble_kron": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
     "spde_loglik_grad": (ctypes.c_int, [vp, vp, ctypes.POINTER(Coeffs), vp, vp, ctypes.c_int64, ctypes.c_int,
                                         ctypes.c_double, vp, ctypes.POINTER(LoglikOut), vp]),
+    "spde_loglik_enqueue": (ctypes.c_int, [vp, vp, ctypes.POINTER(Coeffs), vp, vp, ctypes.c_int64, ctypes.c_int,
+                                           ctypes.c_double, vp, vp]),
+    "spde_loglik_finish": (ctypes.c_int, [vp, ctypes.POINTER(LoglikOut)]),
+    "spde_loglik_mu": (ctypes.c_int, [vp, vp, ctypes.c_int, vp]),
+    "spde_loglik_batch": (ctypes.c_int, [ctypes.c_int, vp, vp, vp, vp, vp, vp, ctypes.c_int, vp, vp, vp, vp,
+                                         ctypes.c_int, vp]),
     "spde_kron_reduce": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_double, vp, vp, ctypes.c_int, vp, vp, vp,
                                         vp]),
     "spde_fields_eval": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int] + [vp] * 8 + [vp]),
@@ -110,6 +123,11 @@ _SIGS = {
     "spde_pcg_refine": (ctypes.c_int, [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                        c_i32p, c_i32p, c_dp, vp]),
     "spde_rademacher": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_uint64, ctypes.c_float, vp, vp]),
+    "spde_pcg_refine_block": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]),
+    "spde_gaussian": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_uint64, vp, vp]),
+    "spde_fform_apply_t": (ctypes.c_int, [vp, ctypes.POINTER(Coeffs), vp, vp, ctypes.c_int, vp]),
+    "spde_add_rows": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int64, vp, vp, vp, vp]),
     "spde_slq": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, vp, vp, vp]),
     "spde_hutch_pattern": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp, vp]),
     "spde_ic_features": (ctypes.c_int, [vp, vp, vp]),

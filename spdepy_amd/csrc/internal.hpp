@@ -170,6 +170,8 @@ struct spde_plan {
   int64_t split_solve = 0;        // ... of the solve launches (current nrhs)
   double* d_gpart = nullptr;      // per (cell, slice) partial windows of the gradient reduction
   void* llws = nullptr;           // buffers and side stream of spde_loglik_grad (loglik.hip)
+  void* extra = nullptr;          // device launch lists of the factor and selected inverse (numeric.hip)
+  double* d_tan = nullptr;        // tangent assembly: templates + dA, dA0, dq of one direction
   int32_t* d_sprow = nullptr;     // one-slice plans: row and column of each entry (Kronecker kernels)
   int32_t* d_spcol = nullptr;
   int64_t work_doubles = 0;       // per-slot workspace size

@@ -72,13 +72,7 @@ struct PlanExtra {            // kept beside the plan (not in the public struct)
   std::vector<hipEvent_t> ev_pre;
 };
 
-std::vector<std::pair<spde_plan*, PlanExtra*>> g_extra;
-
-PlanExtra* extra(spde_plan* p) {
-  for (auto& e : g_extra)
-    if (e.first == p) return e.second;
-  return nullptr;
-}
+PlanExtra* extra(spde_plan* p) { return static_cast<PlanExtra*>(p->extra); }
 
 thread_local bool g_host_only_build = false;  // plan descriptors built for inspection only
 
@@ -629,9 +623,10 @@ Bases make_bases(spde_plan* pl, int slot, double* x) {
 }
 
 // Device arenas of a factor slot, allocated on its first factorisation.  The
-// fronts and Z are cleared once: later factorisations clear only the lower
-// triangles, and nothing writes above a front's diagonal.
-int ensure_slot(spde_plan* pl, int slot) {
+// fronts and Z are cleared once (on the factorisation's stream): later
+// factorisations clear only the lower triangles, and nothing writes above a
+// front's diagonal.
+int ensure_slot(spde_plan* pl, int slot, hipStream_t stream) {
   Slot& sl = pl->slots[slot];
   if (sl.front) return SPDE_OK;
   const Symbolic& sym = pl->sym;
@@ -650,9 +645,11 @@ int ensure_slot(spde_plan* pl, int slot) {
               std::to_string((fb + zb) >> 20) + " MiB failed");
     return SPDE_ERR_ALLOC;
   }
-  SPDE_HIP(hipMemset(sl.front, 0, fb));
-  SPDE_HIP(hipMemset(sl.z, 0, zb));
-  SPDE_HIP(hipMemset(sl.status, 0, sizeof(int)));
+  // cleared on the stream of the factorisation that follows, so the clear is
+  // ordered before it whatever stream the caller uses
+  SPDE_HIP(hipMemsetAsync(sl.front, 0, fb, stream));
+  SPDE_HIP(hipMemsetAsync(sl.z, 0, zb, stream));
+  SPDE_HIP(hipMemsetAsync(sl.status, 0, sizeof(int), stream));
   return SPDE_OK;
 }
 
@@ -699,7 +696,7 @@ int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int 
   pl->host_only = (flags & SPDE_PLAN_HOST_ONLY) != 0;
   pl->nslots = nslots;
   PlanExtra* ex = new PlanExtra();
-  g_extra.push_back({pl, ex});
+  pl->extra = ex;
   if (pl->host_only) {
     // build and bounds-check every launch schedule without touching the device
     g_host_only_build = true;
@@ -765,10 +762,7 @@ int spde_plan_create(int M, int N, int T, int bc, int leaf_max, int nslots, int 
       for (int a = 0; a < nt; ++a)
         for (int b = 0; b <= a && b < pt; ++b) zt.push_back(int4{(int)q, a, b, 0});
     }
-    if (zt.size() > (size_t)INT32_MAX) {
-      set_error("front tile list too long");
-      return SPDE_ERR_STATE;
-    }
+    if (zt.size() > (size_t)INT32_MAX) TRY((set_error("front tile list too long"), SPDE_ERR_STATE));
     ex->nztiles = (int)zt.size();
     TRY(upload(&ex->d_ztiles, zt));
   }
@@ -805,19 +799,14 @@ void spde_plan_destroy(spde_plan* pl) {
   if (!pl) return;
   if (!pl->host_only) spde::loglik_release(pl);
   if (pl->host_only) {
-    for (size_t i = 0; i < g_extra.size(); ++i)
-      if (g_extra[i].first == pl) {
-        delete g_extra[i].second;
-        g_extra.erase(g_extra.begin() + i);
-        break;
-      }
+    delete extra(pl);
   } else {
     hipDeviceSynchronize();
     for (auto* p : {(void*)pl->d_indptr, (void*)pl->d_indices, (void*)pl->d_code, (void*)pl->d_diag,
                     (void*)pl->d_map, (void*)pl->d_mapsym, (void*)pl->d_perm, (void*)pl->d_relpos,
                     (void*)pl->d_bnd, (void*)pl->d_nodes, (void*)pl->d_tmpl, (void*)pl->d_acan,
                     (void*)pl->d_a0can, (void*)pl->d_qdiag, (void*)pl->d_win, (void*)pl->d_red, (void*)pl->d_pos25,
-                    (void*)pl->d_sprow, (void*)pl->d_spcol, (void*)pl->d_gpart})
+                    (void*)pl->d_sprow, (void*)pl->d_spcol, (void*)pl->d_gpart, (void*)pl->d_tan})
       hipFree(p);
     for (auto& s : pl->slots) {
       hipFree(s.front);
@@ -849,22 +838,18 @@ void spde_plan_destroy(spde_plan* pl) {
     for (auto& g : pl->trtri) free_launch(g);
     for (auto e : pl->prof_ev) hipEventDestroy(e);
     if (pl->prof_base) hipEventDestroy(pl->prof_base);
-    for (size_t i = 0; i < g_extra.size(); ++i)
-      if (g_extra[i].first == pl) {
-        PlanExtra* ex = g_extra[i].second;
-        for (auto& d : ex->lvl) {
-          hipFree(d.d_nodes);
-          hipFree(d.d_gtiles);
-        }
-        hipFree(ex->d_children);
-        hipFree(ex->d_ztiles);
-        for (auto e : ex->ev_pre) hipEventDestroy(e);
-        if (ex->ev_start) hipEventDestroy(ex->ev_start);
-        if (ex->pre) hipStreamDestroy(ex->pre);
-        delete ex;
-        g_extra.erase(g_extra.begin() + i);
-        break;
+    if (PlanExtra* ex = extra(pl)) {
+      for (auto& d : ex->lvl) {
+        hipFree(d.d_nodes);
+        hipFree(d.d_gtiles);
       }
+      hipFree(ex->d_children);
+      hipFree(ex->d_ztiles);
+      for (auto e : ex->ev_pre) hipEventDestroy(e);
+      if (ex->ev_start) hipEventDestroy(ex->ev_start);
+      if (ex->pre) hipStreamDestroy(ex->pre);
+      delete ex;
+    }
   }
   delete pl;
 }
@@ -932,7 +917,7 @@ int spde_factor_async(spde_plan* pl, int slot, const double* qval, const double*
     return SPDE_ERR_ARG;
   }
   hipStream_t s = (hipStream_t)stream;
-  rc = ensure_slot(pl, slot);
+  rc = ensure_slot(pl, slot, s);
   if (rc) return rc;
   Slot& sl = pl->slots[slot];
   const Symbolic& sym = pl->sym;
@@ -996,9 +981,33 @@ int spde_factor(spde_plan* pl, int slot, const double* qval, const double* diag_
 int spde_solve(spde_plan* pl, int slot, int mode, double* B, int nrhs, int64_t ldb, void* stream) {
   int rc = check_slot(pl, slot);
   if (rc) return rc;
-  if (!B || nrhs <= 0 || ldb < pl->sym.n || mode < 0 || mode > 2) {
+  if (!B || nrhs <= 0 || ldb < pl->sym.n || mode < 0 || mode > SPDE_SOLVE_LT_NOPERM) {
     set_error("spde_solve: invalid argument");
     return SPDE_ERR_ARG;
+  }
+  if (mode == SPDE_SOLVE_P || mode == SPDE_SOLVE_PT) {
+    // the permutation alone: no factor needed, the slot's staging buffer only
+    Slot& sl = pl->slots[slot];
+    const int64_t n = pl->sym.n;
+    if (sl.xy_cap < n * nrhs) {
+      SPDE_HIP(hipDeviceSynchronize());
+      hipFree(sl.xy);
+      sl.xy = nullptr;
+      sl.xy_cap = 0;
+      SPDE_HIP(hipMalloc((void**)&sl.xy, sizeof(double) * n * nrhs));
+      sl.xy_cap = n * nrhs;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (mode == SPDE_SOLVE_P) {
+      launch_permute_in(B, ldb, pl->d_perm, n, nrhs, sl.xy, s);
+      SPDE_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double), sl.xy, n * sizeof(double), n * sizeof(double), nrhs,
+                                hipMemcpyDeviceToDevice, s));
+    } else {
+      SPDE_HIP(hipMemcpy2DAsync(sl.xy, n * sizeof(double), B, ldb * sizeof(double), n * sizeof(double), nrhs,
+                                hipMemcpyDeviceToDevice, s));
+      launch_permute_out(sl.xy, pl->d_perm, n, nrhs, B, ldb, s);
+    }
+    return SPDE_OK;
   }
   if (!pl->slots[slot].factored) {
     set_error("spde_solve: slot not factored");
@@ -1035,13 +1044,19 @@ int spde_solve(spde_plan* pl, int slot, int mode, double* B, int nrhs, int64_t l
   if (rc) return rc;
   Bases b = make_bases(pl, slot, sl.xy);
   const int nl = (int)pl->levels.size();
-  if (mode == SPDE_SOLVE_LT) {
+  // in: P B for modes A and L; B as it is (already in elimination order) for the others
+  const bool perm_in = mode == SPDE_SOLVE_A || mode == SPDE_SOLVE_L;
+  // out: P^T X for modes A and LT; X in elimination order for L and the unpermuted modes
+  const bool perm_out = mode == SPDE_SOLVE_A || mode == SPDE_SOLVE_LT;
+  const bool fwd = mode == SPDE_SOLVE_A || mode == SPDE_SOLVE_L || mode == SPDE_SOLVE_L_NOPERM;
+  const bool bwd = mode == SPDE_SOLVE_A || mode == SPDE_SOLVE_LT || mode == SPDE_SOLVE_LT_NOPERM;
+  if (perm_in) {
+    launch_permute_in(B, ldb, pl->d_perm, n, nrhs, sl.xy, s);
+  } else {
     SPDE_HIP(hipMemcpy2DAsync(sl.xy, n * sizeof(double), B, ldb * sizeof(double), n * sizeof(double), nrhs,
                               hipMemcpyDeviceToDevice, s));
-  } else {
-    launch_permute_in(B, ldb, pl->d_perm, n, nrhs, sl.xy, s);
   }
-  if (mode != SPDE_SOLVE_LT) {
+  if (fwd) {
     for (int l = 0; l < nl; ++l) {
       LevelPlan& L = pl->levels[l];
       launch_fwd_gather(ex->lvl[l].d_nodes, ex->lvl[l].nn, pl->d_nodes, ex->d_children, pl->d_relpos, sl.xy, n,
@@ -1050,7 +1065,7 @@ int spde_solve(spde_plan* pl, int slot, int mode, double* B, int nrhs, int64_t l
       run_gemm(pl, L.fwd_u, b, s);
     }
   }
-  if (mode != SPDE_SOLVE_L) {
+  if (bwd) {
     for (int l = nl - 1; l >= 0; --l) {
       LevelPlan& L = pl->levels[l];
       launch_bwd_gather(ex->lvl[l].d_nodes, ex->lvl[l].nn, pl->d_nodes, pl->d_bnd, sl.xy, n, nrhs, sl.vec, s);
@@ -1058,11 +1073,11 @@ int spde_solve(spde_plan* pl, int slot, int mode, double* B, int nrhs, int64_t l
       run_gemm(pl, L.bwd_x, b, s);
     }
   }
-  if (mode == SPDE_SOLVE_L) {
+  if (perm_out) {
+    launch_permute_out(sl.xy, pl->d_perm, n, nrhs, B, ldb, s);
+  } else {
     SPDE_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double), sl.xy, n * sizeof(double), n * sizeof(double), nrhs,
                               hipMemcpyDeviceToDevice, s));
-  } else {
-    launch_permute_out(sl.xy, pl->d_perm, n, nrhs, B, ldb, s);
   }
   return SPDE_OK;
 }

@@ -54,8 +54,9 @@ struct spde_ic {
   int* d_active = nullptr;           // [64]
   int* d_iters = nullptr;            // [64]
   float* d_work = nullptr;           // 4 vectors n x 64 (PCG: r, z, p, q; Lanczos: q_prev, q_cur, w, u)
-  double* d_r64 = nullptr;           // refinement residual and solution (n each)
-  float *d_b32 = nullptr, *d_x32 = nullptr;
+  double* d_r64 = nullptr;           // refinement residual (n x 64)
+  float *d_b32 = nullptr, *d_x32 = nullptr;  // inner right-hand side and solution (n x 64)
+  double* d_rn = nullptr;            // refinement: ||r||^2 and threshold per right-hand side [2 * 64]
   double* d_lanczos = nullptr;       // alpha, beta [2 * steps * 64]
   int64_t lanczos_cap = 0;
   double* d_klw = nullptr;           // L'^T x of the KL loss (n x K)
@@ -365,6 +366,67 @@ __global__ __launch_bounds__(256) void correct_kernel(int64_t n, int nblk, const
 __global__ void sum_one_kernel(int nblk, const double* __restrict__ part, double* __restrict__ out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = sum_parts(part, 0, 0, nblk);
 }
+// out[0] = sum over the k right-hand sides of the partials (in order).
+__global__ void sum_rhs_kernel(int nblk, int k, const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int r = 0; r < k; ++r) s += sum_parts(part, 0, r, nblk);
+  out[0] = s;
+}
+
+// FP64 residual R = B - (Q + diag(dadd)) X for k right-hand sides (row-major
+// n x k) with the block partials of ||R_r||^2 per right-hand side.
+__global__ __launch_bounds__(256) void resid64k_kernel(int64_t n, int logk, const int64_t* __restrict__ indptr,
+                                                       const int32_t* __restrict__ indices,
+                                                       const double* __restrict__ qval,
+                                                       const double* __restrict__ dadd, const double* __restrict__ B,
+                                                       const double* __restrict__ X, double* __restrict__ R,
+                                                       double* __restrict__ part) {
+  const int k = 1 << logk;
+  const int64_t units = n << logk;
+  double acc[1] = {0.0};
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = u >> logk;
+    const int r = (int)(u & (k - 1));
+    double s = 0.0;
+    for (int64_t e = indptr[i]; e < indptr[i + 1]; ++e) s += qval[e] * X[((int64_t)indices[e] << logk) + r];
+    if (dadd) s += dadd[i] * X[u];
+    const double v = B[u] - s;
+    R[u] = v;
+    acc[0] += v * v;
+  }
+  block_reduce_store<1>(acc, k, part);
+}
+// out[r] = sum of the partials of right-hand side r (in block order).
+__global__ void norms_kernel(int k, int nblk, const double* __restrict__ part, double* __restrict__ out) {
+  const int r = threadIdx.x;
+  if (r < k) out[r] = sum_parts(part, 0, r, nblk);
+}
+// b32 = r64 / ||r64|| per right-hand side; zero for those already within
+// their threshold (||r||^2 <= thr), whose inner solve is then empty.
+__global__ __launch_bounds__(256) void scale_down_k_kernel(int64_t units, int logk, const double* __restrict__ rn,
+                                                           const double* __restrict__ thr,
+                                                           const double* __restrict__ r64, float* __restrict__ b32) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(u & ((1 << logk) - 1));
+    const double s2 = rn[r];
+    b32[u] = (s2 > thr[r] && s2 > 0.0) ? (float)(r64[u] / sqrt(s2)) : 0.f;
+  }
+}
+// x64 += ||r|| d32 per right-hand side.
+__global__ __launch_bounds__(256) void correct_k_kernel(int64_t units, int logk, const double* __restrict__ rn,
+                                                        const double* __restrict__ thr, const float* __restrict__ d32,
+                                                        double* __restrict__ x64) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(u & ((1 << logk) - 1));
+    const double s2 = rn[r];
+    if (s2 > thr[r] && s2 > 0.0) x64[u] += sqrt(s2) * (double)d32[u];
+  }
+}
+__global__ __launch_bounds__(256) void fill64_kernel(int64_t units, double* __restrict__ x) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x)
+    x[u] = 0.0;
+}
 
 // ------------------------------------------------------------ Lanczos
 // alpha_j = q^T w per right-hand side (partials), then w -= alpha q + beta_prev q_prev
@@ -422,6 +484,20 @@ __global__ __launch_bounds__(256) void rademacher_kernel(int64_t units, uint64_t
   for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t h = splitmix64(seed ^ splitmix64((uint64_t)u));
     v[u] = (h >> 63) ? -scale : scale;
+  }
+}
+
+// z[u] ~ N(0, 1) by Box-Muller from two 53-bit uniforms of the counter:
+// h1 = splitmix64(seed ^ splitmix64(2u)), h2 = splitmix64(seed ^ splitmix64(2u + 1)),
+// U1 = ((h1 >> 11) + 0.5) 2^-53 in (0, 1), U2 = (h2 >> 11) 2^-53,
+// z = sqrt(-2 log U1) cos(2 pi U2).
+__global__ __launch_bounds__(256) void gaussian_kernel(int64_t units, uint64_t seed, double* __restrict__ z) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t h1 = splitmix64(seed ^ splitmix64(2 * (uint64_t)u));
+    const uint64_t h2 = splitmix64(seed ^ splitmix64(2 * (uint64_t)u + 1));
+    const double u1 = ((double)(h1 >> 11) + 0.5) * 0x1.0p-53;
+    const double u2 = (double)(h2 >> 11) * 0x1.0p-53;
+    z[u] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
   }
 }
 
@@ -626,11 +702,12 @@ int ic_bwd(spde_ic* ic, const float* y, float* x, int logk, hipStream_t s) {
 }
 
 int ensure_work(spde_ic* ic) {
-  if (ic->d_work) return SPDE_OK;
-  SPDE_HIP(hipMalloc((void**)&ic->d_work, sizeof(float) * ic->n * 64 * 4));
-  SPDE_HIP(hipMalloc((void**)&ic->d_r64, sizeof(double) * ic->n));
-  SPDE_HIP(hipMalloc((void**)&ic->d_b32, sizeof(float) * ic->n));
-  SPDE_HIP(hipMalloc((void**)&ic->d_x32, sizeof(float) * ic->n));
+  // each buffer once (a failed allocation leaves the others for the next call)
+  if (!ic->d_work) SPDE_HIP(hipMalloc((void**)&ic->d_work, sizeof(float) * ic->n * 64 * 4));
+  if (!ic->d_r64) SPDE_HIP(hipMalloc((void**)&ic->d_r64, sizeof(double) * ic->n * 64));
+  if (!ic->d_b32) SPDE_HIP(hipMalloc((void**)&ic->d_b32, sizeof(float) * ic->n * 64));
+  if (!ic->d_x32) SPDE_HIP(hipMalloc((void**)&ic->d_x32, sizeof(float) * ic->n * 64));
+  if (!ic->d_rn) SPDE_HIP(hipMalloc((void**)&ic->d_rn, sizeof(double) * 64 * 2));
   return SPDE_OK;
 }
 
@@ -864,7 +941,7 @@ void spde_ic_destroy(spde_ic* ic) {
                   (void*)ic->d_adiag, (void*)ic->d_l32, (void*)ic->d_u32, (void*)ic->d_dinv, (void*)ic->d_qc32,
                   (void*)ic->d_status, (void*)ic->d_part, (void*)ic->d_scal, (void*)ic->d_active,
                   (void*)ic->d_iters, (void*)ic->d_work, (void*)ic->d_r64, (void*)ic->d_b32, (void*)ic->d_x32,
-                  (void*)ic->d_lanczos, (void*)ic->d_klw})
+                  (void*)ic->d_rn, (void*)ic->d_lanczos, (void*)ic->d_klw})
     if (p) hipFree(p);
   delete ic;
 }
@@ -1059,6 +1136,74 @@ int spde_pcg_refine(spde_ic* ic, const double* b, double* x, double tol, double 
   return SPDE_OK;
 }
 
+int spde_pcg_refine_block(spde_ic* ic, const double* B, double* X, int k, double tol, double inner_tol,
+                          int max_rounds, int maxit, int32_t* iters, int32_t* rounds, double* relres, void* stream) {
+  const int logk = log2_of(k);
+  if (!ic || !ic->factored || !B || !X || logk < 0 || !(tol > 0.0) || !(inner_tol > 0.0) || max_rounds < 1 ||
+      maxit < 1) {
+    set_error("spde_pcg_refine_block: invalid argument (k a power of two <= 64, after spde_ic_factor)");
+    return SPDE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  int rc = ensure_work(ic);
+  if (rc) return rc;
+  spde_plan* pl = ic->pl;
+  const int64_t n = ic->n, units = n << logk;
+  const unsigned g = grid_for(units);
+  double* rn = ic->d_rn;
+  double* thr = ic->d_rn + 64;
+  hipLaunchKernelGGL(fill64_kernel, dim3(g), dim3(256), 0, s, units, X);
+  hipLaunchKernelGGL(resid64k_kernel, dim3(g), dim3(256), 0, s, n, logk, pl->d_indptr, pl->d_indices, ic->qval,
+                     ic->diag_add, B, X, ic->d_r64, ic->d_part);
+  hipLaunchKernelGGL(norms_kernel, dim3(1), dim3(64), 0, s, k, (int)g, ic->d_part, rn);
+  double bn[64], cur[64], th[64];
+  SPDE_HIP(hipMemcpyAsync(bn, rn, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+  SPDE_HIP(hipStreamSynchronize(s));
+  for (int r = 0; r < k; ++r) {
+    th[r] = tol * tol * bn[r];
+    cur[r] = bn[r];
+    if (iters) iters[r] = 0;
+  }
+  SPDE_HIP(hipMemcpyAsync(thr, th, sizeof(double) * k, hipMemcpyHostToDevice, s));
+  int32_t its[64];
+  int rd = 0;
+  auto pending = [&]() {
+    for (int r = 0; r < k; ++r)
+      if (cur[r] > th[r] && bn[r] > 0.0) return true;
+    return false;
+  };
+  while (rd < max_rounds && pending()) {
+    hipLaunchKernelGGL(scale_down_k_kernel, dim3(g), dim3(256), 0, s, units, logk, rn, thr, ic->d_r64, ic->d_b32);
+    rc = pcg_run(ic, ic->d_b32, ic->d_x32, logk, inner_tol, maxit, its, s);
+    if (rc) return rc;
+    if (iters)
+      for (int r = 0; r < k; ++r) iters[r] += its[r];
+    hipLaunchKernelGGL(correct_k_kernel, dim3(g), dim3(256), 0, s, units, logk, rn, thr, ic->d_x32, X);
+    hipLaunchKernelGGL(resid64k_kernel, dim3(g), dim3(256), 0, s, n, logk, pl->d_indptr, pl->d_indices, ic->qval,
+                       ic->diag_add, B, X, ic->d_r64, ic->d_part);
+    hipLaunchKernelGGL(norms_kernel, dim3(1), dim3(64), 0, s, k, (int)g, ic->d_part, rn);
+    SPDE_HIP(hipMemcpyAsync(cur, rn, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    SPDE_HIP(hipStreamSynchronize(s));
+    ++rd;
+  }
+  SPDE_HIP(hipGetLastError());
+  if (rounds) *rounds = rd;
+  if (relres)
+    for (int r = 0; r < k; ++r) relres[r] = bn[r] > 0.0 ? std::sqrt(cur[r] / bn[r]) : 0.0;
+  return SPDE_OK;
+}
+
+int spde_gaussian(int64_t n, int k, uint64_t seed, double* Z, void* stream) {
+  if (n < 1 || k < 1 || !Z) {
+    set_error("spde_gaussian: invalid argument");
+    return SPDE_ERR_ARG;
+  }
+  const int64_t units = n * k;
+  hipLaunchKernelGGL(gaussian_kernel, dim3(grid_for(units)), dim3(256), 0, (hipStream_t)stream, units, seed, Z);
+  SPDE_HIP(hipGetLastError());
+  return SPDE_OK;
+}
+
 int spde_rademacher(int64_t n, int k, uint64_t seed, float scale, float* V, void* stream) {
   if (n < 1 || k < 1 || !V) {
     set_error("spde_rademacher: invalid argument");
@@ -1202,7 +1347,7 @@ int spde_ic_jeffreys_grad(spde_ic* ic, const double* x, const float* z, int k, d
   ic_bwd(ic, z, Y, logk, s);                                                     // y = L^-T z
   hipLaunchKernelGGL(spmm32_kernel, dim3(g), dim3(256), 0, s, n, logk, ic->pl->d_indptr, ic->pl->d_indices,
                      ic->d_qc32, Y, V, ic->d_part);                               // v = A y, partials of y^T A y
-  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(64), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64 + 2);
+  hipLaunchKernelGGL(sum_rhs_kernel, dim3(1), dim3(64), 0, s, (int)g, k, ic->d_part, ic->d_scal + S_X * 64 + 2);
   ic_fwd(ic, V, U, logk, s);                                                     // u = L^-1 A y
   hipLaunchKernelGGL(jeffreys_grad_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, logk, ic->d_lptr,
                      ic->d_lcol, ic->d_lch, ic->d_dg64, ic->d_adiag, x, ic->d_klw, Y, U, scale, dout);

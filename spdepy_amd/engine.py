@@ -313,10 +313,12 @@ class Engine:
         return -like / (m * r), parts
 
 
-def loglik_fused(engine, coeffs, obs_cols, data, log_tau):
+def loglik_fused(engine, coeffs, obs_cols, data, log_tau, contract=None):
     """The exact-trace evaluation through the single C entry point
-    spde_loglik_grad (closed-form log|Q|); returns (like, parts) like
-    Engine.loglik(..., trace="exact", logdet="closed")."""
+    spde_loglik_grad (closed-form log|Q|): assembly, both factorisations,
+    the selected inverse, mu_c, every reduction and the per-cell adjoints
+    run inside the library; returns (like, parts) like
+    Engine.loglik(..., trace="exact", logdet="closed").  contract as there."""
     torch = _torch()
     y = np.ascontiguousarray(np.asarray(data, dtype=np.float64).reshape(len(obs_cols), -1).T)  # r x m = m x r col-major
     m = len(obs_cols)
@@ -327,9 +329,14 @@ def loglik_fused(engine, coeffs, obs_cols, data, log_tau):
     nat.check(nat.lib().spde_loglik_grad(engine.plan._h, engine.spatial._h, ctypes.byref(coeffs.c),
                                          cols.ctypes.data, y.ctypes.data, m, r, float(log_tau), _ptr(cells),
                                          ctypes.byref(out), _stream(torch)))
-    h = cells.cpu().numpy().reshape(engine.Ns, OUT_WIDTH)
     parts = dict(m=m, r=r, tau=float(np.exp(log_tau)), ld_q=out.logdet_q, ld_qc=out.logdet_qc, like_raw=out.like_raw,
-                 cells=h, colsum=h.sum(0), g_sigma=out.g_sigma, g_tau=out.g_tau)
+                 g_sigma=out.g_sigma, g_tau=out.g_tau)
+    if contract is not None:
+        parts["coef_grad"], parts["colsum"] = contract(cells)
+    else:
+        h = cells.cpu().numpy().reshape(engine.Ns, OUT_WIDTH)
+        parts["cells"] = h
+        parts["colsum"] = h.sum(0)
     return out.like, parts
 
 

@@ -160,6 +160,22 @@ class IterativeSolver:
                                             ctypes.byref(rel), _stream(torch)))
         return x, dict(iters=int(it.value), rounds=int(rounds.value), relres=float(rel.value))
 
+    def refine_block(self, B, tol=1e-10, inner_tol=1e-5, max_rounds=10, maxit=5000):
+        """FP64 X = Q_c^-1 B for k right-hand sides (B: n x k float64 device,
+        row-major; k a power of two <= 64) by iterative refinement around the
+        FP32 block PCG; info holds per-column iterations and ||b - Q_c x|| / ||b||."""
+        torch = _torch()
+        B = B.contiguous()
+        k = self._k(B)
+        X = torch.empty_like(B)
+        its = np.zeros(k, np.int32)
+        rel = np.zeros(k)
+        rounds = np.zeros(1, np.int32)
+        nat.check(nat.lib().spde_pcg_refine_block(self._h, _ptr(B), _ptr(X), k, float(tol), float(inner_tol),
+                                                  int(max_rounds), int(maxit), its.ctypes.data, rounds.ctypes.data,
+                                                  rel.ctypes.data, _stream(torch)))
+        return X, dict(iters=its, rounds=int(rounds[0]), relres=rel)
+
     def rademacher(self, k, seed, scale=1.0):
         torch = _torch()
         V = torch.empty((self.n, k), dtype=torch.float32, device="cuda")
@@ -184,14 +200,84 @@ class IterativeSolver:
         return sig
 
 
+def gaussian(n, k, seed):
+    """n x k standard normal draws (float64 device, row-major) from the
+    library's counter-based generator (spde_gaussian)."""
+    torch = _torch()
+    Z = torch.empty((n, k), dtype=torch.float64, device="cuda")
+    nat.check(nat.lib().spde_gaussian(int(n), int(k), ctypes.c_uint64(seed), _ptr(Z), _stream(torch)))
+    return Z
+
+
+def conditional_rhs(engine, coeffs, obs_cols, y, log_tau, k, seed):
+    """B (n x k float64 device, row-major) = F^T D^(1/2) z1 + S^T (tau y +
+    sqrt(tau) z2) with z1 ~ N(0, I_n) (spde_gaussian with seed) and z2 ~
+    N(0, I_m) (numpy default_rng(seed)): E[B] = tau S^T y = Q_c mu_c and
+    Cov[B] = Q + tau S^T S = Q_c, so Q_c^-1 B holds k draws of x | y.  The
+    operators are those of the last assembly of coeffs on the engine's plan."""
+    torch = _torch()
+    n = engine.n
+    cols = np.asarray(obs_cols, dtype=np.int64)
+    y = np.asarray(y, dtype=np.float64).reshape(-1)
+    if y.size != cols.size:
+        raise ValueError("conditional samples take one realisation (y of length m)")
+    if cols.size and (cols.min() < 0 or cols.max() >= n):
+        raise ValueError("observation index outside the lattice")
+    tau = float(np.exp(log_tau))
+    Z = gaussian(n, k, seed)
+    B = torch.empty_like(Z)
+    nat.check(nat.lib().spde_fform_apply_t(engine.plan._h, ctypes.byref(coeffs.c), _ptr(Z), _ptr(B), int(k),
+                                           _stream(torch)))
+    z2 = np.random.default_rng(seed).normal(size=(cols.size, k))
+    rows, inv = np.unique(cols, return_inverse=True)
+    w = np.zeros((rows.size, k))
+    np.add.at(w, inv, tau * y[:, None] + np.sqrt(tau) * z2)
+    rows_d = torch.as_tensor(rows).cuda()
+    w_d = torch.as_tensor(np.ascontiguousarray(w)).cuda()
+    nat.check(nat.lib().spde_add_rows(int(n), int(k), int(rows.size), _ptr(rows_d), _ptr(w_d), _ptr(B),
+                                      _stream(torch)))
+    torch.cuda.current_stream().synchronize()
+    return B
+
+
+def conditional_samples_pcg(engine, coeffs, obs_cols, y, log_tau, k=64, seed=0, tol=1e-10, shift=0.1,
+                            correction=None, inner_tol=1e-5):
+    """k draws of x | y (n x k float64 device, row-major) without a factor:
+    Q_c^-1 B for the right-hand sides of conditional_rhs, solved by FP64
+    iterative refinement around FP32 PCG with IC(0) (or its learned
+    correction) to ||B - Q_c X|| <= tol ||B|| per column; returns (X, info)."""
+    torch = _torch()
+    from .engine import _obs_device
+    it = engine.iterative()
+    qval = engine.assemble(coeffs)
+    yy = np.asarray(y, dtype=np.float64).reshape(-1)
+    obs = _obs_device(engine, torch, engine.n, np.asarray(obs_cols), yy.reshape(-1, 1))
+    diag_add = obs["counts"] * float(np.exp(log_tau))
+    it.factor(qval, diag_add, shift=shift)
+    if correction is not None:
+        it.correct(correction(engine, coeffs, qval, diag_add))
+    B = conditional_rhs(engine, coeffs, obs_cols, yy, log_tau, k, seed)
+    X, info = it.refine_block(B, tol=tol, inner_tol=inner_tol)
+    info["B"] = B
+    return X, info
+
+
 def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32, slq_steps=50, hutch_probes=64,
-               seed=0, tol=1e-10, probe_tol=1e-6, shift=0.1, correction=None, contract=None, adv_dir=None):
+               seed=0, tol=1e-10, probe_tol=1e-6, shift=0.1, correction=None, contract=None, adv_dir=None,
+               tol_like=None):
     """(-loglik/(m r), parts) as ``Engine.loglik`` returns them, with log|Q_c|
     from stochastic Lanczos quadrature and the trace part of the gradient from
     the Hutchinson estimate.  correction(engine, coeffs, qval, diag_add) ->
     device float tensor (n x 22) applies a learned preconditioner correction.
     parts gains "logdet_qc_se" (standard error of the SLQ estimate),
-    "pcg" (iteration counts) and "ic_shift"."""
+    "pcg" (iteration counts) and "ic_shift".
+
+    tol_like: the relative standard error the value must meet.  The SLQ
+    estimate runs first; when 1/2 r se exceeds tol_like times an upper bound
+    of |loglik| (mu_c minimises mu^T Q mu + tau ||y - S mu||^2, so
+    0 <= mu^T Q mu + tau ||y - S mu||^2 <= tau ||y||^2), the tolerance cannot
+    be met and the call returns (None, parts) with parts["infeasible"] set,
+    before the solves for mu_c and the Hutchinson probes."""
     torch = _torch()
     from .engine import OUT_DOT00, OUT_TAUTR, OUT_WIDTH, _obs_device
     if engine.is_spatial:
@@ -218,6 +304,21 @@ def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32
     ld_m = it.factor(qval, diag_add, shift=shift)
     if correction is not None:
         ld_m = it.correct(correction(engine, coeffs, qval, diag_add))
+    alpha, beta = it.slq(slq_probes, slq_steps, seed)
+    trlog, se, _ = slq_quadrature(alpha, beta, n)
+    ld_qc = ld_m + trlog
+    ld_q0 = sp.factor_finish(0)
+    ld_gram = sp.factor_finish(1)
+    logq = engine.plan.logq_sum(1.0 / (coeffs.c.dt * coeffs.c.sigma))
+    ld_q = ld_q0 + (T - 1) * (ld_gram + logq)
+    if tol_like is not None:
+        known = 0.5 * ld_q * r + m * r * np.log(tau) / 2 - 0.5 * ld_qc * r
+        bound = abs(known) + 0.5 * tau * float((y2 * y2).sum())
+        if 0.5 * r * se > tol_like * bound:
+            if grad:
+                torch.cuda.synchronize()       # the spatial selected inverses are enqueued
+            return None, dict(m=m, r=r, tau=tau, ld_q=ld_q, ld_qc=ld_qc, logdet_qc_se=se, logdet_m=ld_m,
+                              ic_shift=it.shift, infeasible=True, like_bound=bound, pcg=dict(mu=[]))
     b_dev = obs["sty"] * tau
     mus, info = [], []
     for c in range(r):
@@ -226,14 +327,7 @@ def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32
         info.append(inf)
     mu = torch.stack(mus, 1)
     qmu = engine.plan.spmv(qval, mu)
-    alpha, beta = it.slq(slq_probes, slq_steps, seed)
-    trlog, se, _ = slq_quadrature(alpha, beta, n)
-    ld_qc = ld_m + trlog
     resid = obs["y"] - mu[obs["cols"]]
-    ld_q0 = sp.factor_finish(0)
-    ld_gram = sp.factor_finish(1)
-    logq = engine.plan.logq_sum(1.0 / (coeffs.c.dt * coeffs.c.sigma))
-    ld_q = ld_q0 + (T - 1) * (ld_gram + logq)
     quad = float((mu * qmu).sum())
     rss = float((resid * resid).sum())
     like = 0.5 * ld_q * r + m * r * np.log(tau) / 2 - 0.5 * ld_qc * r - 0.5 * quad - tau / 2 * rss

@@ -167,28 +167,28 @@ class Factor:
         return self._solve(b, nat.SOLVE_A)
 
     def solve_L(self, b, use_LDLt_decomposition=False):
-        """L^-1 P b (result in elimination order)."""
-        return self._solve(b, nat.SOLVE_L)
+        """L^-1 b with b in elimination order (CHOLMOD's solve_L)."""
+        return self._solve(b, nat.SOLVE_L_NOPERM)
 
     def solve_Lt(self, b, use_LDLt_decomposition=False):
-        """P^T L^-T b as one call; apply_Pt is then the identity on the result."""
-        return _PtApplied(self._solve(b, nat.SOLVE_LT))
-
-    def apply_Pt(self, b):
-        return np.asarray(b) if isinstance(b, _PtApplied) else self._permute(b, inverse=True)
+        """L^-T b, the result in elimination order (CHOLMOD's solve_Lt)."""
+        return self._solve(b, nat.SOLVE_LT_NOPERM)
 
     def apply_P(self, b):
-        return self._permute(b, inverse=False)
+        """P b: natural order -> elimination order."""
+        return self._solve(b, nat.SOLVE_P)
 
-    def _permute(self, b, inverse):
-        perm = self.engine.plan.export()["perm"]
-        b = np.asarray(b)
-        out = np.empty_like(b)
-        if inverse:
-            out[perm] = b
-        else:
-            out[:] = b[perm]
-        return out
+    def apply_Pt(self, b):
+        """P^T b: elimination order -> natural order."""
+        return self._solve(b, nat.SOLVE_PT)
+
+    def pt_solve_Lt(self, b):
+        """P^T L^-T b in one call (apply_Pt(solve_Lt(b)) without the round trip)."""
+        return self._solve(b, nat.SOLVE_LT)
+
+    def solve_L_p(self, b):
+        """L^-1 P b in one call (solve_L(apply_P(b)))."""
+        return self._solve(b, nat.SOLVE_L)
 
 
 class _SpatialGeom:
@@ -197,11 +197,6 @@ class _SpatialGeom:
 
     def __init__(self, g):
         self.hx, self.hy, self.V, self.dt = g.hx, g.hy, g.V, 1.0
-
-
-class _PtApplied(np.ndarray):
-    def __new__(cls, arr):
-        return np.asarray(arr).view(cls)
 
 
 # --------------------------------------------------------- spatial model
@@ -608,7 +603,21 @@ class _SpaceTimeBase(_StencilMixin):
         self._last_qval = qval
         if not grad:
             return Q, fac, None
-        return Q, fac, _LazyDQ(self, par)
+        dirs = self._directions(par) if hasattr(self, "_directions") else None
+        if dirs is None:
+            return Q, fac, _LazyDQ(self, par)
+        return Q, fac, _TangentDQ(self, par, coeffs, dirs)
+
+    def _mod0_directions(self, par):
+        """dQ of the initial-state parameters carried in par (fitQ0): the
+        Whittle-Matern derivatives placed in block (0, 0)
+        (advection_idiffusion2D.py:157-162)."""
+        p0, kappa0, H0, dH0 = self._mod0_split(par)
+        if p0 is None:
+            return []
+        out = [lambda: _direction(kind=2, dkappa0=kappa0, dH0=np.zeros(4))]
+        out += [(lambda d=d: _direction(kind=2, dkappa0=0.0, dH0=d)) for d in dH0]
+        return out
 
 
     def logLike(self, par, nh1=100, grad=True, trace="exact", logdet="closed", method="exact", **pcg):
@@ -623,23 +632,32 @@ class _SpaceTimeBase(_StencilMixin):
         method="learned": the same with the learned correction of the
         preconditioner (``spdepy_amd.learned``), falling back to the exact
         Cholesky path when the relative standard error of the value exceeds
-        tol_like (default 1e-6).  The parts of the last evaluation are kept
-        in self.last_parts."""
+        tol_like (default 1e-6; None or inf: no fallback) -- decided right
+        after the stochastic log-determinant when a bound on |loglik| already
+        shows the tolerance out of reach.  The parts of the last evaluation
+        are kept in self.last_parts."""
         par = np.asarray(par, dtype=np.float64)
+        if not np.all(np.isfinite(par)):
+            raise ValueError("logLike: parameters must be finite (got %s)" % par[~np.isfinite(par)][:4])
         eng = self._engine()
         coeffs = self._coeffs(par)
         if method in ("pcg", "learned"):
             from .krylov import loglik_pcg
-            tol_like = pcg.pop("tol_like", None)
+            tol_like = pcg.pop("tol_like", 1e-6 if method == "learned" else None)
+            if tol_like is not None and not np.isfinite(tol_like):
+                tol_like = None
             if method == "learned":
                 from .learned import default_preconditioner
                 net = pcg.pop("preconditioner", None) or default_preconditioner()
                 pcg["correction"] = net.corrector()
-                tol_like = 1e-6 if tol_like is None else tol_like
             like, parts = loglik_pcg(eng, coeffs, self._obs_cols(), self.data, par[-1], grad=grad,
-                                     contract=self._device_contract(), adv_dir=self._adv_dir(), **pcg)
-            # relative standard error of the stochastic log|Q_c| in the value
-            parts["like_rel_se"] = 0.5 * parts["r"] * parts["logdet_qc_se"] / abs(parts["like_raw"])
+                                     contract=self._device_contract(), adv_dir=self._adv_dir(), tol_like=tol_like,
+                                     **pcg)
+            if parts.get("infeasible"):
+                parts["like_rel_se"] = 0.5 * parts["r"] * parts["logdet_qc_se"] / parts["like_bound"]
+            else:
+                # relative standard error of the stochastic log|Q_c| in the value
+                parts["like_rel_se"] = 0.5 * parts["r"] * parts["logdet_qc_se"] / abs(parts["like_raw"])
             parts["fallback"] = tol_like is not None and parts["like_rel_se"] > tol_like
             self.last_parts = parts
             if parts["fallback"]:
@@ -656,9 +674,17 @@ class _SpaceTimeBase(_StencilMixin):
         probes = None
         if grad and trace == "hutchinson":
             probes = (2 * np.random.randint(1, 3, self.grid.n * nh1) - 3).reshape(self.grid.n, nh1)
-        like, parts = eng.loglik(coeffs, self._obs_cols(), self.data, par[-1], grad=grad, trace=trace,
-                                 probes=probes, logdet=logdet, contract=self._device_contract(),
-                                 adv_dir=self._adv_dir())
+        adv_dir = self._adv_dir()
+        if grad and trace == "exact" and logdet == "closed" and adv_dir is None:
+            # the whole evaluation behind one library call (spde_loglik_grad)
+            from .engine import loglik_fused
+            like, parts = loglik_fused(eng, coeffs, self._obs_cols(), self.data, par[-1],
+                                       contract=self._device_contract())
+        else:
+            like, parts = eng.loglik(coeffs, self._obs_cols(), self.data, par[-1], grad=grad, trace=trace,
+                                     probes=probes, logdet=logdet, contract=self._device_contract(),
+                                     adv_dir=adv_dir)
+        self.last_parts = parts
         if not grad:
             return like
         g = self._contract(par, coeffs, parts)
@@ -739,6 +765,18 @@ class _StationaryAdvDiff(_SpaceTimeBase):
         _, kappa0, H0, _ = self._mod0_split(par)
         return Coefficients(self.grid, variable=False, kappa=np.exp(kap), H=H, G=ws, sigma=np.exp(sig),
                             kappa0=kappa0, H0=H0)
+
+    def _directions(self, par):
+        """Directions of [log kappa, log gamma, (vx, vy), wx, wy, log sigma,
+        (mod0...)]: dA = dt V kappa I and dQs = 2 V kappa^2 for log kappa;
+        -dt A_H(dH) for the diffusion parameters; the constant advection
+        derivative stencils Aw(., diff = 1, 2) (Acw_2D_b*.cpp) for wx, wy."""
+        kap, gam, vx, vy, ws, sig = self._split(np.asarray(par, dtype=np.float64))
+        _, dH = h_tensor(self.kind, np.exp(gam), vx, vy)
+        out = [lambda: _direction(dkappa_c=np.exp(kap))]
+        out += [(lambda d=d: _direction(dH_c=d)) for d in dH]
+        out += [lambda: _direction(adv_diff=1), lambda: _direction(adv_diff=2), lambda: _direction(kind=1)]
+        return out + self._mod0_directions(par)
 
     def _contract(self, par, coeffs, parts):
         kap, gam, vx, vy, ws, sig = self._split(par)
@@ -889,6 +927,41 @@ class VarAdvectionVarDiffusion2D(_SpaceTimeBase):
             h = res.cpu().numpy()
             return h[:6 * nb], h[6 * nb:]
         return contract
+
+    def _directions(self, par):
+        """Directions of the reference's dQ loop (var_advection_var_diffusion2D.py:
+        131-183): for coefficient i of kappa, dkappa = bs_i kappa; of gamma,
+        dH = I bsH_i gamma per face; of vx / vy, dH = v dv^T + dv v^T with dv
+        the face basis on one component; of wx / wy, the variable advection
+        derivative stencil with the face direction evalAdv(e_i); then log sigma
+        and the initial-state parameters."""
+        g, Np = self.grid, self.Np
+        par = np.asarray(par, dtype=np.float64)
+        kappa, gamma, vx, vy, ws = self.fields(par)
+        vv = np.stack([vx, vy], axis=2)
+        out = [(lambda i=i: _direction(dkappa=g.bs[:, i] * kappa)) for i in range(Np)]
+
+        def dh_gamma(i):
+            d = np.zeros((g.Ns, 4, 2, 2))
+            d[:, :, 0, 0] = d[:, :, 1, 1] = g.bsH[:, :, i] * gamma
+            return _direction(dH=d)
+
+        def dh_v(comp, i):
+            b = g.bsH[:, :, i]
+            z = np.zeros_like(b)
+            dv = np.stack([b, z] if comp == 0 else [z, b], axis=2)
+            d = vv[:, :, :, None] * dv[:, :, None, :] + dv[:, :, :, None] * vv[:, :, None, :]
+            return _direction(dH=d)
+
+        def dw(i):
+            e = np.zeros(2 * Np)
+            e[i] = 1.0
+            return _direction(dG=g.evalAdv(e), adv_diff=1 if i < Np else 2)
+        out += [(lambda i=i: dh_gamma(i)) for i in range(Np)]
+        out += [(lambda c=c, i=i: dh_v(c, i)) for c in (0, 1) for i in range(Np)]
+        out += [(lambda i=i: dw(i)) for i in range(2 * Np)]
+        out.append(lambda: _direction(kind=1))
+        return out + self._mod0_directions(par)
 
     def _contract(self, par, coeffs, parts):
         g, Np = self.grid, self.Np
@@ -1381,6 +1454,77 @@ class CovAdvectionVarDiffusion2D(_CovAdvDiff):
 
 class CovAdvectionVarHaDiffusion2D(_CovAdvDiff):
     kind, dvar, name = "ha", True, "cov-advection-var-ha-diffusion-2D"
+
+
+class _TangentDQ:
+    """The reference's list of dQ_i (one sparse matrix per parameter except
+    tau, in the parameter order; var_advection_var_diffusion2D.py:118-197,
+    advection_idiffusion2D.py:118-163), each assembled on demand on the GPU
+    as the directional derivative of the block formula along the
+    parameter's coefficient direction (``spde_assemble_tangent``), with the
+    reference's derivative stencils.  Returned with the scipy pattern (exact
+    zeros dropped), like Q."""
+
+    def __init__(self, model, par, coeffs, directions):
+        self.model, self.par, self.coeffs, self.dirs = model, par, coeffs, directions
+
+    def __len__(self):
+        return len(self.dirs)
+
+    def __iter__(self):
+        return (self[i] for i in range(len(self)))
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[k] for k in range(*i.indices(len(self)))]
+        import torch
+        from . import _native as nat
+        if i < 0:
+            i += len(self)
+        if not 0 <= i < len(self):
+            raise IndexError(i)
+        eng = self.model._engine()
+        plan = eng.plan
+        base = torch.empty(plan.nnz, dtype=torch.float64, device="cuda")
+        plan.assemble(self.coeffs.c, base)       # the base point of the derivative
+        d, keep = self.dirs[i]()
+        dq = torch.empty(plan.nnz, dtype=torch.float64, device="cuda")
+        nat.check(nat.lib().spde_assemble_tangent(plan._h, ctypes.byref(self.coeffs.c), ctypes.byref(d), _p(dq),
+                                                  ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        del keep
+        return eng.to_scipy(dq)
+
+
+def _direction(kind=0, dkappa=None, dkappa_c=0.0, dH=None, dH_c=None, dG=None, adv_diff=0, dkappa0=0.0, dH0=None):
+    """A spde_direction and the device arrays it points to."""
+    import torch
+    from . import _native as nat
+    d = nat.Direction()
+    d.kind = kind
+    keep = []
+
+    def dev(a):
+        t = torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64).reshape(-1)).cuda()
+        keep.append(t)
+        return t.data_ptr()
+    if dkappa is not None:
+        d.dkappa = dev(dkappa)
+    d.dkappa_c = float(dkappa_c)
+    if dH is not None:
+        d.dH = dev(dH)
+        d.has_dH = 1
+    if dH_c is not None:
+        for q, v in enumerate(np.asarray(dH_c, dtype=np.float64).reshape(4)):
+            d.dH_c[q] = v
+        d.has_dH = 1
+    if dG is not None:
+        d.dG = dev(dG)
+    d.adv_diff = int(adv_diff)
+    d.dkappa0 = float(dkappa0)
+    if dH0 is not None:
+        for q, v in enumerate(np.asarray(dH0, dtype=np.float64).reshape(4)):
+            d.dH0[q] = v
+    return d, keep
 
 
 class _LazyDQ:

@@ -1,0 +1,72 @@
+"""The mixed-precision configuration at 200x200x24 (n = 960,000): the FP32
+PCG path with IC(0), FP64 iterative refinement to a relative residual of
+1e-10 for the conditional mean and 64 conditional samples, and the
+stochastic Lanczos log-determinant (32 Rademacher probes x 50 steps, FP32
+Lanczos vectors, FP64 tridiagonals), each with its error against the FP64
+Cholesky path on the same inputs (non-stationary advection-diffusion, 8x8
+B-spline fields with coefficients ~ N(0, 0.3^2), 24,000 observations of a
+smooth synthetic field with noise sd 0.1)."""
+import numpy as np
+import pytest
+
+from tests.gpu_util import step
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(M=200, N=200, T=24, Nbs=8, m=24000, bc=1):
+    import spdepy_amd as sp
+    from spdepy_amd import spdes
+    x = (np.arange(M) + 0.5) / M
+    y = (np.arange(N) + 0.5) / N
+    g = sp.grid(x=x, y=y, t=np.arange(float(T)), Nbs=Nbs)
+    g0 = sp.grid(x=x, y=y, Nbs=Nbs)
+    mod0 = spdes.WhittleMatern2D(grid=g0, par=np.array([0.0, np.log(0.5), 0.0]), bc=bc)
+    mod = spdes.VarAdvectionVarDiffusion2D(grid=g, mod0=mod0, bc=bc)
+    Np = g.Nbs2
+    rng = np.random.default_rng(3)
+    par = np.hstack([np.zeros(Np), np.full(Np, np.log(0.5)), 0.3 * rng.normal(size=4 * Np),
+                     [0.0, np.log(100.0)]])
+    idx = np.sort(np.random.default_rng(4).choice(M * N * T, m, replace=False))
+    t, rem = idx // (M * N), idx % (M * N)
+    xi, yi = x[rem % M], y[rem // M]
+    field = np.sin(2 * np.pi * xi) * np.cos(2 * np.pi * yi) * np.cos(0.2 * t)
+    data = field + 0.1 * np.random.default_rng(7).normal(size=m)
+    mod.initFit(data, idx=idx, fitQ0=False)
+    return mod, par
+
+
+def test_iterative_path_against_cholesky_at_200x200x24():
+    import torch
+    from spdepy_amd.krylov import conditional_samples_pcg, loglik_pcg
+    step("model and plans 200x200x24")
+    mod, par = _model()
+    eng = mod._engine()
+    coeffs = mod._coeffs(par)
+    cols, data = mod._obs_cols(), mod.data
+    step("exact: FP64 Cholesky of Q_c at n = 960,000")
+    like_ex, p_ex = eng.loglik(coeffs, cols, data, par[-1], grad=False)
+    mu_ex = p_ex["mu"].cpu().numpy().ravel()
+    step("iterative: IC(0), refined PCG for mu_c, SLQ 32 x 50")
+    like_it, p_it = loglik_pcg(eng, coeffs, cols, data, par[-1], grad=False, slq_probes=32, slq_steps=50, seed=1)
+    mu_it = p_it["mu"].cpu().numpy().ravel()
+    mu_err = np.linalg.norm(mu_it - mu_ex) / np.linalg.norm(mu_ex)
+    ld_err = p_it["ld_qc"] - p_ex["ld_qc"]
+    like_rel = abs(like_it - like_ex) / abs(like_ex)
+    print("  log|Q_c| exact %.6f SLQ %.6f (error %.3e, %.2f standard errors; relative %.2e)" % (
+        p_ex["ld_qc"], p_it["ld_qc"], ld_err, ld_err / p_it["logdet_qc_se"], ld_err / abs(p_ex["ld_qc"])))
+    print("  logLike exact %.12f iterative %.12f relative error %.2e" % (like_ex, like_it, like_rel))
+    print("  mu_c: PCG %s, relative error vs Cholesky %.2e" % (p_it["pcg"]["mu"], mu_err))
+    assert p_it["pcg"]["mu"][0]["relres"] <= 1e-10
+    assert mu_err <= 1e-6
+    assert abs(ld_err) <= 4 * p_it["logdet_qc_se"] + 1e-9 * abs(p_ex["ld_qc"])
+    step("64 conditional samples by refined PCG")
+    X, info = conditional_samples_pcg(eng, coeffs, cols, data, par[-1], k=64, seed=2)
+    print("  rounds %d, inner iterations per sample %d..%d, relres max %.2e" % (
+        info["rounds"], info["iters"].min(), info["iters"].max(), info["relres"].max()))
+    assert info["relres"].max() <= 1e-10
+    step("the same 64 right-hand sides by the Cholesky factor")
+    Xc = eng.plan.solve(1, info["B"])
+    err = (torch.linalg.norm(X - Xc, dim=0) / torch.linalg.norm(Xc, dim=0)).cpu().numpy()
+    print("  samples: relative error vs Cholesky max %.2e median %.2e" % (err.max(), np.median(err)))
+    assert err.max() <= 1e-6

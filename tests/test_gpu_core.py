@@ -234,7 +234,7 @@ def test_model_sample_and_update():
     # exact relation L^T P x = z through the factor's own solve
     x = data
     fac = M.Q_fac
-    back = fac.solve_Lt(fac.solve_L(Qd @ x))
+    back = fac.pt_solve_Lt(fac.solve_L_p(Qd @ x))
     assert np.allclose(np.asarray(back), x, rtol=1e-8, atol=1e-10)
     step("Model.update")
     S = g.getS(c["idx"])
@@ -243,6 +243,38 @@ def test_model_sample_and_update():
     Qc = Qd + tau * (S.T @ S).toarray()
     mu_ref = np.linalg.solve(Qc, tau * (S.T @ c["y"]))
     assert np.allclose(M.mu, mu_ref, rtol=1e-9, atol=1e-12)
+
+
+def test_factor_cholmod_calls():
+    """The CHOLMOD Factor calls (model.py:79-80,125-126) with CHOLMOD's
+    meaning: apply_P / apply_Pt permute (P from the plan's ordering),
+    solve_L / solve_Lt solve with L alone in elimination order, and the
+    fused calls equal their compositions; L L^T = P Q P^T is checked
+    against the dense matrix."""
+    c = CASES["var_b1"]()
+    g = c["grid"]
+    m = product_case(c)
+    step("factor for the CHOLMOD calls")
+    Q, fac, _ = m.makeQ(c["par"], grad=False)
+    Qd = Q.toarray()
+    perm = m._engine().plan.export()["perm"]
+    b = np.random.default_rng(9).normal(size=(g.n, 3))
+    pb = fac.apply_P(b)
+    assert np.array_equal(pb, b[perm])
+    assert np.array_equal(fac.apply_Pt(pb), b)
+    assert np.array_equal(fac.apply_P(b[:, 0]), b[perm, 0])
+    y = fac.solve_L(pb)
+    assert np.allclose(y, fac.solve_L_p(b), rtol=1e-13, atol=1e-13 * np.abs(y).max())
+    x = fac.apply_Pt(fac.solve_Lt(y))
+    assert np.allclose(x, np.linalg.solve(Qd, b), rtol=1e-9, atol=1e-11 * np.abs(x).max())
+    assert np.allclose(x, fac.solve_A(b), rtol=1e-12, atol=1e-13 * np.abs(x).max())
+    assert np.allclose(fac.pt_solve_Lt(y), x, rtol=1e-13, atol=1e-13 * np.abs(x).max())
+    # L from L^-1 applied to the identity: L L^T = P Q P^T
+    Linv = fac.solve_L(np.eye(g.n))
+    L = np.linalg.inv(Linv)
+    Pq = Qd[np.ix_(perm, perm)]
+    assert np.allclose(L @ L.T, Pq, rtol=1e-9, atol=1e-11 * np.abs(Pq).max())
+    assert np.allclose(np.triu(L, 1), 0, atol=1e-11 * np.abs(L).max())
 
 
 def test_model_qinv_selected_inverse():
@@ -524,7 +556,7 @@ def test_model_sample_and_update_other_families(family):
     step("Model.sample " + family)
     x = M.sample(n=2, simple=True, seed=3)
     fac = M.Q_fac
-    back = fac.solve_Lt(fac.solve_L(Qd @ x))
+    back = fac.pt_solve_Lt(fac.solve_L_p(Qd @ x))
     assert np.allclose(np.asarray(back), x, rtol=1e-8, atol=1e-10)
     step("Model.update " + family)
     idx = np.sort(rng.choice(g.n, 40, replace=False))

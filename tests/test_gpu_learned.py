@@ -219,3 +219,28 @@ def test_training_beats_plain_ic0():
     gains = np.array(gains)
     assert np.all(gains[:, 0] > 0)          # lower divergence on every held-out draw
     assert gains[:, 1].mean() > 1.0         # fewer PCG iterations on average
+
+
+def test_learned_route_and_cholesky_fallback():
+    """logLike(method="learned") with the shipped weights: without a
+    tolerance the estimate lies within 4 standard errors of the exact value;
+    at the default 1e-6 the stochastic log-determinant cannot meet it, the
+    route sees that right after the Lanczos quadrature and returns the exact
+    Cholesky value and gradient."""
+    c = F.small(12, 10, 6, bc=1, m=60)
+    mod = product_case(c)
+    mod.initFit(c["y"], idx=c["idx"], fitQ0=False)
+    par = c["par"]
+    step("exact logLike 12x10x6")
+    l_ex, j_ex = mod.logLike(par)
+    step("learned route without a tolerance")
+    l_ln, j_ln = mod.logLike(par, method="learned", seed=5, tol_like=np.inf, hutch_probes=256)
+    p = mod.last_parts
+    se = 0.5 * p["r"] * p["logdet_qc_se"] / (p["m"] * p["r"])
+    print("  exact %.10f learned %.10f (se %.2e)" % (l_ex, l_ln, se))
+    assert not p["fallback"] and abs(l_ln - l_ex) <= 4 * se + 1e-9 * abs(l_ex)
+    step("learned route at 1e-6: Cholesky fallback")
+    l_fb, j_fb = mod.logLike(par, method="learned", seed=5)
+    p = mod.last_parts
+    assert p["fallback"] and p["infeasible"] and p["like_rel_se"] > 1e-6
+    assert l_fb == l_ex and np.array_equal(j_fb, j_ex)

@@ -35,7 +35,7 @@ def test_whittle_matern_model_sample_update(kind, bc):
     mod.setModel()
     step("WhittleMatern Model.sample")
     xs = mod.sample(n=2, simple=True, seed=3)
-    back = mod.Q_fac.solve_Lt(mod.Q_fac.solve_L(Qd @ xs))
+    back = mod.Q_fac.pt_solve_Lt(mod.Q_fac.solve_L_p(Qd @ xs))
     assert np.allclose(np.asarray(back), xs, rtol=1e-8, atol=1e-10)
     step("WhittleMatern Model.update")
     idx = np.array([3, 10, 40, 41, 60])
@@ -47,3 +47,74 @@ def test_whittle_matern_model_sample_update(kind, bc):
     step("WhittleMatern Model.qinv")
     mvar = mod.qinv()
     assert np.allclose(mvar, np.diag(np.linalg.inv(Qc)), rtol=1e-10)
+
+
+@pytest.mark.parametrize("two", [True, False])
+def test_model_with_covariates(two):
+    """Model.setModel(useCov=True) (model.py:129-153): intercept + covariate
+    (two sigmas) or intercept only (one sigma) border columns beside the
+    lattice; sample, update and qinv (model.py:73-127) against dense algebra
+    on the augmented precision, the sample through a dense Cholesky factor
+    in this library's ordering extended by the border."""
+    import spdepy_amd as sp
+    from spdepy_amd import spdes
+    M_, N_ = 9, 8
+    x, y = F.axis(M_), F.axis(N_)
+    par = np.array([0.2, np.log(0.5), 0.3, -0.4, 0.0])
+    g = sp.grid(x=x, y=y, Nbs=3)
+    wm = spdes.WhittleMaternAnisotropic2D(grid=g, par=par, bc=3)
+    wm.setQ(par=par)
+    Ql = wm.Q.toarray()
+    n = g.n
+    mod = sp.Model.__new__(sp.Model)
+    mod.grid, mod.mod, mod.Q, mod.mvar, mod.useCov = g, wm, None, None, False
+    mod.mu = np.zeros(n)
+    cov = np.linspace(0.5, 2.0, n) + 0.1 * np.sin(np.arange(n))
+    if two:
+        mod.setModel(mu=cov, sigmas=np.log([0.5, 3.0]), useCov=True, scale=True)
+        C = np.diag([0.5, 3.0])
+        X = np.stack([np.ones(n), cov / cov.max()], 1)
+        mu0 = np.r_[np.zeros(n), 0.0, cov.max()]
+    else:
+        mod.setModel(mu=cov, sigmas=np.log(2.0), useCov=True)
+        C = np.array([[2.0]])
+        X = np.ones((n, 1))
+        mu0 = np.r_[cov, 0.0]
+    k = C.shape[0]
+    Qa = np.block([[Ql, np.zeros((n, k))], [np.zeros((k, n)), C]])
+    assert np.allclose(mod.Q.toarray(), Qa, rtol=1e-13, atol=0)
+    assert np.allclose(mod.mu, mu0, rtol=1e-13, atol=0)
+    S = np.hstack([np.eye(n), X])
+    assert np.allclose(g.getS().toarray(), S)
+
+    def check_sample(Qdense):
+        z = np.random.default_rng(5).normal(size=(n + k) * 2).reshape(n + k, 2)
+        step("covariate sample (k=%d)" % k)
+        xs = mod._sample_aug(z)
+        perm = np.r_[mod._engine().plan.export()["perm"], np.arange(n, n + k)]
+        Lt = np.linalg.cholesky(Qdense[np.ix_(perm, perm)]).T
+        want = np.empty_like(z)
+        want[perm] = np.linalg.solve(Lt, z)
+        assert np.allclose(xs, want, rtol=1e-9, atol=1e-11 * np.abs(want).max())
+        data = mod.sample(n=2, simple=True, seed=5)
+        assert np.allclose(data, S @ (want + mod.mu[:, None]), rtol=1e-9, atol=1e-11 * np.abs(data).max())
+
+    check_sample(Qa)
+    step("covariate qinv (k=%d)" % k)
+    mvar = mod.qinv()
+    want = np.r_[np.diag(np.linalg.inv(S @ Qa @ S.T)), np.zeros(k)]
+    assert mvar.shape == (n + k,)
+    assert np.allclose(mvar, want, rtol=1e-9, atol=1e-12 * np.abs(want).max())
+    idx = np.array([3, 10, 40, 41, 60, 10])
+    yv = np.array([0.5, -0.2, 1.0, 0.3, -0.7, 0.1])
+    step("covariate update (k=%d)" % k)
+    mod.update(yv, idx, tau=4.0)
+    Sd = g.getS(idx).toarray()
+    Qc = Qa + 4.0 * Sd.T @ Sd
+    mu1 = mu0 + 4.0 * np.linalg.solve(Qc, Sd.T @ (yv - Sd @ mu0))
+    assert np.allclose(mod.Q.toarray(), Qc, rtol=1e-13, atol=1e-13 * np.abs(Qc).max())
+    assert np.allclose(mod.mu, mu1, rtol=1e-9, atol=1e-11 * np.abs(mu1).max())
+    check_sample(Qc)
+    mvar = mod.qinv()
+    want = np.r_[np.diag(np.linalg.inv(S @ Qc @ S.T)), np.zeros(k)]
+    assert np.allclose(mvar, want, rtol=1e-8, atol=1e-12 * np.abs(want).max())

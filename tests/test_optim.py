@@ -73,3 +73,23 @@ def test_fit_writes_the_result_vector(tmp_path):
     res = opt.fit(x0=np.zeros(3), lr=0.1, max_steps=50, pol=5, end=stem)
     saved = np.load(stem + ".npy", allow_pickle=False)
     assert saved.dtype == np.float64 and np.array_equal(saved, res["x"])
+
+
+def test_oracle_adam_first_steps_closed_form():
+    """The oracle's Adam loop on f(x) = 1/2 |x|^2: the first step moves each
+    coordinate by lr * g / (|g| (1 - b1) / sqrt(1 - b2) + eps) with m = g,
+    v = g^2; it matches the product rule over a few steps."""
+    from oracle import optim as O
+    from spdepy_amd.optim import Adam
+    fun = lambda x: (0.5 * float(x @ x), x.copy())
+    x0 = np.array([1.0, -2.0, 0.5])
+    xs, fs, gs, xp = O.adam_fit(fun, x0, 0.1, 5, pol=2)
+    g = x0
+    first = x0 - 0.1 * (g / 0.1) / (np.abs(g) / np.sqrt(1 - 0.999) + 1e-8)
+    assert np.allclose(xs[0], first, rtol=1e-14)
+    step = Adam()
+    x = x0.copy()
+    for k in range(5):
+        x = step(x, None, x.copy(), 0.1)
+        assert np.allclose(x, xs[k], rtol=1e-14)
+    assert np.allclose(xp, xs[-2:].mean(0))

@@ -34,6 +34,38 @@ def test_rademacher_signs_balanced_and_seeded():
     assert not np.array_equal(v, K.rademacher(4000, 8, seed=8))
 
 
+def test_gaussian_draws():
+    """Box-Muller on the splitmix64 uniforms: the uniforms lie in (0, 1) and
+    are uniform (first two moments), the draws have the normal moments and
+    the Box-Muller radius identity z^2 + z'^2 = -2 log U1 holds with the
+    sine partner."""
+    u1, u2 = K.uniforms(20000, 4, seed=11)
+    assert u1.min() > 0.0 and u1.max() < 1.0 and u2.min() >= 0.0 and u2.max() < 1.0
+    assert abs(u1.mean() - 0.5) < 0.01 and abs(u1.var() - 1 / 12) < 0.003
+    z = K.gaussian(20000, 4, seed=11)
+    assert abs(z.mean()) < 0.02 and abs(z.var() - 1.0) < 0.03
+    assert abs((z ** 4).mean() - 3.0) < 0.15
+    zs = np.sqrt(-2.0 * np.log(u1)) * np.sin(2.0 * np.pi * u2)
+    assert np.allclose(z ** 2 + zs ** 2, -2.0 * np.log(u1), rtol=1e-12)
+    assert np.array_equal(z, K.gaussian(20000, 4, seed=11))
+    assert not np.array_equal(z, K.gaussian(20000, 4, seed=12))
+
+
+@pytest.mark.parametrize("bc", [1, 3])
+def test_fform_square_root_reproduces_q(bc):
+    """R = F^T D^(1/2) (applied to the identity) gives R R^T = Q, the
+    precision makeQ assembles by the reference's block formula."""
+    c = F.small(7, 6, 4, bc=bc, m=20)
+    g, model, par = c["grid"], c["model"], c["par"]
+    Q, _ = model.makeQ(par, grad=False)
+    A, q, sigma = model.spatial_operator(par)
+    A0 = c["mod0"].A0(c["mod0"].par)[0]
+    M, N, T = g.shape
+    R = K.fform_t(A, q, A0, g.V, T, sigma, g.dt, np.eye(g.n))
+    Qd = Q.toarray()
+    assert np.allclose(R @ R.T, Qd, rtol=1e-12, atol=1e-12 * np.abs(Qd).max())
+
+
 def test_colouring_is_an_independent_set_partition():
     A, c = _qc()
     M, N, T = c["grid"].shape
