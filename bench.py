@@ -182,6 +182,70 @@ def other_routes(mod, par, like_ex, jac_ex, steps=3):
     return out
 
 
+def batch_models(nmodels=64, seed=0):
+    """The batched workload: stationary advection-diffusion models on lattices
+    drawn uniformly from 40x40x5 to 120x120x10, kappa and gamma log-uniform in
+    [0.1, 10] and [0.1, 5], advection uniform in the unit disc, 1000-8000
+    observations each (at most the lattice size) of synthetic N(0, 1) data."""
+    import spdepy_amd as sp
+    from spdepy_amd import spdes
+    rng = np.random.default_rng(seed)
+    models, pars = [], []
+    for i in range(nmodels):
+        M, N, T = int(rng.integers(40, 121)), int(rng.integers(40, 121)), int(rng.integers(5, 11))
+        x, y = (np.arange(M) + 0.5) / M, (np.arange(N) + 0.5) / N
+        g = sp.grid(x=x, y=y, t=np.arange(float(T)))
+        g0 = sp.grid(x=x, y=y)
+        mod0 = spdes.WhittleMatern2D(grid=g0, par=np.array([0.0, np.log(0.5), 0.0]), bc=1)
+        mod = spdes.AdvectionIDiffusion2D(grid=g, mod0=mod0, bc=1)
+        m = int(min(rng.integers(1000, 8001), M * N * T))
+        idx = np.sort(rng.choice(M * N * T, m, replace=False))
+        mod.initFit(rng.normal(size=m), idx=idx, fitQ0=False)
+        r, phi = np.sqrt(rng.uniform()), rng.uniform(0, 2 * np.pi)
+        par = np.array([rng.uniform(np.log(0.1), np.log(10.0)), rng.uniform(np.log(0.1), np.log(5.0)),
+                        r * np.cos(phi), r * np.sin(phi), 0.0, np.log(100.0)])
+        models.append(mod)
+        pars.append(par)
+    return models, pars
+
+
+def batch_route(nmodels=64, nrhs=256, nstreams=8):
+    """Models per second of exact FP64 log-likelihood + gradient and nrhs
+    conditional samples per model: one after another (logLike, then the
+    samples) against the batch (spde_loglik_batch over a pool of streams)."""
+    import torch
+    from spdepy_amd.batch import BatchEvaluator
+    t = time.perf_counter()
+    models, pars = batch_models(nmodels)
+    ev = BatchEvaluator(models, nstreams=nstreams)
+    t_setup = time.perf_counter() - t
+    ev.loglik(pars)                                   # untimed: every plan's buffers
+    for i in range(nmodels):
+        ev.conditional_samples(i, nrhs, seed=i)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    res = ev.loglik(pars)
+    for i in range(nmodels):
+        ev.conditional_samples(i, nrhs, seed=i)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t
+    t = time.perf_counter()
+    seq = []
+    for i, (m, p) in enumerate(zip(models, pars)):
+        seq.append(m.logLike(p))
+        ev.conditional_samples(i, nrhs, seed=i)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t
+    worst = max(abs(a[0] - b[0]) / abs(b[0]) for a, b in zip(res, seq))
+    ns = [m.grid.n for m in models]
+    return {"models_per_s": nmodels / t_batch, "sequential_models_per_s": nmodels / t_seq,
+            "batch_s": t_batch, "sequential_s": t_seq, "setup_s": t_setup, "models": nmodels, "rhs": nrhs,
+            "streams": nstreams, "unknowns_min_max_total": [min(ns), max(ns), sum(ns)],
+            "max_rel_diff_batch_vs_single": worst,
+            "note": "exact FP64 Cholesky log-likelihood + gradient and %d conditional samples per model; "
+                    "plans (symbolic analysis, launch lists) built in setup" % nrhs}
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gemm_fp64_100x100x20_hbm_traffic.txt")
 
 
@@ -207,6 +271,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-paths", action="store_true", help="skip the factorised, iterative and learned routes")
+    ap.add_argument("--batch", action="store_true", help="also time the batch of 64 uneven models")
+    ap.add_argument("--batch-only", action="store_true", help="only the batch of 64 uneven models")
+    ap.add_argument("--only-route", choices=["exact", "pcg", "learned"],
+                    help="only `steps` evaluations of one route on the workload (for profiling)")
     ap.add_argument("--stages", action="store_true", help="print a per-stage breakdown to stderr")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy a 512 MiB buffer 3 times (reference kernel for HBM counter scaling)")
@@ -219,6 +287,18 @@ def main():
     from spdepy_amd import _native as nat
     from spdepy_amd.plan import _ptr
 
+    if args.batch_only:
+        print(json.dumps({"batch": batch_route()}))
+        return
+    if args.only_route:
+        mod, par = build_workload()
+        kw = {"exact": {}, "pcg": dict(method="pcg", seed=11),
+              "learned": dict(method="learned", seed=11, tol_like=np.inf)}[args.only_route]
+        rate, like, jac, parts = time_route(mod, par, args.steps, **kw)
+        info = None if parts is None or "pcg" not in parts else {
+            "mu": parts["pcg"]["mu"], "probe_iterations_mean": float(np.mean(parts["pcg"].get("probes", [0])))}
+        print(json.dumps({"route": args.only_route, "evals_per_s": rate, "like": like, "pcg": info}))
+        return
     if args.copy_calibration:
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -268,6 +348,7 @@ def main():
     kind_all, lms, lfl, lby, _ = dump()
     kinds = sorted(set(kind_all.tolist()), key=lambda q: -lms[kind_all == q].sum())
     dom = kinds[0]
+
     dom_name = "gemm%d_kernel<%s, %s>" % (128 if dom & 4 else 64, "true" if dom & 2 else "false",
                                         "true" if dom & 1 else "false")
     family = lfl.sum() / (lms.sum() * 1e-3) / 1e12
@@ -308,6 +389,7 @@ def main():
         for k in ("pcg", "learned_pcg"):
             if "evals_per_s" in routes.get(k, {}):
                 routes[k]["vs_exact"] = routes[k]["evals_per_s"] / value
+    batch = batch_route() if args.batch else None
     base = None if args.no_cpu_baseline else cpu_baseline(args.cpu_repeats)
     out = {
         "metric": METRIC,
@@ -348,6 +430,7 @@ def main():
         "cpu_baseline": base,
         "symbolic_analysis_s": getattr(build_workload, "symbolic_s", None),
         "routes": routes,
+        "batch": batch,
     }
     print(json.dumps(out))
 

@@ -91,14 +91,20 @@ class BatchEvaluator:
 
     def conditional_samples(self, i, nsamples, seed=0):
         """nsamples draws of x | y for model i after loglik: mu_c + P^T L^-T z
-        with the factor of Q_c of the last evaluation (n x nsamples device)."""
+        with the factor of Q_c of the last evaluation and z from the
+        library's generator (spde_gaussian with seed; n x nsamples device)."""
         torch = _torch()
+        from .krylov import gaussian
         m = self.models[i]
         plan = m._engine().plan
         n = plan.n
-        g = torch.Generator(device="cuda").manual_seed(int(seed))
-        Z = torch.randn((n, nsamples), dtype=torch.float64, device="cuda", generator=g)
-        X = plan.solve(1, Z, mode=nat.SOLVE_LT)
+        Z = gaussian(n, nsamples, seed)
+        # right-hand sides in chunks of at most 64: the solve's front vectors
+        # scale with the chunk, and every plan of a batch keeps its own
+        X = torch.empty_like(Z)
+        for c0 in range(0, nsamples, 64):
+            c1 = min(nsamples, c0 + 64)
+            X[:, c0:c1] = plan.solve(1, Z[:, c0:c1].contiguous(), mode=nat.SOLVE_LT)
         mu = torch.empty(n * self.r, dtype=torch.float64, device="cuda")
         nat.check(nat.lib().spde_loglik_mu(plan._h, _ptr(mu), self.r, ctypes.c_void_p(
             torch.cuda.current_stream().cuda_stream)))

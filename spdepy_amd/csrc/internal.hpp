@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/spdepy_amd.h"
@@ -110,7 +111,7 @@ struct LevelPlan {
   std::vector<Step> steps;
   GemmLaunch syrk;
   // solves / selinv
-  GemmLaunch fwd_y, fwd_u, bwd_t, bwd_x;   // rebuilt per nrhs
+  GemmLaunch fwd_y, fwd_u, bwd_t, bwd_x;   // the current nrhs' launches (owned by solve_cache)
   GemmLaunch si_x, si_s21, si_s11a, si_s11b;
   int64_t si_work = 0;
 };
@@ -180,6 +181,13 @@ struct spde_plan {
   std::vector<spde::GemmLaunch> trtri;  // per recursion height: pairs (X, Z2)
   int trtri_heights = 0;
   int solve_nrhs = -1;            // nrhs the solve launches are built for
+  // solve launches per nrhs (four per level), built once each and swapped in
+  // by value; their device descriptor arrays are freed with the plan
+  struct SolveSet {
+    std::vector<spde::GemmLaunch> launches;
+    int64_t split = 0;
+  };
+  std::map<int, SolveSet> solve_cache;
   bool selinv_built = false;
   // optional event timing of the GEMM launches (measurement only)
   bool prof_on = false;

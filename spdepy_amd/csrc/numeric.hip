@@ -466,42 +466,59 @@ int build_factor_plan(spde_plan* pl) {
   return SPDE_OK;
 }
 
+void free_launch(GemmLaunch& g);
+
 int build_solve_plan(spde_plan* pl, int k) {
+  auto hit = pl->solve_cache.find(k);
+  if (hit != pl->solve_cache.end()) {
+    const auto& v = hit->second.launches;
+    for (size_t l = 0; l < pl->levels.size(); ++l) {
+      LevelPlan& L = pl->levels[l];
+      L.fwd_y = v[4 * l], L.fwd_u = v[4 * l + 1], L.bwd_t = v[4 * l + 2], L.bwd_x = v[4 * l + 3];
+    }
+    pl->split_solve = hit->second.split;
+    pl->solve_nrhs = k;
+    return SPDE_OK;
+  }
   const Symbolic& sym = pl->sym;
   const int n = (int)sym.n;
   // solve launches take the split workspace after the factor and
   // selected-inverse regions (fixed when the plan is made)
   int64_t acc = pl->split_fs;
+  spde_plan::SolveSet set;
   for (auto& L : pl->levels) {
-    for (GemmLaunch* g : {&L.fwd_y, &L.fwd_u, &L.bwd_t, &L.bwd_x}) {
-      if (g->d_probs) hipFree(g->d_probs);
-      if (g->d_tiles) hipFree(g->d_tiles);
-      *g = GemmLaunch();
-    }
-    L.fwd_y.ta = false, L.fwd_y.tb = false;
-    L.fwd_u.ta = false, L.fwd_u.tb = false;
-    L.bwd_t.ta = true, L.bwd_t.tb = false;
-    L.bwd_x.ta = true, L.bwd_x.tb = false;
+    GemmLaunch fy, fu, bt, bx;
+    fy.ta = false, fy.tb = false;
+    fu.ta = false, fu.tb = false;
+    bt.ta = true, bt.tb = false;
+    bx.ta = true, bx.tb = false;
     for (int id : L.nodes) {
       const Node& nd = sym.nodes[id];
       const int nf = nd.n(), p = nd.p, b = (int)nd.B.size();
       const int64_t v = nd.voff * k;
-      add_prob(L.fwd_y, prob(kZ, nd.zoff, p, kVec, v, nf, kX, nd.start, n, p, k, p, 1.0, 0.0), false);
-      add_prob(L.fwd_u, prob(kFront, nd.foff + p, nf, kX, nd.start, n, kVec, v + p, nf, b, k, p, -1.0, 1.0), false);
-      add_prob(L.bwd_t, prob(kFront, nd.foff + p, nf, kVec, v + p, nf, kVec, v, nf, p, k, b, -1.0, 1.0), false);
-      add_prob(L.bwd_x, prob(kZ, nd.zoff, p, kVec, v, nf, kX, nd.start, n, p, k, p, 1.0, 0.0), false);
+      add_prob(fy, prob(kZ, nd.zoff, p, kVec, v, nf, kX, nd.start, n, p, k, p, 1.0, 0.0), false);
+      add_prob(fu, prob(kFront, nd.foff + p, nf, kX, nd.start, n, kVec, v + p, nf, b, k, p, -1.0, 1.0), false);
+      add_prob(bt, prob(kFront, nd.foff + p, nf, kVec, v + p, nf, kVec, v, nf, p, k, b, -1.0, 1.0), false);
+      add_prob(bx, prob(kZ, nd.zoff, p, kVec, v, nf, kX, nd.start, n, p, k, p, 1.0, 0.0), false);
     }
     const ArenaSizes as = arena_sizes(pl, k);
-    for (GemmLaunch* g : {&L.fwd_y, &L.fwd_u, &L.bwd_t, &L.bwd_x}) {
+    for (GemmLaunch* g : {&fy, &fu, &bt, &bx}) {
       int rc = validate(*g, as, "solve");
-      if (rc) return rc;
-      rc = finalize(*g, &acc);
-      if (rc) return rc;
+      if (!rc) rc = finalize(*g, &acc);
+      if (rc) {
+        for (auto& q : set.launches) free_launch(q);
+        for (GemmLaunch* h : {&fy, &fu, &bt, &bx}) free_launch(*h);
+        return rc;
+      }
     }
+    set.launches.push_back(fy);
+    set.launches.push_back(fu);
+    set.launches.push_back(bt);
+    set.launches.push_back(bx);
   }
-  pl->split_solve = acc - pl->split_fs;
-  pl->solve_nrhs = k;
-  return SPDE_OK;
+  set.split = acc - pl->split_fs;
+  pl->solve_cache[k] = set;
+  return build_solve_plan(pl, k);
 }
 
 int build_selinv_plan(spde_plan* pl) {
@@ -831,11 +848,11 @@ void spde_plan_destroy(spde_plan* pl) {
         free_launch(st.panel);
         free_launch(st.update);
       }
-      for (GemmLaunch* g : {&L.syrk, &L.fwd_y, &L.fwd_u, &L.bwd_t, &L.bwd_x, &L.si_x, &L.si_s21, &L.si_s11a,
-                            &L.si_s11b})
-        free_launch(*g);
+      for (GemmLaunch* g : {&L.syrk, &L.si_x, &L.si_s21, &L.si_s11a, &L.si_s11b}) free_launch(*g);
     }
     for (auto& g : pl->trtri) free_launch(g);
+    for (auto& kv : pl->solve_cache)
+      for (auto& g : kv.second.launches) free_launch(g);
     for (auto e : pl->prof_ev) hipEventDestroy(e);
     if (pl->prof_base) hipEventDestroy(pl->prof_base);
     if (PlanExtra* ex = extra(pl)) {
@@ -1018,8 +1035,13 @@ int spde_solve(spde_plan* pl, int slot, int mode, double* B, int nrhs, int64_t l
   const Symbolic& sym = pl->sym;
   const int64_t n = sym.n;
   Slot& sl = pl->slots[slot];
+  if (pl->solve_nrhs != nrhs && pl->solve_cache.count(nrhs)) {
+    // launch lists of this nrhs exist (their descriptors never change): swap them in
+    rc = build_solve_plan(pl, nrhs);
+    if (rc) return rc;
+  }
   if (sl.vec_cap < sym.vec_doubles * nrhs || sl.xy_cap < n * nrhs || pl->solve_nrhs != nrhs) {
-    // buffers and launch lists change: let every stream drain first
+    // buffers change or new launch lists are built: let every stream drain first
     SPDE_HIP(hipDeviceSynchronize());
     if (sl.vec_cap < sym.vec_doubles * nrhs) {
       hipFree(sl.vec);

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 from scipy import sparse
 
+from oracle import krylov as K
 from tests.gpu_util import step
 
 pytestmark = pytest.mark.gpu
@@ -69,8 +70,7 @@ def test_batch_conditional_samples():
     np.add.at(b, cols, tau * np.asarray(mod.data))
     mu = np.linalg.solve(Qc, b)
     plan = mod._engine().plan
-    g = torch.Generator(device="cuda").manual_seed(3)
-    Z = torch.randn((mod.grid.n, 256), dtype=torch.float64, device="cuda", generator=g)
+    Z = torch.as_tensor(K.gaussian(mod.grid.n, 256, 3)).cuda()
     dev = (X - plan.solve(1, Z, mode=nat.SOLVE_LT)).cpu().numpy()
     assert np.allclose(dev, mu[:, None], rtol=1e-9, atol=1e-10 * np.abs(mu).max())
     # x - mu = P^T L^-T z has covariance Q_c^-1: (x - mu)^T Q_c (x - mu) = z^T z per sample
