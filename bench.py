@@ -225,6 +225,8 @@ def batch_route(nmodels=64, nrhs=256, nstreams=8):
     torch.cuda.synchronize()
     t = time.perf_counter()
     res = ev.loglik(pars)
+    torch.cuda.synchronize()
+    t_loglik = time.perf_counter() - t
     for i in range(nmodels):
         ev.conditional_samples(i, nrhs, seed=i)
     torch.cuda.synchronize()
@@ -239,7 +241,8 @@ def batch_route(nmodels=64, nrhs=256, nstreams=8):
     worst = max(abs(a[0] - b[0]) / abs(b[0]) for a, b in zip(res, seq))
     ns = [m.grid.n for m in models]
     return {"models_per_s": nmodels / t_batch, "sequential_models_per_s": nmodels / t_seq,
-            "batch_s": t_batch, "sequential_s": t_seq, "setup_s": t_setup, "models": nmodels, "rhs": nrhs,
+            "batch_s": t_batch, "batch_loglik_s": t_loglik, "sequential_s": t_seq, "setup_s": t_setup,
+            "models": nmodels, "rhs": nrhs,
             "streams": nstreams, "unknowns_min_max_total": [min(ns), max(ns), sum(ns)],
             "max_rel_diff_batch_vs_single": worst,
             "note": "exact FP64 Cholesky log-likelihood + gradient and %d conditional samples per model; "

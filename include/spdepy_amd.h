@@ -268,6 +268,22 @@ typedef struct spde_grad_args {
   const double* adv_dir;
 } spde_grad_args;
 
+/* out[Ns * 40] (device, row-major per spatial cell m) holds d like_raw with
+ * respect to each cell's coefficients, summed over the time slices:
+ *   [0]      kappa_m (the cell's kappa, through A, q and, with a0_variable, A0)
+ *   [1..16]  variable diffusion H of the cell, faces W,E,S,N x (2x2 row-major)
+ *   [17..20] d/dwx of the variable advection per face E,N,W,S
+ *   [21..24] d/dwy of the variable advection per face
+ *   [25]     kappa0 of the initial-state operator A0
+ *   [26..29] H0 (2x2 row-major) of A0
+ *   [30..33] constant diffusion H_c (2x2 row-major; constant models)
+ *   [34], [35] constant advection derivatives d/dwx, d/dwy
+ *   [36]     the cell's share of <G, Q>  [37] of <G00, Q0>
+ *   [38]     of tau sum_k cnt_k Sigma_c,kk (the trace of Q_c^-1 tau S^T S)
+ *   [39]     dt <Abar, Aw(adv_dir)> (with adv_dir; else 0)
+ * The model's gradient is the contraction of these columns with its bases
+ * (spde_fields_contract for the B-spline fields), plus the sigma and tau
+ * entries from the column sums [36..38]. */
 int spde_grad_cells(spde_plan* plan, const spde_grad_args* args, double* out, void* stream);
 
 /* ---------------------------------------------------------------- fused evaluation

@@ -42,9 +42,29 @@ def build(verbose=False):
     tmp = OUT + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, OUT)
+    build_shims()
     if verbose:
         print("built", OUT)
     return OUT
+
+
+# The reference loads one stencil library per kind and boundary condition
+# (ccode/lib_{AcH,AH,Acw,Aw}_2D_b{1,2,3}.so); these carry its C interface over
+# libspdepy_amd.so (csrc/ccode_shim.cpp).
+SHIMS = {"AcH": (1, 0), "AH": (1, 1), "Acw": (0, 0), "Aw": (0, 1)}
+SHIM_DIR = os.path.join(HERE, "ccode")
+
+
+def build_shims():
+    os.makedirs(SHIM_DIR, exist_ok=True)
+    src = os.path.join(CSRC, "ccode_shim.cpp")
+    for kind, (diffusion, variable) in SHIMS.items():
+        for bc in (1, 2, 3):
+            out = os.path.join(SHIM_DIR, "lib_%s_2D_b%d.so" % (kind, bc))
+            _run([HIPCC] + COMMON + ["-D__HIP_PLATFORM_AMD__", "-DSHIM_DIFFUSION=%d" % diffusion,
+                                     "-DSHIM_VARIABLE=%d" % variable, "-DSHIM_BC=%d" % bc, "-shared", src,
+                                     "-o", out + ".tmp", "-L" + HERE, "-lspdepy_amd", "-Wl,-rpath,$ORIGIN/.."])
+            os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

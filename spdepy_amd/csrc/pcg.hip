@@ -259,19 +259,44 @@ __global__ __launch_bounds__(256) void bwd_kernel(int64_t r0, int64_t r1, int lo
 }
 
 // ------------------------------------------------------------ PCG steps
-// Finalise per right-hand side sums of the partials: out[a][r] = sum_b part[a][b][r].
-__device__ inline double sum_parts(const double* __restrict__ part, int a, int r, int nblk) {
-  double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += part[((int64_t)a * kRedBlocks + b) * 64 + r];
-  return s;
+// Per right-hand side sums of the reduction partials, out[a][r] = sum_b
+// part[a][b][r] for a < NA, r < k, by a whole block of kRedThreads: thread
+// (q = tid / 64, r = tid % 64) adds blocks q, q + 16, ..., then the 16 values
+// of each r are added in q order (a fixed order: deterministic).  `out` is
+// shared memory; every thread returns after the block has it.
+constexpr int kRedThreads = 1024;
+template <int NA>
+__device__ inline void reduce_parts(const double* __restrict__ part, int k, int nblk, double (*out)[64]) {
+  __shared__ double red[NA][16][64];
+  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    double acc = 0.0;
+    if (r < k)
+      for (int b = q; b < nblk; b += 16) acc += part[((int64_t)a * kRedBlocks + b) * 64 + r];
+    red[a][q][r] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x < 64)
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      double v = 0.0;
+      for (int qq = 0; qq < 16; ++qq) v += red[a][qq][r];
+      out[a][r] = v;
+    }
+  __syncthreads();
 }
 
 // x = 0, p = z, rz = r^T z, bn = b^T b (r = b), active = bn > 0.
-__global__ void pcg_init_scalars_kernel(int k, int nblk, const double* __restrict__ part, double tol2,
-                                        double* __restrict__ sc, int* __restrict__ active, int* __restrict__ iters) {
+__global__ __launch_bounds__(kRedThreads) void pcg_init_scalars_kernel(int k, int nblk, const double* __restrict__ part,
+                                                                      double tol2, double* __restrict__ sc,
+                                                                      int* __restrict__ active,
+                                                                      int* __restrict__ iters) {
+  __shared__ double sums[2][64];
+  reduce_parts<2>(part, k, nblk, sums);
   const int r = threadIdx.x;
   if (r >= k) return;
-  const double rz = sum_parts(part, 0, r, nblk), bn = sum_parts(part, 1, r, nblk);
+  const double rz = sums[0][r], bn = sums[1][r];
   sc[S_RZ * 64 + r] = rz;
   sc[S_BN * 64 + r] = bn;
   sc[S_TOL2 * 64 + r] = tol2;
@@ -280,11 +305,13 @@ __global__ void pcg_init_scalars_kernel(int k, int nblk, const double* __restric
   iters[r] = 0;
 }
 
-__global__ void pcg_alpha_kernel(int k, int nblk, const double* __restrict__ part, double* __restrict__ sc,
-                                 const int* __restrict__ active) {
+__global__ __launch_bounds__(kRedThreads) void pcg_alpha_kernel(int k, int nblk, const double* __restrict__ part,
+                                                               double* __restrict__ sc, const int* __restrict__ active) {
+  __shared__ double sums[1][64];
+  reduce_parts<1>(part, k, nblk, sums);
   const int r = threadIdx.x;
   if (r >= k) return;
-  const double pq = sum_parts(part, 0, r, nblk);
+  const double pq = sums[0][r];
   sc[S_PQ * 64 + r] = pq;
   sc[S_ALPHA * 64 + r] = (active[r] && pq > 0.0) ? sc[S_RZ * 64 + r] / pq : 0.0;
 }
@@ -312,11 +339,14 @@ __global__ __launch_bounds__(256) void dot2_kernel(int64_t units, int logk, cons
 }
 
 // beta = rz_new / rz; convergence: ||r||^2 <= tol^2 ||b||^2 stops a right-hand side.
-__global__ void pcg_beta_kernel(int k, int nblk, const double* __restrict__ part, double* __restrict__ sc,
-                                int* __restrict__ active, int* __restrict__ iters) {
+__global__ __launch_bounds__(kRedThreads) void pcg_beta_kernel(int k, int nblk, const double* __restrict__ part,
+                                                              double* __restrict__ sc, int* __restrict__ active,
+                                                              int* __restrict__ iters) {
+  __shared__ double sums[2][64];
+  reduce_parts<2>(part, k, nblk, sums);
   const int r = threadIdx.x;
   if (r >= k) return;
-  const double rz = sum_parts(part, 0, r, nblk), rr = sum_parts(part, 1, r, nblk);
+  const double rz = sums[0][r], rr = sums[1][r];
   if (!active[r]) {
     sc[S_BETA * 64 + r] = 0.0;
     return;
@@ -348,29 +378,35 @@ __global__ __launch_bounds__(256) void copy_kernel(int64_t units, const float* _
 }
 
 // ------------------------------------------------------------ refinement
-// b32 = r64 / s (s = ||r64|| on the device), for the FP32 inner solve.
-__global__ __launch_bounds__(256) void scale_down_kernel(int64_t n, int nblk, const double* __restrict__ part,
+// b32 = r64 / s (s = ||r64||, its square at *s2 on the device), for the FP32 inner solve.
+__global__ __launch_bounds__(256) void scale_down_kernel(int64_t n, const double* __restrict__ s2,
                                                          const double* __restrict__ r64, float* __restrict__ b32) {
-  const double s = sqrt(sum_parts(part, 0, 0, nblk));
+  const double s = sqrt(*s2);
   const double inv = s > 0.0 ? 1.0 / s : 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     b32[i] = (float)(r64[i] * inv);
 }
-// x64 += s * d32.
-__global__ __launch_bounds__(256) void correct_kernel(int64_t n, int nblk, const double* __restrict__ part,
+// x64 += s * d32 (s^2 at *s2).
+__global__ __launch_bounds__(256) void correct_kernel(int64_t n, const double* __restrict__ s2,
                                                       const float* __restrict__ d32, double* __restrict__ x64) {
-  const double s = sqrt(sum_parts(part, 0, 0, nblk));
+  const double s = sqrt(*s2);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     x64[i] += s * (double)d32[i];
 }
-__global__ void sum_one_kernel(int nblk, const double* __restrict__ part, double* __restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = sum_parts(part, 0, 0, nblk);
+__global__ __launch_bounds__(kRedThreads) void sum_one_kernel(int nblk, const double* __restrict__ part,
+                                                             double* __restrict__ out) {
+  __shared__ double sums[1][64];
+  reduce_parts<1>(part, 1, nblk, sums);
+  if (threadIdx.x == 0) out[0] = sums[0][0];
 }
 // out[0] = sum over the k right-hand sides of the partials (in order).
-__global__ void sum_rhs_kernel(int nblk, int k, const double* __restrict__ part, double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(kRedThreads) void sum_rhs_kernel(int nblk, int k, const double* __restrict__ part,
+                                                             double* __restrict__ out) {
+  __shared__ double sums[1][64];
+  reduce_parts<1>(part, k, nblk, sums);
+  if (threadIdx.x != 0) return;
   double s = 0.0;
-  for (int r = 0; r < k; ++r) s += sum_parts(part, 0, r, nblk);
+  for (int r = 0; r < k; ++r) s += sums[0][r];
   out[0] = s;
 }
 
@@ -398,9 +434,12 @@ __global__ __launch_bounds__(256) void resid64k_kernel(int64_t n, int logk, cons
   block_reduce_store<1>(acc, k, part);
 }
 // out[r] = sum of the partials of right-hand side r (in block order).
-__global__ void norms_kernel(int k, int nblk, const double* __restrict__ part, double* __restrict__ out) {
+__global__ __launch_bounds__(kRedThreads) void norms_kernel(int k, int nblk, const double* __restrict__ part,
+                                                           double* __restrict__ out) {
+  __shared__ double sums[1][64];
+  reduce_parts<1>(part, k, nblk, sums);
   const int r = threadIdx.x;
-  if (r < k) out[r] = sum_parts(part, 0, r, nblk);
+  if (r < k) out[r] = sums[0][r];
 }
 // b32 = r64 / ||r64|| per right-hand side; zero for those already within
 // their threshold (||r||^2 <= thr), whose inner solve is then empty.
@@ -431,11 +470,14 @@ __global__ __launch_bounds__(256) void fill64_kernel(int64_t units, double* __re
 // ------------------------------------------------------------ Lanczos
 // alpha_j = q^T w per right-hand side (partials), then w -= alpha q + beta_prev q_prev
 // with the partials of ||w||^2; then q_prev = q, q = w / beta_j.
-__global__ void lanczos_alpha_kernel(int k, int nblk, const double* __restrict__ part, double* __restrict__ sc,
-                                     double* __restrict__ alpha_out) {
+__global__ __launch_bounds__(kRedThreads) void lanczos_alpha_kernel(int k, int nblk, const double* __restrict__ part,
+                                                                   double* __restrict__ sc,
+                                                                   double* __restrict__ alpha_out) {
+  __shared__ double sums[1][64];
+  reduce_parts<1>(part, k, nblk, sums);
   const int r = threadIdx.x;
   if (r >= k) return;
-  const double a = sum_parts(part, 0, r, nblk);
+  const double a = sums[0][r];
   sc[S_ALPHA * 64 + r] = a;
   alpha_out[r] = a;
 }
@@ -451,11 +493,14 @@ __global__ __launch_bounds__(256) void lanczos_orth_kernel(int64_t units, int lo
   }
   block_reduce_store<1>(acc, 1 << logk, part);
 }
-__global__ void lanczos_beta_kernel(int k, int nblk, const double* __restrict__ part, double* __restrict__ sc,
-                                    double* __restrict__ beta_out) {
+__global__ __launch_bounds__(kRedThreads) void lanczos_beta_kernel(int k, int nblk, const double* __restrict__ part,
+                                                                  double* __restrict__ sc,
+                                                                  double* __restrict__ beta_out) {
+  __shared__ double sums[1][64];
+  reduce_parts<1>(part, k, nblk, sums);
   const int r = threadIdx.x;
   if (r >= k) return;
-  const double b = sqrt(sum_parts(part, 0, r, nblk));
+  const double b = sqrt(sums[0][r]);
   sc[S_BETA * 64 + r] = b;
   beta_out[r] = b;
 }
@@ -728,7 +773,7 @@ int pcg_run(spde_ic* ic, const float* B, float* X, int logk, double tol, int max
   hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, s, units, B, R);
   ic_apply(ic, R, Z, logk, s);
   hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, R, Z, ic->d_part);
-  hipLaunchKernelGGL(pcg_init_scalars_kernel, dim3(1), dim3(64), 0, s, k, (int)g, ic->d_part, tol * tol, ic->d_scal,
+  hipLaunchKernelGGL(pcg_init_scalars_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, tol * tol, ic->d_scal,
                      ic->d_active, ic->d_iters);
   hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, s, units, Z, P);
   std::vector<int> act(k);
@@ -736,11 +781,11 @@ int pcg_run(spde_ic* ic, const float* B, float* X, int logk, double tol, int max
   for (int it = 0; it < maxit; ++it) {
     hipLaunchKernelGGL(spmm32_kernel, dim3(g), dim3(256), 0, s, n, logk, pl->d_indptr, pl->d_indices, ic->d_qc32, P,
                        Qv, ic->d_part);
-    hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(64), 0, s, k, (int)g, ic->d_part, ic->d_scal, ic->d_active);
+    hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, ic->d_scal, ic->d_active);
     hipLaunchKernelGGL(pcg_update_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, P, Qv, X, R);
     ic_apply(ic, R, Z, logk, s);
     hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, R, Z, ic->d_part);
-    hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(64), 0, s, k, (int)g, ic->d_part, ic->d_scal, ic->d_active,
+    hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, ic->d_scal, ic->d_active,
                        ic->d_iters);
     hipLaunchKernelGGL(pcg_p_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, ic->d_active, Z, P);
     if ((it + 1) % poll == 0 || it + 1 == maxit) {
@@ -1022,7 +1067,7 @@ int spde_ic_correct(spde_ic* ic, const float* corr, double* logdet_m, void* stre
   hipLaunchKernelGGL(ic_cast_diag_kernel, dim3(blocks_of(n)), dim3(256), 0, s, n, ic->d_dg64, ic->d_dinv);
   const unsigned g = grid_for(n);
   hipLaunchKernelGGL(logdet_diag_kernel, dim3(g), dim3(256), 0, s, n, ic->d_dg64, ic->d_part);
-  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(64), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64);
+  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64);
   SPDE_HIP(hipGetLastError());
   double ld = 0.0;
   SPDE_HIP(hipMemcpyAsync(&ld, ic->d_scal + S_X * 64, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1106,25 +1151,23 @@ int spde_pcg_refine(spde_ic* ic, const double* b, double* x, double tol, double 
   hipLaunchKernelGGL(resid64_kernel, dim3(g), dim3(256), 0, s, n, pl->d_indptr, pl->d_indices, pl->d_diag, ic->qval,
                      ic->diag_add, b, x, ic->d_r64, ic->d_part);
   double bn = 0.0, rn = 0.0;
-  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(64), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64);
+  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64);
   SPDE_HIP(hipMemcpyAsync(&bn, ic->d_scal + S_X * 64, sizeof(double), hipMemcpyDeviceToHost, s));
   SPDE_HIP(hipStreamSynchronize(s));
   rn = bn;
   int total = 0, r = 0;
+  // ||r||^2 of the current round stays at d_scal[S_X * 64] (pcg_run does not
+  // touch that slot) for the scaling before and the correction after the solve
   while (r < max_rounds && rn > tol * tol * bn && bn > 0.0) {
-    hipLaunchKernelGGL(scale_down_kernel, dim3(g), dim3(256), 0, s, n, (int)g, ic->d_part, ic->d_r64, ic->d_b32);
+    hipLaunchKernelGGL(scale_down_kernel, dim3(g), dim3(256), 0, s, n, ic->d_scal + S_X * 64, ic->d_r64, ic->d_b32);
     int32_t it = 0;
     rc = pcg_run(ic, ic->d_b32, ic->d_x32, 0, inner_tol, maxit, &it, s);
     if (rc) return rc;
     total += it;
-    // the partials still hold ||r||^2 of this round (pcg_run reuses d_part):
-    // recompute them from r64 before scaling the correction
+    hipLaunchKernelGGL(correct_kernel, dim3(g), dim3(256), 0, s, n, ic->d_scal + S_X * 64, ic->d_x32, x);
     hipLaunchKernelGGL(resid64_kernel, dim3(g), dim3(256), 0, s, n, pl->d_indptr, pl->d_indices, pl->d_diag,
                        ic->qval, ic->diag_add, b, x, ic->d_r64, ic->d_part);
-    hipLaunchKernelGGL(correct_kernel, dim3(g), dim3(256), 0, s, n, (int)g, ic->d_part, ic->d_x32, x);
-    hipLaunchKernelGGL(resid64_kernel, dim3(g), dim3(256), 0, s, n, pl->d_indptr, pl->d_indices, pl->d_diag,
-                       ic->qval, ic->diag_add, b, x, ic->d_r64, ic->d_part);
-    hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(64), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64);
+    hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64);
     SPDE_HIP(hipMemcpyAsync(&rn, ic->d_scal + S_X * 64, sizeof(double), hipMemcpyDeviceToHost, s));
     SPDE_HIP(hipStreamSynchronize(s));
     ++r;
@@ -1155,7 +1198,7 @@ int spde_pcg_refine_block(spde_ic* ic, const double* B, double* X, int k, double
   hipLaunchKernelGGL(fill64_kernel, dim3(g), dim3(256), 0, s, units, X);
   hipLaunchKernelGGL(resid64k_kernel, dim3(g), dim3(256), 0, s, n, logk, pl->d_indptr, pl->d_indices, ic->qval,
                      ic->diag_add, B, X, ic->d_r64, ic->d_part);
-  hipLaunchKernelGGL(norms_kernel, dim3(1), dim3(64), 0, s, k, (int)g, ic->d_part, rn);
+  hipLaunchKernelGGL(norms_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, rn);
   double bn[64], cur[64], th[64];
   SPDE_HIP(hipMemcpyAsync(bn, rn, sizeof(double) * k, hipMemcpyDeviceToHost, s));
   SPDE_HIP(hipStreamSynchronize(s));
@@ -1181,7 +1224,7 @@ int spde_pcg_refine_block(spde_ic* ic, const double* B, double* X, int k, double
     hipLaunchKernelGGL(correct_k_kernel, dim3(g), dim3(256), 0, s, units, logk, rn, thr, ic->d_x32, X);
     hipLaunchKernelGGL(resid64k_kernel, dim3(g), dim3(256), 0, s, n, logk, pl->d_indptr, pl->d_indices, ic->qval,
                        ic->diag_add, B, X, ic->d_r64, ic->d_part);
-    hipLaunchKernelGGL(norms_kernel, dim3(1), dim3(64), 0, s, k, (int)g, ic->d_part, rn);
+    hipLaunchKernelGGL(norms_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, rn);
     SPDE_HIP(hipMemcpyAsync(cur, rn, sizeof(double) * k, hipMemcpyDeviceToHost, s));
     SPDE_HIP(hipStreamSynchronize(s));
     ++rd;
@@ -1251,10 +1294,10 @@ int spde_slq(spde_ic* ic, int k, int steps, uint64_t seed, double* alpha, double
                        U, nullptr);                                              // U = Q_c W
     ic_fwd(ic, U, W, logk, s);                                                   // W = L^-1 U
     hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, Qc, W, ic->d_part);
-    hipLaunchKernelGGL(lanczos_alpha_kernel, dim3(1), dim3(64), 0, s, k, (int)g, ic->d_part, ic->d_scal,
+    hipLaunchKernelGGL(lanczos_alpha_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, ic->d_scal,
                        da + (int64_t)j * 64);
     hipLaunchKernelGGL(lanczos_orth_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, Qc, Qp, W, ic->d_part);
-    hipLaunchKernelGGL(lanczos_beta_kernel, dim3(1), dim3(64), 0, s, k, (int)g, ic->d_part, ic->d_scal,
+    hipLaunchKernelGGL(lanczos_beta_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, ic->d_scal,
                        db + (int64_t)j * 64);
     hipLaunchKernelGGL(lanczos_next_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, Qc, Qp, W);
   }
@@ -1302,12 +1345,12 @@ int spde_ic_kl_grad(spde_ic* ic, const double* x, int k, double scale, float* do
   const unsigned g = grid_for(n << logk);
   hipLaunchKernelGGL(kl_w_kernel, dim3(g), dim3(256), 0, s, n, logk, ic->d_uptr, ic->d_ucol, ic->d_ulow, ic->d_l64,
                      ic->d_dg64, x, ic->d_klw, ic->d_part);
-  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(64), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64 + 1);
+  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64 + 1);
   const unsigned g2 = grid_for(n);
   hipLaunchKernelGGL(kl_grad_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, logk, ic->d_lptr, ic->d_lcol,
                      ic->d_lch, ic->d_dg64, ic->d_adiag, x, ic->d_klw, scale, dout);
   hipLaunchKernelGGL(logdet_diag_kernel, dim3(g2), dim3(256), 0, s, n, ic->d_dg64, ic->d_part);
-  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(64), 0, s, (int)g2, ic->d_part, ic->d_scal + S_X * 64 + 2);
+  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(kRedThreads), 0, s, (int)g2, ic->d_part, ic->d_scal + S_X * 64 + 2);
   SPDE_HIP(hipGetLastError());
   if (loss) {
     double h[2];
@@ -1343,11 +1386,11 @@ int spde_ic_jeffreys_grad(spde_ic* ic, const double* x, const float* z, int k, d
   const unsigned g = grid_for(n << logk);
   hipLaunchKernelGGL(kl_w_kernel, dim3(g), dim3(256), 0, s, n, logk, ic->d_uptr, ic->d_ucol, ic->d_ulow, ic->d_l64,
                      ic->d_dg64, x, ic->d_klw, ic->d_part);
-  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(64), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64 + 1);
+  hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64 + 1);
   ic_bwd(ic, z, Y, logk, s);                                                     // y = L^-T z
   hipLaunchKernelGGL(spmm32_kernel, dim3(g), dim3(256), 0, s, n, logk, ic->pl->d_indptr, ic->pl->d_indices,
                      ic->d_qc32, Y, V, ic->d_part);                               // v = A y, partials of y^T A y
-  hipLaunchKernelGGL(sum_rhs_kernel, dim3(1), dim3(64), 0, s, (int)g, k, ic->d_part, ic->d_scal + S_X * 64 + 2);
+  hipLaunchKernelGGL(sum_rhs_kernel, dim3(1), dim3(kRedThreads), 0, s, (int)g, k, ic->d_part, ic->d_scal + S_X * 64 + 2);
   ic_fwd(ic, V, U, logk, s);                                                     // u = L^-1 A y
   hipLaunchKernelGGL(jeffreys_grad_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, logk, ic->d_lptr,
                      ic->d_lcol, ic->d_lch, ic->d_dg64, ic->d_adiag, x, ic->d_klw, Y, U, scale, dout);

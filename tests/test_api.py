@@ -81,3 +81,17 @@ def test_numeric_model_codes():
     for code, name in ((3, "AdvectionVarHaDiffusion2D"), (6, "VarAdvectionHaDiffusion2D"),
                        (7, "VarAdvectionVarHaDiffusion2D")):
         assert type(spdes.spde_init(code, g, mod0=mod0)).__name__ == name
+
+
+def test_reference_abi_stencil_libraries_export_the_reference_names():
+    """One library per stencil kind and boundary condition, each exporting
+    the reference's C names (AH_new/AH_Row/AH_Col/AH_Val/AH_delete or the
+    Aw_ set), loadable without a GPU."""
+    import ctypes
+    import os
+    d = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "spdepy_amd", "ccode")
+    for kind, pre in (("AcH", "AH"), ("AH", "AH"), ("Acw", "Aw"), ("Aw", "Aw")):
+        for bc in (1, 2, 3):
+            lib = ctypes.CDLL(os.path.join(d, "lib_%s_2D_b%d.so" % (kind, bc)))
+            for nm in ("_new", "_Row", "_Col", "_Val", "_delete"):
+                assert hasattr(lib, pre + nm), (kind, bc, nm)
