@@ -224,12 +224,16 @@ def batch_route(nmodels=64, nrhs=256, nstreams=8):
         ev.conditional_samples(i, nrhs, seed=i)
     torch.cuda.synchronize()
     t = time.perf_counter()
+    torch.cuda.nvtx.range_push("batch_loglik")      # markers for a profiler's timeline
     res = ev.loglik(pars)
     torch.cuda.synchronize()
+    torch.cuda.nvtx.range_pop()
     t_loglik = time.perf_counter() - t
+    torch.cuda.nvtx.range_push("batch_samples")
     for i in range(nmodels):
         ev.conditional_samples(i, nrhs, seed=i)
     torch.cuda.synchronize()
+    torch.cuda.nvtx.range_pop()
     t_batch = time.perf_counter() - t
     t = time.perf_counter()
     seq = []

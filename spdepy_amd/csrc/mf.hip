@@ -139,22 +139,21 @@ __global__ __launch_bounds__(256) void fwd_gather_kernel(const int32_t* __restri
                                                          const int32_t* __restrict__ children, const int32_t* __restrict__ relpos,
                                                          const double* __restrict__ xy, int64_t n, int nrhs,
                                                          double* __restrict__ vec) {
+  // block (node, column c): right-hand sides are independent, so a level's
+  // gather spreads over nodes x columns instead of one block per node
   const int node = lvl_nodes[blockIdx.x];
+  const int c = blockIdx.y;
   const NodeDev nd = nodes[node];
-  double* v = vec + nd.voff * nrhs;
-  for (int c = 0; c < nrhs; ++c)
-    for (int r = threadIdx.x; r < nd.n; r += 256)
-      v[r + (int64_t)c * nd.n] = (r < nd.p) ? xy[nd.start + r + (int64_t)c * n] : 0.0;
+  double* v = vec + nd.voff * nrhs + (int64_t)c * nd.n;
+  for (int r = threadIdx.x; r < nd.n; r += 256) v[r] = (r < nd.p) ? xy[nd.start + r + (int64_t)c * n] : 0.0;
   __syncthreads();
   for (int s = 0; s < 2; ++s) {
     const int ch = children[2 * node + s];
     if (ch < 0) continue;
     const NodeDev cd = nodes[ch];
     const int32_t* rp = relpos + cd.relpos_off;
-    const double* u = vec + cd.voff * nrhs;
-    for (int c = 0; c < nrhs; ++c)
-      for (int r = threadIdx.x; r < cd.nb; r += 256)
-        v[rp[r] + (int64_t)c * nd.n] += u[cd.p + r + (int64_t)c * cd.n];
+    const double* u = vec + cd.voff * nrhs + (int64_t)c * cd.n;
+    for (int r = threadIdx.x; r < cd.nb; r += 256) v[rp[r]] += u[cd.p + r];
     __syncthreads();
   }
 }
@@ -165,14 +164,14 @@ __global__ __launch_bounds__(256) void bwd_gather_kernel(const int32_t* __restri
                                                          const int32_t* __restrict__ bnd, const double* __restrict__ xy,
                                                          int64_t n, int nrhs, double* __restrict__ vec) {
   const int node = lvl_nodes[blockIdx.x];
+  const int c = blockIdx.y;  // one column per block, as the forward gather
   const NodeDev nd = nodes[node];
-  double* v = vec + nd.voff * nrhs;
+  double* v = vec + nd.voff * nrhs + (int64_t)c * nd.n;
   const int32_t* B = bnd + nd.relpos_off;
-  for (int c = 0; c < nrhs; ++c)
-    for (int r = threadIdx.x; r < nd.n; r += 256) {
-      const int64_t g = (r < nd.p) ? (int64_t)nd.start + r : (int64_t)B[r - nd.p];
-      v[r + (int64_t)c * nd.n] = xy[g + (int64_t)c * n];
-    }
+  for (int r = threadIdx.x; r < nd.n; r += 256) {
+    const int64_t g = (r < nd.p) ? (int64_t)nd.start + r : (int64_t)B[r - nd.p];
+    v[r] = xy[g + (int64_t)c * n];
+  }
 }
 
 // --------------------------------------------------------- selected inverse
@@ -271,13 +270,13 @@ void launch_permute_out(const double* xy, const int32_t* perm, int64_t n, int nr
 void launch_fwd_gather(const int32_t* lvl_nodes, int nn, const NodeDev* nodes, const int32_t* children,
                        const int32_t* relpos, const double* xy, int64_t n, int nrhs, double* vec, hipStream_t s) {
   if (nn > 0)
-    hipLaunchKernelGGL(fwd_gather_kernel, dim3(nn), dim3(256), 0, s, lvl_nodes, nodes, children, relpos, xy, n, nrhs,
-                       vec);
+    hipLaunchKernelGGL(fwd_gather_kernel, dim3(nn, nrhs), dim3(256), 0, s, lvl_nodes, nodes, children, relpos, xy, n,
+                       nrhs, vec);
 }
 void launch_bwd_gather(const int32_t* lvl_nodes, int nn, const NodeDev* nodes, const int32_t* bnd, const double* xy,
                        int64_t n, int nrhs, double* vec, hipStream_t s) {
   if (nn > 0)
-    hipLaunchKernelGGL(bwd_gather_kernel, dim3(nn), dim3(256), 0, s, lvl_nodes, nodes, bnd, xy, n, nrhs, vec);
+    hipLaunchKernelGGL(bwd_gather_kernel, dim3(nn, nrhs), dim3(256), 0, s, lvl_nodes, nodes, bnd, xy, n, nrhs, vec);
 }
 void launch_sigma_gather(const int32_t* lvl_nodes, const NodeDev* nodes, const int32_t* relpos, double* S,
                          const int4* tiles, int ntiles, hipStream_t s) {
