@@ -193,3 +193,19 @@ def jeffreys_grad(lower, diag, lptr, lcol, lch, A, col, x, z):
     dd = ((x * w).sum(1) - (y * u).sum(1)) / K
     dout[np.arange(n) * NPAIR] = diag * dd
     return loss, dout.reshape(n, NPAIR)
+
+
+def kaporin_grad(lower, diag, lptr, lcol, lch, A, col, z, ld_m):
+    """(loss, dout) of Kaporin's condition number per row,
+        K/n = log(t / n) + (1/n) log|L' L'^T|  (+ the constant -(1/n) log|A|),
+    t = mean_s z_s^T L'^-1 A L'^-T z_s, from the Jeffreys gradient with zero
+    samples (its first term vanishes): dt = 2 dJ0, and d log L'_ii / d corr[i, 0] = 1.
+    ld_m = log|L' L'^T| = 2 sum log L'_ii."""
+    n = diag.size
+    j0, g0 = jeffreys_grad(lower, diag, lptr, lcol, lch, A, col, np.zeros((n, z.shape[1])), z)
+    t = 2.0 * (j0 + n)
+    loss = np.log(t / n) + ld_m / n
+    dout = (2.0 / t) * g0
+    dout[:, 0] += 2.0 / n
+    return loss, dout
+

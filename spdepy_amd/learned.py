@@ -205,14 +205,41 @@ class LearnedPreconditioner:
         order).  loss="jeffreys": the symmetrised divergence
         KL(p||q) + KL(q||p) (p = N(0, Q_c^-1), q = N(0, (L' L'^T)^-1), with
         K Rademacher probes of seed `seed` for the second term), which bounds
-        both ends of the preconditioned spectrum; loss="kl": KL(p||q) alone.
+        both ends of the preconditioned spectrum; loss="kl": KL(p||q) alone;
+        loss="kaporin": Kaporin's condition number log(mean eig) - mean log eig
+        of L'^-1 Q_c L'^-T, which bounds the CG iteration count and, like CG,
+        ignores the scale of L'; "kaporin+jeffreys" adds 0.1 of the divergence.
         Returns the loss per row before the step."""
         torch = _torch()
         acts = self.forward(self.features(it), shape)
-        it.correct(acts[-1][:, :NPAIR].contiguous())
+        ld_m = it.correct(acts[-1][:, :NPAIR].contiguous())
         dout = torch.empty((it.n, NPAIR), dtype=torch.float32, device="cuda")
         val = ctypes.c_double()
         K = samples.shape[1]
+        if loss.startswith("kaporin"):
+            # Kaporin's condition number per row, log(t / n) + log|L' L'^T| / n with
+            # t = mean z^T L'^-1 A L'^-T z: the Jeffreys gradient with zero samples
+            # is dt / 2, and d log L'_ii / d corr[i, 0] = 1.  Scale-invariant (as
+            # CG is); "kaporin+jeffreys" adds 0.1 of the per-row divergence against
+            # the exact samples.
+            n = it.n
+            z = it.rademacher(K, seed)
+            zeros = torch.zeros_like(samples)
+            nat.check(nat.lib().spde_ic_jeffreys_grad(it._h, _ptr(zeros), _ptr(z), K, 1.0, _ptr(dout),
+                                                      ctypes.byref(val), _stream(torch)))
+            t = 2.0 * (val.value + n)
+            total = float(np.log(t / n) + ld_m / n)
+            dout *= 2.0 / t
+            dout[:, 0] += 2.0 / n
+            if loss == "kaporin+jeffreys":
+                dj = torch.empty_like(dout)
+                nat.check(nat.lib().spde_ic_jeffreys_grad(it._h, _ptr(samples), _ptr(z), K, 0.1 / n, _ptr(dj),
+                                                          ctypes.byref(val), _stream(torch)))
+                dout += dj
+                total += 0.1 * val.value / n
+            self.backward(acts, dout, shape)
+            self.adam_step(lr=lr)
+            return total
         if loss == "jeffreys":
             z = it.rademacher(K, seed)
             nat.check(nat.lib().spde_ic_jeffreys_grad(it._h, _ptr(samples), _ptr(z), K, 1.0 / it.n, _ptr(dout),

@@ -97,7 +97,7 @@ def test_sampler_and_selected_inverse_identities(workload):
     step("factor + sample + selinv")
     eng.plan.factor(0, qval)
     # 16 right-hand sides, as the configuration's 16 conditional samples (the
-    # tile-GEMM solve path; up to 8 take the narrow one)
+    # tile-GEMM solve path; one right-hand side takes the bandwidth-bound one)
     z = torch.as_tensor(np.random.default_rng(0).normal(size=(eng.n, 16))).cuda()
     x = eng.plan.solve(0, z, 2)
     qx = eng.plan.spmv(qval, x)

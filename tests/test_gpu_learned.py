@@ -244,3 +244,83 @@ def test_learned_route_and_cholesky_fallback():
     p = mod.last_parts
     assert p["fallback"] and p["infeasible"] and p["like_rel_se"] > 1e-6
     assert l_fb == l_ex and np.array_equal(j_fb, j_ex)
+
+
+def _held_out_iterations(mod, net, ndraws, seed, extra=()):
+    """Mean PCG iterations (tolerance 1e-5, 8 right-hand sides) with plain
+    IC(0) and with the network's correction, on held-out draws of the
+    B-spline model (observations on 2.5% of the lattice, tau = 100)."""
+    import torch
+    from spdepy_amd.learned import draw_parameters
+    eng = mod._engine()
+    it = eng.iterative()
+    n = eng.n
+    rng = np.random.default_rng(seed)
+    cases = []
+    for _ in range(ndraws):
+        par = draw_parameters(mod.Np, rng)
+        qval = eng.assemble(mod._coeffs(par)).clone()
+        dadd = np.zeros(n)
+        dadd[rng.choice(n, n // 40, replace=False)] = 100.0
+        cases.append((qval, torch.as_tensor(dadd).cuda()))
+    for par, cols in extra:
+        qval = eng.assemble(mod._coeffs(par)).clone()
+        dadd = np.zeros(n)
+        dadd[cols] = np.exp(par[-1])
+        cases.append((qval, torch.as_tensor(dadd).cuda()))
+    B = torch.as_tensor(np.random.default_rng(seed + 1).standard_normal((n, 8)).astype(np.float32)).cuda()
+    out = []
+    for qval, dd in cases:
+        it.factor(qval, dd, shift=0.1)
+        _, plain = it.solve(B, tol=1e-5)
+        it.factor(qval, dd, shift=0.1)
+        it.correct(net.correction_for(it, eng.plan.shape))
+        _, learned = it.solve(B, tol=1e-5)
+        out.append((plain.mean(), learned.mean()))
+    return np.array(out)
+
+
+def test_kaporin_training_beats_plain_ic0():
+    """Training on Kaporin's condition number (log of the mean over the mean
+    log of the eigenvalues of L'^-1 Q_c L'^-T, which bounds CG's iteration
+    count) for 300 draws at 20 x 20 x 6: fewer PCG iterations than plain
+    IC(0) on every held-out draw."""
+    import spdepy_amd as sp
+    from spdepy_amd import spdes
+    from spdepy_amd.learned import LearnedPreconditioner, train
+    ax = (np.arange(20) + 0.5) / 20
+    g = sp.grid(x=ax, y=ax, t=np.arange(6.0), Nbs=4)
+    g0 = sp.grid(x=ax, y=ax, Nbs=4)
+    mod0 = spdes.WhittleMatern2D(grid=g0, par=np.array([0.0, np.log(0.5), 0.0]), bc=1)
+    mod = spdes.VarAdvectionVarDiffusion2D(grid=g, mod0=mod0, bc=1)
+    net = LearnedPreconditioner(seed=0)
+    step("train 300 draws 20x20x6 (Kaporin)")
+    train(net, mod, 300, K=16, seed=1, lr=1e-3, loss="kaporin")
+    res = _held_out_iterations(mod, net, 3, 99)
+    for p, l in res:
+        print("  PCG iterations IC %.1f learned %.1f" % (p, l))
+    assert np.all(res[:, 1] < res[:, 0])
+
+
+def test_shipped_weights_beat_plain_ic0_at_100x100x20():
+    """The shipped network (spdepy_amd/weights/learned_ic.npz) on held-out
+    draws at 100 x 100 x 20 and on the benchmark workload's parameters and
+    observation sites: fewer PCG iterations than plain IC(0) on every case."""
+    import spdepy_amd as sp
+    from spdepy_amd import spdes
+    from spdepy_amd.learned import default_preconditioner
+    M = N = 100
+    ax = (np.arange(M) + 0.5) / M
+    g = sp.grid(x=ax, y=ax, t=np.arange(20.0), Nbs=8)
+    g0 = sp.grid(x=ax, y=ax, Nbs=8)
+    mod0 = spdes.WhittleMatern2D(grid=g0, par=np.array([0.0, np.log(0.5), 0.0]), bc=1)
+    mod = spdes.VarAdvectionVarDiffusion2D(grid=g, mod0=mod0, bc=1)
+    Np = g.Nbs2
+    rb = np.random.default_rng(3)
+    pb = np.hstack([np.zeros(Np), np.full(Np, np.log(0.5)), 0.3 * rb.normal(size=4 * Np), [0.0, np.log(100.0)]])
+    cols = np.sort(np.random.default_rng(4).choice(M * N * 20, 5000, replace=False))
+    step("shipped weights vs plain IC(0) at 100x100x20")
+    res = _held_out_iterations(mod, default_preconditioner(), 2, 77, extra=[(pb, cols)])
+    for p, l in res:
+        print("  PCG iterations IC %.1f learned %.1f (ratio %.3f)" % (p, l, l / p))
+    assert np.all(res[:, 1] < res[:, 0])

@@ -191,3 +191,37 @@ def test_jeffreys_vanishes_at_the_exact_factor():
     lam = np.linalg.eigvalsh(0.5 * (Bm + Bm.T))
     assert abs(loss_ic - (0.5 * (lam + 1 / lam).sum() - n)) <= 1e-8 * n
     assert loss_ic > 0
+
+
+def test_kaporin_gradient_matches_central_differences():
+    """Kaporin's condition number (scale-invariant) of the corrected factor:
+    the composed gradient against central differences of the loss, and the
+    loss is unchanged by a global rescaling of L' (corr[:, 0] += c)."""
+    A, lower, d, lptr, lcol, lch, (M, N, T) = _factor_case()
+    n = A.shape[0]
+    col = K.colours(M, N, T)
+    z = K.rademacher(n, 8, 3)
+    adiag = A.diagonal()
+    corr = np.random.default_rng(11).normal(scale=0.01, size=(n, Le.NPAIR))
+
+    def loss(c):
+        lo, dg = Le.corrected_factor(lower, d, lptr, lcol, lch, adiag, c)
+        return Le.kaporin_grad(lo, dg, lptr, lcol, lch, A, col, z, 2 * np.log(dg).sum())[0]
+    lo, dg = Le.corrected_factor(lower, d, lptr, lcol, lch, adiag, corr)
+    _, dout = Le.kaporin_grad(lo, dg, lptr, lcol, lch, A, col, z, 2 * np.log(dg).sum())
+    rng = np.random.default_rng(12)
+    used = np.concatenate([lch[lch >= 0], np.arange(n) * Le.NPAIR])
+    for e in rng.choice(used, 30, replace=False):
+        i, ch = divmod(int(e), Le.NPAIR)
+        h = 1e-6
+        cp, cm = corr.copy(), corr.copy()
+        cp[i, ch] += h
+        cm[i, ch] -= h
+        fd = (loss(cp) - loss(cm)) / (2 * h)
+        assert abs(fd - dout[i, ch]) <= 1e-5 * max(1.0, abs(fd)), (i, ch, fd, dout[i, ch])
+    # a global scale of L' (with the off-diagonal entries scaled alike) leaves it unchanged
+    lo2, dg2 = 1.7 * lo, 1.7 * dg
+    l1 = Le.kaporin_grad(lo, dg, lptr, lcol, lch, A, col, z, 2 * np.log(dg).sum())[0]
+    l2 = Le.kaporin_grad(lo2, dg2, lptr, lcol, lch, A, col, z, 2 * np.log(dg2).sum())[0]
+    assert abs(l1 - l2) <= 1e-10 * abs(l1)
+
