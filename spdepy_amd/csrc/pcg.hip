@@ -258,6 +258,60 @@ __global__ __launch_bounds__(256) void bwd_kernel(int64_t r0, int64_t r1, int lo
   x[((int64_t)i << logk) + rhs] = s * dinv[i];
 }
 
+// One right-hand side: a row is shared by kRowLanes consecutive lanes (entries
+// strided by kRowLanes, then a fixed butterfly over the group), so a colour's
+// rows are not each a single thread's chain of dependent gathers.
+constexpr int kRowLanes = 8;
+__global__ __launch_bounds__(256) void fwd1_kernel(int64_t r0, int64_t r1, const int32_t* __restrict__ crow,
+                                                   const int64_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
+                                                   const float* __restrict__ l32, const float* __restrict__ dinv,
+                                                   const float* r, float* z) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t rr = r0 + g / kRowLanes;
+  const int lane = (int)(g % kRowLanes);
+  if (rr >= r1) return;  // uniform over the group
+  const int i = crow[rr];
+  float acc = 0.f;
+  for (int64_t p = lptr[i] + lane; p < lptr[i + 1]; p += kRowLanes) acc += l32[p] * z[lcol[p]];
+  for (int o = kRowLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kRowLanes);
+  if (lane == 0) z[i] = (r[i] - acc) * dinv[i];
+}
+// (y and x may be the same array: row i reads only rows of other colours)
+__global__ __launch_bounds__(256) void bwd1_kernel(int64_t r0, int64_t r1, const int32_t* __restrict__ crow,
+                                                   const int64_t* __restrict__ uptr, const int32_t* __restrict__ ucol,
+                                                   const float* __restrict__ u32, const float* __restrict__ dinv,
+                                                   const float* y, float* x) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t rr = r0 + g / kRowLanes;
+  const int lane = (int)(g % kRowLanes);
+  if (rr >= r1) return;
+  const int i = crow[rr];
+  float acc = 0.f;
+  for (int64_t p = uptr[i] + lane; p < uptr[i + 1]; p += kRowLanes) acc += u32[p] * x[ucol[p]];
+  for (int o = kRowLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kRowLanes);
+  if (lane == 0) x[i] = (y[i] - acc) * dinv[i];
+}
+// y = A x for one right-hand side, a group of kRowLanes lanes per row (grid-
+// stride over the rows); the block partials of x^T y as spmm32_kernel's.
+__global__ __launch_bounds__(256) void spmm1_kernel(int64_t n, const int64_t* __restrict__ indptr,
+                                                    const int32_t* __restrict__ indices, const float* __restrict__ a,
+                                                    const float* __restrict__ x, float* __restrict__ y,
+                                                    double* __restrict__ part) {
+  const int lane = (int)(threadIdx.x % kRowLanes);
+  const int64_t groups = (int64_t)gridDim.x * blockDim.x / kRowLanes;
+  double acc[1] = {0.0};
+  for (int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kRowLanes; i < n; i += groups) {
+    float s = 0.f;
+    for (int64_t e = indptr[i] + lane; e < indptr[i + 1]; e += kRowLanes) s += a[e] * x[indices[e]];
+    for (int o = kRowLanes / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kRowLanes);
+    if (lane == 0) {
+      y[i] = s;
+      acc[0] += (double)s * (double)x[i];
+    }
+  }
+  if (part) block_reduce_store<1>(acc, 1, part);
+}
+
 // ------------------------------------------------------------ PCG steps
 // Per right-hand side sums of the reduction partials, out[a][r] = sum_b
 // part[a][b][r] for a < NA, r < k, by a whole block of kRedThreads: thread
@@ -712,25 +766,14 @@ int log2_of(int k) {
 }
 
 // Sweeps of M^-1 = (L L^T)^-1: forward from r into z, then backward in place.
-int ic_apply(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s) {
-  for (int c = 0; c < kColours; ++c) {
-    const int64_t r0 = ic->coff[c], r1 = ic->coff[c + 1];
-    if (r1 > r0)
-      hipLaunchKernelGGL(fwd_kernel, dim3(blocks_of((r1 - r0) << logk)), dim3(256), 0, s, r0, r1, logk, ic->d_crow,
-                         ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dinv, r, z);
-  }
-  for (int c = kColours - 1; c >= 0; --c) {
-    const int64_t r0 = ic->coff[c], r1 = ic->coff[c + 1];
-    if (r1 > r0)
-      hipLaunchKernelGGL(bwd_kernel, dim3(blocks_of((r1 - r0) << logk)), dim3(256), 0, s, r0, r1, logk, ic->d_crow,
-                         ic->d_uptr, ic->d_ucol, ic->d_u32, ic->d_dinv, z, z);
-  }
-  return SPDE_OK;
-}
 int ic_fwd(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s) {
   for (int c = 0; c < kColours; ++c) {
     const int64_t r0 = ic->coff[c], r1 = ic->coff[c + 1];
-    if (r1 > r0)
+    if (r1 <= r0) continue;
+    if (logk == 0)
+      hipLaunchKernelGGL(fwd1_kernel, dim3(blocks_of((r1 - r0) * kRowLanes)), dim3(256), 0, s, r0, r1, ic->d_crow,
+                         ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dinv, r, z);
+    else
       hipLaunchKernelGGL(fwd_kernel, dim3(blocks_of((r1 - r0) << logk)), dim3(256), 0, s, r0, r1, logk, ic->d_crow,
                          ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dinv, r, z);
   }
@@ -739,11 +782,30 @@ int ic_fwd(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s) {
 int ic_bwd(spde_ic* ic, const float* y, float* x, int logk, hipStream_t s) {
   for (int c = kColours - 1; c >= 0; --c) {
     const int64_t r0 = ic->coff[c], r1 = ic->coff[c + 1];
-    if (r1 > r0)
+    if (r1 <= r0) continue;
+    if (logk == 0)
+      hipLaunchKernelGGL(bwd1_kernel, dim3(blocks_of((r1 - r0) * kRowLanes)), dim3(256), 0, s, r0, r1, ic->d_crow,
+                         ic->d_uptr, ic->d_ucol, ic->d_u32, ic->d_dinv, y, x);
+    else
       hipLaunchKernelGGL(bwd_kernel, dim3(blocks_of((r1 - r0) << logk)), dim3(256), 0, s, r0, r1, logk, ic->d_crow,
                          ic->d_uptr, ic->d_ucol, ic->d_u32, ic->d_dinv, y, x);
   }
   return SPDE_OK;
+}
+// Sweeps of M^-1 = (L L^T)^-1: forward from r into z, then backward in place.
+int ic_apply(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s) {
+  ic_fwd(ic, r, z, logk, s);
+  return ic_bwd(ic, z, z, logk, s);
+}
+// y = Q_c x in FP32 (k = 2^logk right-hand sides) with the block partials of
+// x^T y when part is set; `g` blocks (the caller's reduction grid).
+void launch_spmm(spde_ic* ic, int logk, unsigned g, const float* x, float* y, double* part, hipStream_t s) {
+  if (logk == 0)
+    hipLaunchKernelGGL(spmm1_kernel, dim3(g), dim3(256), 0, s, ic->n, ic->pl->d_indptr, ic->pl->d_indices,
+                       ic->d_qc32, x, y, part);
+  else
+    hipLaunchKernelGGL(spmm32_kernel, dim3(g), dim3(256), 0, s, ic->n, logk, ic->pl->d_indptr, ic->pl->d_indices,
+                       ic->d_qc32, x, y, part);
 }
 
 int ensure_work(spde_ic* ic) {
@@ -779,8 +841,7 @@ int pcg_run(spde_ic* ic, const float* B, float* X, int logk, double tol, int max
   std::vector<int> act(k);
   const int poll = 16;
   for (int it = 0; it < maxit; ++it) {
-    hipLaunchKernelGGL(spmm32_kernel, dim3(g), dim3(256), 0, s, n, logk, pl->d_indptr, pl->d_indices, ic->d_qc32, P,
-                       Qv, ic->d_part);
+    launch_spmm(ic, logk, g, P, Qv, ic->d_part, s);
     hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, ic->d_scal, ic->d_active);
     hipLaunchKernelGGL(pcg_update_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, P, Qv, X, R);
     ic_apply(ic, R, Z, logk, s);
@@ -1118,8 +1179,7 @@ int spde_ic_spmm(spde_ic* ic, const float* x, float* y, int k, void* stream) {
     set_error("spde_ic_spmm: invalid argument");
     return SPDE_ERR_ARG;
   }
-  hipLaunchKernelGGL(spmm32_kernel, dim3(grid_for(ic->n << logk)), dim3(256), 0, (hipStream_t)stream, ic->n, logk,
-                     ic->pl->d_indptr, ic->pl->d_indices, ic->d_qc32, x, y, nullptr);
+  launch_spmm(ic, logk, grid_for(ic->n << logk), x, y, nullptr, (hipStream_t)stream);
   SPDE_HIP(hipGetLastError());
   return SPDE_OK;
 }
@@ -1290,8 +1350,7 @@ int spde_slq(spde_ic* ic, int k, int steps, uint64_t seed, double* alpha, double
   double* db = ic->d_lanczos + (int64_t)steps * 64;
   for (int j = 0; j < steps; ++j) {
     ic_bwd(ic, Qc, W, logk, s);                                                  // W = L^-T q
-    hipLaunchKernelGGL(spmm32_kernel, dim3(g), dim3(256), 0, s, n, logk, pl->d_indptr, pl->d_indices, ic->d_qc32, W,
-                       U, nullptr);                                              // U = Q_c W
+    launch_spmm(ic, logk, g, W, U, nullptr, s);                                  // U = Q_c W
     ic_fwd(ic, U, W, logk, s);                                                   // W = L^-1 U
     hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, Qc, W, ic->d_part);
     hipLaunchKernelGGL(lanczos_alpha_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, ic->d_scal,
@@ -1388,8 +1447,7 @@ int spde_ic_jeffreys_grad(spde_ic* ic, const double* x, const float* z, int k, d
                      ic->d_dg64, x, ic->d_klw, ic->d_part);
   hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal + S_X * 64 + 1);
   ic_bwd(ic, z, Y, logk, s);                                                     // y = L^-T z
-  hipLaunchKernelGGL(spmm32_kernel, dim3(g), dim3(256), 0, s, n, logk, ic->pl->d_indptr, ic->pl->d_indices,
-                     ic->d_qc32, Y, V, ic->d_part);                               // v = A y, partials of y^T A y
+  launch_spmm(ic, logk, g, Y, V, ic->d_part, s);                                 // v = A y, partials of y^T A y
   hipLaunchKernelGGL(sum_rhs_kernel, dim3(1), dim3(kRedThreads), 0, s, (int)g, k, ic->d_part, ic->d_scal + S_X * 64 + 2);
   ic_fwd(ic, V, U, logk, s);                                                     // u = L^-1 A y
   hipLaunchKernelGGL(jeffreys_grad_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, logk, ic->d_lptr,

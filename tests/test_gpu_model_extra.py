@@ -118,3 +118,44 @@ def test_model_with_covariates(two):
     mvar = mod.qinv()
     want = np.r_[np.diag(np.linalg.inv(S @ Qc @ S.T)), np.zeros(k)]
     assert np.allclose(mvar, want, rtol=1e-8, atol=1e-12 * np.abs(want).max())
+
+
+def test_kriging_driver_flow_on_an_extended_grid():
+    """The reference's kriging drivers (examples/server/get_results.py:6-14):
+    setModel(mu=cov, sigmas=2 values, useCov=True) on a space-time model over
+    an extended grid, the observed interior cells by getIdx(extend=False),
+    update(y, idx), and the prediction getS() @ mu -- against dense algebra
+    on the covariate-augmented precision."""
+    import spdepy_amd as sp
+    from spdepy_amd import spdes
+    M_, N_, T_ = 8, 7, 3
+    x, y = F.axis(M_), F.axis(N_)
+    g = sp.grid(x=x, y=y, t=np.arange(float(T_)), extend=2)
+    g0 = sp.grid(x=x, y=y, extend=2)
+    mod0 = spdes.WhittleMatern2D(grid=g0, par=np.array([0.0, np.log(0.5), 0.0]), bc=3)
+    st = spdes.AdvectionDiffusion2D(grid=g, mod0=mod0, bc=3)
+    par = np.array([0.1, np.log(0.6), 0.2, -0.1, 0.3, -0.2, 0.0, np.log(20.0)])
+    st.setQ(par=par)
+    mod = sp.Model.__new__(sp.Model)
+    mod.grid, mod.mod, mod.Q, mod.mvar, mod.useCov = g, st, None, None, False
+    n_int = M_ * N_ * T_
+    mod.mu = np.zeros(n_int)
+    cov = 1.0 + 0.5 * np.cos(np.arange(n_int) * 0.37)
+    step("kriging flow: setModel(useCov) on an extended grid")
+    mod.setModel(mu=cov, sigmas=np.log([0.05, 2.0]), useCov=True)
+    Qa = mod.Q.toarray()
+    n = g.n
+    assert Qa.shape == (n + 2, n + 2)
+    mu0 = mod.mu.copy()
+    uidx = np.array([g.getIdx([i, j, 1], extend=False) for i in range(M_) for j in range(N_)])
+    yobs = np.sin(np.arange(uidx.size) * 0.3)
+    step("kriging flow: update + prediction")
+    mod.update(y=yobs, idx=uidx)
+    tau = np.exp(par[-1])
+    Sd = g.getS(uidx).toarray()
+    Qc = Qa + tau * Sd.T @ Sd
+    mu1 = mu0 + tau * np.linalg.solve(Qc, Sd.T @ (yobs - Sd @ mu0))
+    assert np.allclose(mod.mu, mu1, rtol=1e-9, atol=1e-11 * np.abs(mu1).max())
+    pred = g.getS() @ mod.mu
+    want = g.getS().toarray() @ mu1
+    assert np.allclose(pred, want, rtol=1e-9, atol=1e-11 * np.abs(want).max())
