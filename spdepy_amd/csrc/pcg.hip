@@ -67,7 +67,7 @@ struct spde_ic {
 namespace spde {
 
 constexpr int kColours = 18;
-constexpr int kRedBlocks = 512;      // blocks of the reduction kernels
+constexpr int kRedBlocks = 2048;     // blocks of the reduction kernels (8 waves per SIMD on 256 CUs)
 constexpr int kPairs = 22;           // correction channels per cell: diagonal + 21 forward-half pairs
 
 // Scalars per right-hand side in d_scal[s * 64 + r].
@@ -388,30 +388,35 @@ __global__ __launch_bounds__(256) void bwd64_kernel(int64_t r0, int64_t r1, cons
 
 // ------------------------------------------------------------ PCG steps
 // Per right-hand side sums of the reduction partials, out[a][r] = sum_b
-// part[a][b][r] for a < NA, r < k, by a whole block of kRedThreads: thread
-// (q = tid / 64, r = tid % 64) adds blocks q, q + 16, ..., then the 16 values
-// of each r are added in q order (a fixed order: deterministic).  `out` is
-// shared memory; every thread returns after the block has it.
+// part[a][b][r] for a < NA, r < k (k <= 64), by a whole block of kRedThreads:
+// with kp the power of two >= k, thread t = q kp + r adds blocks q, q + G,
+// ... (G = kRedThreads / kp lanes per right-hand side, all 1024 for one), then
+// the G values of each r are added by a fixed tree over q (deterministic).
+// `out` is shared memory; every thread returns after the block has it.
 constexpr int kRedThreads = 1024;
 template <int NA>
 __device__ inline void reduce_parts(const double* __restrict__ part, int k, int nblk, double (*out)[64]) {
-  __shared__ double red[NA][16][64];
-  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+  __shared__ double red[NA][kRedThreads];
+  int kp = 1;
+  while (kp < k) kp <<= 1;
+  const int t = threadIdx.x, r = t & (kp - 1), q = t / kp, G = kRedThreads / kp;
 #pragma unroll
   for (int a = 0; a < NA; ++a) {
     double acc = 0.0;
     if (r < k)
-      for (int b = q; b < nblk; b += 16) acc += part[((int64_t)a * kRedBlocks + b) * 64 + r];
-    red[a][q][r] = acc;
+      for (int b = q; b < nblk; b += G) acc += part[((int64_t)a * kRedBlocks + b) * 64 + r];
+    red[a][t] = acc;
   }
   __syncthreads();
-  if (threadIdx.x < 64)
+  for (int st = kRedThreads / 2; st >= kp; st >>= 1) {
+    if (t < st)
 #pragma unroll
-    for (int a = 0; a < NA; ++a) {
-      double v = 0.0;
-      for (int qq = 0; qq < 16; ++qq) v += red[a][qq][r];
-      out[a][r] = v;
-    }
+      for (int a = 0; a < NA; ++a) red[a][t] += red[a][t + st];
+    __syncthreads();
+  }
+  if (t < kp)
+#pragma unroll
+    for (int a = 0; a < NA; ++a) out[a][t] = red[a][t];
   __syncthreads();
 }
 

@@ -264,7 +264,7 @@ def conditional_samples_pcg(engine, coeffs, obs_cols, y, log_tau, k=64, seed=0, 
 
 def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32, slq_steps=50, hutch_probes=64,
                seed=0, tol=1e-10, probe_tol=1e-6, shift=0.1, correction=None, contract=None, adv_dir=None,
-               tol_like=None):
+               tol_like=None, inner_tol=1e-5):
     """(-loglik/(m r), parts) as ``Engine.loglik`` returns them, with log|Q_c|
     from stochastic Lanczos quadrature and the trace part of the gradient from
     the Hutchinson estimate.  correction(engine, coeffs, qval, diag_add) ->
@@ -322,7 +322,7 @@ def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32
     b_dev = obs["sty"] * tau
     mus, info = [], []
     for c in range(r):
-        x, inf = it.refine(b_dev[:, c].contiguous(), tol=tol)
+        x, inf = it.refine(b_dev[:, c].contiguous(), tol=tol, inner_tol=inner_tol)
         mus.append(x)
         info.append(inf)
     mu = torch.stack(mus, 1)
