@@ -401,6 +401,15 @@ int spde_fields_contract_noalloc(int64_t Ns, int nb, const double* bs, const dou
  * (k a power of two <= 64). */
 typedef struct spde_ic spde_ic;
 int spde_ic_create(spde_plan* plan, spde_ic** ic);
+/* A second handle on the same pattern and factor with its own work buffers
+ * (no counterpart in the reference: it lets two solves with one factor run
+ * at once on different streams, e.g. the conditional mean beside the
+ * Hutchinson probes).  It reads the parent's current factor at every call;
+ * spde_ic_factor / spde_ic_correct on a view return SPDE_ERR_ARG, and the
+ * parent must not be refactored while a view's solve is in flight. */
+int spde_ic_view(spde_ic* parent, spde_ic** view);
+/* Frees a handle; a parent destroyed before its views is freed with the last
+ * of them (the views keep working). */
 void spde_ic_destroy(spde_ic* ic);
 /* nlow: entries of the strictly lower factor (one per off-diagonal pair). */
 int spde_ic_info(const spde_ic* ic, int64_t* nlow, int32_t* ncolours);

@@ -110,6 +110,7 @@ _SIGS = {
                                                             ctypes.c_double, vp, ctypes.c_int, ctypes.c_int, vp]),
     "spde_grad_cells": (ctypes.c_int, [vp, ctypes.POINTER(GradArgs), vp, vp]),
     "spde_ic_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
+    "spde_ic_view": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
     "spde_ic_destroy": (None, [vp]),
     "spde_ic_info": (ctypes.c_int, [vp, c_i64p, c_i32p]),
     "spde_ic_export": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp]),

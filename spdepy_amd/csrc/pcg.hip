@@ -32,15 +32,14 @@
 
 #include "internal.hpp"
 
-struct spde_ic {
+// Pattern, colouring and factor of one precision: built by spde_ic_create,
+// written by spde_ic_factor / spde_ic_correct, read by every solve.  A view
+// (spde_ic_view) takes these fields from its parent at each call.
+struct IcShared {
   spde_plan* pl = nullptr;
   int64_t n = 0, nnz = 0, nlow = 0;
   int M = 0, N = 0, T = 0;
   std::vector<int64_t> coff;        // 19 offsets into crow
-  std::vector<int32_t> h_colour;    // colour per lattice index
-  std::vector<int64_t> h_lptr, h_uptr, h_lsrc, h_lch;
-  std::vector<int32_t> h_lcol, h_ucol;
-  std::vector<int64_t> h_ulow;
   int32_t* d_crow = nullptr;
   int64_t *d_lptr = nullptr, *d_uptr = nullptr, *d_lsrc = nullptr, *d_ulow = nullptr, *d_lch = nullptr;
   int32_t *d_lcol = nullptr, *d_ucol = nullptr;
@@ -48,6 +47,18 @@ struct spde_ic {
   float *d_l32 = nullptr, *d_u32 = nullptr, *d_dinv = nullptr, *d_qc32 = nullptr;
   const double* qval = nullptr;      // Q values of the last factorisation (structural order)
   const double* diag_add = nullptr;
+  bool factored = false;
+};
+// Host copies of the lists (spde_ic_export), kept by the handle that built them.
+struct IcHost {
+  std::vector<int32_t> h_colour;    // colour per lattice index
+  std::vector<int64_t> h_lptr, h_uptr, h_lsrc, h_lch;
+  std::vector<int32_t> h_lcol, h_ucol;
+  std::vector<int64_t> h_ulow;
+};
+// Buffers of the solve in flight: every handle, parent or view, has its own,
+// so a parent and its views can solve on different streams at once.
+struct IcWork {
   int* d_status = nullptr;
   double* d_part = nullptr;          // reduction partials [kRedBlocks * 64 * 2]
   double* d_scal = nullptr;          // per right-hand side scalars [64 * 8]
@@ -61,7 +72,11 @@ struct spde_ic {
   int64_t lanczos_cap = 0;
   double* d_klw = nullptr;           // L'^T x of the KL loss (n x K)
   int64_t kl_cap = 0;
-  bool factored = false;
+};
+struct spde_ic : IcShared, IcHost, IcWork {
+  spde_ic* parent = nullptr;  // set on a view
+  int nviews = 0;             // live views of this handle
+  bool released = false;      // destroyed while views were live: freed with the last view
 };
 
 namespace spde {
@@ -896,6 +911,36 @@ void launch_spmm(spde_ic* ic, int logk, unsigned g, const float* x, float* y, do
                        ic->d_qc32, x, y, part);
 }
 
+// Scalars and partials of one handle (the n-sized buffers come with its first solve).
+int alloc_small_work(spde_ic* ic) {
+  if (hipMalloc((void**)&ic->d_status, sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&ic->d_part, sizeof(double) * kRedBlocks * 64 * 2) != hipSuccess ||
+      hipMalloc((void**)&ic->d_scal, sizeof(double) * 64 * 8) != hipSuccess ||
+      hipMalloc((void**)&ic->d_active, sizeof(int) * 64) != hipSuccess ||
+      hipMalloc((void**)&ic->d_iters, sizeof(int) * 64) != hipSuccess) {
+    set_error("spde_ic: device allocation failed");
+    return SPDE_ERR_ALLOC;
+  }
+  return SPDE_OK;
+}
+void free_work(spde_ic* ic) {
+  for (void* p : {(void*)ic->d_status, (void*)ic->d_part, (void*)ic->d_scal, (void*)ic->d_active,
+                  (void*)ic->d_iters, (void*)ic->d_work, (void*)ic->d_r64, (void*)ic->d_b32, (void*)ic->d_x32,
+                  (void*)ic->d_rn, (void*)ic->d_lanczos, (void*)ic->d_klw})
+    if (p) hipFree(p);
+  static_cast<IcWork&>(*ic) = IcWork();
+}
+void free_shared(spde_ic* ic) {
+  for (void* p : {(void*)ic->d_crow, (void*)ic->d_lptr, (void*)ic->d_uptr, (void*)ic->d_lsrc, (void*)ic->d_ulow,
+                  (void*)ic->d_lch, (void*)ic->d_lcol, (void*)ic->d_ucol, (void*)ic->d_l64, (void*)ic->d_dg64,
+                  (void*)ic->d_adiag, (void*)ic->d_l32, (void*)ic->d_u32, (void*)ic->d_dinv, (void*)ic->d_qc32})
+    if (p) hipFree(p);
+}
+// A view takes its parent's pattern and factor at the start of every call.
+inline void sync_view(spde_ic* ic) {
+  if (ic && ic->parent) static_cast<IcShared&>(*ic) = static_cast<const IcShared&>(*ic->parent);
+}
+
 int ensure_work(spde_ic* ic) {
   // each buffer once (a failed allocation leaves the others for the next call)
   if (!ic->d_work) SPDE_HIP(hipMalloc((void**)&ic->d_work, sizeof(float) * ic->n * 64 * 4));
@@ -1114,29 +1159,52 @@ int spde_ic_create(spde_plan* pl, spde_ic** out) {
       hipMalloc((void**)&ic->d_l32, sizeof(float) * nl) != hipSuccess ||
       hipMalloc((void**)&ic->d_u32, sizeof(float) * nl) != hipSuccess ||
       hipMalloc((void**)&ic->d_dinv, sizeof(float) * n) != hipSuccess ||
-      hipMalloc((void**)&ic->d_qc32, sizeof(float) * ic->nnz) != hipSuccess ||
-      hipMalloc((void**)&ic->d_status, sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&ic->d_part, sizeof(double) * kRedBlocks * 64 * 2) != hipSuccess ||
-      hipMalloc((void**)&ic->d_scal, sizeof(double) * 64 * 8) != hipSuccess ||
-      hipMalloc((void**)&ic->d_active, sizeof(int) * 64) != hipSuccess ||
-      hipMalloc((void**)&ic->d_iters, sizeof(int) * 64) != hipSuccess) {
+      hipMalloc((void**)&ic->d_qc32, sizeof(float) * ic->nnz) != hipSuccess) {
     set_error("spde_ic_create: device allocation failed");
     return fail(SPDE_ERR_ALLOC);
   }
+  if ((rc = alloc_small_work(ic))) return fail(rc);
   *out = ic;
+  return SPDE_OK;
+}
+
+int spde_ic_view(spde_ic* parent, spde_ic** out) {
+  if (!parent || !out || parent->parent || parent->released) {
+    set_error("spde_ic_view: invalid argument (a solver made by spde_ic_create)");
+    return SPDE_ERR_ARG;
+  }
+  *out = nullptr;
+  spde_ic* v = new spde_ic();
+  static_cast<IcShared&>(*v) = static_cast<const IcShared&>(*parent);
+  v->parent = parent;
+  const int rc = alloc_small_work(v);
+  if (rc) {
+    free_work(v);
+    delete v;
+    return rc;
+  }
+  parent->nviews++;
+  *out = v;
   return SPDE_OK;
 }
 
 void spde_ic_destroy(spde_ic* ic) {
   if (!ic) return;
   hipDeviceSynchronize();
-  for (void* p : {(void*)ic->d_crow, (void*)ic->d_lptr, (void*)ic->d_uptr, (void*)ic->d_lsrc, (void*)ic->d_ulow,
-                  (void*)ic->d_lch, (void*)ic->d_lcol, (void*)ic->d_ucol, (void*)ic->d_l64, (void*)ic->d_dg64,
-                  (void*)ic->d_adiag, (void*)ic->d_l32, (void*)ic->d_u32, (void*)ic->d_dinv, (void*)ic->d_qc32,
-                  (void*)ic->d_status, (void*)ic->d_part, (void*)ic->d_scal, (void*)ic->d_active,
-                  (void*)ic->d_iters, (void*)ic->d_work, (void*)ic->d_r64, (void*)ic->d_b32, (void*)ic->d_x32,
-                  (void*)ic->d_rn, (void*)ic->d_lanczos, (void*)ic->d_klw})
-    if (p) hipFree(p);
+  free_work(ic);
+  if (spde_ic* p = ic->parent) {
+    delete ic;
+    if (--p->nviews == 0 && p->released) {
+      free_shared(p);
+      delete p;
+    }
+    return;
+  }
+  if (ic->nviews > 0) {  // the views still read the factor
+    ic->released = true;
+    return;
+  }
+  free_shared(ic);
   delete ic;
 }
 
@@ -1156,6 +1224,7 @@ int spde_ic_export(const spde_ic* ic, int32_t* colour, int64_t* lptr, int32_t* l
     set_error("spde_ic_export: invalid argument");
     return SPDE_ERR_ARG;
   }
+  if (ic->parent) ic = ic->parent;
   if (colour) std::memcpy(colour, ic->h_colour.data(), sizeof(int32_t) * ic->n);
   if (lptr) std::memcpy(lptr, ic->h_lptr.data(), sizeof(int64_t) * (ic->n + 1));
   if (lcol) std::memcpy(lcol, ic->h_lcol.data(), sizeof(int32_t) * ic->nlow);
@@ -1167,6 +1236,10 @@ int spde_ic_export(const spde_ic* ic, int32_t* colour, int64_t* lptr, int32_t* l
 
 int spde_ic_factor(spde_ic* ic, const double* qval, const double* diag_add, double shift, double* logdet_m,
                    void* stream) {
+  if (ic && ic->parent) {
+    set_error("spde_ic_factor: a view cannot change its parent's factor");
+    return SPDE_ERR_ARG;
+  }
   if (!ic || !qval || shift < 0.0) {
     set_error("spde_ic_factor: invalid argument");
     return SPDE_ERR_ARG;
@@ -1199,6 +1272,10 @@ int spde_ic_factor(spde_ic* ic, const double* qval, const double* diag_add, doub
 }
 
 int spde_ic_correct(spde_ic* ic, const float* corr, double* logdet_m, void* stream) {
+  if (ic && ic->parent) {
+    set_error("spde_ic_correct: a view cannot change its parent's factor");
+    return SPDE_ERR_ARG;
+  }
   if (!ic || !ic->qval) {
     set_error("spde_ic_correct: no factorisation");
     return SPDE_ERR_STATE;
@@ -1227,6 +1304,7 @@ int spde_ic_correct(spde_ic* ic, const float* corr, double* logdet_m, void* stre
 }
 
 int spde_ic_values(spde_ic* ic, double* lower, double* diag, double* adiag, void* stream) {
+  sync_view(ic);
   if (!ic || !ic->factored) {
     set_error("spde_ic_values: no factorisation");
     return SPDE_ERR_STATE;
@@ -1239,6 +1317,7 @@ int spde_ic_values(spde_ic* ic, double* lower, double* diag, double* adiag, void
 }
 
 int spde_ic_apply(spde_ic* ic, const float* r, float* z, int k, void* stream) {
+  sync_view(ic);
   const int logk = log2_of(k);
   if (!ic || !ic->factored || !r || !z || logk < 0) {
     set_error("spde_ic_apply: invalid argument (k a power of two <= 64, after spde_ic_factor)");
@@ -1250,6 +1329,7 @@ int spde_ic_apply(spde_ic* ic, const float* r, float* z, int k, void* stream) {
 }
 
 int spde_ic_sweep(spde_ic* ic, int which, const float* in, float* out, int k, void* stream) {
+  sync_view(ic);
   const int logk = log2_of(k);
   if (!ic || !ic->factored || !in || !out || logk < 0 || (which != 1 && which != 2)) {
     set_error("spde_ic_sweep: invalid argument");
@@ -1262,6 +1342,7 @@ int spde_ic_sweep(spde_ic* ic, int which, const float* in, float* out, int k, vo
 }
 
 int spde_ic_spmm(spde_ic* ic, const float* x, float* y, int k, void* stream) {
+  sync_view(ic);
   const int logk = log2_of(k);
   if (!ic || !ic->factored || !x || !y || logk < 0) {
     set_error("spde_ic_spmm: invalid argument");
@@ -1274,6 +1355,7 @@ int spde_ic_spmm(spde_ic* ic, const float* x, float* y, int k, void* stream) {
 
 int spde_pcg_solve(spde_ic* ic, const float* B, float* X, int k, double tol, int maxit, int32_t* iters,
                    void* stream) {
+  sync_view(ic);
   const int logk = log2_of(k);
   if (!ic || !ic->factored || !B || !X || logk < 0 || !(tol > 0.0) || maxit < 1) {
     set_error("spde_pcg_solve: invalid argument (k a power of two <= 64, after spde_ic_factor)");
@@ -1284,6 +1366,7 @@ int spde_pcg_solve(spde_ic* ic, const float* B, float* X, int k, double tol, int
 
 int spde_pcg_refine(spde_ic* ic, const double* b, double* x, double tol, double inner_tol, int max_rounds, int maxit,
                     int32_t* iters, int32_t* rounds, double* relres, void* stream) {
+  sync_view(ic);
   if (!ic || !ic->factored || !b || !x || !(tol > 0.0) || !(inner_tol > 0.0) || max_rounds < 1 || maxit < 1) {
     set_error("spde_pcg_refine: invalid argument");
     return SPDE_ERR_ARG;
@@ -1329,6 +1412,7 @@ int spde_pcg_refine(spde_ic* ic, const double* b, double* x, double tol, double 
 
 int spde_pcg_refine_block(spde_ic* ic, const double* B, double* X, int k, double tol, double inner_tol,
                           int max_rounds, int maxit, int32_t* iters, int32_t* rounds, double* relres, void* stream) {
+  sync_view(ic);
   const int logk = log2_of(k);
   if (!ic || !ic->factored || !B || !X || logk < 0 || !(tol > 0.0) || !(inner_tol > 0.0) || max_rounds < 1 ||
       maxit < 1) {
@@ -1408,6 +1492,7 @@ int spde_rademacher(int64_t n, int k, uint64_t seed, float scale, float* V, void
 }
 
 int spde_slq(spde_ic* ic, int k, int steps, uint64_t seed, double* alpha, double* beta, void* stream) {
+  sync_view(ic);
   const int logk = log2_of(k);
   if (!ic || !ic->factored || logk < 0 || steps < 1 || !alpha || !beta) {
     set_error("spde_slq: invalid argument (k a power of two <= 64, after spde_ic_factor)");
@@ -1463,6 +1548,7 @@ int spde_slq(spde_ic* ic, int k, int steps, uint64_t seed, double* alpha, double
 
 
 int spde_ic_features(spde_ic* ic, float* feat, void* stream) {
+  sync_view(ic);
   if (!ic || !ic->qval || !feat) {
     set_error("spde_ic_features: invalid argument (after spde_ic_factor)");
     return SPDE_ERR_ARG;
@@ -1475,6 +1561,7 @@ int spde_ic_features(spde_ic* ic, float* feat, void* stream) {
 }
 
 int spde_ic_kl_grad(spde_ic* ic, const double* x, int k, double scale, float* dout, double* loss, void* stream) {
+  sync_view(ic);
   const int logk = log2_of(k);
   if (!ic || !ic->factored || !x || !dout || logk < 0) {
     set_error("spde_ic_kl_grad: invalid argument (k a power of two <= 64, after spde_ic_factor)");
@@ -1511,6 +1598,7 @@ int spde_ic_kl_grad(spde_ic* ic, const double* x, int k, double scale, float* do
 
 int spde_ic_jeffreys_grad(spde_ic* ic, const double* x, const float* z, int k, double scale, float* dout,
                           double* loss, void* stream) {
+  sync_view(ic);
   const int logk = log2_of(k);
   if (!ic || !ic->factored || !x || !z || !dout || logk < 0) {
     set_error("spde_ic_jeffreys_grad: invalid argument (k a power of two <= 64, after spde_ic_factor)");
@@ -1552,6 +1640,7 @@ int spde_ic_jeffreys_grad(spde_ic* ic, const double* x, const float* z, int k, d
 
 int spde_hutch_pattern(spde_ic* ic, const float* U, const float* V, int k, double alpha, double beta, double* sigma,
                        void* stream) {
+  sync_view(ic);
   const int logk = log2_of(k);
   if (!ic || !U || !V || !sigma || logk < 0) {
     set_error("spde_hutch_pattern: invalid argument");

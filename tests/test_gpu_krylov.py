@@ -127,6 +127,53 @@ def test_refinement_reaches_fp64(case):
     assert np.allclose(xh, xd, rtol=1e-7, atol=1e-8 * np.abs(xd).max())
 
 
+def test_view_solves_beside_its_parent(case):
+    """spde_ic_view: a second handle on the factor with its own work buffers
+    gives the parent's results bit for bit, also while both solve at once on
+    two streams from two host threads; a view cannot refactor, and it keeps
+    working after its parent's handle is destroyed."""
+    import threading
+    import torch
+    from spdepy_amd import _native as nat
+    from spdepy_amd.krylov import IterativeSolver
+    eng = case["eng"]
+    it = IterativeSolver(eng.plan)
+    it.factor(case["qval"], case["dadd"], shift=0.1)
+    v = it.view()
+    g = torch.Generator(device="cuda").manual_seed(5)
+    b = torch.randn(eng.n, dtype=torch.float64, device="cuda", generator=g)
+    B = torch.randn(eng.n, 64, dtype=torch.float32, device="cuda", generator=g)
+    step("parent and view, one after the other")
+    x0, i0 = it.refine(b, tol=1e-10)
+    X0, j0 = it.solve(B, tol=1e-6)
+    x1, i1 = v.refine(b, tol=1e-10)
+    assert torch.equal(x0, x1) and i0 == i1
+    step("parent and view at once on two streams")
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    box = {}
+
+    def run():
+        with torch.cuda.stream(side):
+            box["x"] = v.refine(b, tol=1e-10)
+    th = threading.Thread(target=run)
+    th.start()
+    X2, j2 = it.solve(B, tol=1e-6)
+    th.join()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    assert torch.equal(box["x"][0], x0) and box["x"][1] == i0
+    assert torch.equal(X2, X0) and np.array_equal(j2, j0)
+    with pytest.raises(nat.SpdeError, match="view"):
+        v.factor(case["qval"], case["dadd"], shift=0.1)
+    step("view after its parent's handle is destroyed")
+    nat.lib().spde_ic_destroy(it._h)
+    it._h = None
+    x3, _ = v.refine(b, tol=1e-10)
+    assert torch.equal(x3, x0)
+    del v
+
+
 def test_rademacher_and_hutch_pattern(case):
     import torch
     eng = case["eng"]
