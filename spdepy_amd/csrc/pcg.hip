@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void resid64_kernel(int64_t n, const int64_t* 
 __global__ __launch_bounds__(256) void fwd_kernel(int64_t r0, int64_t r1, int logk, const int32_t* __restrict__ crow,
                                                   const int64_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
                                                   const float* __restrict__ l32, const float* __restrict__ dinv,
-                                                  const float* r, float* z) {
+                                                  const float* r, float* z, int fold) {
   const int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t rr = r0 + (u >> logk);
   if (rr >= r1) return;
@@ -254,7 +254,9 @@ __global__ __launch_bounds__(256) void fwd_kernel(int64_t r0, int64_t r1, int lo
   const int i = crow[rr];
   float s = r[((int64_t)i << logk) + rhs];
   for (int64_t p = lptr[i]; p < lptr[i + 1]; ++p) s -= l32[p] * z[((int64_t)lcol[p] << logk) + rhs];
-  z[((int64_t)i << logk) + rhs] = s * dinv[i];
+  float v = s * dinv[i];
+  if (fold) v *= dinv[i];
+  z[((int64_t)i << logk) + rhs] = v;
 }
 
 // Backward sweep of one colour: x_i = (y_i - sum_upper L_ji x_j) / L_ii
@@ -280,7 +282,7 @@ constexpr int kRowLanes = 8;
 __global__ __launch_bounds__(256) void fwd1_kernel(int64_t r0, int64_t r1, const int32_t* __restrict__ crow,
                                                    const int64_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
                                                    const float* __restrict__ l32, const float* __restrict__ dinv,
-                                                   const float* r, float* z) {
+                                                   const float* r, float* z, int fold) {
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t rr = r0 + g / kRowLanes;
   const int lane = (int)(g % kRowLanes);
@@ -289,7 +291,11 @@ __global__ __launch_bounds__(256) void fwd1_kernel(int64_t r0, int64_t r1, const
   float acc = 0.f;
   for (int64_t p = lptr[i] + lane; p < lptr[i + 1]; p += kRowLanes) acc += l32[p] * z[lcol[p]];
   for (int o = kRowLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kRowLanes);
-  if (lane == 0) z[i] = (r[i] - acc) * dinv[i];
+  if (lane == 0) {
+    float v = (r[i] - acc) * dinv[i];
+    if (fold) v *= dinv[i];
+    z[i] = v;
+  }
 }
 // (y and x may be the same array: row i reads only rows of other colours)
 __global__ __launch_bounds__(256) void bwd1_kernel(int64_t r0, int64_t r1, const int32_t* __restrict__ crow,
@@ -361,7 +367,7 @@ __global__ __launch_bounds__(256) void spmm64_kernel(int64_t n, const int64_t* _
 __global__ __launch_bounds__(256) void fwd64_kernel(int64_t r0, int64_t r1, const int32_t* __restrict__ crow,
                                                     const int64_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
                                                     const float* __restrict__ l32, const float* __restrict__ dinv,
-                                                    const float* r, float* z) {
+                                                    const float* r, float* z, int fold) {
   const int lane = threadIdx.x & 63;
   const int64_t rw = r0 + (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (rw >= r1) return;  // uniform over the wave
@@ -377,7 +383,9 @@ __global__ __launch_bounds__(256) void fwd64_kernel(int64_t r0, int64_t r1, cons
     for (int q = 0; q < 4; ++q) s -= l32[p + q] * v[q];
   }
   for (; p < p1; ++p) s -= l32[p] * z[((int64_t)lcol[p] << 6) + lane];
-  z[((int64_t)i << 6) + lane] = s * dinv[i];
+  float v = s * dinv[i];
+  if (fold) v *= dinv[i];
+  z[((int64_t)i << 6) + lane] = v;
 }
 __global__ __launch_bounds__(256) void bwd64_kernel(int64_t r0, int64_t r1, const int32_t* __restrict__ crow,
                                                     const int64_t* __restrict__ uptr, const int32_t* __restrict__ ucol,
@@ -859,27 +867,40 @@ int log2_of(int k) {
   return (1 << l) == k && k <= 64 ? l : -1;
 }
 
-// Sweeps of M^-1 = (L L^T)^-1: forward from r into z, then backward in place.
-int ic_fwd(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s) {
+// The last colour with rows: its rows have no neighbour of a later colour.
+int last_colour(const spde_ic* ic) {
+  int c = kColours - 1;
+  while (c > 0 && ic->coff[c + 1] <= ic->coff[c]) --c;
+  return c;
+}
+// Forward sweep z = L^-1 r, colour by colour.  fold: the rows of the last
+// colour also take their backward step (x_i = z_i / L_ii: no later colour
+// couples to them), which the backward sweep of ic_apply then skips.
+int ic_fwd(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s, bool fold = false) {
+  const int last = last_colour(ic);
   for (int c = 0; c < kColours; ++c) {
     const int64_t r0 = ic->coff[c], r1 = ic->coff[c + 1];
     if (r1 <= r0) continue;
+    const int f = fold && c == last ? 1 : 0;
     if (logk == 0)
       hipLaunchKernelGGL(fwd1_kernel, dim3(blocks_of((r1 - r0) * kRowLanes)), dim3(256), 0, s, r0, r1, ic->d_crow,
-                         ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dinv, r, z);
+                         ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dinv, r, z, f);
     else if (logk == 6)
       hipLaunchKernelGGL(fwd64_kernel, dim3(blocks_of((r1 - r0) << 6)), dim3(256), 0, s, r0, r1, ic->d_crow,
-                         ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dinv, r, z);
+                         ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dinv, r, z, f);
     else
       hipLaunchKernelGGL(fwd_kernel, dim3(blocks_of((r1 - r0) << logk)), dim3(256), 0, s, r0, r1, logk, ic->d_crow,
-                         ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dinv, r, z);
+                         ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dinv, r, z, f);
   }
   return SPDE_OK;
 }
-int ic_bwd(spde_ic* ic, const float* y, float* x, int logk, hipStream_t s) {
+// Backward sweep x = L^-T y (x may be y), colour by colour from the last;
+// skip_last: the last colour was folded into the forward sweep.
+int ic_bwd(spde_ic* ic, const float* y, float* x, int logk, hipStream_t s, bool skip_last = false) {
+  const int last = last_colour(ic);
   for (int c = kColours - 1; c >= 0; --c) {
     const int64_t r0 = ic->coff[c], r1 = ic->coff[c + 1];
-    if (r1 <= r0) continue;
+    if (r1 <= r0 || (skip_last && c == last)) continue;
     if (logk == 0)
       hipLaunchKernelGGL(bwd1_kernel, dim3(blocks_of((r1 - r0) * kRowLanes)), dim3(256), 0, s, r0, r1, ic->d_crow,
                          ic->d_uptr, ic->d_ucol, ic->d_u32, ic->d_dinv, y, x);
@@ -892,10 +913,11 @@ int ic_bwd(spde_ic* ic, const float* y, float* x, int logk, hipStream_t s) {
   }
   return SPDE_OK;
 }
-// Sweeps of M^-1 = (L L^T)^-1: forward from r into z, then backward in place.
+// z = M^-1 r = (L L^T)^-1 r: forward from r into z, then backward in place,
+// the last colour's two steps in one launch.
 int ic_apply(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s) {
-  ic_fwd(ic, r, z, logk, s);
-  return ic_bwd(ic, z, z, logk, s);
+  ic_fwd(ic, r, z, logk, s, true);
+  return ic_bwd(ic, z, z, logk, s, true);
 }
 // y = Q_c x in FP32 (k = 2^logk right-hand sides) with the block partials of
 // x^T y when part is set; `g` blocks (the caller's reduction grid).

@@ -283,6 +283,38 @@ def _side_solver(it):
     return it._side
 
 
+def _start_side(it, job):
+    """Runs job(view of it) on the side stream from a second host thread,
+    after the work enqueued so far on the current stream; returns a function
+    that joins the thread, makes the current stream wait for the side stream
+    and returns the job's result (tensors marked for the current stream), or
+    raises the job's exception."""
+    torch = _torch()
+    view, stream = _side_solver(it)
+    main = torch.cuda.current_stream()
+    stream.wait_stream(main)
+    box = {}
+
+    def run():
+        try:
+            with torch.cuda.stream(stream):
+                box["res"] = job(view)
+        except BaseException as e:
+            box["err"] = e
+    worker = threading.Thread(target=run, name="spdepy-side-solve")
+    worker.start()
+
+    def join():
+        worker.join()
+        if "err" in box:
+            raise box["err"]
+        main.wait_stream(stream)
+        for x, _ in box["res"]:
+            x.record_stream(main)
+        return box["res"]
+    return join
+
+
 def _hutchinson(it, probes, seed, tol):
     """Estimate of Q_c^-1 on the pattern from `probes` Rademacher probes, in
     batches of up to 64 (the probes of batch j from seed + 1 + its first
@@ -346,56 +378,44 @@ def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32
     ld_m = it.factor(qval, diag_add, shift=shift)
     if correction is not None:
         ld_m = it.correct(correction(engine, coeffs, qval, diag_add))
-    alpha, beta = it.slq(slq_probes, slq_steps, seed)
-    trlog, se, _ = slq_quadrature(alpha, beta, n)
-    ld_qc = ld_m + trlog
-    ld_q0 = sp.factor_finish(0)
-    ld_gram = sp.factor_finish(1)
-    logq = engine.plan.logq_sum(1.0 / (coeffs.c.dt * coeffs.c.sigma))
-    ld_q = ld_q0 + (T - 1) * (ld_gram + logq)
-    if tol_like is not None:
-        known = 0.5 * ld_q * r + m * r * np.log(tau) / 2 - 0.5 * ld_qc * r
-        bound = abs(known) + 0.5 * tau * float((y2 * y2).sum())
-        if 0.5 * r * se > tol_like * bound:
-            if grad:
-                torch.cuda.synchronize()       # the spatial selected inverses are enqueued
-            return None, dict(m=m, r=r, tau=tau, ld_q=ld_q, ld_qc=ld_qc, logdet_qc_se=se, logdet_m=ld_m,
-                              ic_shift=it.shift, infeasible=True, like_bound=bound, pcg=dict(mu=[]))
     b_dev = obs["sty"] * tau
 
     def mean_solves(solver):
         return [solver.refine(b_dev[:, c].contiguous(), tol=tol, inner_tol=inner_tol) for c in range(r)]
 
-    side = None
-    if grad and hutch_probes > 0:
-        # mu_c (one right-hand side: short, latency-bound launches) on a second
-        # stream from a second host thread, beside the Hutchinson probes'
-        # 64-column solves on this one; each has its own work buffers
-        side, side_stream = _side_solver(it)
-        main = torch.cuda.current_stream()
-        side_stream.wait_stream(main)
-        box = {}
-
-        def run_side():
-            try:
-                with torch.cuda.stream(side_stream):
-                    box["res"] = mean_solves(side)
-            except BaseException as e:  # re-raised by the caller's thread
-                box["err"] = e
-        worker = threading.Thread(target=run_side, name="spdepy-mu-c")
-        worker.start()
-        try:
+    # mu_c (one right-hand side: short, latency-bound launches) runs on a
+    # second stream from a second host thread beside the SLQ and the
+    # Hutchinson probes' 64-column solves on this one, each handle with its
+    # own work buffers; with a tolerance on the value it starts only once the
+    # SLQ has shown the tolerance within reach
+    side, results = None, None
+    try:
+        if grad and hutch_probes > 0 and tol_like is None:
+            side = _start_side(it, mean_solves)
+        alpha, beta = it.slq(slq_probes, slq_steps, seed)
+        trlog, se, _ = slq_quadrature(alpha, beta, n)
+        ld_qc = ld_m + trlog
+        ld_q0 = sp.factor_finish(0)
+        ld_gram = sp.factor_finish(1)
+        logq = engine.plan.logq_sum(1.0 / (coeffs.c.dt * coeffs.c.sigma))
+        ld_q = ld_q0 + (T - 1) * (ld_gram + logq)
+        if tol_like is not None:
+            known = 0.5 * ld_q * r + m * r * np.log(tau) / 2 - 0.5 * ld_qc * r
+            bound = abs(known) + 0.5 * tau * float((y2 * y2).sum())
+            if 0.5 * r * se > tol_like * bound:
+                if grad:
+                    torch.cuda.synchronize()       # the spatial selected inverses are enqueued
+                return None, dict(m=m, r=r, tau=tau, ld_q=ld_q, ld_qc=ld_qc, logdet_qc_se=se, logdet_m=ld_m,
+                                  ic_shift=it.shift, infeasible=True, like_bound=bound, pcg=dict(mu=[]))
+            if grad and hutch_probes > 0:
+                side = _start_side(it, mean_solves)
+        if side is not None:
             sqc, iters = _hutchinson(it, hutch_probes, seed, probe_tol)
-        finally:
-            worker.join()
-        if "err" in box:
-            raise box["err"]
-        main.wait_stream(side_stream)
-        results = box["res"]
-        for x, _ in results:
-            x.record_stream(main)
-    else:
-        results = mean_solves(it)
+        else:
+            results = mean_solves(it)
+    finally:
+        if side is not None:       # joined on every exit, so the view's buffers are free again
+            results = side()
     mus = [x for x, _ in results]
     info = [inf for _, inf in results]
     mu = torch.stack(mus, 1)
