@@ -72,6 +72,7 @@ struct IcWork {
   int64_t lanczos_cap = 0;
   double* d_klw = nullptr;           // L'^T x of the KL loss (n x K)
   int64_t kl_cap = 0;
+  hipGraphExec_t it_graph[7] = {};   // PCG iterations on the refinement buffers, per log2 of the rhs count
 };
 struct spde_ic : IcShared, IcHost, IcWork {
   spde_ic* parent = nullptr;  // set on a view
@@ -950,6 +951,8 @@ void free_work(spde_ic* ic) {
                   (void*)ic->d_iters, (void*)ic->d_work, (void*)ic->d_r64, (void*)ic->d_b32, (void*)ic->d_x32,
                   (void*)ic->d_rn, (void*)ic->d_lanczos, (void*)ic->d_klw})
     if (p) hipFree(p);
+  for (hipGraphExec_t g : ic->it_graph)
+    if (g) hipGraphExecDestroy(g);
   static_cast<IcWork&>(*ic) = IcWork();
 }
 void free_shared(spde_ic* ic) {
@@ -973,9 +976,57 @@ int ensure_work(spde_ic* ic) {
   return SPDE_OK;
 }
 
+constexpr int kPoll = 16;  // iterations between the host's looks at the active flags
+
+// One PCG iteration on the handle's work vectors (R, Z, P, Qv) and X.
+void enqueue_iteration(spde_ic* ic, int logk, unsigned g, float* X, hipStream_t s) {
+  const int k = 1 << logk;
+  const int64_t n = ic->n, units = n << logk;
+  float* R = ic->d_work;
+  float* Z = R + n * 64;
+  float* P = Z + n * 64;
+  float* Qv = P + n * 64;
+  launch_spmm(ic, logk, g, P, Qv, ic->d_part, s);
+  hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, ic->d_scal, ic->d_active);
+  hipLaunchKernelGGL(pcg_update_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, P, Qv, X, R);
+  ic_apply(ic, R, Z, logk, s);
+  hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, R, Z, ic->d_part);
+  hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, ic->d_scal, ic->d_active,
+                     ic->d_iters);
+  hipLaunchKernelGGL(pcg_p_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, ic->d_active, Z, P);
+}
+
+// kPoll iterations with X = the handle's d_x32 as one HIP graph, captured the
+// first time and replayed afterwards (the buffers it names never move): one
+// launch instead of about forty per iteration, which is what bounds a solve
+// with one right-hand side.  Null when the stream cannot be captured (the
+// legacy default stream): the caller then launches kernel by kernel.
+hipGraphExec_t iteration_graph(spde_ic* ic, int logk, unsigned g, hipStream_t s) {
+  if (ic->it_graph[logk]) return ic->it_graph[logk];
+  if (s == nullptr) return nullptr;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  for (int j = 0; j < kPoll; ++j) enqueue_iteration(ic, logk, g, ic->d_x32, s);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &graph);
+  hipGraphExec_t exec = nullptr;
+  if (e == hipSuccess && graph) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (graph) hipGraphDestroy(graph);
+  if (e != hipSuccess || !exec) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  ic->it_graph[logk] = exec;
+  return exec;
+}
+
 // FP32 PCG with k = 2^logk right-hand sides; X (initial guess 0) and B in
-// row-major n x k.  Host polls the active flags every `poll` iterations.
-int pcg_run(spde_ic* ic, const float* B, float* X, int logk, double tol, int maxit, int32_t* iters, hipStream_t s) {
+// row-major n x k.  Host polls the active flags every kPoll iterations.
+// graph: iterate through iteration_graph (X must be the handle's d_x32).
+int pcg_run(spde_ic* ic, const float* B, float* X, int logk, double tol, int maxit, int32_t* iters, hipStream_t s,
+            bool graph = false) {
   const int k = 1 << logk;
   const int64_t n = ic->n, units = n << logk;
   int rc = ensure_work(ic);
@@ -983,8 +1034,6 @@ int pcg_run(spde_ic* ic, const float* B, float* X, int logk, double tol, int max
   float* R = ic->d_work;
   float* Z = R + n * 64;
   float* P = Z + n * 64;
-  float* Qv = P + n * 64;
-  spde_plan* pl = ic->pl;
   const unsigned g = grid_for(units);
   hipLaunchKernelGGL(fill_kernel, dim3(g), dim3(256), 0, s, units, 0.f, X);
   hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, s, units, B, R);
@@ -993,18 +1042,18 @@ int pcg_run(spde_ic* ic, const float* B, float* X, int logk, double tol, int max
   hipLaunchKernelGGL(pcg_init_scalars_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, tol * tol, ic->d_scal,
                      ic->d_active, ic->d_iters);
   hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, s, units, Z, P);
+  SPDE_HIP(hipGetLastError());
+  hipGraphExec_t ge = graph && X == ic->d_x32 ? iteration_graph(ic, logk, g, s) : nullptr;
   std::vector<int> act(k);
-  const int poll = 16;
-  for (int it = 0; it < maxit; ++it) {
-    launch_spmm(ic, logk, g, P, Qv, ic->d_part, s);
-    hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, ic->d_scal, ic->d_active);
-    hipLaunchKernelGGL(pcg_update_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, P, Qv, X, R);
-    ic_apply(ic, R, Z, logk, s);
-    hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, R, Z, ic->d_part);
-    hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, ic->d_scal, ic->d_active,
-                       ic->d_iters);
-    hipLaunchKernelGGL(pcg_p_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, ic->d_active, Z, P);
-    if ((it + 1) % poll == 0 || it + 1 == maxit) {
+  for (int it = 0; it < maxit;) {
+    if (ge && maxit - it >= kPoll) {
+      SPDE_HIP(hipGraphLaunch(ge, s));
+      it += kPoll;
+    } else {
+      enqueue_iteration(ic, logk, g, X, s);
+      ++it;
+    }
+    if (it % kPoll == 0 || it == maxit) {
       SPDE_HIP(hipMemcpyAsync(act.data(), ic->d_active, sizeof(int) * k, hipMemcpyDeviceToHost, s));
       SPDE_HIP(hipStreamSynchronize(s));
       bool any = false;
@@ -1414,7 +1463,7 @@ int spde_pcg_refine(spde_ic* ic, const double* b, double* x, double tol, double 
   while (r < max_rounds && rn > tol * tol * bn && bn > 0.0) {
     hipLaunchKernelGGL(scale_down_kernel, dim3(g), dim3(256), 0, s, n, ic->d_scal + S_X * 64, ic->d_r64, ic->d_b32);
     int32_t it = 0;
-    rc = pcg_run(ic, ic->d_b32, ic->d_x32, 0, inner_tol, maxit, &it, s);
+    rc = pcg_run(ic, ic->d_b32, ic->d_x32, 0, inner_tol, maxit, &it, s, true);
     if (rc) return rc;
     total += it;
     hipLaunchKernelGGL(correct_kernel, dim3(g), dim3(256), 0, s, n, ic->d_scal + S_X * 64, ic->d_x32, x);
@@ -1471,7 +1520,7 @@ int spde_pcg_refine_block(spde_ic* ic, const double* B, double* X, int k, double
   };
   while (rd < max_rounds && pending()) {
     hipLaunchKernelGGL(scale_down_k_kernel, dim3(g), dim3(256), 0, s, units, logk, rn, thr, ic->d_r64, ic->d_b32);
-    rc = pcg_run(ic, ic->d_b32, ic->d_x32, logk, inner_tol, maxit, its, s);
+    rc = pcg_run(ic, ic->d_b32, ic->d_x32, logk, inner_tol, maxit, its, s, true);
     if (rc) return rc;
     if (iters)
       for (int r = 0; r < k; ++r) iters[r] += its[r];

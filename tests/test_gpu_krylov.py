@@ -131,7 +131,10 @@ def test_view_solves_beside_its_parent(case):
     """spde_ic_view: a second handle on the factor with its own work buffers
     gives the parent's results bit for bit, also while both solve at once on
     two streams from two host threads; a view cannot refactor, and it keeps
-    working after its parent's handle is destroyed."""
+    working after its parent's handle is destroyed.  On a created stream the
+    refinement's inner iterations replay a captured HIP graph (kernel by
+    kernel on the default stream): every call, each of many replays with host
+    polls between them, equals the launches one by one."""
     import threading
     import torch
     from spdepy_amd import _native as nat
@@ -144,10 +147,13 @@ def test_view_solves_beside_its_parent(case):
     b = torch.randn(eng.n, dtype=torch.float64, device="cuda", generator=g)
     B = torch.randn(eng.n, 64, dtype=torch.float32, device="cuda", generator=g)
     step("parent and view, one after the other")
+    Bd = B[:, :4].double().contiguous()
     x0, i0 = it.refine(b, tol=1e-10)
     X0, j0 = it.solve(B, tol=1e-6)
+    Y0, k0 = it.refine_block(Bd, tol=1e-10)
     x1, i1 = v.refine(b, tol=1e-10)
     assert torch.equal(x0, x1) and i0 == i1
+    print("  refinement", i0, "block", k0["iters"])
     step("parent and view at once on two streams")
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
@@ -155,14 +161,18 @@ def test_view_solves_beside_its_parent(case):
 
     def run():
         with torch.cuda.stream(side):
-            box["x"] = v.refine(b, tol=1e-10)
+            box["x"] = [v.refine(b, tol=1e-10) for _ in range(2)]
+            box["Y"] = [v.refine_block(Bd, tol=1e-10) for _ in range(2)]
     th = threading.Thread(target=run)
     th.start()
     X2, j2 = it.solve(B, tol=1e-6)
     th.join()
     torch.cuda.current_stream().wait_stream(side)
     torch.cuda.synchronize()
-    assert torch.equal(box["x"][0], x0) and box["x"][1] == i0
+    for x, inf in box["x"]:
+        assert torch.equal(x, x0) and inf == i0
+    for Y, inf in box["Y"]:
+        assert torch.equal(Y, Y0) and np.array_equal(inf["iters"], k0["iters"])
     assert torch.equal(X2, X0) and np.array_equal(j2, j0)
     with pytest.raises(nat.SpdeError, match="view"):
         v.factor(case["qval"], case["dadd"], shift=0.1)
