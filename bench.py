@@ -115,7 +115,10 @@ def cpu_baseline(repeats=3):
 
     sample(F.var_model(50, 50, 10, m=625))                # warm-up: BLAS threads, allocator, imports
     c = F.config2()
-    runs = [sample(c) for _ in range(repeats)]
+    runs = []
+    for i in range(repeats):
+        runs.append(sample(c))
+        print("cpu baseline sample %d/%d: %.1f s" % (i + 1, repeats, runs[-1][0]), file=sys.stderr, flush=True)
     totals = sorted(r[0] for r in runs)
     med = totals[len(totals) // 2]
     t_q, t_dq, t_f, flops = min(runs, key=lambda r: abs(r[0] - med))[1]
@@ -169,8 +172,8 @@ def other_routes(mod, par, like_ex, jac_ex, steps=3):
                                         "probes_mean": float(np.mean(parts["pcg"]["probes"]))},
                      "ic_shift": parts["ic_shift"],
                      "estimators": "log|Q_c|: 32 Rademacher probes x 50 Lanczos steps on L^-1 Q_c L^-T + "
-                                   "log|L L^T|; gradient trace: 64-probe Hutchinson from FP32 PCG (tol 1e-6); "
-                                   "mu_c: FP64 refinement to 1e-10"}
+                                   "log|L L^T|; gradient trace: 64-probe Hutchinson from FP32 PCG (tol 1e-4); "
+                                   "mu_c: FP64 refinement to 1e-10 on a second stream beside the probes"}
     if "evals_per_s" in out.get("learned_pcg", {}):
         lp = out["learned_pcg"]
         lp["meets_1e-6"] = bool(lp["rel_se_like"] <= 1e-6)
