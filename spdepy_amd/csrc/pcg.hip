@@ -280,6 +280,29 @@ __global__ __launch_bounds__(256) void bwd_kernel(int64_t r0, int64_t r1, int lo
 // strided by kRowLanes, then a fixed butterfly over the group), so a colour's
 // rows are not each a single thread's chain of dependent gathers.
 constexpr int kRowLanes = 8;
+// Sum over p = p0, p0 + kRowLanes, ... < p1 of w[p] v[col[p]] in that order,
+// four entries' loads issued before their products, so a row's few entries
+// per lane cost one round of dependent loads instead of one per entry.
+__device__ inline float strided_dot(int64_t p0, int64_t p1, const int32_t* __restrict__ col,
+                                    const float* __restrict__ w, const float* v) {
+  float acc = 0.f;
+  for (int64_t p = p0; p < p1; p += 4 * kRowLanes) {
+    int c[4];
+    float a[4], b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const bool in = p + q * kRowLanes < p1;
+      c[q] = in ? col[p + q * kRowLanes] : 0;
+      a[q] = in ? w[p + q * kRowLanes] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) b[q] = v[c[q]];  // index 0 past the end: a valid address, value unused
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (p + q * kRowLanes < p1) acc += a[q] * b[q];
+  }
+  return acc;
+}
 __global__ __launch_bounds__(256) void fwd1_kernel(int64_t r0, int64_t r1, const int32_t* __restrict__ crow,
                                                    const int64_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
                                                    const float* __restrict__ l32, const float* __restrict__ dinv,
@@ -289,8 +312,7 @@ __global__ __launch_bounds__(256) void fwd1_kernel(int64_t r0, int64_t r1, const
   const int lane = (int)(g % kRowLanes);
   if (rr >= r1) return;  // uniform over the group
   const int i = crow[rr];
-  float acc = 0.f;
-  for (int64_t p = lptr[i] + lane; p < lptr[i + 1]; p += kRowLanes) acc += l32[p] * z[lcol[p]];
+  float acc = strided_dot(lptr[i] + lane, lptr[i + 1], lcol, l32, z);
   for (int o = kRowLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kRowLanes);
   if (lane == 0) {
     float v = (r[i] - acc) * dinv[i];
@@ -308,8 +330,7 @@ __global__ __launch_bounds__(256) void bwd1_kernel(int64_t r0, int64_t r1, const
   const int lane = (int)(g % kRowLanes);
   if (rr >= r1) return;
   const int i = crow[rr];
-  float acc = 0.f;
-  for (int64_t p = uptr[i] + lane; p < uptr[i + 1]; p += kRowLanes) acc += u32[p] * x[ucol[p]];
+  float acc = strided_dot(uptr[i] + lane, uptr[i + 1], ucol, u32, x);
   for (int o = kRowLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kRowLanes);
   if (lane == 0) x[i] = (y[i] - acc) * dinv[i];
 }
@@ -323,8 +344,7 @@ __global__ __launch_bounds__(256) void spmm1_kernel(int64_t n, const int64_t* __
   const int64_t groups = (int64_t)gridDim.x * blockDim.x / kRowLanes;
   double acc[1] = {0.0};
   for (int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kRowLanes; i < n; i += groups) {
-    float s = 0.f;
-    for (int64_t e = indptr[i] + lane; e < indptr[i + 1]; e += kRowLanes) s += a[e] * x[indices[e]];
+    float s = strided_dot(indptr[i] + lane, indptr[i + 1], indices, a, x);
     for (int o = kRowLanes / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kRowLanes);
     if (lane == 0) {
       y[i] = s;
