@@ -449,6 +449,21 @@ int spde_pcg_solve(spde_ic* ic, const float* B, float* X, int k, double tol, int
  * iterations, *rounds, *relres = ||b - Q_c x|| / ||b|| (host). */
 int spde_pcg_refine(spde_ic* ic, const double* b, double* x, double tol, double inner_tol, int max_rounds,
                     int maxit, int32_t* iters, int32_t* rounds, double* relres, void* stream);
+/* FP64 x = Q_c^-1 b (one right-hand side, device) by FP32 PCG with FP64
+ * residual replacement (van der Vorst and Ye, SIAM J. Sci. Comput. 22, 2000):
+ * iterative refinement without restarting the recurrence.  Whenever the
+ * FP32 recursive residual has fallen by delta (checked every 16 iterations)
+ * or 512 iterations have passed, the FP32 iterate is added into x,
+ * r = b - Q_c x is formed in FP64 and replaces the recurrence's residual
+ * (search direction kept), until ||r|| <= tol ||b||, maxit iterations, or
+ * three replacements without a 10% fall.  A segment that leaves the true
+ * residual larger is taken back; after it, a breakdown or a stall the
+ * recurrence restarts from r.
+ * *iters, *updates (replacements), *relres = ||b - Q_c x|| / ||b|| (host).
+ * Replaces the refinement rounds' restarts of spde_pcg_refine, whose inner
+ * solves each start a new Krylov space. */
+int spde_pcg_replace(spde_ic* ic, const double* b, double* x, double tol, double delta, int maxit,
+                     int32_t* iters, int32_t* updates, double* relres, void* stream);
 /* X = Q_c^-1 B for k right-hand sides (k a power of two <= 64; B, X device
  * FP64, row-major n x k, entry i * k + r) by iterative refinement around the
  * FP32 block PCG: per column r = b - Q_c x in FP64, an inner solve of

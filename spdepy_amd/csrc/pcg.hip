@@ -515,6 +515,18 @@ __global__ __launch_bounds__(256) void dot2_kernel(int64_t units, int logk, cons
   block_reduce_store<2>(acc, 1 << logk, part);
 }
 
+// After a residual replacement (one right-hand side): r^T z and r^T r of the
+// replaced residual; the search direction is kept.
+__global__ __launch_bounds__(kRedThreads) void pcg_reseed_kernel(int nblk, const double* __restrict__ part,
+                                                                double* __restrict__ sc, int* __restrict__ active) {
+  __shared__ double sums[2][64];
+  reduce_parts<2>(part, 1, nblk, sums);
+  if (threadIdx.x != 0) return;
+  sc[S_RZ * 64] = sums[0][0];
+  sc[S_RR * 64] = sums[1][0];
+  active[0] = sums[0][0] > 0.0 ? 1 : 0;
+}
+
 // beta = rz_new / rz; convergence: ||r||^2 <= tol^2 ||b||^2 stops a right-hand side.
 __global__ __launch_bounds__(kRedThreads) void pcg_beta_kernel(int k, int nblk, const double* __restrict__ part,
                                                               double* __restrict__ sc, int* __restrict__ active,
@@ -569,6 +581,18 @@ __global__ __launch_bounds__(256) void correct_kernel(int64_t n, const double* _
   const double s = sqrt(*s2);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     x64[i] += s * (double)d32[i];
+}
+// r32 = r64 in FP32 (residual replacement: the FP64 residual becomes the recurrence's).
+__global__ __launch_bounds__(256) void cast_down_kernel(int64_t n, const double* __restrict__ r64,
+                                                        float* __restrict__ r32) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    r32[i] = (float)r64[i];
+}
+// x64 += a x32 (a = 1: a segment's FP32 iterate into the FP64 solution; -1: taken back).
+__global__ __launch_bounds__(256) void axpy32_kernel(int64_t n, double a, const float* __restrict__ x32,
+                                                     double* __restrict__ x64) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x64[i] += a * (double)x32[i];
 }
 __global__ __launch_bounds__(kRedThreads) void sum_one_kernel(int nblk, const double* __restrict__ part,
                                                              double* __restrict__ out) {
@@ -996,7 +1020,8 @@ int ensure_work(spde_ic* ic) {
   return SPDE_OK;
 }
 
-constexpr int kPoll = 16;  // iterations between the host's looks at the active flags
+constexpr int kPoll = 16;     // iterations between the host's looks at the active flags
+constexpr int kSegMax = 512;  // residual replacement: at most this many iterations between replacements
 
 // One PCG iteration on the handle's work vectors (R, Z, P, Qv) and X.
 void enqueue_iteration(spde_ic* ic, int logk, unsigned g, float* X, hipStream_t s) {
@@ -1497,6 +1522,109 @@ int spde_pcg_refine(spde_ic* ic, const double* b, double* x, double tol, double 
   SPDE_HIP(hipGetLastError());
   if (iters) *iters = total;
   if (rounds) *rounds = r;
+  if (relres) *relres = bn > 0.0 ? std::sqrt(rn / bn) : 0.0;
+  return SPDE_OK;
+}
+
+int spde_pcg_replace(spde_ic* ic, const double* b, double* x, double tol, double delta, int maxit, int32_t* iters,
+                     int32_t* updates, double* relres, void* stream) {
+  sync_view(ic);
+  if (!ic || !ic->factored || !b || !x || !(tol > 0.0) || !(delta > 0.0 && delta < 1.0) || maxit < 1) {
+    set_error("spde_pcg_replace: invalid argument (tol > 0, 0 < delta < 1, maxit >= 1, after spde_ic_factor)");
+    return SPDE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  int rc = ensure_work(ic);
+  if (rc) return rc;
+  spde_plan* pl = ic->pl;
+  const int64_t n = ic->n;
+  const unsigned g = grid_for(n);
+  float* R = ic->d_work;
+  float* Z = R + n * 64;
+  float* P = Z + n * 64;
+  double* rn_dev = ic->d_scal + S_X * 64;
+  auto residual = [&](double* out) -> int {  // r64 = b - Q_c x, *out = ||r64||^2
+    hipLaunchKernelGGL(resid64_kernel, dim3(g), dim3(256), 0, s, n, pl->d_indptr, pl->d_indices, pl->d_diag,
+                       ic->qval, ic->diag_add, b, x, ic->d_r64, ic->d_part);
+    hipLaunchKernelGGL(sum_one_kernel, dim3(1), dim3(kRedThreads), 0, s, (int)g, ic->d_part, rn_dev);
+    SPDE_HIP(hipMemcpyAsync(out, rn_dev, sizeof(double), hipMemcpyDeviceToHost, s));
+    SPDE_HIP(hipStreamSynchronize(s));
+    return SPDE_OK;
+  };
+  // r32 = r64, z = M^-1 r32 and the partials of r^T z, r^T r
+  auto precondition = [&]() {
+    hipLaunchKernelGGL(cast_down_kernel, dim3(g), dim3(256), 0, s, n, ic->d_r64, R);
+    ic_apply(ic, R, Z, 0, s);
+    hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, n, 0, R, Z, ic->d_part);
+  };
+  // a fresh recurrence from the current residual (tolerance 0: the host ends the solve)
+  auto restart = [&]() {
+    precondition();
+    hipLaunchKernelGGL(pcg_init_scalars_kernel, dim3(1), dim3(kRedThreads), 0, s, 1, (int)g, ic->d_part, 0.0,
+                       ic->d_scal, ic->d_active, ic->d_iters);
+    hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, s, n, Z, P);
+  };
+  SPDE_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+  double bn = 0.0;
+  if ((rc = residual(&bn))) return rc;
+  double rn = bn;
+  int it = 0, upd = 0, stall = 0;
+  if (bn > 0.0) {
+    hipLaunchKernelGGL(fill_kernel, dim3(g), dim3(256), 0, s, n, 0.f, ic->d_x32);
+    restart();
+    SPDE_HIP(hipGetLastError());
+    hipGraphExec_t ge = iteration_graph(ic, 0, g, s);
+    double seg = bn;   // ||r||^2 at the start of the segment
+    int seg_it = 0;    // iterations of the segment
+    while (it < maxit) {
+      const int before = it;
+      if (ge && maxit - it >= kPoll) {
+        SPDE_HIP(hipGraphLaunch(ge, s));
+        it += kPoll;
+      } else {
+        for (int j = 0; j < kPoll && it < maxit; ++j, ++it) enqueue_iteration(ic, 0, g, ic->d_x32, s);
+      }
+      seg_it += it - before;
+      int act = 0;
+      double rr = 0.0;
+      SPDE_HIP(hipMemcpyAsync(&act, ic->d_active, sizeof(int), hipMemcpyDeviceToHost, s));
+      SPDE_HIP(hipMemcpyAsync(&rr, ic->d_scal + S_RR * 64, sizeof(double), hipMemcpyDeviceToHost, s));
+      SPDE_HIP(hipStreamSynchronize(s));
+      if (act && rr > delta * delta * seg && seg_it < kSegMax && it < maxit) continue;
+      // residual replacement: the segment's FP32 iterate into x, the true FP64 residual into the recurrence
+      hipLaunchKernelGGL(axpy32_kernel, dim3(g), dim3(256), 0, s, n, 1.0, ic->d_x32, x);
+      double rn_new = 0.0;
+      if ((rc = residual(&rn_new))) return rc;
+      ++upd;
+      seg_it = 0;
+      if (!(rn_new <= rn)) {
+        // the segment made the true residual worse (the FP32 recurrence lost its
+        // way): take it back and restart the recurrence from the previous residual
+        hipLaunchKernelGGL(axpy32_kernel, dim3(g), dim3(256), 0, s, n, -1.0, ic->d_x32, x);
+        hipLaunchKernelGGL(fill_kernel, dim3(g), dim3(256), 0, s, n, 0.f, ic->d_x32);
+        if ((rc = residual(&rn))) return rc;
+        if (++stall >= 3 || it >= maxit) break;
+        restart();
+        seg = rn;
+        continue;
+      }
+      hipLaunchKernelGGL(fill_kernel, dim3(g), dim3(256), 0, s, n, 0.f, ic->d_x32);
+      stall = rn_new > 0.81 * rn ? stall + 1 : 0;  // the true residual fell by less than 10%
+      rn = rn_new;
+      if (rn <= tol * tol * bn || it >= maxit || stall >= 3) break;
+      if (act && stall == 0) {
+        precondition();
+        hipLaunchKernelGGL(pcg_reseed_kernel, dim3(1), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal,
+                           ic->d_active);
+      } else {
+        restart();  // breakdown of the recurrence, or little progress: new search directions
+      }
+      seg = rn;
+    }
+  }
+  SPDE_HIP(hipGetLastError());
+  if (iters) *iters = it;
+  if (updates) *updates = upd;
   if (relres) *relres = bn > 0.0 ? std::sqrt(rn / bn) : 0.0;
   return SPDE_OK;
 }

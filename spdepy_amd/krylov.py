@@ -172,6 +172,18 @@ class IterativeSolver:
                                             ctypes.byref(rel), _stream(torch)))
         return x, dict(iters=int(it.value), rounds=int(rounds.value), relres=float(rel.value))
 
+    def replace(self, b, tol=1e-10, delta=0.1, maxit=10000):
+        """FP64 solution of Q_c x = b by FP32 PCG with FP64 residual
+        replacement whenever the recursive residual has fallen by delta."""
+        torch = _torch()
+        b = b.contiguous()
+        x = torch.empty_like(b)
+        it, upd, rel = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        nat.check(nat.lib().spde_pcg_replace(self._h, _ptr(b), _ptr(x), float(tol), float(delta), int(maxit),
+                                             ctypes.byref(it), ctypes.byref(upd), ctypes.byref(rel),
+                                             _stream(torch)))
+        return x, dict(iters=int(it.value), updates=int(upd.value), relres=float(rel.value))
+
     def refine_block(self, B, tol=1e-10, inner_tol=1e-5, max_rounds=10, maxit=5000):
         """FP64 X = Q_c^-1 B for k right-hand sides (B: n x k float64 device,
         row-major; k a power of two <= 64) by iterative refinement around the
@@ -333,13 +345,18 @@ def _hutchinson(it, probes, seed, tol):
 
 def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32, slq_steps=50, hutch_probes=64,
                seed=0, tol=1e-10, probe_tol=1e-4, shift=0.1, correction=None, contract=None, adv_dir=None,
-               tol_like=None, inner_tol=1e-5):
+               tol_like=None, inner_tol=1e-5, mean_solver="replace", delta=0.1):
     """(-loglik/(m r), parts) as ``Engine.loglik`` returns them, with log|Q_c|
     from stochastic Lanczos quadrature and the trace part of the gradient from
     the Hutchinson estimate.  correction(engine, coeffs, qval, diag_add) ->
     device float tensor (n x 22) applies a learned preconditioner correction.
     parts gains "logdet_qc_se" (standard error of the SLQ estimate),
     "pcg" (iteration counts) and "ic_shift".
+
+    mean_solver: "replace" (default) solves for mu_c by FP32 PCG with FP64
+    residual replacement each time the residual has fallen by `delta`
+    (1,088 iterations at 100 x 100 x 20); "refine" by restarted FP32 solves
+    to `inner_tol` inside FP64 refinement rounds (1,383).  Both to `tol`.
 
     probe_tol: relative residual of the FP32 solves Q_c U = V of the
     Hutchinson probes.  At 100 x 100 x 20 with 64 probes, 1e-4 instead of
@@ -380,7 +397,12 @@ def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32
         ld_m = it.correct(correction(engine, coeffs, qval, diag_add))
     b_dev = obs["sty"] * tau
 
+    if mean_solver not in ("replace", "refine"):
+        raise ValueError("mean_solver must be 'replace' or 'refine'")
+
     def mean_solves(solver):
+        if mean_solver == "replace":
+            return [solver.replace(b_dev[:, c].contiguous(), tol=tol, delta=delta) for c in range(r)]
         return [solver.refine(b_dev[:, c].contiguous(), tol=tol, inner_tol=inner_tol) for c in range(r)]
 
     # mu_c (one right-hand side: short, latency-bound launches) runs on a

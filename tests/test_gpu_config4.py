@@ -1,6 +1,7 @@
 """The mixed-precision configuration at 200x200x24 (n = 960,000): the FP32
 PCG path with IC(0), FP64 iterative refinement to a relative residual of
-1e-10 for the conditional mean and 64 conditional samples, and the
+1e-10 for the conditional mean (FP64 residual replacement, and restarted
+refinement rounds) and 64 conditional samples, and the
 stochastic Lanczos log-determinant (32 Rademacher probes x 50 steps, FP32
 Lanczos vectors, FP64 tridiagonals), each with its error against the FP64
 Cholesky path on the same inputs (non-stationary advection-diffusion, 8x8
@@ -47,7 +48,7 @@ def test_iterative_path_against_cholesky_at_200x200x24():
     step("exact: FP64 Cholesky of Q_c at n = 960,000")
     like_ex, p_ex = eng.loglik(coeffs, cols, data, par[-1], grad=False)
     mu_ex = p_ex["mu"].cpu().numpy().ravel()
-    step("iterative: IC(0), refined PCG for mu_c, SLQ 32 x 50")
+    step("iterative: IC(0), PCG with FP64 residual replacement for mu_c, SLQ 32 x 50")
     like_it, p_it = loglik_pcg(eng, coeffs, cols, data, par[-1], grad=False, slq_probes=32, slq_steps=50, seed=1)
     mu_it = p_it["mu"].cpu().numpy().ravel()
     mu_err = np.linalg.norm(mu_it - mu_ex) / np.linalg.norm(mu_ex)
@@ -59,6 +60,13 @@ def test_iterative_path_against_cholesky_at_200x200x24():
     print("  mu_c: PCG %s, relative error vs Cholesky %.2e" % (p_it["pcg"]["mu"], mu_err))
     assert p_it["pcg"]["mu"][0]["relres"] <= 1e-10
     assert mu_err <= 1e-6
+    step("mu_c by restarted refinement rounds")
+    _, p_rf = loglik_pcg(eng, coeffs, cols, data, par[-1], grad=False, slq_probes=2, slq_steps=5, seed=1,
+                         mean_solver="refine")
+    mu_rf = p_rf["mu"].cpu().numpy().ravel()
+    err_rf = np.linalg.norm(mu_rf - mu_ex) / np.linalg.norm(mu_ex)
+    print("  mu_c: restarted rounds %s, relative error vs Cholesky %.2e" % (p_rf["pcg"]["mu"], err_rf))
+    assert p_rf["pcg"]["mu"][0]["relres"] <= 1e-10 and err_rf <= 1e-6
     assert abs(ld_err) <= 4 * p_it["logdet_qc_se"] + 1e-9 * abs(p_ex["ld_qc"])
     step("64 conditional samples by refined PCG")
     X, info = conditional_samples_pcg(eng, coeffs, cols, data, par[-1], k=64, seed=2)

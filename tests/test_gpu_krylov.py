@@ -127,6 +127,28 @@ def test_refinement_reaches_fp64(case):
     assert np.allclose(xh, xd, rtol=1e-7, atol=1e-8 * np.abs(xd).max())
 
 
+@pytest.mark.parametrize("delta", [0.1, 1e-3])
+def test_residual_replacement_reaches_fp64(case, delta):
+    """spde_pcg_replace: FP32 PCG with the FP64 residual replacing the
+    recursive one reaches the FP64 tolerance and the direct solution, also
+    with long segments (delta 1e-3), where a segment that makes the true
+    residual worse is taken back."""
+    import torch
+    eng = case["eng"]
+    it = eng.iterative()
+    it.factor(case["qval"], case["dadd"], shift=0.1)
+    b = torch.randn(eng.n, dtype=torch.float64, device="cuda")
+    step("residual replacement delta %g" % delta)
+    x, info = it.replace(b, tol=1e-10, delta=delta)
+    _, info_r = it.refine(b, tol=1e-10)
+    bh, xh = b.cpu().numpy(), x.cpu().numpy()
+    rel = np.linalg.norm(bh - case["A"] @ xh) / np.linalg.norm(bh)
+    print("  replacement", info, "restarted refinement", info_r, "host relres %.2e" % rel)
+    assert info["relres"] <= 1e-10 and rel <= 2e-10 and info["updates"] >= 1
+    xd = spla.spsolve(case["A"].tocsc(), bh)
+    assert np.allclose(xh, xd, rtol=1e-7, atol=1e-8 * np.abs(xd).max())
+
+
 def test_view_solves_beside_its_parent(case):
     """spde_ic_view: a second handle on the factor with its own work buffers
     gives the parent's results bit for bit, also while both solve at once on
