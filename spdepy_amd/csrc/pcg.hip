@@ -464,16 +464,42 @@ __device__ inline void reduce_parts(const double* __restrict__ part, int k, int 
   __syncthreads();
 }
 
+// out[a] = sum_b part[a][b][r] for one right-hand side r by a whole block of
+// kRedThreads (thread t adds blocks t, t + kRedThreads, ..., then a fixed
+// tree: deterministic); valid in thread 0.  The per-column scalar kernels
+// run one such block per right-hand side.
+template <int NA>
+__device__ inline void reduce_col(const double* __restrict__ part, int r, int nblk, double (&out)[NA]) {
+  __shared__ double red[NA][kRedThreads];
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    double acc = 0.0;
+    for (int b = t; b < nblk; b += kRedThreads) acc += part[((int64_t)a * kRedBlocks + b) * 64 + r];
+    red[a][t] = acc;
+  }
+  __syncthreads();
+  for (int st = kRedThreads / 2; st > 0; st >>= 1) {
+    if (t < st)
+#pragma unroll
+      for (int a = 0; a < NA; ++a) red[a][t] += red[a][t + st];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < NA; ++a) out[a] = red[a][0];
+}
+
 // x = 0, p = z, rz = r^T z, bn = b^T b (r = b), active = bn > 0.
-__global__ __launch_bounds__(kRedThreads) void pcg_init_scalars_kernel(int k, int nblk, const double* __restrict__ part,
+// One block per right-hand side r = blockIdx.x (as the kernels below).
+__global__ __launch_bounds__(kRedThreads) void pcg_init_scalars_kernel(int nblk, const double* __restrict__ part,
                                                                       double tol2, double* __restrict__ sc,
                                                                       int* __restrict__ active,
                                                                       int* __restrict__ iters) {
-  __shared__ double sums[2][64];
-  reduce_parts<2>(part, k, nblk, sums);
-  const int r = threadIdx.x;
-  if (r >= k) return;
-  const double rz = sums[0][r], bn = sums[1][r];
+  double v[2];
+  reduce_col<2>(part, blockIdx.x, nblk, v);
+  const int r = blockIdx.x;
+  if (threadIdx.x) return;
+  const double rz = v[0], bn = v[1];
   sc[S_RZ * 64 + r] = rz;
   sc[S_BN * 64 + r] = bn;
   sc[S_TOL2 * 64 + r] = tol2;
@@ -482,13 +508,13 @@ __global__ __launch_bounds__(kRedThreads) void pcg_init_scalars_kernel(int k, in
   iters[r] = 0;
 }
 
-__global__ __launch_bounds__(kRedThreads) void pcg_alpha_kernel(int k, int nblk, const double* __restrict__ part,
+__global__ __launch_bounds__(kRedThreads) void pcg_alpha_kernel(int nblk, const double* __restrict__ part,
                                                                double* __restrict__ sc, const int* __restrict__ active) {
-  __shared__ double sums[1][64];
-  reduce_parts<1>(part, k, nblk, sums);
-  const int r = threadIdx.x;
-  if (r >= k) return;
-  const double pq = sums[0][r];
+  double v[1];
+  reduce_col<1>(part, blockIdx.x, nblk, v);
+  const int r = blockIdx.x;
+  if (threadIdx.x) return;
+  const double pq = v[0];
   sc[S_PQ * 64 + r] = pq;
   sc[S_ALPHA * 64 + r] = (active[r] && pq > 0.0) ? sc[S_RZ * 64 + r] / pq : 0.0;
 }
@@ -528,14 +554,14 @@ __global__ __launch_bounds__(kRedThreads) void pcg_reseed_kernel(int nblk, const
 }
 
 // beta = rz_new / rz; convergence: ||r||^2 <= tol^2 ||b||^2 stops a right-hand side.
-__global__ __launch_bounds__(kRedThreads) void pcg_beta_kernel(int k, int nblk, const double* __restrict__ part,
+__global__ __launch_bounds__(kRedThreads) void pcg_beta_kernel(int nblk, const double* __restrict__ part,
                                                               double* __restrict__ sc, int* __restrict__ active,
                                                               int* __restrict__ iters) {
-  __shared__ double sums[2][64];
-  reduce_parts<2>(part, k, nblk, sums);
-  const int r = threadIdx.x;
-  if (r >= k) return;
-  const double rz = sums[0][r], rr = sums[1][r];
+  double v[2];
+  reduce_col<2>(part, blockIdx.x, nblk, v);
+  const int r = blockIdx.x;
+  if (threadIdx.x) return;
+  const double rz = v[0], rr = v[1];
   if (!active[r]) {
     sc[S_BETA * 64 + r] = 0.0;
     return;
@@ -635,12 +661,11 @@ __global__ __launch_bounds__(256) void resid64k_kernel(int64_t n, int logk, cons
   block_reduce_store<1>(acc, k, part);
 }
 // out[r] = sum of the partials of right-hand side r (in block order).
-__global__ __launch_bounds__(kRedThreads) void norms_kernel(int k, int nblk, const double* __restrict__ part,
+__global__ __launch_bounds__(kRedThreads) void norms_kernel(int nblk, const double* __restrict__ part,
                                                            double* __restrict__ out) {
-  __shared__ double sums[1][64];
-  reduce_parts<1>(part, k, nblk, sums);
-  const int r = threadIdx.x;
-  if (r < k) out[r] = sums[0][r];
+  double v[1];
+  reduce_col<1>(part, blockIdx.x, nblk, v);
+  if (threadIdx.x == 0) out[blockIdx.x] = v[0];
 }
 // b32 = r64 / ||r64|| per right-hand side; zero for those already within
 // their threshold (||r||^2 <= thr), whose inner solve is then empty.
@@ -671,14 +696,14 @@ __global__ __launch_bounds__(256) void fill64_kernel(int64_t units, double* __re
 // ------------------------------------------------------------ Lanczos
 // alpha_j = q^T w per right-hand side (partials), then w -= alpha q + beta_prev q_prev
 // with the partials of ||w||^2; then q_prev = q, q = w / beta_j.
-__global__ __launch_bounds__(kRedThreads) void lanczos_alpha_kernel(int k, int nblk, const double* __restrict__ part,
+__global__ __launch_bounds__(kRedThreads) void lanczos_alpha_kernel(int nblk, const double* __restrict__ part,
                                                                    double* __restrict__ sc,
                                                                    double* __restrict__ alpha_out) {
-  __shared__ double sums[1][64];
-  reduce_parts<1>(part, k, nblk, sums);
-  const int r = threadIdx.x;
-  if (r >= k) return;
-  const double a = sums[0][r];
+  double v[1];
+  reduce_col<1>(part, blockIdx.x, nblk, v);
+  const int r = blockIdx.x;
+  if (threadIdx.x) return;
+  const double a = v[0];
   sc[S_ALPHA * 64 + r] = a;
   alpha_out[r] = a;
 }
@@ -694,14 +719,14 @@ __global__ __launch_bounds__(256) void lanczos_orth_kernel(int64_t units, int lo
   }
   block_reduce_store<1>(acc, 1 << logk, part);
 }
-__global__ __launch_bounds__(kRedThreads) void lanczos_beta_kernel(int k, int nblk, const double* __restrict__ part,
+__global__ __launch_bounds__(kRedThreads) void lanczos_beta_kernel(int nblk, const double* __restrict__ part,
                                                                   double* __restrict__ sc,
                                                                   double* __restrict__ beta_out) {
-  __shared__ double sums[1][64];
-  reduce_parts<1>(part, k, nblk, sums);
-  const int r = threadIdx.x;
-  if (r >= k) return;
-  const double b = sqrt(sums[0][r]);
+  double v[1];
+  reduce_col<1>(part, blockIdx.x, nblk, v);
+  const int r = blockIdx.x;
+  if (threadIdx.x) return;
+  const double b = sqrt(v[0]);
   sc[S_BETA * 64 + r] = b;
   beta_out[r] = b;
 }
@@ -1032,11 +1057,11 @@ void enqueue_iteration(spde_ic* ic, int logk, unsigned g, float* X, hipStream_t 
   float* P = Z + n * 64;
   float* Qv = P + n * 64;
   launch_spmm(ic, logk, g, P, Qv, ic->d_part, s);
-  hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, ic->d_scal, ic->d_active);
+  hipLaunchKernelGGL(pcg_alpha_kernel, dim3(k), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal, ic->d_active);
   hipLaunchKernelGGL(pcg_update_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, P, Qv, X, R);
   ic_apply(ic, R, Z, logk, s);
   hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, R, Z, ic->d_part);
-  hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, ic->d_scal, ic->d_active,
+  hipLaunchKernelGGL(pcg_beta_kernel, dim3(k), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal, ic->d_active,
                      ic->d_iters);
   hipLaunchKernelGGL(pcg_p_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, ic->d_active, Z, P);
 }
@@ -1084,7 +1109,7 @@ int pcg_run(spde_ic* ic, const float* B, float* X, int logk, double tol, int max
   hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, s, units, B, R);
   ic_apply(ic, R, Z, logk, s);
   hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, R, Z, ic->d_part);
-  hipLaunchKernelGGL(pcg_init_scalars_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, tol * tol, ic->d_scal,
+  hipLaunchKernelGGL(pcg_init_scalars_kernel, dim3(k), dim3(kRedThreads), 0, s, (int)g, ic->d_part, tol * tol, ic->d_scal,
                      ic->d_active, ic->d_iters);
   hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, s, units, Z, P);
   SPDE_HIP(hipGetLastError());
@@ -1560,7 +1585,7 @@ int spde_pcg_replace(spde_ic* ic, const double* b, double* x, double tol, double
   // a fresh recurrence from the current residual (tolerance 0: the host ends the solve)
   auto restart = [&]() {
     precondition();
-    hipLaunchKernelGGL(pcg_init_scalars_kernel, dim3(1), dim3(kRedThreads), 0, s, 1, (int)g, ic->d_part, 0.0,
+    hipLaunchKernelGGL(pcg_init_scalars_kernel, dim3(1), dim3(kRedThreads), 0, s, (int)g, ic->d_part, 0.0,
                        ic->d_scal, ic->d_active, ic->d_iters);
     hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, s, n, Z, P);
   };
@@ -1649,7 +1674,7 @@ int spde_pcg_refine_block(spde_ic* ic, const double* B, double* X, int k, double
   hipLaunchKernelGGL(fill64_kernel, dim3(g), dim3(256), 0, s, units, X);
   hipLaunchKernelGGL(resid64k_kernel, dim3(g), dim3(256), 0, s, n, logk, pl->d_indptr, pl->d_indices, ic->qval,
                      ic->diag_add, B, X, ic->d_r64, ic->d_part);
-  hipLaunchKernelGGL(norms_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, rn);
+  hipLaunchKernelGGL(norms_kernel, dim3(k), dim3(kRedThreads), 0, s, (int)g, ic->d_part, rn);
   double bn[64], cur[64], th[64];
   SPDE_HIP(hipMemcpyAsync(bn, rn, sizeof(double) * k, hipMemcpyDeviceToHost, s));
   SPDE_HIP(hipStreamSynchronize(s));
@@ -1675,7 +1700,7 @@ int spde_pcg_refine_block(spde_ic* ic, const double* B, double* X, int k, double
     hipLaunchKernelGGL(correct_k_kernel, dim3(g), dim3(256), 0, s, units, logk, rn, thr, ic->d_x32, X);
     hipLaunchKernelGGL(resid64k_kernel, dim3(g), dim3(256), 0, s, n, logk, pl->d_indptr, pl->d_indices, ic->qval,
                        ic->diag_add, B, X, ic->d_r64, ic->d_part);
-    hipLaunchKernelGGL(norms_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, rn);
+    hipLaunchKernelGGL(norms_kernel, dim3(k), dim3(kRedThreads), 0, s, (int)g, ic->d_part, rn);
     SPDE_HIP(hipMemcpyAsync(cur, rn, sizeof(double) * k, hipMemcpyDeviceToHost, s));
     SPDE_HIP(hipStreamSynchronize(s));
     ++rd;
@@ -1745,10 +1770,10 @@ int spde_slq(spde_ic* ic, int k, int steps, uint64_t seed, double* alpha, double
     launch_spmm(ic, logk, g, W, U, nullptr, s);                                  // U = Q_c W
     ic_fwd(ic, U, W, logk, s);                                                   // W = L^-1 U
     hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, Qc, W, ic->d_part);
-    hipLaunchKernelGGL(lanczos_alpha_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, ic->d_scal,
+    hipLaunchKernelGGL(lanczos_alpha_kernel, dim3(k), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal,
                        da + (int64_t)j * 64);
     hipLaunchKernelGGL(lanczos_orth_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, Qc, Qp, W, ic->d_part);
-    hipLaunchKernelGGL(lanczos_beta_kernel, dim3(1), dim3(kRedThreads), 0, s, k, (int)g, ic->d_part, ic->d_scal,
+    hipLaunchKernelGGL(lanczos_beta_kernel, dim3(k), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal,
                        db + (int64_t)j * 64);
     hipLaunchKernelGGL(lanczos_next_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, Qc, Qp, W);
   }
