@@ -78,3 +78,70 @@ def test_iterative_path_against_cholesky_at_200x200x24():
     err = (torch.linalg.norm(X - Xc, dim=0) / torch.linalg.norm(Xc, dim=0)).cpu().numpy()
     print("  samples: relative error vs Cholesky max %.2e median %.2e" % (err.max(), np.median(err)))
     assert err.max() <= 1e-6
+
+
+def test_exact_path_with_gradient_at_200x200x24():
+    """The exact FP64 log-likelihood and gradient at n = 960,000 (Cholesky and
+    selected inverse of Q_c, log|Q| from the spatial factors) fit one GPU's
+    memory: log|Q| equals the oracle's closed form, the selected inverse
+    satisfies sum_j (Q_c)_ij (Q_c^-1)_ij = 1 on every row, and the gradient
+    agrees with Richardson central differences of the value."""
+    import torch
+    from oracle import fixtures as F, grid as G, models as Mo
+    step("model 200x200x24, exact logLike + gradient")
+    mod, par = _model()
+    like, jac = mod.logLike(par)
+    parts = mod.last_parts
+    eng = mod._engine()
+    import time
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    like2, jac2 = mod.logLike(par)
+    torch.cuda.synchronize()
+    print("  one evaluation with gradient: %.3f s" % (time.perf_counter() - t0))
+    assert like2 == like and np.array_equal(jac2, jac)
+    info = eng.plan.info
+    print("  like %.12f; plan: nnz(L) %.3e, front doubles %.3e (%.0f GB per slot), selected-inverse arena "
+          "the same" % (like, info["nnz_L"], info["front_doubles"], 8e-9 * info["front_doubles"]))
+    step("oracle closed-form log|Q| at 200x200x24")
+    M, N, T = 200, 200, 24
+    g = G.STGrid(F.axis(M), F.axis(N), np.arange(float(T)), Nbs=8)
+    mod0 = F.mod0_iso(M, N, Nbs=8)
+    A, q, s = Mo.VarAdvVarDiff(g, mod0, bc=1).spatial_operator(par)
+    A0 = mod0.A0(mod0.par)[0]
+    cf = Mo.logdet_closed_form(A, q, A0, g.V, T, s, g.dt)
+    print("  log|Q| GPU %.10f oracle %.10f" % (parts["ld_q"], cf))
+    assert abs(parts["ld_q"] - cf) <= 1e-11 * abs(cf)
+    step("selected inverse identity on every row")
+    coeffs = mod._coeffs(par)
+    qval = eng.assemble(coeffs)
+    cols = np.asarray(mod._obs_cols())
+    tau = np.exp(par[-1])
+    dadd = torch.as_tensor(np.bincount(cols, minlength=eng.n).astype(np.float64) * tau).cuda()
+    sig = eng.plan.selinv(1)
+    indptr, indices = eng.plan.structure()
+    rows = torch.as_tensor(np.repeat(np.arange(eng.n), np.diff(indptr))).cuda()
+    qc = qval.clone()
+    diag = rows == torch.as_tensor(indices).cuda()
+    qc[diag] += dadd
+    row_sum = torch.zeros(eng.n, dtype=torch.float64, device="cuda").index_add_(0, rows, qc * sig)
+    err = float((row_sum - 1.0).abs().max())
+    print("  max |sum_j (Q_c)_ij Sigma_ij - 1| = %.3e" % err)
+    assert err < 1e-8
+    del sig, rows, qc, diag, row_sum
+
+    def value(p):
+        return mod.logLike(p, grad=False)
+    scale = np.abs(jac).max()
+    Np = mod.Np
+    for i in [3 * Np + 11, 6 * Np + 1]:
+        step("central differences, parameter %d" % i)
+
+        def central(h):
+            pp, pm = par.copy(), par.copy()
+            pp[i] += h
+            pm[i] -= h
+            return (value(pp) - value(pm)) / (2 * h)
+        fd = (4 * central(1e-4) - central(2e-4)) / 3
+        print("  param %d grad %+.10e fd %+.10e err/scale %.2e" % (i, jac[i], fd, abs(fd - jac[i]) / scale))
+        assert abs(fd - jac[i]) <= 2e-6 * abs(jac[i]) + 2e-6 * scale
