@@ -12,7 +12,8 @@ timing starts.  Prints one JSON line.  Beside the headline (exact path) it
 reports, on the same workload: the route that factorises Q as the reference
 does, the iterative path (FP32 PCG with IC(0), FP64 refinement, stochastic
 Lanczos log|Q_c|, Hutchinson trace) and its learned-preconditioner form, each
-with its error against the exact path; and the symbolic-analysis time.
+with its error against the exact path; the time of 16 conditional samples
+from the exact path's factor; and the symbolic-analysis time.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-paths]
 """
@@ -183,6 +184,46 @@ def other_routes(mod, par, like_ex, jac_ex, steps=3):
                                "decided_after_slq": bool(parts.get("infeasible")),
                                "rel_err_like": abs(like - like_ex) / abs(like_ex)}
     return out
+
+
+def conditional_samples_route(mod, par, k=16, steps=10):
+    """ms per k conditional samples x = mu_c + P^T L^-T z of the exact path on
+    the workload (the factor of Q_c the evaluation leaves, z from the
+    library's generator, one k-column solve), with the largest relative error
+    of the identity (x - mu_c)^T Q_c (x - mu_c) = z^T z over the columns."""
+    import ctypes
+    import torch
+    from spdepy_amd import _native as nat
+    from spdepy_amd.krylov import gaussian
+    from spdepy_amd.plan import _ptr
+    mod.logLike(par)
+    eng = mod._engine()
+    plan = eng.plan
+    n = plan.n
+
+    def draw(seed):
+        Z = gaussian(n, k, seed)
+        X = plan.solve(1, Z, mode=nat.SOLVE_LT)
+        mu = torch.empty(n, dtype=torch.float64, device="cuda")
+        nat.check(nat.lib().spde_loglik_mu(plan._h, _ptr(mu), 1,
+                                           ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        return Z, X + mu[:, None], mu
+    draw(0)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for i in range(steps):
+        Z, X, mu = draw(i + 1)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) / steps * 1e3
+    qval = eng.assemble(mod._coeffs(par))
+    tau = float(np.exp(par[-1]))
+    dadd = torch.as_tensor(np.bincount(np.asarray(mod._obs_cols()), minlength=n) * tau).cuda()
+    D = X - mu[:, None]
+    QD = plan.spmv(qval, D) + dadd[:, None] * D
+    rel = float((((D * QD).sum(0) - (Z * Z).sum(0)).abs() / (Z * Z).sum(0)).max())
+    return {"samples": k, "ms": ms, "identity_max_rel_err": rel,
+            "what": "mu_c + P^T L^-T z with the FP64 factor of Q_c: %d Box-Muller draws, one %d-column "
+                    "triangular solve (MFMA tiles), mean %d evaluations" % (k, k, steps)}
 
 
 def batch_models(nmodels=64, seed=0):
@@ -395,6 +436,7 @@ def main():
                           "per_kind_ms_per_eval": {int(q): lms[kind_all == q].sum() / prof_steps for q in kinds}}),
               file=sys.stderr)
     routes = None if args.no_paths else other_routes(mod, par, like, jac)
+    samples = None if args.no_paths else conditional_samples_route(mod, par)
     if routes:
         for k in ("pcg", "learned_pcg"):
             if "evals_per_s" in routes.get(k, {}):
@@ -440,6 +482,7 @@ def main():
         "cpu_baseline": base,
         "symbolic_analysis_s": getattr(build_workload, "symbolic_s", None),
         "routes": routes,
+        "conditional_samples": samples,
         "batch": batch,
     }
     print(json.dumps(out))
