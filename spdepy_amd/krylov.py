@@ -65,9 +65,13 @@ class IterativeSolver:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
+        if h is None or not h.value:
+            return
+        self._h = None
+        try:
             nat.lib().spde_ic_destroy(h)
-            self._h = None
+        except (AttributeError, TypeError):  # at interpreter exit the modules are gone; the process frees the device
+            pass
 
     def view(self):
         """A second handle on this factor with its own work buffers, for a

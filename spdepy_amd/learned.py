@@ -316,8 +316,10 @@ def train(net, model, draws, K=16, shift=0.1, lr=1e-3, obs_frac=0.025, seed=0, l
     """Trains `net` on `draws` parameter settings of the B-spline model
     `model` (its grid and boundary condition): for each draw, Q_c with
     observations at a random obs_frac of the lattice (tau = 100), K exact
-    samples, the IC(0) factor, one Adam step on `loss` (see
-    LearnedPreconditioner.train_step).  Returns the per-draw losses."""
+    samples from the FP64 Cholesky path (for the divergence losses; the
+    Kaporin loss uses K Rademacher probes and no samples), the IC(0) factor,
+    one Adam step on `loss` (see LearnedPreconditioner.train_step).  Returns
+    the per-draw losses."""
     torch = _torch()
     eng = model._engine()
     it = eng.iterative()
@@ -333,7 +335,10 @@ def train(net, model, draws, K=16, shift=0.1, lr=1e-3, obs_frac=0.025, seed=0, l
         dadd = np.zeros(n)
         dadd[obs] = np.exp(par[-1])
         dadd_d = torch.as_tensor(dadd).cuda()
-        x = exact_samples(eng, qval, dadd_d, K, seed * 100003 + d)
+        # Kaporin's loss needs Q_c and the factor only (its K probes are
+        # Rademacher); the divergences compare against exact samples
+        x = torch.zeros((n, K), dtype=torch.float64, device="cuda") if loss == "kaporin" else \
+            exact_samples(eng, qval, dadd_d, K, seed * 100003 + d)
         it.factor(qval, dadd_d, shift=shift)
         losses.append(net.train_step(it, x, shape, lr=lr, loss=loss, seed=seed * 100003 + d + 7))
         if log is not None and (d % 50 == 0 or d == draws - 1):

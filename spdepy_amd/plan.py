@@ -40,9 +40,13 @@ class Plan:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
+        if h is None or not h.value:
+            return
+        self._h = None
+        try:
             nat.lib().spde_plan_destroy(h)
-            self._h = None
+        except (AttributeError, TypeError):  # at interpreter exit the modules are gone; the process frees the device
+            pass
 
     # ------------------------------------------------------------ host data
     def export(self):

@@ -4,10 +4,10 @@
 
 Each draw is a seeded parameter setting of the B-spline advection /
 anisotropic-diffusion model (``learned.draw_parameters``) with observations
-at 2.5% of the lattice (tau = 100); the exact FP64 Cholesky path supplies
-K samples of N(0, Q_c^-1), and one Adam step lowers the loss of the corrected
-IC(0) factor (``learned.train``; --loss: Kaporin's condition number by
-default, or the divergences against the exact samples).  Progress goes to stdout, the
+at 2.5% of the lattice (tau = 100), and one Adam step lowers the loss of the
+corrected IC(0) factor (``learned.train``; --loss: Kaporin's condition number
+by default, from K Rademacher probes, or the divergences against K samples of
+N(0, Q_c^-1) that the exact FP64 Cholesky path supplies).  Progress goes to stdout, the
 weights to spdepy_amd/weights/learned_ic.npz (or --out)."""
 import argparse
 import os
