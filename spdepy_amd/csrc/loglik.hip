@@ -10,6 +10,8 @@
 // examples/server/get_results.py:6-51).
 #include <hip/hip_runtime.h>
 
+#include <thread>
+
 #include <cmath>
 #include <string>
 #include <vector>
@@ -395,17 +397,30 @@ int spde_loglik_batch(int B, spde_plan* const* pls, spde_plan* const* sps, const
       busy.push_back(sps[i1]);
       ++i1;
     }
+    // enqueued from one host thread per stream (each model's plans are its
+    // own within a wave, so the threads share no library state; the error
+    // text is per thread)
     std::vector<int> enq(i1 - i0, 0);
-    for (int i = i0; i < i1; ++i) {
-      const int rc = spde_loglik_enqueue(pls[i], sps[i], &cs[i], obs_cols[i], ys[i], ms[i], r, log_taus[i], cells[i],
-                                         streams[(i - i0) % nstreams]);
-      status[i] = rc;
-      enq[i - i0] = rc == SPDE_OK;
-      if (rc && !first_rc) {
-        first_rc = rc;
-        first_msg = spde_last_error();
+    std::vector<std::string> msg(i1 - i0);
+    const int nt = std::min(nstreams, i1 - i0);
+    auto enqueue_part = [&](int t) {
+      for (int i = i0 + t; i < i1; i += nt) {
+        const int rc = spde_loglik_enqueue(pls[i], sps[i], &cs[i], obs_cols[i], ys[i], ms[i], r, log_taus[i],
+                                           cells[i], streams[(i - i0) % nstreams]);
+        status[i] = rc;
+        enq[i - i0] = rc == SPDE_OK;
+        if (rc) msg[i - i0] = spde_last_error();
       }
-    }
+    };
+    std::vector<std::thread> workers;
+    for (int t = 1; t < nt; ++t) workers.emplace_back(enqueue_part, t);
+    enqueue_part(0);
+    for (auto& w : workers) w.join();
+    for (int i = i0; i < i1; ++i)
+      if (status[i] && !first_rc) {
+        first_rc = status[i];
+        first_msg = msg[i - i0];
+      }
     for (int i = i0; i < i1; ++i) {
       if (!enq[i - i0]) continue;
       const int rc = spde_loglik_finish(pls[i], &outs[i]);
