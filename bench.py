@@ -174,14 +174,16 @@ def other_routes(mod, par, like_ex, jac_ex, steps=3):
                      "ic_shift": parts["ic_shift"],
                      "estimators": "log|Q_c|: 32 Rademacher probes x 50 Lanczos steps on L^-1 Q_c L^-T + "
                                    "log|L L^T|; gradient trace: 64-probe Hutchinson from FP32 PCG (tol 1e-4); "
-                                   "mu_c: FP64 refinement to 1e-10 on a second stream beside the probes"}
+                                   "mu_c: FP32 PCG with FP64 residual replacement to 1e-10, on a second "
+                                   "stream beside the probes"}
     if "evals_per_s" in out.get("learned_pcg", {}):
         lp = out["learned_pcg"]
         lp["meets_1e-6"] = bool(lp["rel_se_like"] <= 1e-6)
         # the product route falls back to the exact Cholesky path when the stated error is missed
         rate, like, jac, parts = time_route(mod, par, steps, method="learned", seed=11)
         lp["with_fallback"] = {"evals_per_s": rate, "fallback": bool(parts.get("fallback")),
-                               "decided_after_slq": bool(parts.get("infeasible")),
+                               "decided_before_solves": bool(parts.get("infeasible")),
+                               "decided_by_pilot": bool(parts.get("pilot")),
                                "rel_err_like": abs(like - like_ex) / abs(like_ex)}
     return out
 
@@ -324,7 +326,7 @@ def main():
     ap.add_argument("--no-paths", action="store_true", help="skip the factorised, iterative and learned routes")
     ap.add_argument("--batch", action="store_true", help="also time the batch of 64 uneven models")
     ap.add_argument("--batch-only", action="store_true", help="only the batch of 64 uneven models")
-    ap.add_argument("--only-route", choices=["exact", "pcg", "learned"],
+    ap.add_argument("--only-route", choices=["exact", "pcg", "learned", "learned_fallback"],
                     help="only `steps` evaluations of one route on the workload (for profiling)")
     ap.add_argument("--stages", action="store_true", help="print a per-stage breakdown to stderr")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -344,7 +346,8 @@ def main():
     if args.only_route:
         mod, par = build_workload()
         kw = {"exact": {}, "pcg": dict(method="pcg", seed=11),
-              "learned": dict(method="learned", seed=11, tol_like=np.inf)}[args.only_route]
+              "learned": dict(method="learned", seed=11, tol_like=np.inf),
+              "learned_fallback": dict(method="learned", seed=11)}[args.only_route]
         rate, like, jac, parts = time_route(mod, par, args.steps, **kw)
         info = None if parts is None or "pcg" not in parts else {
             "mu": parts["pcg"]["mu"], "probe_iterations_mean": float(np.mean(parts["pcg"].get("probes", [0])))}

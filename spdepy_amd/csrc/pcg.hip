@@ -43,6 +43,7 @@ struct IcShared {
   int32_t* d_crow = nullptr;
   int64_t *d_lptr = nullptr, *d_uptr = nullptr, *d_lsrc = nullptr, *d_ulow = nullptr, *d_lch = nullptr;
   int32_t *d_lcol = nullptr, *d_ucol = nullptr;
+  int64_t* d_lkey = nullptr;         // colour * n + index of each lower entry's column
   double *d_l64 = nullptr, *d_dg64 = nullptr, *d_adiag = nullptr;  // factor (FP64), Q_c diagonal
   float *d_l32 = nullptr, *d_u32 = nullptr, *d_dinv = nullptr, *d_qc32 = nullptr;
   const double* qval = nullptr;      // Q values of the last factorisation (structural order)
@@ -99,10 +100,13 @@ __device__ inline int lat_colour(int64_t idx, int M, int N) {
 // ------------------------------------------------------------ IC(0)
 // One thread per row of colour c.  L_ij = (A_ij - sum_k L_ik L_jk) / L_jj over
 // the k adjacent to both with colour(k) < colour(j), in (colour, index) order
-// of the row's lower list; L_ii = sqrt((1 + shift) A_ii - sum_j L_ij^2).
+// of the row's lower list; L_ii = sqrt((1 + shift) A_ii - sum_j L_ij^2).  The
+// common k are found by one merge of the two sorted lists (lkey = colour * n +
+// index of each entry), in row j's order.
 __global__ __launch_bounds__(256) void ic_factor_kernel(int64_t r0, int64_t r1, const int32_t* __restrict__ crow,
                                                         const int64_t* __restrict__ lptr,
                                                         const int32_t* __restrict__ lcol,
+                                                        const int64_t* __restrict__ lkey,
                                                         const int64_t* __restrict__ lsrc,
                                                         const double* __restrict__ qval,
                                                         const double* __restrict__ adiag, double shift,
@@ -116,13 +120,19 @@ __global__ __launch_bounds__(256) void ic_factor_kernel(int64_t r0, int64_t r1, 
   for (int64_t q = lo; q < hi; ++q) {
     const int j = lcol[q];
     double a = qval[lsrc[q]];
-    for (int64_t p = lptr[j]; p < lptr[j + 1]; ++p) {
-      const int kk = lcol[p];
-      for (int64_t s = lo; s < q; ++s)
-        if (lcol[s] == kk) {
-          a -= l64[s] * l64[p];
-          break;
-        }
+    int64_t sq = lo, p = lptr[j];
+    const int64_t pe = lptr[j + 1];
+    while (sq < q && p < pe) {
+      const int64_t ks = lkey[sq], kp = lkey[p];
+      if (ks == kp) {
+        a -= l64[sq] * l64[p];
+        ++sq;
+        ++p;
+      } else if (ks < kp) {
+        ++sq;
+      } else {
+        ++p;
+      }
     }
     const double v = a / dg64[j];
     l64[q] = v;
@@ -1026,7 +1036,8 @@ void free_work(spde_ic* ic) {
 }
 void free_shared(spde_ic* ic) {
   for (void* p : {(void*)ic->d_crow, (void*)ic->d_lptr, (void*)ic->d_uptr, (void*)ic->d_lsrc, (void*)ic->d_ulow,
-                  (void*)ic->d_lch, (void*)ic->d_lcol, (void*)ic->d_ucol, (void*)ic->d_l64, (void*)ic->d_dg64,
+                  (void*)ic->d_lch, (void*)ic->d_lcol, (void*)ic->d_lkey, (void*)ic->d_ucol, (void*)ic->d_l64,
+                  (void*)ic->d_dg64,
                   (void*)ic->d_adiag, (void*)ic->d_l32, (void*)ic->d_u32, (void*)ic->d_dinv, (void*)ic->d_qc32})
     if (p) hipFree(p);
 }
@@ -1289,6 +1300,11 @@ int spde_ic_create(spde_plan* pl, spde_ic** out) {
   if ((rc = upload_vec(&ic->d_lptr, ic->h_lptr))) return fail(rc);
   if ((rc = upload_vec(&ic->d_uptr, ic->h_uptr))) return fail(rc);
   if ((rc = upload_vec(&ic->d_lcol, ic->h_lcol))) return fail(rc);
+  {
+    std::vector<int64_t> lkey(ic->nlow);
+    for (int64_t q = 0; q < ic->nlow; ++q) lkey[q] = (int64_t)ic->h_colour[ic->h_lcol[q]] * n + ic->h_lcol[q];
+    if ((rc = upload_vec(&ic->d_lkey, lkey))) return fail(rc);
+  }
   if ((rc = upload_vec(&ic->d_lsrc, ic->h_lsrc))) return fail(rc);
   if ((rc = upload_vec(&ic->d_lch, ic->h_lch))) return fail(rc);
   if ((rc = upload_vec(&ic->d_ucol, ic->h_ucol))) return fail(rc);
@@ -1398,7 +1414,8 @@ int spde_ic_factor(spde_ic* ic, const double* qval, const double* diag_add, doub
     const int64_t r0 = ic->coff[c], r1 = ic->coff[c + 1];
     if (r1 > r0)
       hipLaunchKernelGGL(ic_factor_kernel, dim3(blocks_of(r1 - r0)), dim3(256), 0, s, r0, r1, ic->d_crow, ic->d_lptr,
-                         ic->d_lcol, ic->d_lsrc, qval, ic->d_adiag, shift, ic->d_l64, ic->d_dg64, ic->d_status);
+                         ic->d_lcol, ic->d_lkey, ic->d_lsrc, qval, ic->d_adiag, shift, ic->d_l64, ic->d_dg64,
+                         ic->d_status);
   }
   SPDE_HIP(hipGetLastError());
   int st = 0;

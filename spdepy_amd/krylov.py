@@ -372,7 +372,9 @@ def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32
     of |loglik| (mu_c minimises mu^T Q mu + tau ||y - S mu||^2, so
     0 <= mu^T Q mu + tau ||y - S mu||^2 <= tau ||y||^2), the tolerance cannot
     be met and the call returns (None, parts) with parts["infeasible"] set,
-    before the solves for mu_c and the Hutchinson probes."""
+    before the solves for mu_c and the Hutchinson probes.  A pilot of 4
+    probes decides first when even a tenth of its error, scaled to
+    slq_probes, is out of reach (parts["pilot"])."""
     torch = _torch()
     from .engine import OUT_DOT00, OUT_TAUTR, OUT_WIDTH, _obs_device
     if engine.is_spatial:
@@ -418,21 +420,40 @@ def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32
     try:
         if grad and hutch_probes > 0 and tol_like is None:
             side = _start_side(it, mean_solves)
-        alpha, beta = it.slq(slq_probes, slq_steps, seed)
-        trlog, se, _ = slq_quadrature(alpha, beta, n)
-        ld_qc = ld_m + trlog
         ld_q0 = sp.factor_finish(0)
         ld_gram = sp.factor_finish(1)
         logq = engine.plan.logq_sum(1.0 / (coeffs.c.dt * coeffs.c.sigma))
         ld_q = ld_q0 + (T - 1) * (ld_gram + logq)
-        if tol_like is not None:
+
+        def out_of_reach(ld_qc, se, margin=1.0):
+            """(bound, whether margin * (1/2) r se exceeds tol_like * bound)."""
             known = 0.5 * ld_q * r + m * r * np.log(tau) / 2 - 0.5 * ld_qc * r
             bound = abs(known) + 0.5 * tau * float((y2 * y2).sum())
-            if 0.5 * r * se > tol_like * bound:
-                if grad:
-                    torch.cuda.synchronize()       # the spatial selected inverses are enqueued
-                return None, dict(m=m, r=r, tau=tau, ld_q=ld_q, ld_qc=ld_qc, logdet_qc_se=se, logdet_m=ld_m,
-                                  ic_shift=it.shift, infeasible=True, like_bound=bound, pcg=dict(mu=[]))
+            return bound, margin * 0.5 * r * se > tol_like * bound
+
+        def infeasible(ld_qc, se, bound, pilot):
+            if grad:
+                torch.cuda.synchronize()       # the spatial selected inverses are enqueued
+            return None, dict(m=m, r=r, tau=tau, ld_q=ld_q, ld_qc=ld_qc, logdet_qc_se=se, logdet_m=ld_m,
+                              ic_shift=it.shift, infeasible=True, like_bound=bound, pilot=pilot,
+                              pcg=dict(mu=[]))
+        if tol_like is not None and slq_probes > 4:
+            # a pilot of 4 probes: when a tenth of its standard error, scaled to
+            # slq_probes probes, already exceeds the tolerance, the full
+            # estimate cannot meet it (the tenth covers the pilot's own spread)
+            a_p, b_p = it.slq(4, slq_steps, seed + 7919)
+            tr_p, se_p, _ = slq_quadrature(a_p, b_p, n)
+            se_est = se_p * np.sqrt(4.0 / slq_probes)
+            bound, out = out_of_reach(ld_m + tr_p, se_est, margin=0.1)
+            if out:
+                return infeasible(ld_m + tr_p, se_est, bound, True)
+        alpha, beta = it.slq(slq_probes, slq_steps, seed)
+        trlog, se, _ = slq_quadrature(alpha, beta, n)
+        ld_qc = ld_m + trlog
+        if tol_like is not None:
+            bound, out = out_of_reach(ld_qc, se)
+            if out:
+                return infeasible(ld_qc, se, bound, False)
             if grad and hutch_probes > 0:
                 side = _start_side(it, mean_solves)
         if side is not None:

@@ -225,8 +225,9 @@ def test_learned_route_and_cholesky_fallback():
     """logLike(method="learned") with the shipped weights: without a
     tolerance the estimate lies within 4 standard errors of the exact value;
     at the default 1e-6 the stochastic log-determinant cannot meet it, the
-    route sees that right after the Lanczos quadrature and returns the exact
-    Cholesky value and gradient."""
+    route sees that from a 4-probe pilot of the Lanczos quadrature and returns
+    the exact Cholesky value and gradient; at a loose tolerance neither the
+    pilot nor the full estimate falls back."""
     c = F.small(12, 10, 6, bc=1, m=60)
     mod = product_case(c)
     mod.initFit(c["y"], idx=c["idx"], fitQ0=False)
@@ -242,8 +243,13 @@ def test_learned_route_and_cholesky_fallback():
     step("learned route at 1e-6: Cholesky fallback")
     l_fb, j_fb = mod.logLike(par, method="learned", seed=5)
     p = mod.last_parts
-    assert p["fallback"] and p["infeasible"] and p["like_rel_se"] > 1e-6
+    assert p["fallback"] and p["infeasible"] and p["pilot"] and p["like_rel_se"] > 1e-6
     assert l_fb == l_ex and np.array_equal(j_fb, j_ex)
+    step("learned route at 0.5: no fallback")
+    l_lo, _ = mod.logLike(par, method="learned", seed=5, tol_like=0.5, hutch_probes=64)
+    p = mod.last_parts
+    assert not p["fallback"] and not p.get("infeasible") and p["like_rel_se"] <= 0.5
+    assert abs(l_lo - l_ex) <= 4 * 0.5 * p["r"] * p["logdet_qc_se"] / (p["m"] * p["r"]) + 1e-9 * abs(l_ex)
 
 
 def _held_out_iterations(mod, net, ndraws, seed, extra=()):
