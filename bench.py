@@ -173,7 +173,7 @@ def other_routes(mod, par, like_ex, jac_ex, steps=3):
                                         "probes_mean": float(np.mean(parts["pcg"]["probes"]))},
                      "ic_shift": parts["ic_shift"],
                      "estimators": "log|Q_c|: 32 Rademacher probes x 50 Lanczos steps on L^-1 Q_c L^-T + "
-                                   "log|L L^T|; gradient trace: 64-probe Hutchinson from FP32 PCG (tol 1e-4); "
+                                   "log|L L^T|; gradient trace: 64-probe Hutchinson from FP32 PCG (tol 1e-3); "
                                    "mu_c: FP32 PCG with FP64 residual replacement to 1e-10, on a second "
                                    "stream beside the probes"}
     if "evals_per_s" in out.get("learned_pcg", {}):
