@@ -465,6 +465,15 @@ int spde_pcg_refine(spde_ic* ic, const double* b, double* x, double tol, double 
 int spde_pcg_replace(spde_ic* ic, const double* b, double* x, double tol, double delta, int maxit,
                      int32_t* iters, int32_t* updates, double* relres, void* stream);
 /* X = Q_c^-1 B for k right-hand sides (k a power of two <= 64; B, X device
+ * FP64, row-major n x k) by the FP32 block PCG with FP64 residual
+ * replacement (as spde_pcg_replace): every column is replaced together once
+ * every live column's recursive residual has fallen by delta, or after 512
+ * iterations; per column a worse segment is taken back and the column
+ * restarted, a column within tol (or stalled three times) stops.  iters[k],
+ * relres[k] (host) per column; *updates (host). */
+int spde_pcg_replace_block(spde_ic* ic, const double* B, double* X, int k, double tol, double delta, int maxit,
+                           int32_t* iters, int32_t* updates, double* relres, void* stream);
+/* X = Q_c^-1 B for k right-hand sides (k a power of two <= 64; B, X device
  * FP64, row-major n x k, entry i * k + r) by iterative refinement around the
  * FP32 block PCG: per column r = b - Q_c x in FP64, an inner solve of
  * Q_c d = r / ||r|| to inner_tol, x += ||r|| d, until ||r|| <= tol ||b|| for

@@ -125,6 +125,8 @@ _SIGS = {
                                        c_i32p, c_i32p, c_dp, vp]),
     "spde_pcg_replace": (ctypes.c_int, [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, c_i32p, c_i32p,
                                         c_dp, vp]),
+    "spde_pcg_replace_block": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_int, vp, c_i32p, vp, vp]),
     "spde_rademacher": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_uint64, ctypes.c_float, vp, vp]),
     "spde_pcg_refine_block": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]),

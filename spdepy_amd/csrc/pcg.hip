@@ -65,6 +65,7 @@ struct IcWork {
   double* d_scal = nullptr;          // per right-hand side scalars [64 * 8]
   int* d_active = nullptr;           // [64]
   int* d_iters = nullptr;            // [64]
+  int* d_state = nullptr;            // [64] per right-hand side after a block residual replacement
   float* d_work = nullptr;           // 4 vectors n x 64 (PCG: r, z, p, q; Lanczos: q_prev, q_cur, w, u)
   double* d_r64 = nullptr;           // refinement residual (n x 64)
   float *d_b32 = nullptr, *d_x32 = nullptr;  // inner right-hand side and solution (n x 64)
@@ -563,6 +564,26 @@ __global__ __launch_bounds__(kRedThreads) void pcg_reseed_kernel(int nblk, const
   active[0] = sums[0][0] > 0.0 ? 1 : 0;
 }
 
+// After a block residual replacement, one block per right-hand side r: state
+// 0 stops the column, 1 keeps its search direction with r^T z and r^T r of
+// the replaced residual, 2 restarts it from them (p = z by copy_cols_kernel).
+__global__ __launch_bounds__(kRedThreads) void pcg_replace_scalars_kernel(int nblk, const double* __restrict__ part,
+                                                                         const int* __restrict__ state,
+                                                                         double* __restrict__ sc,
+                                                                         int* __restrict__ active) {
+  double v[2];
+  reduce_col<2>(part, blockIdx.x, nblk, v);
+  const int r = blockIdx.x;
+  if (threadIdx.x) return;
+  if (state[r] == 0) {
+    active[r] = 0;
+    return;
+  }
+  sc[S_RZ * 64 + r] = v[0];
+  sc[S_RR * 64 + r] = v[1];
+  active[r] = v[0] > 0.0 ? 1 : 0;
+}
+
 // beta = rz_new / rz; convergence: ||r||^2 <= tol^2 ||b||^2 stops a right-hand side.
 __global__ __launch_bounds__(kRedThreads) void pcg_beta_kernel(int nblk, const double* __restrict__ part,
                                                               double* __restrict__ sc, int* __restrict__ active,
@@ -629,6 +650,20 @@ __global__ __launch_bounds__(256) void axpy32_kernel(int64_t n, double a, const 
                                                      double* __restrict__ x64) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     x64[i] += a * (double)x32[i];
+}
+// x64 += coef[r] x32 per right-hand side (block residual replacement).
+__global__ __launch_bounds__(256) void axpy32k_kernel(int64_t units, int logk, const double* __restrict__ coef,
+                                                      const float* __restrict__ x32, double* __restrict__ x64) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
+    const double c = coef[(int)(u & ((1 << logk) - 1))];
+    if (c != 0.0) x64[u] += c * (double)x32[u];
+  }
+}
+// p = z for the right-hand sides whose recurrence restarts (state 2).
+__global__ __launch_bounds__(256) void copy_cols_kernel(int64_t units, int logk, const int* __restrict__ state,
+                                                        const float* __restrict__ z, float* __restrict__ p) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x)
+    if (state[(int)(u & ((1 << logk) - 1))] == 2) p[u] = z[u];
 }
 __global__ __launch_bounds__(kRedThreads) void sum_one_kernel(int nblk, const double* __restrict__ part,
                                                              double* __restrict__ out) {
@@ -1019,7 +1054,8 @@ int alloc_small_work(spde_ic* ic) {
       hipMalloc((void**)&ic->d_part, sizeof(double) * kRedBlocks * 64 * 2) != hipSuccess ||
       hipMalloc((void**)&ic->d_scal, sizeof(double) * 64 * 8) != hipSuccess ||
       hipMalloc((void**)&ic->d_active, sizeof(int) * 64) != hipSuccess ||
-      hipMalloc((void**)&ic->d_iters, sizeof(int) * 64) != hipSuccess) {
+      hipMalloc((void**)&ic->d_iters, sizeof(int) * 64) != hipSuccess ||
+      hipMalloc((void**)&ic->d_state, sizeof(int) * 64) != hipSuccess) {
     set_error("spde_ic: device allocation failed");
     return SPDE_ERR_ALLOC;
   }
@@ -1027,7 +1063,7 @@ int alloc_small_work(spde_ic* ic) {
 }
 void free_work(spde_ic* ic) {
   for (void* p : {(void*)ic->d_status, (void*)ic->d_part, (void*)ic->d_scal, (void*)ic->d_active,
-                  (void*)ic->d_iters, (void*)ic->d_work, (void*)ic->d_r64, (void*)ic->d_b32, (void*)ic->d_x32,
+                  (void*)ic->d_iters, (void*)ic->d_state, (void*)ic->d_work, (void*)ic->d_r64, (void*)ic->d_b32, (void*)ic->d_x32,
                   (void*)ic->d_rn, (void*)ic->d_lanczos, (void*)ic->d_klw})
     if (p) hipFree(p);
   for (hipGraphExec_t g : ic->it_graph)
@@ -1668,6 +1704,149 @@ int spde_pcg_replace(spde_ic* ic, const double* b, double* x, double tol, double
   if (iters) *iters = it;
   if (updates) *updates = upd;
   if (relres) *relres = bn > 0.0 ? std::sqrt(rn / bn) : 0.0;
+  return SPDE_OK;
+}
+
+int spde_pcg_replace_block(spde_ic* ic, const double* B, double* X, int k, double tol, double delta, int maxit,
+                           int32_t* iters, int32_t* updates, double* relres, void* stream) {
+  sync_view(ic);
+  const int logk = log2_of(k);
+  if (!ic || !ic->factored || !B || !X || logk < 0 || !(tol > 0.0) || !(delta > 0.0 && delta < 1.0) || maxit < 1) {
+    set_error("spde_pcg_replace_block: invalid argument (k a power of two <= 64, tol > 0, 0 < delta < 1)");
+    return SPDE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  int rc = ensure_work(ic);
+  if (rc) return rc;
+  spde_plan* pl = ic->pl;
+  const int64_t n = ic->n, units = n << logk;
+  const unsigned g = grid_for(units);
+  float* R = ic->d_work;
+  float* Z = R + n * 64;
+  float* P = Z + n * 64;
+  double* rn_dev = ic->d_rn;
+  double* coef_dev = ic->d_rn + 64;
+  // ||B - Q_c X||^2 per column into out (host)
+  auto residual = [&](double* out) -> int {
+    hipLaunchKernelGGL(resid64k_kernel, dim3(g), dim3(256), 0, s, n, logk, pl->d_indptr, pl->d_indices, ic->qval,
+                       ic->diag_add, B, X, ic->d_r64, ic->d_part);
+    hipLaunchKernelGGL(norms_kernel, dim3(k), dim3(kRedThreads), 0, s, (int)g, ic->d_part, rn_dev);
+    SPDE_HIP(hipMemcpyAsync(out, rn_dev, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    SPDE_HIP(hipStreamSynchronize(s));
+    return SPDE_OK;
+  };
+  auto axpy = [&](const double* coef) -> int {
+    SPDE_HIP(hipMemcpyAsync(coef_dev, coef, sizeof(double) * k, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(axpy32k_kernel, dim3(g), dim3(256), 0, s, units, logk, coef_dev, ic->d_x32, X);
+    SPDE_HIP(hipStreamSynchronize(s));  // coef lives on the host stack
+    return SPDE_OK;
+  };
+  // r32 = r64, z = M^-1 r32, partials of r^T z and r^T r
+  auto precondition = [&]() {
+    hipLaunchKernelGGL(cast_down_kernel, dim3(g), dim3(256), 0, s, units, ic->d_r64, R);
+    ic_apply(ic, R, Z, logk, s);
+    hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, R, Z, ic->d_part);
+  };
+  hipLaunchKernelGGL(fill64_kernel, dim3(g), dim3(256), 0, s, units, X);
+  double bn[64], cur[64], fresh[64], coef[64];
+  int state[64], stall[64], act[64];
+  bool done[64];
+  if ((rc = residual(bn))) return rc;
+  for (int r = 0; r < k; ++r) {
+    cur[r] = bn[r];
+    stall[r] = 0;
+    done[r] = !(bn[r] > 0.0);
+  }
+  int it = 0, upd = 0;
+  auto all_done = [&]() {
+    for (int r = 0; r < k; ++r)
+      if (!done[r]) return false;
+    return true;
+  };
+  if (!all_done()) {
+    hipLaunchKernelGGL(fill_kernel, dim3(g), dim3(256), 0, s, units, 0.f, ic->d_x32);
+    precondition();
+    hipLaunchKernelGGL(pcg_init_scalars_kernel, dim3(k), dim3(kRedThreads), 0, s, (int)g, ic->d_part, 0.0,
+                       ic->d_scal, ic->d_active, ic->d_iters);
+    hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, s, units, Z, P);
+    SPDE_HIP(hipGetLastError());
+    hipGraphExec_t ge = iteration_graph(ic, logk, g, s);
+    double seg[64], rr[64];
+    for (int r = 0; r < k; ++r) seg[r] = bn[r];
+    int seg_it = 0;
+    while (it < maxit) {
+      const int before = it;
+      if (ge && maxit - it >= kPoll) {
+        SPDE_HIP(hipGraphLaunch(ge, s));
+        it += kPoll;
+      } else {
+        for (int j = 0; j < kPoll && it < maxit; ++j, ++it) enqueue_iteration(ic, logk, g, ic->d_x32, s);
+      }
+      seg_it += it - before;
+      SPDE_HIP(hipMemcpyAsync(act, ic->d_active, sizeof(int) * k, hipMemcpyDeviceToHost, s));
+      SPDE_HIP(hipMemcpyAsync(rr, ic->d_scal + S_RR * 64, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+      SPDE_HIP(hipStreamSynchronize(s));
+      bool ready = seg_it >= kSegMax || it >= maxit;
+      if (!ready) {
+        ready = true;  // every live column's recursive residual has fallen by delta (or it broke down)
+        for (int r = 0; r < k; ++r)
+          if (!done[r] && act[r] && rr[r] > delta * delta * seg[r]) ready = false;
+      }
+      if (!ready) continue;
+      // replacement for every column: the segment's FP32 iterate into X, the FP64 residual into the recurrence
+      for (int r = 0; r < k; ++r) coef[r] = 1.0;
+      if ((rc = axpy(coef))) return rc;
+      if ((rc = residual(fresh))) return rc;
+      ++upd;
+      seg_it = 0;
+      bool back = false;
+      for (int r = 0; r < k; ++r) {
+        coef[r] = 0.0;
+        if (done[r]) {
+          state[r] = 0;
+        } else if (!(fresh[r] <= cur[r])) {
+          coef[r] = -1.0;  // the segment made this column's residual worse: take it back, restart the column
+          back = true;
+          state[r] = 2;
+          if (++stall[r] >= 3) done[r] = true, state[r] = 0;
+        } else {
+          stall[r] = fresh[r] > 0.81 * cur[r] ? stall[r] + 1 : 0;
+          cur[r] = fresh[r];
+          if (cur[r] <= tol * tol * bn[r] || stall[r] >= 3) {
+            done[r] = true;
+            state[r] = 0;
+          } else {
+            state[r] = act[r] && stall[r] == 0 ? 1 : 2;
+          }
+        }
+      }
+      if (back) {
+        if ((rc = axpy(coef))) return rc;
+        if ((rc = residual(fresh))) return rc;
+        for (int r = 0; r < k; ++r)
+          if (coef[r] != 0.0) cur[r] = fresh[r];
+      }
+      hipLaunchKernelGGL(fill_kernel, dim3(g), dim3(256), 0, s, units, 0.f, ic->d_x32);
+      if (all_done() || it >= maxit) break;
+      SPDE_HIP(hipMemcpyAsync(ic->d_state, state, sizeof(int) * k, hipMemcpyHostToDevice, s));
+      precondition();
+      hipLaunchKernelGGL(pcg_replace_scalars_kernel, dim3(k), dim3(kRedThreads), 0, s, (int)g, ic->d_part,
+                         ic->d_state, ic->d_scal, ic->d_active);
+      hipLaunchKernelGGL(copy_cols_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_state, Z, P);
+      SPDE_HIP(hipStreamSynchronize(s));  // state lives on the host stack
+      for (int r = 0; r < k; ++r) seg[r] = cur[r];
+    }
+    if (iters) {
+      SPDE_HIP(hipMemcpyAsync(iters, ic->d_iters, sizeof(int) * k, hipMemcpyDeviceToHost, s));
+      SPDE_HIP(hipStreamSynchronize(s));
+    }
+  } else if (iters) {
+    for (int r = 0; r < k; ++r) iters[r] = 0;
+  }
+  SPDE_HIP(hipGetLastError());
+  if (updates) *updates = upd;
+  if (relres)
+    for (int r = 0; r < k; ++r) relres[r] = bn[r] > 0.0 ? std::sqrt(cur[r] / bn[r]) : 0.0;
   return SPDE_OK;
 }
 

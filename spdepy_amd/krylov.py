@@ -188,6 +188,22 @@ class IterativeSolver:
                                              _stream(torch)))
         return x, dict(iters=int(it.value), updates=int(upd.value), relres=float(rel.value))
 
+    def replace_block(self, B, tol=1e-10, delta=0.1, maxit=10000):
+        """FP64 X = Q_c^-1 B for k right-hand sides (as refine_block) by the
+        FP32 block PCG with FP64 residual replacement; info holds per-column
+        iterations, ||b - Q_c x|| / ||b|| and the number of replacements."""
+        torch = _torch()
+        B = B.contiguous()
+        k = self._k(B)
+        X = torch.empty_like(B)
+        its = np.zeros(k, np.int32)
+        rel = np.zeros(k)
+        upd = ctypes.c_int32()
+        nat.check(nat.lib().spde_pcg_replace_block(self._h, _ptr(B), _ptr(X), k, float(tol), float(delta), int(maxit),
+                                                   its.ctypes.data, ctypes.byref(upd), rel.ctypes.data,
+                                                   _stream(torch)))
+        return X, dict(iters=its, updates=int(upd.value), relres=rel)
+
     def refine_block(self, B, tol=1e-10, inner_tol=1e-5, max_rounds=10, maxit=5000):
         """FP64 X = Q_c^-1 B for k right-hand sides (B: n x k float64 device,
         row-major; k a power of two <= 64) by iterative refinement around the
@@ -269,11 +285,13 @@ def conditional_rhs(engine, coeffs, obs_cols, y, log_tau, k, seed):
 
 
 def conditional_samples_pcg(engine, coeffs, obs_cols, y, log_tau, k=64, seed=0, tol=1e-10, shift=0.1,
-                            correction=None, inner_tol=1e-5):
+                            correction=None, inner_tol=1e-5, solver="replace", delta=0.1):
     """k draws of x | y (n x k float64 device, row-major) without a factor:
-    Q_c^-1 B for the right-hand sides of conditional_rhs, solved by FP64
-    iterative refinement around FP32 PCG with IC(0) (or its learned
-    correction) to ||B - Q_c X|| <= tol ||B|| per column; returns (X, info)."""
+    Q_c^-1 B for the right-hand sides of conditional_rhs, solved by the FP32
+    block PCG with IC(0) (or its learned correction) and FP64 residual
+    replacement (solver="replace", every `delta` of residual fall) or FP64
+    refinement rounds of restarted solves to `inner_tol` (solver="refine"),
+    to ||B - Q_c X|| <= tol ||B|| per column; returns (X, info)."""
     torch = _torch()
     from .engine import _obs_device
     it = engine.iterative()
@@ -285,7 +303,12 @@ def conditional_samples_pcg(engine, coeffs, obs_cols, y, log_tau, k=64, seed=0, 
     if correction is not None:
         it.correct(correction(engine, coeffs, qval, diag_add))
     B = conditional_rhs(engine, coeffs, obs_cols, yy, log_tau, k, seed)
-    X, info = it.refine_block(B, tol=tol, inner_tol=inner_tol)
+    if solver == "replace":
+        X, info = it.replace_block(B, tol=tol, delta=delta)
+    elif solver == "refine":
+        X, info = it.refine_block(B, tol=tol, inner_tol=inner_tol)
+    else:
+        raise ValueError("solver must be 'replace' or 'refine'")
     info["B"] = B
     return X, info
 

@@ -68,16 +68,19 @@ def test_iterative_path_against_cholesky_at_200x200x24():
     print("  mu_c: restarted rounds %s, relative error vs Cholesky %.2e" % (p_rf["pcg"]["mu"], err_rf))
     assert p_rf["pcg"]["mu"][0]["relres"] <= 1e-10 and err_rf <= 1e-6
     assert abs(ld_err) <= 4 * p_it["logdet_qc_se"] + 1e-9 * abs(p_ex["ld_qc"])
-    step("64 conditional samples by refined PCG")
-    X, info = conditional_samples_pcg(eng, coeffs, cols, data, par[-1], k=64, seed=2)
-    print("  rounds %d, inner iterations per sample %d..%d, relres max %.2e" % (
-        info["rounds"], info["iters"].min(), info["iters"].max(), info["relres"].max()))
-    assert info["relres"].max() <= 1e-10
-    step("the same 64 right-hand sides by the Cholesky factor")
-    Xc = eng.plan.solve(1, info["B"])
-    err = (torch.linalg.norm(X - Xc, dim=0) / torch.linalg.norm(Xc, dim=0)).cpu().numpy()
-    print("  samples: relative error vs Cholesky max %.2e median %.2e" % (err.max(), np.median(err)))
-    assert err.max() <= 1e-6
+    for solver in ("replace", "refine"):
+        step("64 conditional samples by %s PCG" % solver)
+        X, info = conditional_samples_pcg(eng, coeffs, cols, data, par[-1], k=64, seed=2, solver=solver)
+        print("  %s, iterations per sample %d..%d, relres max %.2e" % (
+            "%d replacements" % info["updates"] if solver == "replace" else "%d rounds" % info["rounds"],
+            info["iters"].min(), info["iters"].max(), info["relres"].max()))
+        assert info["relres"].max() <= 1e-10
+        step("the same 64 right-hand sides by the Cholesky factor")
+        Xc = eng.plan.solve(1, info["B"])
+        err = (torch.linalg.norm(X - Xc, dim=0) / torch.linalg.norm(Xc, dim=0)).cpu().numpy()
+        print("  samples: relative error vs Cholesky max %.2e median %.2e" % (err.max(), np.median(err)))
+        assert err.max() <= 1e-6
+        del X, Xc
 
 
 def test_exact_path_with_gradient_at_200x200x24():

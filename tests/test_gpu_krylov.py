@@ -311,16 +311,20 @@ def test_fform_square_root_matches_oracle(bc):
     assert np.allclose(R @ R.T, Qd, rtol=1e-12, atol=1e-12 * np.abs(Qd).max())
 
 
-def test_conditional_samples_by_refined_pcg(case):
-    """Q_c^-1 B for the conditional right-hand sides by FP64 refinement of
-    the block PCG: every column within 1e-10 relative residual, equal to the
-    sparse direct solve of the same B; E[B] = tau S^T y checked through the
-    draws' mean against mu_c."""
+@pytest.mark.parametrize("solver", ["replace", "refine"])
+def test_conditional_samples_by_refined_pcg(case, solver):
+    """Q_c^-1 B for the conditional right-hand sides by the block PCG with
+    FP64 residual replacement, and by FP64 refinement rounds: every column
+    within 1e-10 relative residual, equal to the sparse direct solve of the
+    same B; E[B] = tau S^T y checked through the draws' mean against mu_c."""
     from spdepy_amd.krylov import conditional_samples_pcg
     mod, par, eng = case["mod"], case["c"]["par"], case["eng"]
-    step("conditional samples by refined PCG 12x10x6 (k=16)")
-    X, info = conditional_samples_pcg(eng, mod._coeffs(par), mod._obs_cols(), mod.data, par[-1], k=16, seed=4)
-    print("  rounds %d, iterations %s, relres max %.2e" % (info["rounds"], info["iters"], info["relres"].max()))
+    step("conditional samples by %s PCG 12x10x6 (k=16)" % solver)
+    X, info = conditional_samples_pcg(eng, mod._coeffs(par), mod._obs_cols(), mod.data, par[-1], k=16, seed=4,
+                                      solver=solver)
+    print("  %s, iterations %s, relres max %.2e" % (
+        "%d replacements" % info["updates"] if solver == "replace" else "%d rounds" % info["rounds"],
+        info["iters"], info["relres"].max()))
     assert info["relres"].max() <= 1e-10
     Xh, Bh = X.cpu().numpy(), info["B"].cpu().numpy()
     A = case["A"].tocsc()
