@@ -365,15 +365,18 @@ __global__ __launch_bounds__(256) void spmm1_kernel(int64_t n, const int64_t* __
   if (part) block_reduce_store<1>(acc, 1, part);
 }
 
-// Sixty-four right-hand sides: a wave per row, lanes = right-hand sides.  The
-// row index is made wave-uniform (readfirstlane), so the row's pattern and
-// values come through the scalar unit and each neighbour costs one contiguous
-// 256-byte vector load; eight neighbours' loads are issued before their sums.
-__global__ __launch_bounds__(256) void spmm64_kernel(int64_t n, const int64_t* __restrict__ indptr,
-                                                     const int32_t* __restrict__ indices,
-                                                     const float* __restrict__ a, const float* __restrict__ x,
-                                                     float* __restrict__ y, double* __restrict__ part) {
+// K = 16, 32 or 64 right-hand sides: a wave per row, lane r < K = right-hand
+// side r (the other lanes idle).  The row index is made wave-uniform
+// (readfirstlane), so the row's pattern and values come through the scalar
+// unit and each neighbour costs one contiguous 4K-byte vector load; eight
+// (SpMM) or four (sweeps) neighbours' loads are issued before their sums.
+template <int K>
+__global__ __launch_bounds__(256) void spmmw_kernel(int64_t n, const int64_t* __restrict__ indptr,
+                                                    const int32_t* __restrict__ indices,
+                                                    const float* __restrict__ a, const float* __restrict__ x,
+                                                    float* __restrict__ y, double* __restrict__ part) {
   const int lane = threadIdx.x & 63;
+  const bool on = lane < K;
   const int wpb = blockDim.x >> 6;
   double acc[1] = {0.0};
   for (int64_t row = (int64_t)blockIdx.x * wpb + (threadIdx.x >> 6); row < n; row += (int64_t)gridDim.x * wpb) {
@@ -384,61 +387,65 @@ __global__ __launch_bounds__(256) void spmm64_kernel(int64_t n, const int64_t* _
     for (; e + 8 <= e1; e += 8) {
       float v[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) v[q] = x[((int64_t)indices[e + q] << 6) + lane];
+      for (int q = 0; q < 8; ++q) v[q] = on ? x[(int64_t)indices[e + q] * K + lane] : 0.f;
 #pragma unroll
       for (int q = 0; q < 8; ++q) s += a[e + q] * v[q];
     }
-    for (; e < e1; ++e) s += a[e] * x[((int64_t)indices[e] << 6) + lane];
-    const int64_t u = ((int64_t)i << 6) + lane;
-    y[u] = s;
-    acc[0] += (double)s * (double)x[u];
+    for (; e < e1; ++e) s += a[e] * (on ? x[(int64_t)indices[e] * K + lane] : 0.f);
+    if (on) {
+      const int64_t u = (int64_t)i * K + lane;
+      y[u] = s;
+      acc[0] += (double)s * (double)x[u];
+    }
   }
-  if (part) block_reduce_store<1>(acc, 64, part);
+  if (part) block_reduce_store<1>(acc, K, part);
 }
-// The colour sweeps with sixty-four right-hand sides, a wave per row as above.
-__global__ __launch_bounds__(256) void fwd64_kernel(int64_t r0, int64_t r1, const int32_t* __restrict__ crow,
-                                                    const int64_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
-                                                    const float* __restrict__ l32, const float* __restrict__ dinv,
-                                                    const float* r, float* z, int fold) {
+// The colour sweeps with K right-hand sides, a wave per row as above.
+template <int K>
+__global__ __launch_bounds__(256) void fwdw_kernel(int64_t r0, int64_t r1, const int32_t* __restrict__ crow,
+                                                   const int64_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
+                                                   const float* __restrict__ l32, const float* __restrict__ dinv,
+                                                   const float* r, float* z, int fold) {
   const int lane = threadIdx.x & 63;
   const int64_t rw = r0 + (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (rw >= r1) return;  // uniform over the wave
+  if (rw >= r1 || lane >= K) return;  // the row is uniform over the wave
   const int i = crow[__builtin_amdgcn_readfirstlane((int)rw)];
-  float s = r[((int64_t)i << 6) + lane];
+  float s = r[(int64_t)i * K + lane];
   const int64_t p1 = lptr[i + 1];
   int64_t p = lptr[i];
   for (; p + 4 <= p1; p += 4) {
     float v[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = z[((int64_t)lcol[p + q] << 6) + lane];
+    for (int q = 0; q < 4; ++q) v[q] = z[(int64_t)lcol[p + q] * K + lane];
 #pragma unroll
     for (int q = 0; q < 4; ++q) s -= l32[p + q] * v[q];
   }
-  for (; p < p1; ++p) s -= l32[p] * z[((int64_t)lcol[p] << 6) + lane];
+  for (; p < p1; ++p) s -= l32[p] * z[(int64_t)lcol[p] * K + lane];
   float v = s * dinv[i];
   if (fold) v *= dinv[i];
-  z[((int64_t)i << 6) + lane] = v;
+  z[(int64_t)i * K + lane] = v;
 }
-__global__ __launch_bounds__(256) void bwd64_kernel(int64_t r0, int64_t r1, const int32_t* __restrict__ crow,
-                                                    const int64_t* __restrict__ uptr, const int32_t* __restrict__ ucol,
-                                                    const float* __restrict__ u32, const float* __restrict__ dinv,
-                                                    const float* y, float* x) {
+template <int K>
+__global__ __launch_bounds__(256) void bwdw_kernel(int64_t r0, int64_t r1, const int32_t* __restrict__ crow,
+                                                   const int64_t* __restrict__ uptr, const int32_t* __restrict__ ucol,
+                                                   const float* __restrict__ u32, const float* __restrict__ dinv,
+                                                   const float* y, float* x) {
   const int lane = threadIdx.x & 63;
   const int64_t rw = r0 + (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (rw >= r1) return;
+  if (rw >= r1 || lane >= K) return;
   const int i = crow[__builtin_amdgcn_readfirstlane((int)rw)];
-  float s = y[((int64_t)i << 6) + lane];
+  float s = y[(int64_t)i * K + lane];
   const int64_t p1 = uptr[i + 1];
   int64_t p = uptr[i];
   for (; p + 4 <= p1; p += 4) {
     float v[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = x[((int64_t)ucol[p + q] << 6) + lane];
+    for (int q = 0; q < 4; ++q) v[q] = x[(int64_t)ucol[p + q] * K + lane];
 #pragma unroll
     for (int q = 0; q < 4; ++q) s -= u32[p + q] * v[q];
   }
-  for (; p < p1; ++p) s -= u32[p] * x[((int64_t)ucol[p] << 6) + lane];
-  x[((int64_t)i << 6) + lane] = s * dinv[i];
+  for (; p < p1; ++p) s -= u32[p] * x[(int64_t)ucol[p] * K + lane];
+  x[(int64_t)i * K + lane] = s * dinv[i];
 }
 
 // ------------------------------------------------------------ PCG steps
@@ -1000,9 +1007,11 @@ int ic_fwd(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s, bool 
     if (logk == 0)
       hipLaunchKernelGGL(fwd1_kernel, dim3(blocks_of((r1 - r0) * kRowLanes)), dim3(256), 0, s, r0, r1, ic->d_crow,
                          ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dinv, r, z, f);
-    else if (logk == 6)
-      hipLaunchKernelGGL(fwd64_kernel, dim3(blocks_of((r1 - r0) << 6)), dim3(256), 0, s, r0, r1, ic->d_crow,
-                         ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dinv, r, z, f);
+    else if (logk >= 4) {
+      auto kern = logk == 6 ? fwdw_kernel<64> : logk == 5 ? fwdw_kernel<32> : fwdw_kernel<16>;
+      hipLaunchKernelGGL(kern, dim3(blocks_of((r1 - r0) << 6)), dim3(256), 0, s, r0, r1, ic->d_crow, ic->d_lptr,
+                         ic->d_lcol, ic->d_l32, ic->d_dinv, r, z, f);
+    }
     else
       hipLaunchKernelGGL(fwd_kernel, dim3(blocks_of((r1 - r0) << logk)), dim3(256), 0, s, r0, r1, logk, ic->d_crow,
                          ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dinv, r, z, f);
@@ -1019,9 +1028,11 @@ int ic_bwd(spde_ic* ic, const float* y, float* x, int logk, hipStream_t s, bool 
     if (logk == 0)
       hipLaunchKernelGGL(bwd1_kernel, dim3(blocks_of((r1 - r0) * kRowLanes)), dim3(256), 0, s, r0, r1, ic->d_crow,
                          ic->d_uptr, ic->d_ucol, ic->d_u32, ic->d_dinv, y, x);
-    else if (logk == 6)
-      hipLaunchKernelGGL(bwd64_kernel, dim3(blocks_of((r1 - r0) << 6)), dim3(256), 0, s, r0, r1, ic->d_crow,
-                         ic->d_uptr, ic->d_ucol, ic->d_u32, ic->d_dinv, y, x);
+    else if (logk >= 4) {
+      auto kern = logk == 6 ? bwdw_kernel<64> : logk == 5 ? bwdw_kernel<32> : bwdw_kernel<16>;
+      hipLaunchKernelGGL(kern, dim3(blocks_of((r1 - r0) << 6)), dim3(256), 0, s, r0, r1, ic->d_crow, ic->d_uptr,
+                         ic->d_ucol, ic->d_u32, ic->d_dinv, y, x);
+    }
     else
       hipLaunchKernelGGL(bwd_kernel, dim3(blocks_of((r1 - r0) << logk)), dim3(256), 0, s, r0, r1, logk, ic->d_crow,
                          ic->d_uptr, ic->d_ucol, ic->d_u32, ic->d_dinv, y, x);
@@ -1040,9 +1051,9 @@ void launch_spmm(spde_ic* ic, int logk, unsigned g, const float* x, float* y, do
   if (logk == 0)
     hipLaunchKernelGGL(spmm1_kernel, dim3(g), dim3(256), 0, s, ic->n, ic->pl->d_indptr, ic->pl->d_indices,
                        ic->d_qc32, x, y, part);
-  else if (logk == 6)
-    hipLaunchKernelGGL(spmm64_kernel, dim3(g), dim3(256), 0, s, ic->n, ic->pl->d_indptr, ic->pl->d_indices,
-                       ic->d_qc32, x, y, part);
+  else if (logk >= 4)
+    hipLaunchKernelGGL(logk == 6 ? spmmw_kernel<64> : logk == 5 ? spmmw_kernel<32> : spmmw_kernel<16>, dim3(g),
+                       dim3(256), 0, s, ic->n, ic->pl->d_indptr, ic->pl->d_indices, ic->d_qc32, x, y, part);
   else
     hipLaunchKernelGGL(spmm32_kernel, dim3(g), dim3(256), 0, s, ic->n, logk, ic->pl->d_indptr, ic->pl->d_indices,
                        ic->d_qc32, x, y, part);
