@@ -255,7 +255,7 @@ def batch_models(nmodels=64, seed=0):
     return models, pars
 
 
-def batch_route(nmodels=64, nrhs=256, nstreams=8):
+def batch_route(nmodels=64, nrhs=256, nstreams=8, chunk=256):
     """Models per second of exact FP64 log-likelihood + gradient and nrhs
     conditional samples per model: one after another (logLike, then the
     samples) against the batch (spde_loglik_batch over a pool of streams)."""
@@ -263,7 +263,7 @@ def batch_route(nmodels=64, nrhs=256, nstreams=8):
     from spdepy_amd.batch import BatchEvaluator
     t = time.perf_counter()
     models, pars = batch_models(nmodels)
-    ev = BatchEvaluator(models, nstreams=nstreams)
+    ev = BatchEvaluator(models, nstreams=nstreams, chunk=chunk)
     t_setup = time.perf_counter() - t
     ev.loglik(pars)                                   # untimed: every plan's buffers
     for i in range(nmodels):

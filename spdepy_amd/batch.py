@@ -27,10 +27,12 @@ class BatchEvaluator:
     """models: drop-in space-time models (stationary or B-spline advection-
     diffusion) whose data are set (initFit); leaf: optional per-model
     supernode leaf sizes (the tuner's choice); nstreams: HIP streams the
-    batch spreads over."""
+    batch spreads over; chunk: right-hand sides per solve of the conditional
+    samples (each plan keeps front vectors for that many)."""
 
-    def __init__(self, models, nstreams=8, leaf=None):
+    def __init__(self, models, nstreams=8, leaf=None, chunk=256):
         torch = _torch()
+        self.chunk = int(chunk)
         self.models = list(models)
         for i, m in enumerate(self.models):
             if leaf is not None and leaf[i] is not None:
@@ -99,11 +101,11 @@ class BatchEvaluator:
         plan = m._engine().plan
         n = plan.n
         Z = gaussian(n, nsamples, seed)
-        # right-hand sides in chunks of at most 64: the solve's front vectors
-        # scale with the chunk, and every plan of a batch keeps its own
+        # right-hand sides in chunks of at most self.chunk: the solve's front
+        # vectors scale with the chunk, and every plan of a batch keeps its own
         X = torch.empty_like(Z)
-        for c0 in range(0, nsamples, 64):
-            c1 = min(nsamples, c0 + 64)
+        for c0 in range(0, nsamples, self.chunk):
+            c1 = min(nsamples, c0 + self.chunk)
             X[:, c0:c1] = plan.solve(1, Z[:, c0:c1].contiguous(), mode=nat.SOLVE_LT)
         mu = torch.empty(n * self.r, dtype=torch.float64, device="cuda")
         nat.check(nat.lib().spde_loglik_mu(plan._h, _ptr(mu), self.r, ctypes.c_void_p(
