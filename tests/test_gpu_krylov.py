@@ -149,6 +149,29 @@ def test_residual_replacement_reaches_fp64(case, delta):
     assert np.allclose(xh, xd, rtol=1e-7, atol=1e-8 * np.abs(xd).max())
 
 
+def test_residual_replacement_edge_cases(case):
+    """A zero right-hand side returns zero with no iteration; a zero column
+    of a block stops at once while the others converge; an iteration cap
+    returns the residual reached instead of failing."""
+    import torch
+    eng = case["eng"]
+    it = eng.iterative()
+    it.factor(case["qval"], case["dadd"], shift=0.1)
+    step("residual replacement: zero right-hand side, zero column, iteration cap")
+    x, info = it.replace(torch.zeros(eng.n, dtype=torch.float64, device="cuda"))
+    assert info["iters"] == 0 and info["relres"] == 0.0 and float(x.abs().max()) == 0.0
+    B = torch.randn(eng.n, 4, dtype=torch.float64, device="cuda")
+    B[:, 2] = 0.0
+    X, info = it.replace_block(B, tol=1e-10)
+    print("  block with a zero column:", info["iters"], info["relres"])
+    assert info["iters"][2] == 0 and info["relres"][2] == 0.0 and float(X[:, 2].abs().max()) == 0.0
+    assert np.all(np.delete(info["relres"], 2) <= 1e-10)
+    b = torch.randn(eng.n, dtype=torch.float64, device="cuda")
+    x, info = it.replace(b, tol=1e-10, maxit=8)
+    print("  capped at 8 iterations:", info)
+    assert info["iters"] == 8 and 0.0 < info["relres"] <= 1.0
+
+
 def test_view_solves_beside_its_parent(case):
     """spde_ic_view: a second handle on the factor with its own work buffers
     gives the parent's results bit for bit, also while both solve at once on
