@@ -172,10 +172,11 @@ def other_routes(mod, par, like_ex, jac_ex, steps=3):
                      "pcg_iterations": {"mu_refinement": parts["pcg"]["mu"][0],
                                         "probes_mean": float(np.mean(parts["pcg"]["probes"]))},
                      "ic_shift": parts["ic_shift"],
-                     "estimators": "log|Q_c|: 32 Rademacher probes x 50 Lanczos steps on L^-1 Q_c L^-T + "
-                                   "log|L L^T|; gradient trace: 64-probe Hutchinson from FP32 PCG (tol 1e-3); "
-                                   "mu_c: FP32 PCG with FP64 residual replacement to 1e-10, on a second "
-                                   "stream beside the probes"}
+                     "estimators": "64 probes v = L w (w Rademacher) solved by FP32 PCG to 1e-3: their PCG "
+                                   "coefficients give 64 Lanczos tridiagonals of L^-1 Q_c L^-T (50 steps) for "
+                                   "log|Q_c| - log|L L^T|, and u = Q_c^-1 v with y = L^-T w the trace estimate "
+                                   "(E[u y^T] = Q_c^-1); mu_c: FP32 PCG with FP64 residual replacement to "
+                                   "1e-10, on a second stream beside the probes"}
     if "evals_per_s" in out.get("learned_pcg", {}):
         lp = out["learned_pcg"]
         lp["meets_1e-6"] = bool(lp["rel_se_like"] <= 1e-6)

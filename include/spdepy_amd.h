@@ -443,6 +443,17 @@ int spde_ic_spmm(spde_ic* ic, const float* x, float* y, int k, void* stream);
  * ||r|| <= tol ||b|| or maxit iterations; iters[k] (host) per right-hand side. */
 int spde_pcg_solve(spde_ic* ic, const float* B, float* X, int k, double tol, int maxit, int32_t* iters,
                    void* stream);
+/* spde_pcg_solve that also returns each right-hand side's first cap step
+ * coefficients (host, alpha[r * cap + j], beta[r * cap + j], zero past its
+ * iterations): with M = L L^T they give the Lanczos tridiagonal of
+ * L^-1 Q_c L^-T started at L^-1 b / ||L^-1 b|| (diagonal 1/alpha_1,
+ * 1/alpha_j + beta_(j-1)/alpha_(j-1); off-diagonal sqrt(beta_j)/alpha_j), so
+ * the solves of b = L w for Rademacher w give stochastic Lanczos quadrature
+ * of log|Q_c| - log|M| on the probes w at no extra cost. */
+int spde_pcg_solve_coeffs(spde_ic* ic, const float* B, float* X, int k, double tol, int maxit, int32_t* iters,
+                          double* alpha, double* beta, int cap, void* stream);
+/* Y = L W for the current FP32 factor (k right-hand sides, row-major n x k). */
+int spde_ic_lmul(spde_ic* ic, const float* W, float* Y, int k, void* stream);
 /* FP64 x = Q_c^-1 b (one right-hand side, device) by iterative refinement:
  * r = b - Q_c x in FP64, an FP32 PCG solve of Q_c d = r / ||r|| to inner_tol,
  * x += ||r|| d, until ||r|| <= tol ||b|| or max_rounds; *iters total inner

@@ -127,6 +127,9 @@ _SIGS = {
                                         c_dp, vp]),
     "spde_pcg_replace_block": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                               ctypes.c_int, vp, c_i32p, vp, vp]),
+    "spde_pcg_solve_coeffs": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, vp, vp, vp,
+                                             ctypes.c_int, vp]),
+    "spde_ic_lmul": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, vp]),
     "spde_rademacher": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_uint64, ctypes.c_float, vp, vp]),
     "spde_pcg_refine_block": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]),

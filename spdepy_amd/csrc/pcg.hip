@@ -74,6 +74,8 @@ struct IcWork {
   int64_t lanczos_cap = 0;
   double* d_klw = nullptr;           // L'^T x of the KL loss (n x K)
   int64_t kl_cap = 0;
+  double* d_hist = nullptr;          // PCG alpha, beta per iteration and right-hand side [2 cap 64]
+  int hist_cap = 0;
   hipGraphExec_t it_graph[7] = {};   // PCG iterations on the refinement buffers, per log2 of the rhs count
 };
 struct spde_ic : IcShared, IcHost, IcWork {
@@ -592,9 +594,13 @@ __global__ __launch_bounds__(kRedThreads) void pcg_replace_scalars_kernel(int nb
 }
 
 // beta = rz_new / rz; convergence: ||r||^2 <= tol^2 ||b||^2 stops a right-hand side.
+// hist (optional, cap iterations): the step's alpha and beta per right-hand
+// side at hist[(2 j) 64 + r], hist[(2 j + 1) 64 + r] for its j-th iteration,
+// the coefficients that give the Lanczos tridiagonal of L^-1 Q_c L^-T.
 __global__ __launch_bounds__(kRedThreads) void pcg_beta_kernel(int nblk, const double* __restrict__ part,
                                                               double* __restrict__ sc, int* __restrict__ active,
-                                                              int* __restrict__ iters) {
+                                                              int* __restrict__ iters, double* __restrict__ hist,
+                                                              int cap) {
   double v[2];
   reduce_col<2>(part, blockIdx.x, nblk, v);
   const int r = blockIdx.x;
@@ -604,11 +610,17 @@ __global__ __launch_bounds__(kRedThreads) void pcg_beta_kernel(int nblk, const d
     sc[S_BETA * 64 + r] = 0.0;
     return;
   }
-  iters[r] += 1;
+  const int j = iters[r];
+  iters[r] = j + 1;
   sc[S_RR * 64 + r] = rr;
   const double old = sc[S_RZ * 64 + r];
-  sc[S_BETA * 64 + r] = old != 0.0 ? rz / old : 0.0;
+  const double beta = old != 0.0 ? rz / old : 0.0;
+  sc[S_BETA * 64 + r] = beta;
   sc[S_RZ * 64 + r] = rz;
+  if (hist && j < cap) {
+    hist[(int64_t)(2 * j) * 64 + r] = sc[S_ALPHA * 64 + r];
+    hist[(int64_t)(2 * j + 1) * 64 + r] = beta;
+  }
   if (rr <= sc[S_TOL2 * 64 + r] * sc[S_BN * 64 + r] || !(rz > 0.0)) active[r] = 0;
 }
 
@@ -664,6 +676,19 @@ __global__ __launch_bounds__(256) void axpy32k_kernel(int64_t units, int logk, c
   for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
     const double c = coef[(int)(u & ((1 << logk) - 1))];
     if (c != 0.0) x64[u] += c * (double)x32[u];
+  }
+}
+// Y = L W (row-major n x k, FP32): the diagonal of the factor and its lower entries.
+__global__ __launch_bounds__(256) void ic_lmul_kernel(int64_t units, int logk, const int64_t* __restrict__ lptr,
+                                                      const int32_t* __restrict__ lcol, const float* __restrict__ l32,
+                                                      const double* __restrict__ dg64, const float* __restrict__ w,
+                                                      float* __restrict__ y) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < units; u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = u >> logk;
+    const int r = (int)(u & ((1 << logk) - 1));
+    float s = (float)dg64[i] * w[u];
+    for (int64_t p = lptr[i]; p < lptr[i + 1]; ++p) s += l32[p] * w[((int64_t)lcol[p] << logk) + r];
+    y[u] = s;
   }
 }
 // p = z for the right-hand sides whose recurrence restarts (state 2).
@@ -1075,7 +1100,7 @@ int alloc_small_work(spde_ic* ic) {
 void free_work(spde_ic* ic) {
   for (void* p : {(void*)ic->d_status, (void*)ic->d_part, (void*)ic->d_scal, (void*)ic->d_active,
                   (void*)ic->d_iters, (void*)ic->d_state, (void*)ic->d_work, (void*)ic->d_r64, (void*)ic->d_b32, (void*)ic->d_x32,
-                  (void*)ic->d_rn, (void*)ic->d_lanczos, (void*)ic->d_klw})
+                  (void*)ic->d_rn, (void*)ic->d_lanczos, (void*)ic->d_klw, (void*)ic->d_hist})
     if (p) hipFree(p);
   for (hipGraphExec_t g : ic->it_graph)
     if (g) hipGraphExecDestroy(g);
@@ -1107,7 +1132,8 @@ constexpr int kPoll = 16;     // iterations between the host's looks at the acti
 constexpr int kSegMax = 512;  // residual replacement: at most this many iterations between replacements
 
 // One PCG iteration on the handle's work vectors (R, Z, P, Qv) and X.
-void enqueue_iteration(spde_ic* ic, int logk, unsigned g, float* X, hipStream_t s) {
+void enqueue_iteration(spde_ic* ic, int logk, unsigned g, float* X, hipStream_t s, double* hist = nullptr,
+                       int cap = 0) {
   const int k = 1 << logk;
   const int64_t n = ic->n, units = n << logk;
   float* R = ic->d_work;
@@ -1120,7 +1146,7 @@ void enqueue_iteration(spde_ic* ic, int logk, unsigned g, float* X, hipStream_t 
   ic_apply(ic, R, Z, logk, s);
   hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, R, Z, ic->d_part);
   hipLaunchKernelGGL(pcg_beta_kernel, dim3(k), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal, ic->d_active,
-                     ic->d_iters);
+                     ic->d_iters, hist, cap);
   hipLaunchKernelGGL(pcg_p_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, ic->d_active, Z, P);
 }
 
@@ -1154,7 +1180,7 @@ hipGraphExec_t iteration_graph(spde_ic* ic, int logk, unsigned g, hipStream_t s)
 // row-major n x k.  Host polls the active flags every kPoll iterations.
 // graph: iterate through iteration_graph (X must be the handle's d_x32).
 int pcg_run(spde_ic* ic, const float* B, float* X, int logk, double tol, int maxit, int32_t* iters, hipStream_t s,
-            bool graph = false) {
+            bool graph = false, double* hist = nullptr, int cap = 0) {
   const int k = 1 << logk;
   const int64_t n = ic->n, units = n << logk;
   int rc = ensure_work(ic);
@@ -1171,14 +1197,14 @@ int pcg_run(spde_ic* ic, const float* B, float* X, int logk, double tol, int max
                      ic->d_active, ic->d_iters);
   hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, s, units, Z, P);
   SPDE_HIP(hipGetLastError());
-  hipGraphExec_t ge = graph && X == ic->d_x32 ? iteration_graph(ic, logk, g, s) : nullptr;
+  hipGraphExec_t ge = graph && !hist && X == ic->d_x32 ? iteration_graph(ic, logk, g, s) : nullptr;
   std::vector<int> act(k);
   for (int it = 0; it < maxit;) {
     if (ge && maxit - it >= kPoll) {
       SPDE_HIP(hipGraphLaunch(ge, s));
       it += kPoll;
     } else {
-      enqueue_iteration(ic, logk, g, X, s);
+      enqueue_iteration(ic, logk, g, X, s, hist, cap);
       ++it;
     }
     if (it % kPoll == 0 || it == maxit) {
@@ -1567,6 +1593,51 @@ int spde_pcg_solve(spde_ic* ic, const float* B, float* X, int k, double tol, int
     return SPDE_ERR_ARG;
   }
   return pcg_run(ic, B, X, logk, tol, maxit, iters, (hipStream_t)stream);
+}
+
+int spde_pcg_solve_coeffs(spde_ic* ic, const float* B, float* X, int k, double tol, int maxit, int32_t* iters,
+                          double* alpha, double* beta, int cap, void* stream) {
+  sync_view(ic);
+  const int logk = log2_of(k);
+  if (!ic || !ic->factored || !B || !X || logk < 0 || !(tol > 0.0) || maxit < 1 || cap < 1 || !alpha || !beta ||
+      !iters) {
+    set_error("spde_pcg_solve_coeffs: invalid argument (k a power of two <= 64, cap >= 1, after spde_ic_factor)");
+    return SPDE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (ic->hist_cap < cap) {
+    if (ic->d_hist) SPDE_HIP(hipFree(ic->d_hist));
+    ic->d_hist = nullptr;
+    ic->hist_cap = 0;
+    SPDE_HIP(hipMalloc((void**)&ic->d_hist, sizeof(double) * 2 * 64 * cap));
+    ic->hist_cap = cap;
+  }
+  int rc = pcg_run(ic, B, X, logk, tol, maxit, iters, s, false, ic->d_hist, cap);
+  if (rc) return rc;
+  std::vector<double> h((size_t)2 * 64 * cap);
+  SPDE_HIP(hipMemcpyAsync(h.data(), ic->d_hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+  SPDE_HIP(hipStreamSynchronize(s));
+  for (int r = 0; r < k; ++r)
+    for (int j = 0; j < cap; ++j) {
+      const bool ok = j < iters[r];
+      alpha[(int64_t)r * cap + j] = ok ? h[(size_t)(2 * j) * 64 + r] : 0.0;
+      beta[(int64_t)r * cap + j] = ok ? h[(size_t)(2 * j + 1) * 64 + r] : 0.0;
+    }
+  return SPDE_OK;
+}
+
+int spde_ic_lmul(spde_ic* ic, const float* W, float* Y, int k, void* stream) {
+  sync_view(ic);
+  const int logk = log2_of(k);
+  if (!ic || !ic->factored || !W || !Y || logk < 0) {
+    set_error("spde_ic_lmul: invalid argument (k a power of two <= 64, after spde_ic_factor)");
+    return SPDE_ERR_ARG;
+  }
+  const int64_t units = ic->n << logk;
+  hipLaunchKernelGGL(ic_lmul_kernel, dim3(grid_for(units)), dim3(256), 0, (hipStream_t)stream, units, logk,
+                     ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dg64, W, Y);
+  SPDE_HIP(hipGetLastError());
+  return SPDE_OK;
 }
 
 int spde_pcg_refine(spde_ic* ic, const double* b, double* x, double tol, double inner_tol, int max_rounds, int maxit,

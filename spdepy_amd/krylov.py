@@ -30,6 +30,29 @@ def _torch():
     return torch
 
 
+def cg_lanczos(alpha, beta, iters):
+    """The Lanczos tridiagonals (diagonal, off-diagonal; k x cap, as
+    slq_quadrature takes them) of L^-1 A L^-T from each column's PCG step
+    coefficients alpha_j, beta_j (Saad, Iterative Methods for Sparse Linear
+    Systems, 2nd ed., sec. 6.7.3): d_1 = 1/alpha_1, d_j = 1/alpha_j +
+    beta_(j-1)/alpha_(j-1), e_j = sqrt(beta_j)/alpha_j; a column stops at
+    its last iteration (its off-diagonal there is zero)."""
+    alpha = np.asarray(alpha, dtype=np.float64)
+    beta = np.asarray(beta, dtype=np.float64)
+    k, cap = alpha.shape
+    d = np.zeros((k, cap))
+    e = np.zeros((k, cap))
+    for r in range(k):
+        m = int(min(iters[r], cap))
+        a, b = alpha[r, :m], beta[r, :m]
+        if m == 0:
+            continue
+        d[r, :m] = 1.0 / a
+        d[r, 1:m] += b[:m - 1] / a[:m - 1]
+        e[r, :m - 1] = np.sqrt(np.maximum(b[:m - 1], 0.0)) / a[:m - 1]
+    return d, e
+
+
 def slq_quadrature(alpha, beta, n):
     """Gauss quadrature of log over each probe's Lanczos tridiagonal:
     n * sum_i (e_1^T y_i)^2 log(theta_i); returns (mean, standard error,
@@ -164,6 +187,29 @@ class IterativeSolver:
         nat.check(nat.lib().spde_pcg_solve(self._h, _ptr(B), _ptr(X), k, float(tol), int(maxit), it.ctypes.data,
                                            _stream(torch)))
         return X, it
+
+    def solve_coeffs(self, B, tol=1e-6, maxit=5000, cap=50):
+        """solve() with each column's first `cap` PCG step coefficients
+        (k x cap alpha and beta, zero past its iterations)."""
+        torch = _torch()
+        B = B.contiguous()
+        k = self._k(B)
+        X = torch.empty_like(B)
+        it = np.zeros(k, np.int32)
+        a = np.zeros((k, cap))
+        b = np.zeros((k, cap))
+        nat.check(nat.lib().spde_pcg_solve_coeffs(self._h, _ptr(B), _ptr(X), k, float(tol), int(maxit),
+                                                  it.ctypes.data, a.ctypes.data, b.ctypes.data, int(cap),
+                                                  _stream(torch)))
+        return X, it, a, b
+
+    def lmul(self, W):
+        """L W for the current FP32 factor (n x k float32)."""
+        torch = _torch()
+        W = W.contiguous()
+        Y = torch.empty_like(W)
+        nat.check(nat.lib().spde_ic_lmul(self._h, _ptr(W), _ptr(Y), self._k(W), _stream(torch)))
+        return Y
 
     def refine(self, b, tol=1e-10, inner_tol=1e-5, max_rounds=10, maxit=5000):
         """FP64 solution of Q_c x = b by iterative refinement around FP32 PCG."""
@@ -354,6 +400,28 @@ def _start_side(it, job):
     return join
 
 
+def _hutchinson_cg(it, probes, seed, tol, steps):
+    """The estimate of Q_c^-1 on the pattern and the Lanczos tridiagonals
+    of L^-1 Q_c L^-T (first `steps` steps) from the same PCG solves: probes
+    v = L w for Rademacher w in batches of up to 64 (batch j from seed + 1 +
+    its first index), u = Q_c^-1 v and y = L^-T w, so E[u y^T] = Q_c^-1;
+    returns (sigma, iterations per probe, diagonals, off-diagonals)."""
+    sqc, iters, D, E, done = None, [], [], [], 0
+    while done < probes:
+        k = min(64, probes - done)
+        k = 1 << (k.bit_length() - 1)
+        W = it.rademacher(k, seed + 1 + done)
+        U, its, a, b = it.solve_coeffs(it.lmul(W), tol=tol, cap=steps)
+        Y = it.sweep(2, W)
+        iters += list(its)
+        sqc = it.hutch_pattern(U, Y, out=sqc, alpha=k / probes, beta=0.0 if sqc is None else 1.0)
+        d, e = cg_lanczos(a, b, its)
+        D.append(d)
+        E.append(e)
+        done += k
+    return sqc, iters, np.vstack(D), np.vstack(E)
+
+
 def _hutchinson(it, probes, seed, tol):
     """Estimate of Q_c^-1 on the pattern from `probes` Rademacher probes, in
     batches of up to 64 (the probes of batch j from seed + 1 + its first
@@ -372,7 +440,7 @@ def _hutchinson(it, probes, seed, tol):
 
 def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32, slq_steps=50, hutch_probes=64,
                seed=0, tol=1e-10, probe_tol=1e-3, shift=0.1, correction=None, contract=None, adv_dir=None,
-               tol_like=None, inner_tol=1e-5, mean_solver="replace", delta=0.1):
+               tol_like=None, inner_tol=1e-5, mean_solver="replace", delta=0.1, slq="cg"):
     """(-loglik/(m r), parts) as ``Engine.loglik`` returns them, with log|Q_c|
     from stochastic Lanczos quadrature and the trace part of the gradient from
     the Hutchinson estimate.  correction(engine, coeffs, qval, diag_add) ->
@@ -384,6 +452,15 @@ def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32
     residual replacement each time the residual has fallen by `delta`
     (1,088 iterations at 100 x 100 x 20); "refine" by restarted FP32 solves
     to `inner_tol` inside FP64 refinement rounds (1,383).  Both to `tol`.
+
+    slq: "cg" (default when the gradient is wanted and no tolerance waits
+    on the value) takes the Lanczos quadrature from the Hutchinson probes'
+    own PCG solves -- probes v = L w for Rademacher w, whose PCG coefficients
+    give the Lanczos tridiagonal of L^-1 Q_c L^-T started at w, and whose
+    solutions u = Q_c^-1 L w with y = L^-T w estimate Q_c^-1 (E[u y^T] =
+    Q_c^-1) -- so log|Q_c| comes from hutch_probes probes x slq_steps steps
+    at no extra cost (the batched-CG estimator of Gardner et al., NeurIPS
+    2018); "lanczos" runs slq_probes separate Lanczos probes first.
 
     probe_tol: relative residual of the FP32 solves Q_c U = V of the
     Hutchinson probes.  At 100 x 100 x 20 with 64 probes, 1e-3 instead of
@@ -470,22 +547,29 @@ def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32
             bound, out = out_of_reach(ld_m + tr_p, se_est, margin=0.1)
             if out:
                 return infeasible(ld_m + tr_p, se_est, bound, True)
-        alpha, beta = it.slq(slq_probes, slq_steps, seed)
-        trlog, se, _ = slq_quadrature(alpha, beta, n)
-        ld_qc = ld_m + trlog
+        cg_slq = slq == "cg" and side is not None
+        if not cg_slq:
+            alpha, beta = it.slq(slq_probes, slq_steps, seed)
+            trlog, se, _ = slq_quadrature(alpha, beta, n)
+            ld_qc = ld_m + trlog
         if tol_like is not None:
             bound, out = out_of_reach(ld_qc, se)
             if out:
                 return infeasible(ld_qc, se, bound, False)
             if grad and hutch_probes > 0:
                 side = _start_side(it, mean_solves)
-        if side is not None:
+        if cg_slq:
+            sqc, iters, alpha, beta = _hutchinson_cg(it, hutch_probes, seed, probe_tol, slq_steps)
+        elif side is not None:
             sqc, iters = _hutchinson(it, hutch_probes, seed, probe_tol)
         else:
             results = mean_solves(it)
     finally:
         if side is not None:       # joined on every exit, so the view's buffers are free again
             results = side()
+    if cg_slq:
+        trlog, se, _ = slq_quadrature(alpha, beta, n)
+        ld_qc = ld_m + trlog
     mus = [x for x, _ in results]
     info = [inf for _, inf in results]
     mu = torch.stack(mus, 1)

@@ -269,6 +269,29 @@ def test_lanczos_and_slq(case):
     assert abs(est - exact) <= 4 * se + 1e-5 * abs(np.linalg.slogdet(Ao)[1])
 
 
+def test_lanczos_from_pcg_coefficients(case):
+    """The PCG step coefficients of Q_c u = L w give the Lanczos tridiagonal
+    of L^-1 Q_c L^-T started at w: equal to the explicit Lanczos (spde_slq,
+    same Rademacher probes) over the first steps, and their quadrature
+    estimates log|Q_c| - log|L L^T| within its standard error."""
+    from spdepy_amd.krylov import cg_lanczos, slq_quadrature
+    eng = case["eng"]
+    it = eng.iterative()
+    ld_m = it.factor(case["qval"], case["dadd"], shift=0.1)
+    step("PCG coefficients vs explicit Lanczos")
+    W = it.rademacher(8, seed=21)
+    U, its, a, b = it.solve_coeffs(it.lmul(W), tol=1e-6, cap=30)
+    d, e = cg_lanczos(a, b, its)
+    la, lb = it.slq(8, 30, seed=21)
+    print("  iterations", its, "diag cg", d[0, :4], "lanczos", la[0, :4])
+    assert np.allclose(d[:, :6], la[:, :6], rtol=2e-3)
+    assert np.allclose(e[:, :5], lb[:, :5], rtol=2e-3)
+    est, se, _ = slq_quadrature(d, e, eng.n)
+    exact = np.linalg.slogdet(case["A"].toarray())[1] - ld_m
+    print("  log|Q_c| - log|M|: CG quadrature %.6f +- %.6f, exact %.6f" % (est, se, exact))
+    assert abs(est - exact) <= 4 * se + 1e-6 * abs(exact)
+
+
 def test_pcg_loglike_against_exact(case):
     mod, par = case["mod"], case["c"]["par"]
     step("logLike exact vs pcg 12x10x6")
