@@ -1023,7 +1023,9 @@ int last_colour(const spde_ic* ic) {
 // Forward sweep z = L^-1 r, colour by colour.  fold: the rows of the last
 // colour also take their backward step (x_i = z_i / L_ii: no later colour
 // couples to them), which the backward sweep of ic_apply then skips.
-int ic_fwd(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s, bool fold = false) {
+// lds: bytes of LDS each block of the 64-column sweeps reserves (0: none), which
+// caps how many of their blocks a CU holds (see kShareLds).
+int ic_fwd(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s, bool fold = false, unsigned lds = 0) {
   const int last = last_colour(ic);
   for (int c = 0; c < kColours; ++c) {
     const int64_t r0 = ic->coff[c], r1 = ic->coff[c + 1];
@@ -1034,7 +1036,8 @@ int ic_fwd(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s, bool 
                          ic->d_lptr, ic->d_lcol, ic->d_l32, ic->d_dinv, r, z, f);
     else if (logk >= 4) {
       auto kern = logk == 6 ? fwdw_kernel<64> : logk == 5 ? fwdw_kernel<32> : fwdw_kernel<16>;
-      hipLaunchKernelGGL(kern, dim3(blocks_of((r1 - r0) << 6)), dim3(256), 0, s, r0, r1, ic->d_crow, ic->d_lptr,
+      hipLaunchKernelGGL(kern, dim3(blocks_of((r1 - r0) << 6)), dim3(256), logk == 6 ? lds : 0, s, r0, r1, ic->d_crow,
+                         ic->d_lptr,
                          ic->d_lcol, ic->d_l32, ic->d_dinv, r, z, f);
     }
     else
@@ -1045,7 +1048,8 @@ int ic_fwd(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s, bool 
 }
 // Backward sweep x = L^-T y (x may be y), colour by colour from the last;
 // skip_last: the last colour was folded into the forward sweep.
-int ic_bwd(spde_ic* ic, const float* y, float* x, int logk, hipStream_t s, bool skip_last = false) {
+int ic_bwd(spde_ic* ic, const float* y, float* x, int logk, hipStream_t s, bool skip_last = false,
+           unsigned lds = 0) {
   const int last = last_colour(ic);
   for (int c = kColours - 1; c >= 0; --c) {
     const int64_t r0 = ic->coff[c], r1 = ic->coff[c + 1];
@@ -1055,7 +1059,8 @@ int ic_bwd(spde_ic* ic, const float* y, float* x, int logk, hipStream_t s, bool 
                          ic->d_uptr, ic->d_ucol, ic->d_u32, ic->d_dinv, y, x);
     else if (logk >= 4) {
       auto kern = logk == 6 ? bwdw_kernel<64> : logk == 5 ? bwdw_kernel<32> : bwdw_kernel<16>;
-      hipLaunchKernelGGL(kern, dim3(blocks_of((r1 - r0) << 6)), dim3(256), 0, s, r0, r1, ic->d_crow, ic->d_uptr,
+      hipLaunchKernelGGL(kern, dim3(blocks_of((r1 - r0) << 6)), dim3(256), logk == 6 ? lds : 0, s, r0, r1, ic->d_crow,
+                         ic->d_uptr,
                          ic->d_ucol, ic->d_u32, ic->d_dinv, y, x);
     }
     else
@@ -1066,9 +1071,9 @@ int ic_bwd(spde_ic* ic, const float* y, float* x, int logk, hipStream_t s, bool 
 }
 // z = M^-1 r = (L L^T)^-1 r: forward from r into z, then backward in place,
 // the last colour's two steps in one launch.
-int ic_apply(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s) {
-  ic_fwd(ic, r, z, logk, s, true);
-  return ic_bwd(ic, z, z, logk, s, true);
+int ic_apply(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s, unsigned lds = 0) {
+  ic_fwd(ic, r, z, logk, s, true, lds);
+  return ic_bwd(ic, z, z, logk, s, true, lds);
 }
 // y = Q_c x in FP32 (k = 2^logk right-hand sides) with the block partials of
 // x^T y when part is set; `g` blocks (the caller's reduction grid).
@@ -1129,11 +1134,17 @@ int ensure_work(spde_ic* ic) {
 }
 
 constexpr int kPoll = 16;     // iterations between the host's looks at the active flags
+// LDS each block of the 64-column sweeps reserves in spde_pcg_solve_coeffs, whose
+// probe solves run beside another stream's solve (the conditional mean): 32 KB
+// of the CU's 160 KB holds five instead of eight of their blocks per CU and
+// leaves wave slots to the other stream's short launches (2.95 -> 3.11 evals/s
+// at 100x100x20; 24 KB 3.06, 40 KB 3.05).
+constexpr unsigned kShareLds = 32 * 1024;
 constexpr int kSegMax = 512;  // residual replacement: at most this many iterations between replacements
 
 // One PCG iteration on the handle's work vectors (R, Z, P, Qv) and X.
 void enqueue_iteration(spde_ic* ic, int logk, unsigned g, float* X, hipStream_t s, double* hist = nullptr,
-                       int cap = 0) {
+                       int cap = 0, unsigned lds = 0) {
   const int k = 1 << logk;
   const int64_t n = ic->n, units = n << logk;
   float* R = ic->d_work;
@@ -1143,7 +1154,7 @@ void enqueue_iteration(spde_ic* ic, int logk, unsigned g, float* X, hipStream_t 
   launch_spmm(ic, logk, g, P, Qv, ic->d_part, s);
   hipLaunchKernelGGL(pcg_alpha_kernel, dim3(k), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal, ic->d_active);
   hipLaunchKernelGGL(pcg_update_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, P, Qv, X, R);
-  ic_apply(ic, R, Z, logk, s);
+  ic_apply(ic, R, Z, logk, s, lds);
   hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, R, Z, ic->d_part);
   hipLaunchKernelGGL(pcg_beta_kernel, dim3(k), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal, ic->d_active,
                      ic->d_iters, hist, cap);
@@ -1180,7 +1191,7 @@ hipGraphExec_t iteration_graph(spde_ic* ic, int logk, unsigned g, hipStream_t s)
 // row-major n x k.  Host polls the active flags every kPoll iterations.
 // graph: iterate through iteration_graph (X must be the handle's d_x32).
 int pcg_run(spde_ic* ic, const float* B, float* X, int logk, double tol, int maxit, int32_t* iters, hipStream_t s,
-            bool graph = false, double* hist = nullptr, int cap = 0) {
+            bool graph = false, double* hist = nullptr, int cap = 0, unsigned lds = 0) {
   const int k = 1 << logk;
   const int64_t n = ic->n, units = n << logk;
   int rc = ensure_work(ic);
@@ -1204,7 +1215,7 @@ int pcg_run(spde_ic* ic, const float* B, float* X, int logk, double tol, int max
       SPDE_HIP(hipGraphLaunch(ge, s));
       it += kPoll;
     } else {
-      enqueue_iteration(ic, logk, g, X, s, hist, cap);
+      enqueue_iteration(ic, logk, g, X, s, hist, cap, lds);
       ++it;
     }
     if (it % kPoll == 0 || it == maxit) {
@@ -1612,7 +1623,7 @@ int spde_pcg_solve_coeffs(spde_ic* ic, const float* B, float* X, int k, double t
     SPDE_HIP(hipMalloc((void**)&ic->d_hist, sizeof(double) * 2 * 64 * cap));
     ic->hist_cap = cap;
   }
-  int rc = pcg_run(ic, B, X, logk, tol, maxit, iters, s, false, ic->d_hist, cap);
+  int rc = pcg_run(ic, B, X, logk, tol, maxit, iters, s, false, ic->d_hist, cap, kShareLds);
   if (rc) return rc;
   std::vector<double> h((size_t)2 * 64 * cap);
   SPDE_HIP(hipMemcpyAsync(h.data(), ic->d_hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
