@@ -1024,7 +1024,7 @@ int last_colour(const spde_ic* ic) {
 // colour also take their backward step (x_i = z_i / L_ii: no later colour
 // couples to them), which the backward sweep of ic_apply then skips.
 // lds: bytes of LDS each block of the 64-column sweeps reserves (0: none), which
-// caps how many of their blocks a CU holds (see kShareLds).
+// caps how many of their blocks a CU holds (kShareLds).
 int ic_fwd(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s, bool fold = false, unsigned lds = 0) {
   const int last = last_colour(ic);
   for (int c = 0; c < kColours; ++c) {
@@ -1077,13 +1077,15 @@ int ic_apply(spde_ic* ic, const float* r, float* z, int logk, hipStream_t s, uns
 }
 // y = Q_c x in FP32 (k = 2^logk right-hand sides) with the block partials of
 // x^T y when part is set; `g` blocks (the caller's reduction grid).
-void launch_spmm(spde_ic* ic, int logk, unsigned g, const float* x, float* y, double* part, hipStream_t s) {
+void launch_spmm(spde_ic* ic, int logk, unsigned g, const float* x, float* y, double* part, hipStream_t s,
+                 unsigned lds = 0) {
   if (logk == 0)
     hipLaunchKernelGGL(spmm1_kernel, dim3(g), dim3(256), 0, s, ic->n, ic->pl->d_indptr, ic->pl->d_indices,
                        ic->d_qc32, x, y, part);
   else if (logk >= 4)
     hipLaunchKernelGGL(logk == 6 ? spmmw_kernel<64> : logk == 5 ? spmmw_kernel<32> : spmmw_kernel<16>, dim3(g),
-                       dim3(256), 0, s, ic->n, ic->pl->d_indptr, ic->pl->d_indices, ic->d_qc32, x, y, part);
+                       dim3(256), logk == 6 ? lds : 0, s, ic->n, ic->pl->d_indptr, ic->pl->d_indices, ic->d_qc32, x,
+                       y, part);
   else
     hipLaunchKernelGGL(spmm32_kernel, dim3(g), dim3(256), 0, s, ic->n, logk, ic->pl->d_indptr, ic->pl->d_indices,
                        ic->d_qc32, x, y, part);
@@ -1134,11 +1136,12 @@ int ensure_work(spde_ic* ic) {
 }
 
 constexpr int kPoll = 16;     // iterations between the host's looks at the active flags
-// LDS each block of the 64-column sweeps reserves in spde_pcg_solve_coeffs, whose
-// probe solves run beside another stream's solve (the conditional mean): 32 KB
-// of the CU's 160 KB holds five instead of eight of their blocks per CU and
-// leaves wave slots to the other stream's short launches (2.95 -> 3.11 evals/s
-// at 100x100x20; 24 KB 3.06, 40 KB 3.05).
+// LDS each block of the 64-column launches (SpMM, sweeps, vector steps)
+// reserves in spde_pcg_solve_coeffs, whose probe solves run beside another
+// stream's solve (the conditional mean): 32 KB of the CU's 160 KB holds five
+// instead of eight of their blocks per CU and leaves wave slots to the other
+// stream's short launches (iterative log-likelihood at 100x100x20: 2.95 ->
+// 3.35 evals/s; 24 KB 3.11, 40 KB 3.07, 48 KB 2.83).
 constexpr unsigned kShareLds = 32 * 1024;
 constexpr int kSegMax = 512;  // residual replacement: at most this many iterations between replacements
 
@@ -1151,14 +1154,14 @@ void enqueue_iteration(spde_ic* ic, int logk, unsigned g, float* X, hipStream_t 
   float* Z = R + n * 64;
   float* P = Z + n * 64;
   float* Qv = P + n * 64;
-  launch_spmm(ic, logk, g, P, Qv, ic->d_part, s);
+  launch_spmm(ic, logk, g, P, Qv, ic->d_part, s, lds);
   hipLaunchKernelGGL(pcg_alpha_kernel, dim3(k), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal, ic->d_active);
-  hipLaunchKernelGGL(pcg_update_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, P, Qv, X, R);
+  hipLaunchKernelGGL(pcg_update_kernel, dim3(g), dim3(256), lds, s, units, logk, ic->d_scal, P, Qv, X, R);
   ic_apply(ic, R, Z, logk, s, lds);
-  hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), 0, s, units, logk, R, Z, ic->d_part);
+  hipLaunchKernelGGL(dot2_kernel, dim3(g), dim3(256), lds, s, units, logk, R, Z, ic->d_part);
   hipLaunchKernelGGL(pcg_beta_kernel, dim3(k), dim3(kRedThreads), 0, s, (int)g, ic->d_part, ic->d_scal, ic->d_active,
                      ic->d_iters, hist, cap);
-  hipLaunchKernelGGL(pcg_p_kernel, dim3(g), dim3(256), 0, s, units, logk, ic->d_scal, ic->d_active, Z, P);
+  hipLaunchKernelGGL(pcg_p_kernel, dim3(g), dim3(256), lds, s, units, logk, ic->d_scal, ic->d_active, Z, P);
 }
 
 // kPoll iterations with X = the handle's d_x32 as one HIP graph, captured the
