@@ -293,7 +293,7 @@ __global__ __launch_bounds__(NW * 64) void gemv_kernel(const GemmProb* __restric
   }
 }
 
-void launch_gemm(const GemmLaunch& g, const Bases& b, hipStream_t s) {
+void launch_gemm(const GemmLaunch& g, const Bases& b, hipStream_t s, unsigned lds) {
   if (g.ntiles == 0) return;
   dim3 grid(g.ntiles), block(256);
   if (g.gemv) {
@@ -321,22 +321,22 @@ void launch_gemm(const GemmLaunch& g, const Bases& b, hipStream_t s) {
   }
   if (g.tile == 128) {
     if (!g.ta && !g.tb)
-      hipLaunchKernelGGL((gemm128_kernel<false, false>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+      hipLaunchKernelGGL((gemm128_kernel<false, false>), grid, block, lds, s, g.d_probs, g.d_tiles, b);
     else if (!g.ta && g.tb)
-      hipLaunchKernelGGL((gemm128_kernel<false, true>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+      hipLaunchKernelGGL((gemm128_kernel<false, true>), grid, block, lds, s, g.d_probs, g.d_tiles, b);
     else if (g.ta && !g.tb)
-      hipLaunchKernelGGL((gemm128_kernel<true, false>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+      hipLaunchKernelGGL((gemm128_kernel<true, false>), grid, block, lds, s, g.d_probs, g.d_tiles, b);
     else
-      hipLaunchKernelGGL((gemm128_kernel<true, true>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+      hipLaunchKernelGGL((gemm128_kernel<true, true>), grid, block, lds, s, g.d_probs, g.d_tiles, b);
   } else {
     if (!g.ta && !g.tb)
-      hipLaunchKernelGGL((gemm64_kernel<false, false>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+      hipLaunchKernelGGL((gemm64_kernel<false, false>), grid, block, lds, s, g.d_probs, g.d_tiles, b);
     else if (!g.ta && g.tb)
-      hipLaunchKernelGGL((gemm64_kernel<false, true>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+      hipLaunchKernelGGL((gemm64_kernel<false, true>), grid, block, lds, s, g.d_probs, g.d_tiles, b);
     else if (g.ta && !g.tb)
-      hipLaunchKernelGGL((gemm64_kernel<true, false>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+      hipLaunchKernelGGL((gemm64_kernel<true, false>), grid, block, lds, s, g.d_probs, g.d_tiles, b);
     else
-      hipLaunchKernelGGL((gemm64_kernel<true, true>), grid, block, 0, s, g.d_probs, g.d_tiles, b);
+      hipLaunchKernelGGL((gemm64_kernel<true, true>), grid, block, lds, s, g.d_probs, g.d_tiles, b);
   }
 }
 
@@ -646,7 +646,7 @@ extern "C" int spde_debug_gemm(int ta, int tb, int M, int N, int K, double alpha
   bs.p[0] = const_cast<double*>(A);
   bs.p[1] = const_cast<double*>(B);
   bs.p[2] = C;
-  launch_gemm(g, bs, (hipStream_t)stream);
+  launch_gemm(g, bs, (hipStream_t)stream, 0);
   SPDE_HIP(hipStreamSynchronize((hipStream_t)stream));
   hipFree(g.d_probs);
   hipFree(g.d_tiles);
