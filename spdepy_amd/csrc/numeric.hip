@@ -581,11 +581,13 @@ void free_launch(GemmLaunch& g) {
   g.d_tiles = nullptr;
 }
 
-// LDS each block of the selected inverse's second-stream products (X = L21 Z,
-// Z^T Z, one level ahead of the chain) reserves: 48 KB holds one 128-tile or
-// one 64-tile block of them per CU, so the chain's own products on the first
-// stream keep most of the chip (exact evaluation at 100x100x20: 14.68 -> 14.80
-// evals/s, mean of three alternating runs each).
+// LDS each 64-tile block of the selected inverse's second-stream products
+// (X = L21 Z, Z^T Z, one level ahead of the chain) reserves: 48 KB holds one
+// such block per CU, so the chain's own products on the first stream keep
+// most of the chip.  128-tile launches run at full width: capping them too
+// gained 0.5% of the evaluation rate but slowed each of their launches by a
+// fifth (exact evaluation at 100x100x20, three alternating runs each: no cap
+// 14.72 evals/s, 64-tiles capped 14.78, all capped 14.86).
 constexpr unsigned kPreLds = 48 * 1024;
 // GEMM launch, bracketed by HIP events when the plan's timing is enabled.
 void run_gemm(spde_plan* pl, const GemmLaunch& g, const Bases& b, hipStream_t s, unsigned lds = 0) {
@@ -1229,8 +1231,7 @@ int spde_selinv(spde_plan* pl, int slot, double* sigma, void* stream) {
   for (int l = (int)pl->levels.size() - 1; l >= 0; --l) {
     LevelPlan& L = pl->levels[l];
     LevelDev& D = ex->lvl[l];
-    run_gemm(pl, L.si_x, b, ex->pre, kPreLds);
-    run_gemm(pl, L.si_s11a, b, ex->pre, kPreLds);
+    for (const GemmLaunch* g : {&L.si_x, &L.si_s11a}) run_gemm(pl, *g, b, ex->pre, g->tile == 128 ? 0 : kPreLds);
     SPDE_HIP(hipEventRecord(ex->ev_pre[l], ex->pre));
     launch_sigma_gather(D.d_nodes, pl->d_nodes, pl->d_relpos, sl.sigma, D.d_gtiles, D.ngtiles, s);
     SPDE_HIP(hipStreamWaitEvent(s, ex->ev_pre[l], 0));
