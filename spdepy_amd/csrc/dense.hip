@@ -233,6 +233,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // quarters of K, partial sums combined in LDS in wave order.
 // A transposed (K x M stored): one wave per output row m (a column of A),
 // lanes stride over k, butterfly sum across the wave.
+// loads each lane issues before their products in the GEMV's inner loop
+constexpr int kGemvDepth = 16;
 template <bool TA, int NW = 4>
 __global__ __launch_bounds__(NW * 64) void gemv_kernel(const GemmProb* __restrict__ probs,
                                                        const GemmTile* __restrict__ tiles, Bases bases) {
@@ -250,14 +252,29 @@ __global__ __launch_bounds__(NW * 64) void gemv_kernel(const GemmProb* __restric
     __shared__ double red[NW][kGemvMaxN][64];
     const int m = t.tm * 64 + lane;
     const int q = (K + NW - 1) / NW, k0 = wave * q, k1 = min(K, k0 + q);
-    if (m < P.M)
-#pragma unroll 8
-      for (int k = k0; k < k1; ++k) {
-        const double a = A[m + (int64_t)k * P.lda];
+    if (m < P.M) {
+      // batches of kGemvDepth loads issued before their products, so that
+      // many columns' worth of A are in flight per lane; the sums still run
+      // in ascending k
+      const double* __restrict__ a = A + m + (int64_t)k0 * P.lda;
+      int k = k0;
+      for (; k + kGemvDepth <= k1; k += kGemvDepth, a += (int64_t)kGemvDepth * P.lda) {
+        double av[kGemvDepth];
+#pragma unroll
+        for (int u = 0; u < kGemvDepth; ++u) av[u] = a[(int64_t)u * P.lda];
+#pragma unroll
+        for (int u = 0; u < kGemvDepth; ++u)
+#pragma unroll
+          for (int c = 0; c < kGemvMaxN; ++c)
+            if (c < N) acc[c] += av[u] * X[k + u + (int64_t)c * P.ldb];
+      }
+      for (; k < k1; ++k, a += P.lda) {
+        const double av = *a;
 #pragma unroll
         for (int c = 0; c < kGemvMaxN; ++c)
-          if (c < N) acc[c] += a * X[k + (int64_t)c * P.ldb];
+          if (c < N) acc[c] += av * X[k + (int64_t)c * P.ldb];
       }
+    }
 #pragma unroll
     for (int c = 0; c < kGemvMaxN; ++c) red[wave][c][lane] = acc[c];
     __syncthreads();
