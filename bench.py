@@ -115,7 +115,7 @@ def cpu_baseline(repeats=3):
         return t_q + t_dq + 2 * t_f, (t_q, t_dq, t_f, mf.flops)
 
     sample(F.var_model(50, 50, 10, m=625))                # warm-up: BLAS threads, allocator, imports
-    c = F.config2()
+    c = F.bspline_100x100x20()
     runs = []
     for i in range(repeats):
         runs.append(sample(c))
@@ -133,7 +133,7 @@ def cpu_baseline(repeats=3):
                        "bound on the CPU rate); the leg took %.0fs"
                        % (repeats, t_q, t_dq, flops, t_f, time.perf_counter() - t_all)),
             "exact_trace_variant": "the oracle's exact-trace route at this size (Woodbury form, "
-                                   "tests/golden/make_config2.py) took 455 s for one evaluation on 8 cores of the "
+                                   "tests/golden/make_bspline_100x100x20.py) took 455 s for one evaluation on 8 cores of the "
                                    "build container (not re-timed here)"}
 
 

@@ -1,12 +1,12 @@
 """Seeded workloads of the hot path (test infrastructure).
 
-config1: stationary isotropic advection-diffusion, 30x30 cells x 10 steps
+stationary_30x30x10: stationary isotropic advection-diffusion, 30x30 cells x 10 steps
          (n = 9,000), bc=1, par = [log 1, log 0.5, 0.3, -0.2, log 1, log 100],
          mod0 = isotropic Whittle-Matern [0, log 0.5, 0]; one seeded sample,
          200 observations at uniform random (cell, time) indices + N(0, 0.1^2).
-config2: B-spline (Nbs=8) advection and anisotropic diffusion on 100x100 x 20
+bspline_100x100x20: B-spline (Nbs=8) advection and anisotropic diffusion on 100x100 x 20
          steps (n = 200,000), 386 parameters, 5,000 observations.
-small(): the config-2 model class on reduced grids, for dense checks.
+small(): the 100x100x20 B-spline model class on reduced grids, for dense checks.
 field_model(): the families with some coefficients as B-spline fields and
          the rest constant, on small grids (dense sampling).
 """
@@ -68,11 +68,11 @@ def var_model(M=100, N=100, T=20, Nbs=8, bc=1, m=5000, seeds=(3, 4, 6, 7), ne=No
     return dict(grid=g, mod0=mod0, model=model, par=par, idx=idx, y=y, x=x)
 
 
-def config1():
+def stationary_30x30x10():
     return stationary()
 
 
-def config2():
+def bspline_100x100x20():
     return var_model()
 
 

@@ -1,7 +1,7 @@
-"""Writes tests/golden/config2.npz: the oracle's exact log-likelihood and full
+"""Writes tests/golden/bspline_100x100x20.npz: the oracle's exact log-likelihood and full
 386-entry gradient at 100 x 100 x 20 (n = 200,000; B-spline advection and
 anisotropic diffusion, Nbs = 8, bc = 1, 5,000 observations:
-``oracle.fixtures.config2()``), with the observation indices and values.
+``oracle.fixtures.bspline_100x100x20()``), with the observation indices and values.
 
 The exact trace at this size comes from the Woodbury form
 (``oracle.models.loglike_woodbury``): Q^{-1} S^T through Q = F^T D F with dense
@@ -11,7 +11,7 @@ matrices dQ_i built by the oracle's block formula -- a route independent of
 the GPU's Cholesky and selected inverse of Q_c.  About 15 minutes and 20 GB
 on 8 cores.  Run in the CPU container:
 
-    python tests/golden/make_config2.py
+    python tests/golden/make_bspline_100x100x20.py
 """
 import os
 import sys
@@ -26,7 +26,7 @@ from oracle import fixtures as F, models as Mo  # noqa: E402
 
 def main():
     t0 = time.time()
-    c = F.config2()
+    c = F.bspline_100x100x20()
     g, model, par = c["grid"], c["model"], c["par"]
     M, N, T = g.shape
     Q, _ = model.makeQ(par, grad=False)
@@ -37,7 +37,7 @@ def main():
     like, jac, mu = Mo.loglike_woodbury(Q, model.dQ_iter(par), A, q, A0, g.V, T, sigma, g.dt, M, N, cols,
                                         c["y"], par[-1])
     ld_q = Mo.logdet_closed_form(A, q, A0, g.V, T, sigma, g.dt)
-    np.savez_compressed(os.path.join(HERE, "config2.npz"), par=par, idx=c["idx"], y=c["y"],
+    np.savez_compressed(os.path.join(HERE, "bspline_100x100x20.npz"), par=par, idx=c["idx"], y=c["y"],
                         like=np.array(like), jac=jac, logdet_q=np.array(ld_q),
                         mu_sum=np.array(mu.sum()), mu_sq=np.array((mu ** 2).sum()))
     print("like %.15e |jac| %.6e  (%.0fs)" % (like, np.linalg.norm(jac), time.time() - t0))

@@ -1,4 +1,4 @@
-"""Writes tests/golden/config1.npz: the oracle's log-likelihood and exact
+"""Writes tests/golden/stationary_30x30x10.npz: the oracle's log-likelihood and exact
 gradient at the CPU-runnable configuration (stationary isotropic
 advection-diffusion, 30 x 30 cells x 10 steps, n = 9,000, bc = 1,
 par = [log 1, log 0.5, 0.3, -0.2, log 1, log 100], mod0 = isotropic
@@ -9,7 +9,7 @@ checksums of its values.  The exact trace uses dense inverses
 of Q and Q_c (n = 9,000: about a minute on 8 cores).  Run in the CPU
 container:
 
-    python tests/golden/make_config1.py
+    python tests/golden/make_stationary_30x30x10.py
 """
 import os
 import sys
@@ -29,7 +29,7 @@ def main():
     Q.sort_indices()
     S = g.getS(c["idx"])
     like, jac, _ = Mo.loglike(Q.tocsc(), dQ, S, c["y"], c["par"][-1], trace="exact")
-    np.savez_compressed(os.path.join(HERE, "config1.npz"), par=c["par"], idx=c["idx"], y=c["y"],
+    np.savez_compressed(os.path.join(HERE, "stationary_30x30x10.npz"), par=c["par"], idx=c["idx"], y=c["y"],
                         like=np.array(like), jac=jac, q_nnz=np.array(Q.nnz),
                         q_abs_sum=np.array(np.abs(Q.data).sum()), q_diag=Q.diagonal(),
                         q_indptr=Q.indptr.astype(np.int32), q_indices=Q.indices.astype(np.int32))

@@ -1,8 +1,8 @@
 """Parity at the headline configuration (B-spline advection and anisotropic
 diffusion, 100 x 100 x 20, n = 200,000, Nbs = 8, 386 parameters, 5,000
 observations, bc = 1): the drop-in model's exact log-likelihood and all 386
-gradient entries against the oracle's values in tests/golden/config2.npz,
-which tests/golden/make_config2.py computes by the Woodbury form (Q^-1 S^T
+gradient entries against the oracle's values in tests/golden/bspline_100x100x20.npz,
+which tests/golden/make_bspline_100x100x20.py computes by the Woodbury form (Q^-1 S^T
 through Q = F^T D F, no Cholesky of Q_c) -- a route independent of the GPU's
 factorisation and selected inverse."""
 import os
@@ -14,7 +14,7 @@ from tests.gpu_util import step
 
 pytestmark = pytest.mark.gpu
 
-GOLD = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config2.npz"))
+GOLD = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "bspline_100x100x20.npz"))
 
 
 def _model():
@@ -29,10 +29,10 @@ def _model():
     return mod
 
 
-def test_config2_loglike_and_full_gradient():
+def test_loglike_and_full_gradient_100x100x20():
     mod = _model()
     par = GOLD["par"]
-    step("config2 exact logLike + 386-entry gradient vs Woodbury oracle")
+    step("100x100x20 exact logLike + 386-entry gradient vs Woodbury oracle")
     like, jac = mod.logLike(par)
     like_o, jac_o = float(GOLD["like"]), GOLD["jac"]
     scale = np.abs(jac_o).max()
@@ -52,6 +52,6 @@ def test_config2_loglike_and_full_gradient():
     assert np.allclose(jac[others], jac_o[others], rtol=1e-9, atol=1e-9 * scale)
     assert abs(jac[sig] - jac_o[sig]) <= 1e-8 * scale
     # the factorised route to log|Q| agrees too
-    step("config2 logLike, log|Q| factorised")
+    step("100x100x20 logLike, log|Q| factorised")
     like_f = mod.logLike(par, grad=False, logdet="factor")
     assert abs(like_f - like_o) <= 1e-9 * abs(like_o)

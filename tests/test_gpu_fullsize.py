@@ -25,7 +25,7 @@ def test_logdet_matches_closed_form(workload):
     step("factor Q at 100x100x20")
     Q, fac, _ = mod.makeQ(par, grad=False)
     ld = fac.logdet()
-    c = F.config2()
+    c = F.bspline_100x100x20()
     assert np.allclose(c["par"], par)
     A, q, s = c["model"].spatial_operator(c["par"])
     A0 = c["mod0"].A0(c["mod0"].par)[0]

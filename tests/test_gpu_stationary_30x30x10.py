@@ -1,7 +1,7 @@
 """Parity at the CPU-runnable configuration (stationary isotropic
 advection-diffusion, 30 x 30 x 10, n = 9,000, bc = 1, 200 observations):
 the drop-in model's Q, logLike and exact gradient against the oracle's
-values in tests/golden/config1.npz (written by tests/golden/make_config1.py;
+values in tests/golden/stationary_30x30x10.npz (written by tests/golden/make_stationary_30x30x10.py;
 the oracle's exact trace there uses dense inverses)."""
 import os
 
@@ -13,15 +13,15 @@ from tests.gpu_util import product_case, step
 
 pytestmark = pytest.mark.gpu
 
-GOLD = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config1.npz"))
+GOLD = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "stationary_30x30x10.npz"))
 
 
-def test_config1_q_loglike_gradient():
+def test_q_loglike_gradient_30x30x10():
     c = F.stationary()
     assert np.array_equal(c["idx"], GOLD["idx"]) and np.array_equal(c["y"], GOLD["y"])
     par = GOLD["par"]
     m = product_case(c)
-    step("config1 makeQ (n = 9,000)")
+    step("30x30x10 makeQ (n = 9,000)")
     Q, _, _ = m.makeQ(par, grad=False)
     Q = Q.tocsr()
     Q.sort_indices()
@@ -33,7 +33,7 @@ def test_config1_q_loglike_gradient():
     qs = float(GOLD["q_abs_sum"])
     assert abs(np.abs(Q.data).sum() - qs) <= 1e-12 * qs
     m.initFit(GOLD["y"], idx=GOLD["idx"], fitQ0=False)
-    step("config1 logLike exact")
+    step("30x30x10 logLike exact")
     like, jac = m.logLike(par)
     like_o, jac_o = float(GOLD["like"]), GOLD["jac"]
     assert abs(like - like_o) <= 1e-9 * abs(like_o)

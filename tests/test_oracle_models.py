@@ -222,7 +222,7 @@ def test_field_families_reduce_to_stationary(kind, dvar, gvar):
 
 
 def test_field_family_ani_all_fields_is_var_model():
-    """All-field anisotropic case = the config-2 model (pinned separately)."""
+    """All-field anisotropic case = the 100x100x20 B-spline model (pinned separately)."""
     c = F.small(8, 7, 4, bc=1)
     fm = Mo.FieldAdvDiff(c["grid"], c["mod0"], "ani", True, True, bc=1)
     Qv, _ = c["model"].makeQ(c["par"], grad=False)
@@ -322,11 +322,11 @@ def test_cov_family_derivatives(kind, dvar):
         assert abs(fd - dQ[i]).max() <= 1e-6 * max(abs(dQ[i]).max(), 1e-3 * abs(Q).max()), i
 
 
-def test_config1_golden_is_the_oracle():
-    """tests/golden/config1.npz (the GPU parity target at 30 x 30 x 10) is what
+def test_golden_30x30x10_is_the_oracle():
+    """tests/golden/stationary_30x30x10.npz (the GPU parity target at 30 x 30 x 10) is what
     the oracle computes now (dense inverses at n = 9,000: ~15 s)."""
     import os
-    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config1.npz"))
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "stationary_30x30x10.npz"))
     c = F.stationary()
     assert np.array_equal(c["idx"], gold["idx"]) and np.array_equal(c["y"], gold["y"])
     Q, dQ = c["model"].makeQ(c["par"], grad=True)
@@ -336,7 +336,7 @@ def test_config1_golden_is_the_oracle():
     assert np.allclose(jac, gold["jac"], rtol=1e-10, atol=1e-12 * np.abs(jac).max())
 
 
-def test_sparse_lu_logdet_branch_matches_closed_form_at_config1():
+def test_sparse_lu_logdet_branch_matches_closed_form_at_30x30x10():
     """The oracle's n > 4,000 branch (sparse LU with symmetric pivoting, the
     CHOLMOD stand-in for log|Q| and log|Q_c|) against the closed form and the
     determinant lemma at 30 x 30 x 10 (n = 9,000)."""
@@ -357,11 +357,11 @@ def test_sparse_lu_logdet_branch_matches_closed_form_at_config1():
     assert abs(Mo.logdet_spd(Qc) - lemma) <= 1e-11 * abs(lemma)
 
 
-def test_woodbury_route_reproduces_config1_golden():
+def test_woodbury_route_reproduces_golden_30x30x10():
     """The Woodbury form that writes the 100 x 100 x 20 fixture reproduces the
-    dense-inverse values of tests/golden/config1.npz at n = 9,000."""
+    dense-inverse values of tests/golden/stationary_30x30x10.npz at n = 9,000."""
     import os
-    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config1.npz"))
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "stationary_30x30x10.npz"))
     c = F.stationary()
     g, model, par = c["grid"], c["model"], c["par"]
     M, N, T = g.shape

@@ -182,7 +182,7 @@ def test_wide_time_plan_reproduces_dense_factor_and_inverse(M, N, T, bc, leaf):
     assert np.allclose(flat[e["map"][m]], inv[rows[m], e["indices"][m]], rtol=1e-9, atol=1e-12 * np.abs(inv).max())
 
 
-def test_config2_plan_statistics():
+def test_plan_statistics_100x100x20():
     """100x100x20 lattice: the analysis the GPU factorisation is sized from."""
     pl = Plan(100, 100, 20, 1, leaf_max=128, nslots=0, host_only=True)
     info = pl.info
