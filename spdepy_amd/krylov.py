@@ -440,7 +440,7 @@ def _hutchinson(it, probes, seed, tol):
 
 def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32, slq_steps=50, hutch_probes=64,
                seed=0, tol=1e-10, probe_tol=1e-3, shift=0.1, correction=None, contract=None, adv_dir=None,
-               tol_like=None, inner_tol=1e-5, mean_solver="replace", delta=0.1, slq="cg"):
+               tol_like=None, inner_tol=1e-5, mean_solver="replace", delta=0.1, slq="cg", pilot_steps=20):
     """(-loglik/(m r), parts) as ``Engine.loglik`` returns them, with log|Q_c|
     from stochastic Lanczos quadrature and the trace part of the gradient from
     the Hutchinson estimate.  correction(engine, coeffs, qval, diag_add) ->
@@ -473,8 +473,10 @@ def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32
     0 <= mu^T Q mu + tau ||y - S mu||^2 <= tau ||y||^2), the tolerance cannot
     be met and the call returns (None, parts) with parts["infeasible"] set,
     before the solves for mu_c and the Hutchinson probes.  A pilot of 4
-    probes decides first when even a tenth of its error, scaled to
-    slq_probes, is out of reach (parts["pilot"])."""
+    probes of pilot_steps Lanczos steps decides first when even a tenth of
+    its error, scaled to slq_probes, is out of reach (parts["pilot"]); the
+    spread of the probes' quadratures, which sets that error, is fixed
+    after the first steps."""
     torch = _torch()
     from .engine import OUT_DOT00, OUT_TAUTR, OUT_WIDTH, _obs_device
     if engine.is_spatial:
@@ -493,11 +495,7 @@ def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32
     gram = sp.assemble_spatial(engine.plan, 1)
     sp.factor_async(0, q0, owner=engine)
     sp.factor_async(1, gram, owner=engine)
-    if grad:
-        s_q0 = torch.empty(sp.nnz, dtype=torch.float64, device="cuda")
-        s_gram = torch.empty(sp.nnz, dtype=torch.float64, device="cuda")
-        sp.selinv(0, s_q0)
-        sp.selinv(1, s_gram)
+    s_q0 = s_gram = None
     ld_m = it.factor(qval, diag_add, shift=shift)
     if correction is not None:
         ld_m = it.correct(correction(engine, coeffs, qval, diag_add))
@@ -532,8 +530,6 @@ def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32
             return bound, margin * 0.5 * r * se > tol_like * bound
 
         def infeasible(ld_qc, se, bound, pilot):
-            if grad:
-                torch.cuda.synchronize()       # the spatial selected inverses are enqueued
             return None, dict(m=m, r=r, tau=tau, ld_q=ld_q, ld_qc=ld_qc, logdet_qc_se=se, logdet_m=ld_m,
                               ic_shift=it.shift, infeasible=True, like_bound=bound, pilot=pilot,
                               pcg=dict(mu=[]))
@@ -541,7 +537,7 @@ def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32
             # a pilot of 4 probes: when a tenth of its standard error, scaled to
             # slq_probes probes, already exceeds the tolerance, the full
             # estimate cannot meet it (the tenth covers the pilot's own spread)
-            a_p, b_p = it.slq(4, slq_steps, seed + 7919)
+            a_p, b_p = it.slq(4, min(pilot_steps, slq_steps), seed + 7919)
             tr_p, se_p, _ = slq_quadrature(a_p, b_p, n)
             se_est = se_p * np.sqrt(4.0 / slq_probes)
             bound, out = out_of_reach(ld_m + tr_p, se_est, margin=0.1)
@@ -558,6 +554,13 @@ def loglik_pcg(engine, coeffs, obs_cols, data, log_tau, grad=True, slq_probes=32
                 return infeasible(ld_qc, se, bound, False)
             if grad and hutch_probes > 0:
                 side = _start_side(it, mean_solves)
+        if grad:
+            # the spatial selected inverses of the log|Q| part of the gradient,
+            # enqueued once the value is known to be wanted
+            s_q0 = torch.empty(sp.nnz, dtype=torch.float64, device="cuda")
+            s_gram = torch.empty(sp.nnz, dtype=torch.float64, device="cuda")
+            sp.selinv(0, s_q0)
+            sp.selinv(1, s_gram)
         if cg_slq:
             sqc, iters, alpha, beta = _hutchinson_cg(it, hutch_probes, seed, probe_tol, slq_steps)
         elif side is not None:
