@@ -256,19 +256,21 @@ def batch_models(nmodels=64, seed=0):
     return models, pars
 
 
-def batch_route(nmodels=64, nrhs=256, nstreams=8, chunk=256):
+def batch_route(nmodels=64, nrhs=256, nstreams=8, chunk=256, warm=2):
     """Models per second of exact FP64 log-likelihood + gradient and nrhs
     conditional samples per model: one after another (logLike, then the
-    samples) against the batch (spde_loglik_batch over a pool of streams)."""
+    samples) against the batch (spde_loglik_batch over a pool of streams),
+    after `warm` untimed batches."""
     import torch
     from spdepy_amd.batch import BatchEvaluator
     t = time.perf_counter()
     models, pars = batch_models(nmodels)
     ev = BatchEvaluator(models, nstreams=nstreams, chunk=chunk)
     t_setup = time.perf_counter() - t
-    ev.loglik(pars)                                   # untimed: every plan's buffers
-    for i in range(nmodels):
-        ev.conditional_samples(i, nrhs, seed=i)
+    for _ in range(warm):                             # untimed: every plan's buffers, steady clocks
+        ev.loglik(pars)
+        for i in range(nmodels):
+            ev.conditional_samples(i, nrhs, seed=i)
     torch.cuda.synchronize()
     t = time.perf_counter()
     torch.cuda.nvtx.range_push("batch_loglik")      # markers for a profiler's timeline
@@ -294,7 +296,7 @@ def batch_route(nmodels=64, nrhs=256, nstreams=8, chunk=256):
     return {"models_per_s": nmodels / t_batch, "sequential_models_per_s": nmodels / t_seq,
             "batch_s": t_batch, "batch_loglik_s": t_loglik, "sequential_s": t_seq, "setup_s": t_setup,
             "models": nmodels, "rhs": nrhs,
-            "streams": nstreams, "unknowns_min_max_total": [min(ns), max(ns), sum(ns)],
+            "streams": nstreams, "warm_batches": warm, "unknowns_min_max_total": [min(ns), max(ns), sum(ns)],
             "max_rel_diff_batch_vs_single": worst,
             "note": "exact FP64 Cholesky log-likelihood + gradient and %d conditional samples per model; "
                     "plans (symbolic analysis, launch lists) built in setup" % nrhs}
@@ -327,6 +329,8 @@ def main():
     ap.add_argument("--no-paths", action="store_true", help="skip the factorised, iterative and learned routes")
     ap.add_argument("--batch", action="store_true", help="also time the batch of 64 uneven models")
     ap.add_argument("--batch-only", action="store_true", help="only the batch of 64 uneven models")
+    ap.add_argument("--batch-streams", type=int, default=8, help="HIP streams of the batch")
+    ap.add_argument("--batch-warm", type=int, default=2, help="untimed batches before the timed one")
     ap.add_argument("--only-route", choices=["exact", "pcg", "learned", "learned_fallback"],
                     help="only `steps` evaluations of one route on the workload (for profiling)")
     ap.add_argument("--stages", action="store_true", help="print a per-stage breakdown to stderr")
@@ -342,7 +346,7 @@ def main():
     from spdepy_amd.plan import _ptr
 
     if args.batch_only:
-        print(json.dumps({"batch": batch_route()}))
+        print(json.dumps({"batch": batch_route(nstreams=args.batch_streams, warm=args.batch_warm)}))
         return
     if args.only_route:
         mod, par = build_workload()
@@ -445,7 +449,7 @@ def main():
         for k in ("pcg", "learned_pcg"):
             if "evals_per_s" in routes.get(k, {}):
                 routes[k]["vs_exact"] = routes[k]["evals_per_s"] / value
-    batch = batch_route() if args.batch else None
+    batch = batch_route(nstreams=args.batch_streams, warm=args.batch_warm) if args.batch else None
     base = None if args.no_cpu_baseline else cpu_baseline(args.cpu_repeats)
     out = {
         "metric": METRIC,
