@@ -38,6 +38,7 @@
 struct IcShared {
   spde_plan* pl = nullptr;
   int64_t n = 0, nnz = 0, nlow = 0;
+  int64_t max_low = 0;               // longest lower list of a row
   int M = 0, N = 0, T = 0;
   std::vector<int64_t> coff;        // 19 offsets into crow
   int32_t* d_crow = nullptr;
@@ -147,6 +148,66 @@ __global__ __launch_bounds__(256) void ic_factor_kernel(int64_t r0, int64_t r1, 
     dg64[i] = 1.0;
   } else {
     dg64[i] = sqrt(d);
+  }
+}
+
+// The same factor with one wave per row (rows of at most kIcRowMax lower
+// entries): for each entry q in turn, the lanes take the entries of row j's
+// lower list, find each one's column among the row's finished entries lo..q-1
+// by binary search on lkey, and the products are summed over the wave in a
+// fixed order.  The row's finished values stay in LDS (written by lane 0, read
+// by the other lanes of the same wave later in program order).
+constexpr int kIcRowMax = 128;
+__global__ __launch_bounds__(256) void ic_factorw_kernel(int64_t r0, int64_t r1, const int32_t* __restrict__ crow,
+                                                         const int64_t* __restrict__ lptr,
+                                                         const int32_t* __restrict__ lcol,
+                                                         const int64_t* __restrict__ lkey,
+                                                         const int64_t* __restrict__ lsrc,
+                                                         const double* __restrict__ qval,
+                                                         const double* __restrict__ adiag, double shift,
+                                                         double* __restrict__ l64, double* __restrict__ dg64,
+                                                         int* __restrict__ status) {
+  __shared__ double vals[4][kIcRowMax];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t rr = r0 + blockIdx.x * (int64_t)4 + wave;
+  if (rr >= r1) return;
+  const int i = crow[rr];
+  const int64_t lo = lptr[i], hi = lptr[i + 1];
+  double* v_row = vals[wave];
+  double ssq = 0.0;
+  for (int64_t q = lo; q < hi; ++q) {
+    const int j = lcol[q];
+    const double aq = qval[lsrc[q]];
+    const int64_t pe = lptr[j + 1];
+    double part = 0.0;
+    for (int64_t p = lptr[j] + lane; p < pe; p += 64) {
+      const int64_t kp = lkey[p];
+      int64_t a = lo, b = q;  // first key >= kp in [lo, q)
+      while (a < b) {
+        const int64_t mid = (a + b) >> 1;
+        if (lkey[mid] < kp) a = mid + 1;
+        else b = mid;
+      }
+      if (a < q && lkey[a] == kp) part += v_row[a - lo] * l64[p];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+    const double v = (aq - part) / dg64[j];
+    if (lane == 0) {
+      v_row[q - lo] = v;
+      l64[q] = v;
+    }
+    __builtin_amdgcn_wave_barrier();
+    ssq += v * v;
+  }
+  if (lane == 0) {
+    const double d = (1.0 + shift) * adiag[i] - ssq;
+    if (!(d > 0.0)) {
+      atomicMax(status, i + 1);
+      dg64[i] = 1.0;
+    } else {
+      dg64[i] = sqrt(d);
+    }
   }
 }
 
@@ -1298,6 +1359,7 @@ int spde_ic_create(spde_plan* pl, spde_ic** out) {
     }
   }
   for (int64_t v = 0; v < n; ++v) {
+    ic->max_low = std::max(ic->max_low, ic->h_lptr[v + 1]);
     ic->h_lptr[v + 1] += ic->h_lptr[v];
     ic->h_uptr[v + 1] += ic->h_uptr[v];
   }
@@ -1499,7 +1561,12 @@ int spde_ic_factor(spde_ic* ic, const double* qval, const double* diag_add, doub
   SPDE_HIP(hipMemsetAsync(ic->d_status, 0, sizeof(int), s));
   for (int c = 0; c < kColours; ++c) {
     const int64_t r0 = ic->coff[c], r1 = ic->coff[c + 1];
-    if (r1 > r0)
+    if (r1 <= r0) continue;
+    if (ic->max_low <= kIcRowMax)
+      hipLaunchKernelGGL(ic_factorw_kernel, dim3((unsigned)((r1 - r0 + 3) / 4)), dim3(256), 0, s, r0, r1, ic->d_crow,
+                         ic->d_lptr, ic->d_lcol, ic->d_lkey, ic->d_lsrc, qval, ic->d_adiag, shift, ic->d_l64,
+                         ic->d_dg64, ic->d_status);
+    else
       hipLaunchKernelGGL(ic_factor_kernel, dim3(blocks_of(r1 - r0)), dim3(256), 0, s, r0, r1, ic->d_crow, ic->d_lptr,
                          ic->d_lcol, ic->d_lkey, ic->d_lsrc, qval, ic->d_adiag, shift, ic->d_l64, ic->d_dg64,
                          ic->d_status);
